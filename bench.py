@@ -1,0 +1,145 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X compressible FV step (one JSON line on stdout).
+
+Workload (the single-GPU configuration the metric is quoted on): 3-D periodic
+256^3 ideal gas (gamma 1.4) on [0, 2pi]^3, Taylor-Green velocity at Mach 0.3,
+p = 1/gamma, rho = 1 + 0.1 * sum of 8 seeded Fourier modes (SplitMix64 seed
+0x5EED); plain method = WCNS (characteristic WENO5-Z) + HLLC everywhere
+(FluxMode::Weno) with the positivity limiter, SSP-RK3, dt = 0.5 / CFL rate
+recomputed on the device every step.
+
+A "step" is one SSP-RK3 step (3 stages) over the whole grid with the state
+resident in HBM.  value = cells * steps * 3 / seconds (cell-updates per second
+per RK stage).  The roofline entry is that of the dominant kernel (the fused
+axis pass), timed with HIP events on the solver's stream inside the timed
+region.  The CPU baseline is the oracle (bit-faithful restatement of the
+reference CPU solver) timed for one step of the same 256^3 workload on the
+host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N = 256
+CFL = 0.5
+# Algorithmic FP64 work of the WCNS (Weno-mode) residual, counted from the
+# reference formulas with sparse eigenvectors, div/sqrt counted as 1 FLOP:
+# 3,500 FLOP per cell per RK stage in 3-D, i.e. 3,500/3 per cell per axis pass.
+FLOP_PER_CELL_STAGE = 3500.0
+BYTES_PER_CELL_STAGE = 120.0  # read U^(s), read U^n, write U^(s+1) (5 x f64 each)
+PEAK_FP64 = 78.6e12
+PEAK_HBM = 8.0e12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=N)
+    ap.add_argument("--fast", action="store_true",
+                    help="FMA-contracted variant (differs from the reference arithmetic by rounding)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    return ap.parse_args()
+
+
+def cpu_baseline(n, u, steps):
+    """Oracle (restated reference CPU solver) on the host cores: `steps`
+    SSP-RK3 steps of the same workload, CFL dt."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle  # noqa: E402  (CPU-baseline leg only)
+
+    lib = _oracle.load()
+    threads = lib.emref_max_threads()
+    o = _oracle.OracleSolver((n, n, n), (0, 0, 0), (2 * np.pi,) * 3)
+    o.set_u(u)
+    o.set_opts(mode="weno", cfl=CFL)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        o.step(o.dt_cfl())
+    el = time.perf_counter() - t0
+    return {
+        "value": n ** 3 * steps * 3 / el,
+        "unit": "cell-updates/s per RK stage",
+        "cores": int(threads),
+        "kind": "port",
+        "sample": f"{steps} SSP-RK3 step(s) (dt_cfl + 3 stages) of the same {n}^3 Weno workload, "
+                  f"OpenMP over pencils, {el:.1f} s wall",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU decomposition is a design note only)", file=sys.stderr)
+        sys.exit(2)
+    import emergence_amd as em
+
+    n = args.n
+    lo, hi = (0.0, 0.0, 0.0), (2 * np.pi,) * 3
+    u = em.case_init(1, (n, n, n), lo, hi, 6, [0.3, float(0x5EED)])
+    g = em.HipSolver((n, n, n), lo, hi, u, mode="weno", cfl=CFL, precise=not args.fast)
+    tiles = g.autotune(3)
+    g.upload(u)
+    if args.warmup:
+        g.run_steps(args.warmup)
+    g.synchronize()
+    t0 = time.perf_counter()
+    pass_ms = g.run_steps(args.steps, timed=True)
+    g.synchronize()
+    el = time.perf_counter() - t0
+    cells = n ** 3
+    value = cells * args.steps * 3 / el
+    ms_per_step = el / args.steps * 1e3
+    # dominant kernel: the slowest axis pass
+    dom = int(np.argmax(pass_ms))
+    flop_launch = FLOP_PER_CELL_STAGE / 3.0 * cells
+    achieved = flop_launch / (pass_ms[dom] * 1e-3)
+    rec = {
+        "metric": "FP64 cell-updates/s per RK stage (256³, batched AMR); adjoint/forward cost ratio",
+        "value": value,
+        "unit": "cell-updates/s per RK stage",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Fourier-mode density, Taylor-Green velocity)",
+        "config": {
+            "workload": f"3-D periodic {n}^3 ideal gas, WCNS+HLLC+positivity (FluxMode::Weno), SSP-RK3, CFL {CFL}",
+            "cells": cells,
+            "variant": "fast" if args.fast else "precise",
+            "tiles": tiles,
+            "pass_ms": pass_ms,
+        },
+        "roofline": {
+            "bound": "fp64",
+            "kernel": f"axis_pass d={dom}",
+            "achieved": achieved / 1e12,
+            "peak": PEAK_FP64 / 1e12,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP64,
+            "traffic": None,
+            "hbm_fraction": value * BYTES_PER_CELL_STAGE / PEAK_HBM,
+            "fp64_fraction": value * FLOP_PER_CELL_STAGE / PEAK_FP64,
+        },
+    }
+    if not args.no_cpu_baseline:
+        rec["cpu_baseline"] = cpu_baseline(n, u, args.cpu_steps)
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,87 @@
+"""ctypes loader of the HIP library (emergence_amd/libemhip.so).
+
+The library is the product: every call below runs gfx950 kernels.  There is
+no CPU fallback; a missing or unloadable library raises.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libemhip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "em_hip.h")
+
+# every entry point declared in include/em_hip.h
+EXPORTS = (
+    "em_last_error",
+    "em_solver_create",
+    "em_solver_destroy",
+    "em_solver_set_variant",
+    "em_solver_set_tiles",
+    "em_solver_autotune",
+    "em_solver_upload",
+    "em_solver_download",
+    "em_solver_rhs",
+    "em_solver_step",
+    "em_solver_dt_cfl",
+    "em_solver_run_to",
+    "em_solver_run_steps",
+    "em_solver_counters",
+    "em_solver_time",
+    "em_solver_synchronize",
+    "em_case_init",
+)
+
+_lib = None
+
+
+class EmError(RuntimeError):
+    """Failure reported by the HIP library (code, message)."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def load():
+    """Load libemhip.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                           "(make -C emergence_amd/csrc)")
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    D = C.POINTER(C.c_double)
+    I64 = C.POINTER(C.c_int64)
+    I32 = C.POINTER(C.c_int32)
+    U64 = C.POINTER(C.c_uint64)
+    lib.em_last_error.restype = C.c_char_p
+    lib.em_last_error.argtypes = []
+    lib.em_solver_create.argtypes = [C.POINTER(P), I64, D, D, C.c_int32, D, I32, D, C.c_int32, C.c_int32,
+                                     C.c_double, C.c_double, C.c_double, D]
+    lib.em_solver_destroy.argtypes = [P]
+    lib.em_solver_set_variant.argtypes = [P, C.c_int32]
+    lib.em_solver_set_tiles.argtypes = [P, I32]
+    lib.em_solver_autotune.argtypes = [P, C.c_int32, I32]
+    lib.em_solver_upload.argtypes = [P, D]
+    lib.em_solver_download.argtypes = [P, D]
+    lib.em_solver_rhs.argtypes = [P, C.c_double, D, U64]
+    lib.em_solver_step.argtypes = [P, C.c_double]
+    lib.em_solver_dt_cfl.argtypes = [P, D]
+    lib.em_solver_run_to.argtypes = [P, C.c_double, I64]
+    lib.em_solver_run_steps.argtypes = [P, C.c_int64, D]
+    lib.em_solver_counters.argtypes = [P, U64]
+    lib.em_solver_time.argtypes = [P, D]
+    lib.em_solver_synchronize.argtypes = [P]
+    lib.em_case_init.argtypes = [C.c_int32, I64, D, D, C.c_int32, D, D]
+    for name in EXPORTS:
+        getattr(lib, name).restype = C.c_char_p if name == "em_last_error" else C.c_int
+    _lib = lib
+    return lib
+
+
+def check(rc):
+    if rc != 0:
+        raise EmError(rc, load().em_last_error().decode())
+    return rc

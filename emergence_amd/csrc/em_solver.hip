@@ -1,0 +1,831 @@
+// Host side of the uniform-grid solver: the GpuSolver object behind the C ABI
+// (include/em_hip.h), its small kernels (CFL rate, device-resident dt, ghost
+// padding for download), graph-replayed device-driven stepping, the tile
+// autotuner and the workload initialisers.
+//
+// Reference: fluid/crates/compressible-cuda/src/lib.rs:155-477 (GpuSolver),
+// fluid/crates/compressible/src/solver.rs (Solver: step, dt_cfl, run_to).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/em_hip.h"
+#include "em_pass.h"
+#include "em_types.h"
+
+namespace em {
+
+static thread_local std::string g_last_error;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+static void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Error(EM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPCHK(x) hip_check((x), #x)
+
+// ------------------------------------------------------------------ kernels
+
+__global__ void k_set_dt(DevScalars* sc, double dt) { sc->dt = dt; }
+
+// Zero fill on the solver's stream (a non-blocking stream does not order
+// against work on the NULL stream, so no NULL-stream memset is used).
+__global__ void k_fill_zero(double* p, long long n) {
+    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (long long)gridDim.x * 256ll) p[i] = 0.0;
+}
+
+__global__ void k_set_run(DevScalars* sc, double t_end) {
+    sc->t_end = t_end;
+    sc->active = 1;
+    sc->err = 0;
+    sc->steps = 0;
+}
+
+__global__ void k_zero_rate(DevScalars* sc) { sc->rate = 0ull; }
+
+// t += dt after a step (solver.rs:737)
+__global__ void k_advance(DevScalars* sc) {
+    if (sc->active == 0) return;
+    sc->t = sc->t + sc->dt;
+    sc->steps += 1;
+}
+
+// dt of the next step from the CFL rate of the current state.
+//   run_to == 0: dt = cfl / rate (fixed number of steps)
+//   run_to == 1: Solver::run_to rule (solver.rs:758-768): stop once
+//                t >= t_end - 1e-10 |t_end|; dt = min(cfl / rate, t_end - t);
+//                collapse if dt is not finite or <= 1e-12 max(|t_end|, 1e-300)
+__global__ void k_dt_from_rate(DevScalars* sc, double cfl, int run_to) {
+    if (sc->active == 0) return;
+    const double rate = __longlong_as_double((long long)sc->rate);
+    double dt = cfl / rate;
+    if (run_to) {
+        const double te = sc->t_end;
+        if (!(sc->t < te - 1e-10 * fabs(te))) {
+            sc->active = 0;
+            return;
+        }
+        dt = fmin(dt, te - sc->t);
+        if (!(isfinite(dt) && dt > 1e-12 * fmax(fabs(te), 1e-300))) {
+            sc->err |= 1;
+            sc->active = 0;
+            return;
+        }
+    } else if (!(isfinite(dt) && dt > 0.0)) {
+        sc->err |= 1;
+        sc->active = 0;
+        return;
+    }
+    sc->dt = dt;
+    sc->rate = 0ull;
+}
+
+// CFL rate max over interior cells: dt_cfl (solver.rs:478-523).  Wave
+// reduction by shuffles, one atomicMax per workgroup on the IEEE bits of a
+// non-negative double.
+__global__ __launch_bounds__(256) void k_rate(const double* __restrict__ U, long long ncell, Eos e, double dx0,
+                                              double dx1, double dx2, int a0, int a1, int a2, DevScalars* sc) {
+    __shared__ double red[4];
+    double r = 0.0;
+    for (long long c = blockIdx.x * 256ll + threadIdx.x; c < ncell; c += (long long)gridDim.x * 256ll) {
+        double u[5], w[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) u[k] = U[k * ncell + c];
+        to_prim(e, u, w);
+        const double cs = sound_speed(e, w[0], w[4]);
+        double ac = -0.0;
+        if (a0) ac += (fabs(w[1]) + cs) / dx0;
+        if (a1) ac += (fabs(w[2]) + cs) / dx1;
+        if (a2) ac += (fabs(w[3]) + cs) / dx2;
+        r = fmax(r, ac);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) r = fmax(r, __shfl_xor(r, off));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = r;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        m = m > 0.0 ? m : 0.0;
+        atomicMax(&sc->rate, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// Interior SoA -> ghost-padded SoA (padded extents m[3], ghost offsets g[3]).
+__global__ void k_scatter(const double* __restrict__ U, long long ncell, int n0, int n1, int n2,
+                          double* __restrict__ Pd, long long len, int m1, int m2, int g0, int g1, int g2) {
+    const long long c = blockIdx.x * 256ll + threadIdx.x;
+    if (c >= ncell) return;
+    const int k = (int)(c % n2);
+    const int j = (int)((c / n2) % n1);
+    const int i = (int)(c / ((long long)n1 * n2));
+    const long long q = ((long long)(i + g0) * m1 + (j + g1)) * m2 + (k + g2);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) Pd[v * len + q] = U[v * ncell + c];
+}
+
+// Ghost sweep along axis D of a ghost-padded SoA array: one thread per
+// pencil over the whole padded transverse plane, layers gi = 0..ng-1 low and
+// high (solver.rs:278-348).  Sweeps run x, y, z in turn so edges and corners
+// come out as in the reference.
+template <int D>
+__global__ void k_ghost_sweep(double* __restrict__ Pd, long long len, int m0, int m1, int m2, int n, int ng, Eos e,
+                              int k0, int k1, const double* __restrict__ st) {
+    const int mm[3] = {m0, m1, m2};
+    constexpr int E0 = D == 0 ? 1 : 0;
+    constexpr int E1 = D == 2 ? 1 : 2;
+    const long long p = blockIdx.x * 128ll + threadIdx.x;
+    if (p >= (long long)mm[E0] * mm[E1]) return;
+    const int a = (int)(p / mm[E1]), b = (int)(p % mm[E1]);
+    const long long sd[3] = {(long long)m1 * m2, (long long)m2, 1ll};
+    const long long base = (long long)a * sd[E0] + (long long)b * sd[E1];
+    const int kind[2] = {k0, k1};
+    double state[2][5];
+#pragma unroll
+    for (int q = 0; q < 10; ++q) state[q / 5][q % 5] = st[q];
+    auto ld = [&](int x, double* o) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v) o[v] = Pd[v * len + base + (long long)(ng + x) * sd[D]];
+    };
+    for (int gi = 0; gi < ng; ++gi) {
+        double u[5];
+        bc_ghost<D>(e, n, gi - ng, kind, state, ld, u);
+        const long long lo = base + (long long)gi * sd[D];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) Pd[v * len + lo] = u[v];
+        bc_ghost<D>(e, n, n + gi, kind, state, ld, u);
+        const long long hi = base + (long long)(n + ng + gi) * sd[D];
+#pragma unroll
+        for (int v = 0; v < 5; ++v) Pd[v * len + hi] = u[v];
+    }
+}
+
+// ------------------------------------------------------------------ solver
+
+struct Solver {
+    int64_t n[3];
+    double lo[3], dx[3];
+    int ng;
+    Eos eos;
+    int bc_kind[3][2];
+    double bc_state[3][2][5];
+    int mode, positivity;
+    double cfl, tol;
+    int precise = 1;  // reference arithmetic by default
+    int cfg[3] = {1, 1, 2};
+    long long ncell;
+    long long stride[3];
+    int nactive;
+    int act[3];
+    hipStream_t stream = nullptr;
+    double* U[3] = {nullptr, nullptr, nullptr};  // U^n, stage 1, stage 2
+    double* R = nullptr;
+    double* pad = nullptr;
+    double* bcdev = nullptr;  // [3][2][5] boundary states for the ghost sweep
+    DevScalars* sc = nullptr;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};  // per run mode
+    int graph_variant[2] = {-1, -1};
+    int graph_cfg[2][3];
+    bool rate_valid = false;
+
+    bool active(int d) const { return n[d] > 1; }
+    int m(int d) const { return active(d) ? (int)(n[d] + 2 * ng) : 1; }
+    int g(int d) const { return active(d) ? ng : 0; }
+    long long len() const { return (long long)m(0) * m(1) * m(2); }
+
+    Solver(const int64_t nn[3], const double lo_[3], const double hi[3], int ng_, const double e[4],
+           const int32_t bk[6], const double bs[30], int mode_, int pos, double cfl_, double tol_, double t0) {
+        for (int d = 0; d < 3; ++d) {
+            if (nn[d] < 1) throw Error(EM_ERR_ARG, "mesh extents must be >= 1");
+            n[d] = nn[d];
+            lo[d] = lo_[d];
+            dx[d] = (hi[d] - lo_[d]) / (double)nn[d];  // Mesh::new, solver.rs:32
+        }
+        ng = ng_;
+        eos = Eos{e[0], e[1], e[2], e[3]};
+        mode = mode_;
+        positivity = pos;
+        cfl = cfl_;
+        tol = tol_;
+        if (mode < 0 || mode > 2) throw Error(EM_ERR_ARG, "mode must be 0 (Hybrid), 1 (Weno) or 2 (Central)");
+        if (ng < (mode == MODE_CENTRAL ? 3 : 5))
+            throw Error(EM_ERR_ARG, "ghost layers: need ng >= 5 for Hybrid/Weno, >= 3 for Central");
+        nactive = 0;
+        for (int d = 0; d < 3; ++d) {
+            act[d] = active(d) ? 1 : 0;
+            nactive += act[d];
+            for (int sd = 0; sd < 2; ++sd) {
+                bc_kind[d][sd] = bk[2 * d + sd];
+                if (bc_kind[d][sd] < 0 || bc_kind[d][sd] > 4) throw Error(EM_ERR_ARG, "boundary kind out of range");
+                for (int q = 0; q < 5; ++q) bc_state[d][sd][q] = bs ? bs[(2 * d + sd) * 5 + q] : 0.0;
+            }
+            if (active(d) && n[d] < 2 && (bc_kind[d][0] == BC_CHARACTERISTIC || bc_kind[d][1] == BC_CHARACTERISTIC))
+                throw Error(EM_ERR_ARG, "characteristic boundaries need at least 2 cells");
+        }
+        if (nactive == 0) throw Error(EM_ERR_ARG, "mesh has no active axis");
+        ncell = n[0] * n[1] * n[2];
+        stride[0] = n[1] * n[2];
+        stride[1] = n[2];
+        stride[2] = 1;
+        HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        for (int b = 0; b < 3; ++b) HIPCHK(hipMalloc(&U[b], sizeof(double) * 5 * ncell));
+        HIPCHK(hipMalloc(&R, sizeof(double) * 5 * ncell));
+        HIPCHK(hipMalloc(&sc, sizeof(DevScalars)));
+        HIPCHK(hipMalloc(&bcdev, sizeof(double) * 30));
+        HIPCHK(hipMemcpyAsync(bcdev, &bc_state[0][0][0], sizeof(double) * 30, hipMemcpyHostToDevice, stream));
+        DevScalars h{};
+        h.t = t0;
+        h.active = 1;
+        HIPCHK(hipMemcpyAsync(sc, &h, sizeof h, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        // default tiles: pencils along the contiguous axis take long tiles
+        for (int d = 0; d < 3; ++d) cfg[d] = (stride[d] == 1) ? 2 : 0;
+    }
+
+    ~Solver() {
+        for (auto& gx : graph)
+            if (gx) (void)hipGraphExecDestroy(gx);
+        for (auto* p : U)
+            if (p) (void)hipFree(p);
+        if (R) (void)hipFree(R);
+        if (pad) (void)hipFree(pad);
+        if (sc) (void)hipFree(sc);
+        if (bcdev) (void)hipFree(bcdev);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void upload(const double* u) {
+        std::vector<double> h(5 * ncell);
+        const int m1 = m(1), m2 = m(2);
+        for (long long i = 0; i < n[0]; ++i)
+            for (long long j = 0; j < n[1]; ++j)
+                for (long long k = 0; k < n[2]; ++k) {
+                    const long long q = ((i + g(0)) * m1 + (j + g(1))) * m2 + (k + g(2));
+                    const long long c = (i * n[1] + j) * n[2] + k;
+                    for (int v = 0; v < 5; ++v) h[v * ncell + c] = u[q * 5 + v];
+                }
+        HIPCHK(hipMemcpyAsync(U[0], h.data(), sizeof(double) * 5 * ncell, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        rate_valid = false;
+    }
+
+    // Ghost-padded SoA image of U[0] in `pad` (ghosts by the reference's
+    // axis-ordered sweeps).
+    void fill_pad() {
+        const long long L = len();
+        if (!pad) {
+            HIPCHK(hipMalloc(&pad, sizeof(double) * 5 * L));
+            const long long nz = 5 * L;
+            hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)std::min<long long>((nz + 255) / 256, 4096)), dim3(256), 0,
+                               stream, pad, nz);
+            HIPCHK(hipGetLastError());
+        }
+        const unsigned nb = (unsigned)((ncell + 255) / 256);
+        hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, stream, U[0], ncell, (int)n[0], (int)n[1], (int)n[2],
+                           pad, L, m(1), m(2), g(0), g(1), g(2));
+        HIPCHK(hipGetLastError());
+        for (int d = 0; d < 3; ++d) {
+            if (!active(d)) continue;
+            const int e0 = d == 0 ? 1 : 0, e1 = d == 2 ? 1 : 2;
+            const long long np = (long long)m(e0) * m(e1);
+            const unsigned gb = (unsigned)((np + 127) / 128);
+            const double* st = bcdev + d * 10;
+            if (d == 0)
+                hipLaunchKernelGGL(k_ghost_sweep<0>, dim3(gb), dim3(128), 0, stream, pad, L, m(0), m(1), m(2),
+                                   (int)n[0], ng, eos, bc_kind[0][0], bc_kind[0][1], st);
+            else if (d == 1)
+                hipLaunchKernelGGL(k_ghost_sweep<1>, dim3(gb), dim3(128), 0, stream, pad, L, m(0), m(1), m(2),
+                                   (int)n[1], ng, eos, bc_kind[1][0], bc_kind[1][1], st);
+            else
+                hipLaunchKernelGGL(k_ghost_sweep<2>, dim3(gb), dim3(128), 0, stream, pad, L, m(0), m(1), m(2),
+                                   (int)n[2], ng, eos, bc_kind[2][0], bc_kind[2][1], st);
+            HIPCHK(hipGetLastError());
+        }
+    }
+
+    void download(double* u) {
+        fill_pad();
+        const long long L = len();
+        std::vector<double> h(5 * L);
+        HIPCHK(hipMemcpyAsync(h.data(), pad, sizeof(double) * 5 * L, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        for (long long q = 0; q < L; ++q)
+            for (int v = 0; v < 5; ++v) u[q * 5 + v] = h[v * L + q];
+    }
+
+    PassArgs pass_args(int d) const {
+        PassArgs a{};
+        a.sc = sc;
+        for (int e = 0; e < 3; ++e) {
+            a.n[e] = (int)n[e];
+            a.stride[e] = stride[e];
+            a.dx_all[e] = dx[e];
+            a.active[e] = act[e];
+        }
+        a.ncell = ncell;
+        a.eos = eos;
+        for (int sd = 0; sd < 2; ++sd) {
+            a.bc_kind[sd] = bc_kind[d][sd];
+            for (int q = 0; q < 5; ++q) a.bc_state[sd][q] = bc_state[d][sd][q];
+        }
+        a.inv_dx = 1.0 / dx[d];
+        a.dx_d = dx[d];
+        a.ndim_pos = positivity ? (double)nactive : 0.0;
+        a.tol = tol;
+        // transverse axes: the pencils of a workgroup are neighbours along the
+        // innermost active transverse axis
+        int t[2], nt = 0;
+        for (int e = 0; e < 3; ++e)
+            if (e != d) t[nt++] = e;
+        if (active(t[1]) || !active(t[0])) {
+            a.fast_axis = t[1];
+            a.slow_axis = t[0];
+        } else {
+            a.fast_axis = t[0];
+            a.slow_axis = t[1];
+        }
+        a.dfast = stride[d] == 1 ? 1 : 0;
+        const TileCfg tc = kTileCfgs[cfg[d]];
+        a.ntile = (int)((n[d] + tc.T - 1) / tc.T);
+        a.ngroup = (int)((n[a.fast_axis] + tc.P - 1) / tc.P);
+        return a;
+    }
+
+    hipError_t launch(int d, const PassArgs& a) {
+        return precise ? precise::launch_pass(d, mode, cfg[d], a, stream) : fast::launch_pass(d, mode, cfg[d], a, stream);
+    }
+
+    // One stage: residual of Uin over the active axes in order, fused with
+    // the SSP-RK3 combination on the last axis (rk_stage 0: residual only).
+    void stage(const double* Uin, double* Uout, int rk_stage, bool want_rate, int stage_dt_host, double stage_dt,
+               hipEvent_t* ev) {
+        int last = -1;
+        for (int d = 0; d < 3; ++d)
+            if (active(d)) last = d;
+        bool first = true;
+        for (int d = 0; d < 3; ++d) {
+            if (!active(d)) continue;
+            PassArgs a = pass_args(d);
+            a.U = Uin;
+            a.R = R;
+            a.U0 = U[0];
+            a.Uout = Uout;
+            a.first = first ? 1 : 0;
+            a.rk_stage = d == last ? rk_stage : 0;
+            a.want_rate = (d == last && want_rate) ? 1 : 0;
+            a.stage_dt_host = stage_dt_host;
+            a.stage_dt = stage_dt;
+            if (ev) HIPCHK(hipEventRecord(ev[2 * d], stream));
+            HIPCHK(launch(d, a));
+            if (ev) HIPCHK(hipEventRecord(ev[2 * d + 1], stream));
+            first = false;
+        }
+    }
+
+    // SSP-RK3 step with the dt already in sc->dt (solver.rs:659-738):
+    // U1 = U + dt L(U); U2 = 3/4 U + 1/4 (U1 + dt L(U1));
+    // U <- U/3 + 2/3 (U2 + dt L(U2)), the last stage also leaving the CFL
+    // rate of the new state in sc->rate.
+    void step_kernels(std::vector<hipEvent_t>* evs = nullptr) {
+        hipEvent_t* e[3] = {nullptr, nullptr, nullptr};
+        if (evs)
+            for (int s = 0; s < 3; ++s) e[s] = evs->data() + 6 * s;
+        stage(U[0], U[1], 1, false, 0, 0.0, e[0]);
+        stage(U[1], U[2], 2, false, 0, 0.0, e[1]);
+        hipLaunchKernelGGL(k_zero_rate, dim3(1), dim3(1), 0, stream, sc);
+        stage(U[2], U[0], 3, true, 0, 0.0, e[2]);
+        hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, stream, sc);
+        HIPCHK(hipGetLastError());
+    }
+
+    void reset_run(double t_end) {
+        hipLaunchKernelGGL(k_set_run, dim3(1), dim3(1), 0, stream, sc, t_end);
+        HIPCHK(hipGetLastError());
+    }
+
+    void compute_rate() {
+        hipLaunchKernelGGL(k_zero_rate, dim3(1), dim3(1), 0, stream, sc);
+        long long nb = std::min<long long>((ncell + 255) / 256, 2048);
+        hipLaunchKernelGGL(k_rate, dim3((unsigned)nb), dim3(256), 0, stream, U[0], ncell, eos, dx[0], dx[1], dx[2],
+                           act[0], act[1], act[2], sc);
+        HIPCHK(hipGetLastError());
+        rate_valid = true;
+    }
+
+    DevScalars read_scalars() {
+        DevScalars h;
+        HIPCHK(hipMemcpyAsync(&h, sc, sizeof h, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return h;
+    }
+
+    void rhs(double stage_dt, double* out, uint64_t stats[4]) {
+        DevScalars before = read_scalars();
+        reset_run(0.0);
+        stage(U[0], nullptr, 0, false, 1, stage_dt, nullptr);
+        std::vector<double> h(5 * ncell);
+        HIPCHK(hipMemcpyAsync(h.data(), R, sizeof(double) * 5 * ncell, hipMemcpyDeviceToHost, stream));
+        DevScalars after = read_scalars();
+        const long long L = len();
+        std::fill(out, out + 5 * L, 0.0);
+        const int m1 = m(1), m2 = m(2);
+        for (long long i = 0; i < n[0]; ++i)
+            for (long long j = 0; j < n[1]; ++j)
+                for (long long k = 0; k < n[2]; ++k) {
+                    const long long q = ((i + g(0)) * m1 + (j + g(1))) * m2 + (k + g(2));
+                    const long long c = (i * n[1] + j) * n[2] + k;
+                    for (int v = 0; v < 5; ++v) out[q * 5 + v] = h[v * ncell + c];
+                }
+        if (stats) {
+            stats[0] = faces_per_stage();
+            for (int q = 0; q < 3; ++q) stats[1 + q] = after.counters[q] - before.counters[q];
+        }
+    }
+
+    uint64_t faces_per_stage() const {
+        uint64_t f = 0;
+        for (int d = 0; d < 3; ++d)
+            if (active(d)) f += (uint64_t)(n[d] + 1) * (ncell / n[d]);
+        return f;
+    }
+
+    void step(double dt) {
+        reset_run(0.0);
+        hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, stream, sc, dt);
+        step_kernels();
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    double dt_cfl() {
+        compute_rate();
+        DevScalars h = read_scalars();
+        const double rate = __builtin_bit_cast(double, h.rate);
+        return cfl / rate;  // solver.rs:517
+    }
+
+    // One step driven by device-resident dt: dt kernel + stages + advance.
+    void device_step(int run_to) {
+        hipLaunchKernelGGL(k_dt_from_rate, dim3(1), dim3(1), 0, stream, sc, cfl, run_to);
+        step_kernels();
+    }
+
+    hipGraphExec_t get_graph(int run_to) {
+        bool same = graph[run_to] && graph_variant[run_to] == precise;
+        for (int d = 0; d < 3 && same; ++d) same = graph_cfg[run_to][d] == cfg[d];
+        if (same) return graph[run_to];
+        if (graph[run_to]) {
+            HIPCHK(hipGraphExecDestroy(graph[run_to]));
+            graph[run_to] = nullptr;
+        }
+        hipGraph_t gr;
+        HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        device_step(run_to);
+        HIPCHK(hipStreamEndCapture(stream, &gr));
+        HIPCHK(hipGraphInstantiate(&graph[run_to], gr, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(gr));
+        graph_variant[run_to] = precise;
+        for (int d = 0; d < 3; ++d) graph_cfg[run_to][d] = cfg[d];
+        return graph[run_to];
+    }
+
+    int64_t run_to(double t_end) {
+        reset_run(t_end);
+        compute_rate();
+        hipGraphExec_t gx = get_graph(1);
+        const int chunk = 16;
+        for (;;) {
+            for (int q = 0; q < chunk; ++q) HIPCHK(hipGraphLaunch(gx, stream));
+            DevScalars h = read_scalars();
+            if (h.err & 1) throw Error(EM_ERR_COLLAPSE, "time step collapsed: the solution lost admissibility");
+            if (!h.active) return h.steps;
+        }
+    }
+
+    void run_steps(int64_t nsteps, double pass_ms[3]) {
+        reset_run(0.0);
+        compute_rate();
+        if (!pass_ms) {
+            hipGraphExec_t gx = get_graph(0);
+            for (int64_t q = 0; q < nsteps; ++q) HIPCHK(hipGraphLaunch(gx, stream));
+        } else {
+            std::vector<hipEvent_t> ev(18 * (size_t)nsteps);
+            for (auto& x : ev) HIPCHK(hipEventCreate(&x));
+            for (int64_t q = 0; q < nsteps; ++q) {
+                std::vector<hipEvent_t> step_ev(ev.begin() + 18 * q, ev.begin() + 18 * (q + 1));
+                hipLaunchKernelGGL(k_dt_from_rate, dim3(1), dim3(1), 0, stream, sc, cfl, 0);
+                step_kernels(&step_ev);
+            }
+            HIPCHK(hipStreamSynchronize(stream));
+            for (int d = 0; d < 3; ++d) {
+                double tot = 0.0;
+                int cnt = 0;
+                if (active(d)) {
+                    for (int64_t q = 0; q < nsteps; ++q)
+                        for (int s = 0; s < 3; ++s) {
+                            float ms = 0.f;
+                            HIPCHK(hipEventElapsedTime(&ms, ev[18 * q + 6 * s + 2 * d], ev[18 * q + 6 * s + 2 * d + 1]));
+                            tot += ms;
+                            ++cnt;
+                        }
+                }
+                pass_ms[d] = cnt ? tot / cnt : 0.0;
+            }
+            for (auto& x : ev) (void)hipEventDestroy(x);
+        }
+        DevScalars h = read_scalars();
+        if (h.err & 1) throw Error(EM_ERR_COLLAPSE, "time step collapsed: the solution lost admissibility");
+    }
+
+    // Times every tile shape of every active axis on residual-only passes
+    // of the current state and keeps the fastest; the state is untouched and
+    // the counters are restored.
+    void autotune(int repeats, int32_t out[3]) {
+        DevScalars before = read_scalars();
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        for (int d = 0; d < 3; ++d) {
+            if (!active(d)) continue;
+            double best = 1e300;
+            int bestc = cfg[d];
+            for (int c = 0; c < kNumTileCfgs; ++c) {
+                if (kTileCfgs[c].T > 2 * n[d] && c != 0) continue;
+                cfg[d] = c;
+                PassArgs a = pass_args(d);
+                a.U = U[0];
+                a.R = R;
+                a.U0 = U[0];
+                a.Uout = nullptr;
+                a.first = 1;
+                a.rk_stage = 0;
+                a.stage_dt_host = 1;
+                a.stage_dt = 0.0;
+                HIPCHK(launch(d, a));  // warm-up
+                HIPCHK(hipEventRecord(e0, stream));
+                for (int r = 0; r < repeats; ++r) HIPCHK(launch(d, a));
+                HIPCHK(hipEventRecord(e1, stream));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) {
+                    best = ms;
+                    bestc = c;
+                }
+            }
+            cfg[d] = bestc;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        HIPCHK(hipMemcpyAsync(&sc->counters[0], &before.counters[0], sizeof before.counters, hipMemcpyHostToDevice,
+                              stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        if (out)
+            for (int d = 0; d < 3; ++d) out[d] = cfg[d];
+    }
+};
+
+// ------------------------------------------------------------------ cases
+
+static uint64_t splitmix64(uint64_t& s) {  // Steele, Lea & Flood 2014
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+static double unit(uint64_t& s) { return (double)(splitmix64(s) >> 11) * 0x1.0p-53; }
+
+static void prim_to_cons(const double w[5], double gam, double out[5]) {
+    const double rho = w[0];
+    const double ke = 0.5 * rho * (w[1] * w[1] + w[2] * w[2] + w[3] * w[3]);
+    out[0] = rho;
+    out[1] = rho * w[1];
+    out[2] = rho * w[2];
+    out[3] = rho * w[3];
+    out[4] = rho * (w[4] * (1.0 / rho) / (gam - 1.0)) + ke;  // ideal gas rho_e, eos.rs:43-53
+}
+
+static void case_init(int id, const int64_t n[3], const double lo[3], const double hi[3], int ng, const double* prm,
+                      double* u) {
+    auto act = [&](int d) { return n[d] > 1; };
+    const int64_t m[3] = {act(0) ? n[0] + 2 * ng : 1, act(1) ? n[1] + 2 * ng : 1, act(2) ? n[2] + 2 * ng : 1};
+    const int64_t g[3] = {act(0) ? ng : 0, act(1) ? ng : 0, act(2) ? ng : 0};
+    double dx[3];
+    for (int d = 0; d < 3; ++d) dx[d] = (hi[d] - lo[d]) / (double)n[d];
+    std::memset(u, 0, sizeof(double) * 5 * m[0] * m[1] * m[2]);
+    const double gam = 1.4;
+    // Fourier modes of case 1
+    int km[8][3];
+    double ph[8];
+    if (id == 1) {
+        uint64_t s = (uint64_t)(prm ? prm[1] : 0x5EED);
+        for (int q = 0; q < 8; ++q) {
+            for (;;) {
+                int k[3];
+                for (int d = 0; d < 3; ++d) k[d] = (int)(splitmix64(s) % 9) - 4;
+                const int k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+                if (k2 > 0 && k2 <= 16) {
+                    for (int d = 0; d < 3; ++d) km[q][d] = k[d];
+                    break;
+                }
+            }
+            ph[q] = 6.283185307179586 * unit(s);
+        }
+    }
+    for (int64_t i = 0; i < n[0]; ++i)
+        for (int64_t j = 0; j < n[1]; ++j)
+            for (int64_t k = 0; k < n[2]; ++k) {
+                const double x[3] = {lo[0] + ((double)i + 0.5) * dx[0], lo[1] + ((double)j + 0.5) * dx[1],
+                                     lo[2] + ((double)k + 0.5) * dx[2]};
+                double w[5];
+                if (id == 0) {
+                    const double xs = prm ? prm[0] : 0.8, ys = prm ? prm[1] : 0.8;
+                    const bool e = x[0] > xs, nn = x[1] > ys;
+                    if (e && nn) {
+                        w[0] = 1.5; w[1] = 0.0; w[2] = 0.0; w[4] = 1.5;
+                    } else if (!e && nn) {
+                        w[0] = 0.5323; w[1] = 1.206; w[2] = 0.0; w[4] = 0.3;
+                    } else if (!e && !nn) {
+                        w[0] = 0.138; w[1] = 1.206; w[2] = 1.206; w[4] = 0.029;
+                    } else {
+                        w[0] = 0.5323; w[1] = 0.0; w[2] = 1.206; w[4] = 0.3;
+                    }
+                    w[3] = 0.0;
+                } else if (id == 1) {
+                    const double U0 = prm ? prm[0] : 0.3;
+                    double s = 0.0;
+                    for (int q = 0; q < 8; ++q)
+                        s += std::cos(km[q][0] * x[0] + km[q][1] * x[1] + km[q][2] * x[2] + ph[q]);
+                    w[0] = 1.0 + 0.1 * s;
+                    w[1] = U0 * std::sin(x[0]) * std::cos(x[1]) * std::cos(x[2]);
+                    w[2] = -U0 * std::cos(x[0]) * std::sin(x[1]) * std::cos(x[2]);
+                    w[3] = 0.0;
+                    w[4] = 1.0 / gam;
+                } else if (id == 2) {
+                    const double TAU = 6.283185307179586;
+                    w[0] = 1.0 + 0.1 * std::sin(TAU * x[0]) * std::cos(TAU * x[1]);
+                    w[1] = 0.3; w[2] = 0.2; w[3] = 0.1; w[4] = 1.0;
+                } else {
+                    throw Error(EM_ERR_ARG, "unknown case id");
+                }
+                const int64_t q = ((i + g[0]) * m[1] + (j + g[1])) * m[2] + (k + g[2]);
+                prim_to_cons(w, gam, u + 5 * q);
+            }
+}
+
+}  // namespace em
+
+// ------------------------------------------------------------------ C ABI
+
+struct em_solver {
+    em::Solver* s;
+};
+
+template <class F>
+static int guarded(F&& f) {
+    try {
+        f();
+        em::g_last_error.clear();
+        return EM_OK;
+    } catch (const em::Error& e) {
+        em::g_last_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        em::g_last_error = e.what();
+        return EM_ERR_STATE;
+    }
+}
+
+#define NEED(h)                                                              \
+    if (!(h) || !(h)->s) {                                                   \
+        em::g_last_error = "null solver handle";                             \
+        return EM_ERR_ARG;                                                   \
+    }
+
+extern "C" {
+
+const char* em_last_error(void) { return em::g_last_error.c_str(); }
+
+int em_solver_create(em_solver** out, const int64_t n[3], const double lo[3], const double hi[3], int32_t ng,
+                     const double eos[4], const int32_t bc_kind[6], const double bc_state[30], int32_t mode,
+                     int32_t positivity, double cfl, double tol, double t0, const double* u) {
+    if (!out || !n || !lo || !hi || !eos || !bc_kind || !u) {
+        em::g_last_error = "null argument";
+        return EM_ERR_ARG;
+    }
+    return guarded([&] {
+        auto* s = new em::Solver(n, lo, hi, ng, eos, bc_kind, bc_state, mode, positivity, cfl, tol, t0);
+        try {
+            s->upload(u);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = new em_solver{s};
+    });
+}
+
+int em_solver_destroy(em_solver* h) {
+    if (!h) return EM_OK;
+    delete h->s;
+    delete h;
+    return EM_OK;
+}
+
+int em_solver_set_variant(em_solver* h, int32_t precise) {
+    NEED(h);
+    h->s->precise = precise ? 1 : 0;
+    return EM_OK;
+}
+
+int em_solver_set_tiles(em_solver* h, const int32_t cfg[3]) {
+    NEED(h);
+    for (int d = 0; d < 3; ++d)
+        if (cfg[d] < 0 || cfg[d] >= em::kNumTileCfgs) {
+            em::g_last_error = "tile configuration out of range";
+            return EM_ERR_ARG;
+        }
+    for (int d = 0; d < 3; ++d) h->s->cfg[d] = cfg[d];
+    return EM_OK;
+}
+
+int em_solver_autotune(em_solver* h, int32_t repeats, int32_t cfg_out[3]) {
+    NEED(h);
+    return guarded([&] { h->s->autotune(repeats > 0 ? repeats : 3, cfg_out); });
+}
+
+int em_solver_upload(em_solver* h, const double* u) {
+    NEED(h);
+    return guarded([&] { h->s->upload(u); });
+}
+
+int em_solver_download(em_solver* h, double* u) {
+    NEED(h);
+    return guarded([&] { h->s->download(u); });
+}
+
+int em_solver_rhs(em_solver* h, double stage_dt, double* rhs, uint64_t stats[4]) {
+    NEED(h);
+    return guarded([&] { h->s->rhs(stage_dt, rhs, stats); });
+}
+
+int em_solver_step(em_solver* h, double dt) {
+    NEED(h);
+    return guarded([&] { h->s->step(dt); });
+}
+
+int em_solver_dt_cfl(em_solver* h, double* dt) {
+    NEED(h);
+    return guarded([&] { *dt = h->s->dt_cfl(); });
+}
+
+int em_solver_run_to(em_solver* h, double t_end, int64_t* steps) {
+    NEED(h);
+    return guarded([&] {
+        int64_t k = h->s->run_to(t_end);
+        if (steps) *steps = k;
+    });
+}
+
+int em_solver_run_steps(em_solver* h, int64_t nsteps, double pass_ms[3]) {
+    NEED(h);
+    return guarded([&] { h->s->run_steps(nsteps, pass_ms); });
+}
+
+int em_solver_counters(em_solver* h, uint64_t out[3]) {
+    NEED(h);
+    return guarded([&] {
+        em::DevScalars d = h->s->read_scalars();
+        for (int q = 0; q < 3; ++q) out[q] = d.counters[q];
+    });
+}
+
+int em_solver_time(em_solver* h, double* t) {
+    NEED(h);
+    return guarded([&] { *t = h->s->read_scalars().t; });
+}
+
+int em_solver_synchronize(em_solver* h) {
+    NEED(h);
+    return guarded([&] { em::hip_check(hipStreamSynchronize(h->s->stream), "hipStreamSynchronize"); });
+}
+
+int em_case_init(int32_t case_id, const int64_t n[3], const double lo[3], const double hi[3], int32_t ng,
+                 const double* params, double* u) {
+    if (!n || !lo || !hi || !u) {
+        em::g_last_error = "null argument";
+        return EM_ERR_ARG;
+    }
+    return guarded([&] { em::case_init(case_id, n, lo, hi, ng, params, u); });
+}
+
+}  // extern "C"

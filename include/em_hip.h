@@ -1,0 +1,132 @@
+/* C ABI of the MI355X (gfx950) compressible finite-volume solver.
+ *
+ * Drop-in for the reference's GPU solver object
+ *   GpuSolver<f64> (fluid/crates/compressible-cuda/src/lib.rs:155-477),
+ * which a host binds for the Euler system the way the reference's CPU
+ * `Solver` (fluid/crates/compressible/src/solver.rs:139-795) is driven:
+ * plain pointers and sizes, caller-owned host buffers, library-owned device
+ * buffers, one handle = one HIP stream, not thread-safe per handle.
+ *
+ * Host state arrays use the reference's CPU layout: ghost-padded AoS float64,
+ * cell index (i*m1 + j)*m2 + k with m_d = n_d + 2*ng on active axes (n_d > 1)
+ * and 1 otherwise, five conserved values (rho, rho u, rho v, rho w, E) per cell
+ * (solver.rs:40-77).
+ *
+ * Error convention: every call returns 0 on success and a negative code on
+ * failure (the reference returns Result<_, Box<dyn Error>>); em_last_error()
+ * gives the message of the calling thread's last failure.
+ */
+#ifndef EM_HIP_H
+#define EM_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct em_solver em_solver;
+
+enum {
+    EM_OK = 0,
+    EM_ERR_ARG = -1,       /* invalid argument or unsupported option */
+    EM_ERR_HIP = -2,       /* HIP runtime / launch failure */
+    EM_ERR_COLLAPSE = -3,  /* "time step collapsed" (lib.rs:447-449) */
+    EM_ERR_STATE = -4      /* call order / state error */
+};
+
+/* Boundary kinds (solver.rs:8-20): 0 periodic, 1 outflow, 2 reflect,
+ * 3 Dirichlet (fixed conserved state), 4 characteristic (far-field state). */
+/* Flux modes (scheme.rs:39-47): 0 Hybrid, 1 Weno (WCNS everywhere),
+ * 2 Central (KEEP-PE everywhere). */
+
+/* Thread-local message of the last failure ("" if none). */
+const char* em_last_error(void);
+
+/* GpuSolver::from_cpu (lib.rs:186-282): snapshot of mesh, boundary
+ * conditions and scheme options plus the conserved state.
+ *   n[3], lo[3], hi[3]  mesh (Mesh::new, solver.rs:31-34), ng ghost layers
+ *                       (>= 5 for Hybrid/Weno, >= 3 for Central)
+ *   eos[4]              gamma, p_inf, b, q (eos.rs:12-24)
+ *   bc_kind[6]          [axis][side]
+ *   bc_state[30]        [axis][side][5] conserved state for Dirichlet and
+ *                       characteristic sides (ignored otherwise)
+ *   mode, positivity, cfl, tol, t0   Solver fields (solver.rs:144-152)
+ *   u                   ghost-padded AoS conserved state (ghosts ignored) */
+int em_solver_create(em_solver** out, const int64_t n[3], const double lo[3], const double hi[3], int32_t ng,
+                     const double eos[4], const int32_t bc_kind[6], const double bc_state[30], int32_t mode,
+                     int32_t positivity, double cfl, double tol, double t0, const double* u);
+
+int em_solver_destroy(em_solver* s);
+
+/* Arithmetic variant: precise != 0 evaluates every expression without FMA
+ * contraction (reproduces the reference CPU arithmetic; the default); 0 =
+ * contracted (fast).  No counterpart in the reference (its GPU path has one
+ * build). */
+int em_solver_set_variant(em_solver* s, int32_t precise);
+
+/* Tile shape per axis (index into the library's tile table), and the
+ * project autotuner that times every shape on the current state and keeps
+ * the fastest per axis.  cfg_out may be NULL. */
+int em_solver_set_tiles(em_solver* s, const int32_t cfg[3]);
+int em_solver_autotune(em_solver* s, int32_t repeats, int32_t cfg_out[3]);
+
+/* Replace the conserved state (ghosts ignored). */
+int em_solver_upload(em_solver* s, const double* u);
+
+/* GpuSolver::download (lib.rs:457-461): conserved state in the CPU layout;
+ * ghost cells are filled from the current interior with the boundary
+ * conditions, axes in order, as Solver::fill_ghosts (solver.rs:259-348). */
+int em_solver_download(em_solver* s, double* u);
+
+/* Per-stage residual -div F of the current state (Solver::rhs,
+ * solver.rs:351-475) with the positivity limiter's lambda from stage_dt;
+ * written in the CPU layout with zero ghosts.  stats (may be NULL) receives
+ * [faces, shock-path faces, first-order fallbacks, limited faces]
+ * (scheme.rs:101-117) of this evaluation. */
+int em_solver_rhs(em_solver* s, double stage_dt, double* rhs, uint64_t stats[4]);
+
+/* GpuSolver::step (lib.rs:401-412): one SSP-RK3 step of size dt. */
+int em_solver_step(em_solver* s, double dt);
+
+/* GpuSolver::dt_cfl (lib.rs:415-441): cfl / max over interior cells of
+ * sum_d (|u_d| + c) / dx_d. */
+int em_solver_dt_cfl(em_solver* s, double* dt);
+
+/* GpuSolver::run_to (lib.rs:443-454): CFL steps until t_end (stop rule of
+ * Solver::run_to, solver.rs:758-768).  dt lives on the device; steps run as
+ * replayed HIP graphs with one host synchronisation per chunk of steps. */
+int em_solver_run_to(em_solver* s, double t_end, int64_t* steps);
+
+/* nsteps CFL-limited steps (dt = cfl / rate of the state at each step),
+ * device-resident dt, graph-replayed.  With pass_ms != NULL the steps are
+ * launched eagerly with HIP events around every axis pass on the solver's
+ * stream and pass_ms[3] receives the mean duration (ms) of one pass along
+ * each axis.  Returns EM_ERR_COLLAPSE if dt became non-finite or <= 0. */
+int em_solver_run_steps(em_solver* s, int64_t nsteps, double pass_ms[3]);
+
+/* GpuSolver::counters (lib.rs:464-467): [shock-path faces, first-order
+ * fallbacks, positivity-limited faces] accumulated since creation. */
+int em_solver_counters(em_solver* s, uint64_t out[3]);
+
+/* Solver time t (lib.rs:164). */
+int em_solver_time(em_solver* s, double* t);
+
+/* GpuSolver::synchronize (lib.rs:469-472). */
+int em_solver_synchronize(em_solver* s);
+
+/* Workload initialisers (host, ghost-padded AoS output of the given mesh),
+ * mirroring the reference's example set-ups:
+ *   case 0  2-D four-quadrant Riemann problem, params {x_split, y_split}
+ *           (compressible-cuda/examples/gpu_validate.rs:100-116)
+ *   case 1  3-D periodic Taylor-Green velocity (U0 = params[0]) with
+ *           p = 1/gamma and rho = 1 + 0.1 * sum of 8 random Fourier modes
+ *           (|k| <= 4, SplitMix64 seed params[1] as an integer) on [0,2pi]^3
+ *   case 2  cpu_validate smooth wave (examples/cpu_validate.rs:30-39)
+ * lo/hi give the domain (Mesh::x cell centres). */
+int em_case_init(int32_t case_id, const int64_t n[3], const double lo[3], const double hi[3], int32_t ng,
+                 const double* params, double* u);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EM_HIP_H */
