@@ -47,8 +47,26 @@ def parse():
     ap.add_argument("--fast", action="store_true",
                     help="FMA-contracted variant (differs from the reference arithmetic by rounding)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shapes (short profiled runs)")
     return ap.parse_args()
+
+
+def profiled_traffic(variant, d):
+    """HBM bytes per launch of the axis pass along d, from the committed
+    rocprofv3 summary (FETCH_SIZE and WRITE_SIZE runs scaled by a plain-copy
+    calibration); None when no summary for this variant exists."""
+    import re
+
+    path = os.path.join(ROOT, "profiles", f"axis_pass_256cubed_weno_{variant}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if f"axis_pass<{d}," in line and "GB |" in line:
+            m = re.search(r"\| ([0-9.]+) GB \|\s*$", line.strip())
+            if m:
+                return float(m.group(1)) * 1e9
+    return None
 
 
 def cpu_baseline(n, u, steps):
@@ -88,7 +106,7 @@ def main():
     lo, hi = (0.0, 0.0, 0.0), (2 * np.pi,) * 3
     u = em.case_init(1, (n, n, n), lo, hi, 6, [0.3, float(0x5EED)])
     g = em.HipSolver((n, n, n), lo, hi, u, mode="weno", cfl=CFL, precise=not args.fast)
-    tiles = g.autotune(3)
+    tiles = None if args.no_autotune else g.autotune(3)
     g.upload(u)
     if args.warmup:
         g.run_steps(args.warmup)
@@ -131,7 +149,7 @@ def main():
             "peak": PEAK_FP64 / 1e12,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64,
-            "traffic": None,
+            "traffic": profiled_traffic("fast" if args.fast else "precise", dom),
             "hbm_fraction": value * BYTES_PER_CELL_STAGE / PEAK_HBM,
             "fp64_fraction": value * FLOP_PER_CELL_STAGE / PEAK_FP64,
         },
