@@ -7,14 +7,29 @@
 namespace em {
 
 // Tile shapes (cells along the axis x pencils, threads); index = cfg id.
-constexpr TileCfg kTileCfgs[] = {{32, 16, 640}, {64, 8, 576}, {128, 4, 576}, {256, 2, 576}};
-constexpr int kNumTileCfgs = 4;
+// Shapes 4..8 give exactly one midpoint per thread ((T+5)*P == NT).
+constexpr TileCfg kTileCfgs[] = {{32, 16, 640}, {64, 8, 576}, {128, 4, 576}, {256, 2, 576}, {27, 8, 256},
+                                 {59, 4, 256},  {59, 8, 512}, {27, 16, 512}, {123, 4, 512}};
+constexpr int kNumTileCfgs = 9;
+
+#define EM_DECLARE_LAUNCHERS                                                               \
+    hipError_t launch_pass_hybrid(int d, int cfg, const PassArgs& a, hipStream_t s);      \
+    hipError_t launch_pass_weno(int d, int cfg, const PassArgs& a, hipStream_t s);        \
+    hipError_t launch_pass_central(int d, int cfg, const PassArgs& a, hipStream_t s);     \
+    inline hipError_t launch_pass(int d, int mode, int cfg, const PassArgs& a, hipStream_t s) { \
+        switch (mode) {                                                                    \
+            case MODE_HYBRID: return launch_pass_hybrid(d, cfg, a, s);                     \
+            case MODE_WENO: return launch_pass_weno(d, cfg, a, s);                         \
+            case MODE_CENTRAL: return launch_pass_central(d, cfg, a, s);                   \
+            default: return hipErrorInvalidValue;                                          \
+        }                                                                                  \
+    }
 
 namespace precise {
-hipError_t launch_pass(int d, int mode, int cfg, const PassArgs& a, hipStream_t s);
+EM_DECLARE_LAUNCHERS
 }
 namespace fast {
-hipError_t launch_pass(int d, int mode, int cfg, const PassArgs& a, hipStream_t s);
+EM_DECLARE_LAUNCHERS
 }
 
 }  // namespace em

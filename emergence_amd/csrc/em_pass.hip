@@ -19,9 +19,10 @@
 // fluid/crates/compressible/src/solver.rs:97-137 (rk_combine), 351-475 (rhs),
 // 259-348 (ghosts), 478-523 (dt_cfl).
 //
-// This file is compiled twice: EM_VARIANT=precise with -ffp-contract=off
-// (reproduces the reference CPU arithmetic) and EM_VARIANT=fast with FMA
-// contraction.
+// This file is compiled once per (variant, flux mode): EM_VARIANT=precise
+// with -ffp-contract=off (reproduces the reference CPU arithmetic) or
+// EM_VARIANT=fast with FMA contraction; EM_MODE = 0 hybrid, 1 weno,
+// 2 central.  Kernels ask for 4 waves per SIMD (<= 128 VGPRs).
 #include <hip/hip_runtime.h>
 
 #include "em_pass.h"
@@ -29,6 +30,19 @@
 
 #ifndef EM_VARIANT
 #error "EM_VARIANT must name the arithmetic variant"
+#endif
+#ifndef EM_MODE
+#error "EM_MODE must name the flux mode instantiated by this translation unit"
+#endif
+#if EM_MODE == 0
+#define EM_LAUNCHER launch_pass_hybrid
+#elif EM_MODE == 1
+#define EM_LAUNCHER launch_pass_weno
+#else
+#define EM_LAUNCHER launch_pass_central
+#endif
+#ifndef EM_WAVES_PER_EU
+#define EM_WAVES_PER_EU 4
 #endif
 
 namespace em {
@@ -47,7 +61,7 @@ __device__ __forceinline__ void ghost_state(const PassArgs& a, long long pen, in
 }
 
 template <int D, int MODE, int T, int P, int NT>
-__global__ __launch_bounds__(NT) void axis_pass(PassArgs a) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_WAVES_PER_EU))) void axis_pass(PassArgs a) {
     constexpr int C = T + 10;  // cells held in LDS
     constexpr int M = T + 5;   // midpoints
     constexpr int NF = T + 1;  // faces
@@ -367,14 +381,15 @@ __global__ __launch_bounds__(NT) void axis_pass(PassArgs a) {
     }
     __syncthreads();
     if (tid == 0) {
-        if (scnt[0]) atomicAdd(&a.sc->counters[0], (unsigned long long)scnt[0]);
-        if (scnt[1]) atomicAdd(&a.sc->counters[1], (unsigned long long)scnt[1]);
-        if (scnt[2]) atomicAdd(&a.sc->counters[2], (unsigned long long)scnt[2]);
+        const int slot = bid % kSlots;
+        if (scnt[0]) atomicAdd(&a.sc->counters[0][slot][0], (unsigned long long)scnt[0]);
+        if (scnt[1]) atomicAdd(&a.sc->counters[1][slot][0], (unsigned long long)scnt[1]);
+        if (scnt[2]) atomicAdd(&a.sc->counters[2][slot][0], (unsigned long long)scnt[2]);
         if (a.want_rate) {
             double rm = 0.0;
             for (int w = 0; w < (NT + 63) / 64; ++w) rm = fmax(rm, sred[w]);
             rm = rm > 0.0 ? rm : 0.0;  // IEEE bits of +x order like x (no -0.0)
-            atomicMax(&a.sc->rate, (unsigned long long)__double_as_longlong(rm));
+            atomicMax(&a.sc->rate[slot][0], (unsigned long long)__double_as_longlong(rm));
         }
     }
 }
@@ -386,6 +401,7 @@ static hipError_t launch_one(const PassArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Tile table: must list the shapes of kTileCfgs (em_pass.h) in order.
 template <int D, int MODE>
 static hipError_t launch_cfg(int cfg, const PassArgs& a, hipStream_t s) {
     switch (cfg) {
@@ -393,25 +409,21 @@ static hipError_t launch_cfg(int cfg, const PassArgs& a, hipStream_t s) {
         case 1: return launch_one<D, MODE, 64, 8, 576>(a, s);
         case 2: return launch_one<D, MODE, 128, 4, 576>(a, s);
         case 3: return launch_one<D, MODE, 256, 2, 576>(a, s);
+        case 4: return launch_one<D, MODE, 27, 8, 256>(a, s);
+        case 5: return launch_one<D, MODE, 59, 4, 256>(a, s);
+        case 6: return launch_one<D, MODE, 59, 8, 512>(a, s);
+        case 7: return launch_one<D, MODE, 27, 16, 512>(a, s);
+        case 8: return launch_one<D, MODE, 123, 4, 512>(a, s);
         default: return hipErrorInvalidValue;
     }
 }
 
-template <int D>
-static hipError_t launch_mode(int mode, int cfg, const PassArgs& a, hipStream_t s) {
-    switch (mode) {
-        case MODE_HYBRID: return launch_cfg<D, MODE_HYBRID>(cfg, a, s);
-        case MODE_WENO: return launch_cfg<D, MODE_WENO>(cfg, a, s);
-        case MODE_CENTRAL: return launch_cfg<D, MODE_CENTRAL>(cfg, a, s);
-        default: return hipErrorInvalidValue;
-    }
-}
-
-hipError_t launch_pass(int d, int mode, int cfg, const PassArgs& a, hipStream_t s) {
+// This translation unit instantiates one flux mode (EM_MODE).
+hipError_t EM_LAUNCHER(int d, int cfg, const PassArgs& a, hipStream_t s) {
     switch (d) {
-        case 0: return launch_mode<0>(mode, cfg, a, s);
-        case 1: return launch_mode<1>(mode, cfg, a, s);
-        case 2: return launch_mode<2>(mode, cfg, a, s);
+        case 0: return launch_cfg<0, EM_MODE>(cfg, a, s);
+        case 1: return launch_cfg<1, EM_MODE>(cfg, a, s);
+        case 2: return launch_cfg<2, EM_MODE>(cfg, a, s);
         default: return hipErrorInvalidValue;
     }
 }

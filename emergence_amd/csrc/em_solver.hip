@@ -52,7 +52,17 @@ __global__ void k_set_run(DevScalars* sc, double t_end) {
     sc->steps = 0;
 }
 
-__global__ void k_zero_rate(DevScalars* sc) { sc->rate = 0ull; }
+__global__ void k_zero_rate(DevScalars* sc) {
+    if (threadIdx.x < kSlots) sc->rate[threadIdx.x][0] = 0ull;
+}
+
+// max over the rate slots (IEEE bits of non-negative doubles order like the
+// values)
+__device__ __forceinline__ unsigned long long rate_bits(const DevScalars* sc) {
+    unsigned long long m = 0ull;
+    for (int i = 0; i < kSlots; ++i) m = sc->rate[i][0] > m ? sc->rate[i][0] : m;
+    return m;
+}
 
 // t += dt after a step (solver.rs:737)
 __global__ void k_advance(DevScalars* sc) {
@@ -68,7 +78,7 @@ __global__ void k_advance(DevScalars* sc) {
 //                collapse if dt is not finite or <= 1e-12 max(|t_end|, 1e-300)
 __global__ void k_dt_from_rate(DevScalars* sc, double cfl, int run_to) {
     if (sc->active == 0) return;
-    const double rate = __longlong_as_double((long long)sc->rate);
+    const double rate = __longlong_as_double((long long)rate_bits(sc));
     double dt = cfl / rate;
     if (run_to) {
         const double te = sc->t_end;
@@ -88,7 +98,7 @@ __global__ void k_dt_from_rate(DevScalars* sc, double cfl, int run_to) {
         return;
     }
     sc->dt = dt;
-    sc->rate = 0ull;
+    for (int i = 0; i < kSlots; ++i) sc->rate[i][0] = 0ull;
 }
 
 // CFL rate max over interior cells: dt_cfl (solver.rs:478-523).  Wave
@@ -117,7 +127,7 @@ __global__ __launch_bounds__(256) void k_rate(const double* __restrict__ U, long
     if (threadIdx.x == 0) {
         double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
         m = m > 0.0 ? m : 0.0;
-        atomicMax(&sc->rate, (unsigned long long)__double_as_longlong(m));
+        atomicMax(&sc->rate[blockIdx.x % kSlots][0], (unsigned long long)__double_as_longlong(m));
     }
 }
 
@@ -182,7 +192,7 @@ struct Solver {
     int mode, positivity;
     double cfl, tol;
     int precise = 1;  // reference arithmetic by default
-    int cfg[3] = {1, 1, 2};
+    int cfg[3] = {6, 6, 6};
     long long ncell;
     long long stride[3];
     int nactive;
@@ -243,13 +253,15 @@ struct Solver {
         HIPCHK(hipMalloc(&sc, sizeof(DevScalars)));
         HIPCHK(hipMalloc(&bcdev, sizeof(double) * 30));
         HIPCHK(hipMemcpyAsync(bcdev, &bc_state[0][0][0], sizeof(double) * 30, hipMemcpyHostToDevice, stream));
-        DevScalars h{};
-        h.t = t0;
-        h.active = 1;
-        HIPCHK(hipMemcpyAsync(sc, &h, sizeof h, hipMemcpyHostToDevice, stream));
+        std::vector<DevScalars> h(1);
+        std::memset(h.data(), 0, sizeof(DevScalars));
+        h[0].t = t0;
+        h[0].active = 1;
+        HIPCHK(hipMemcpyAsync(sc, h.data(), sizeof(DevScalars), hipMemcpyHostToDevice, stream));
         HIPCHK(hipStreamSynchronize(stream));
-        // default tiles: pencils along the contiguous axis take long tiles
-        for (int d = 0; d < 3; ++d) cfg[d] = (stride[d] == 1) ? 2 : 0;
+        // default tile: 59 cells x 8 pencils, 512 threads (one midpoint per
+        // thread, two workgroups per CU); the autotuner may pick another
+        for (int d = 0; d < 3; ++d) cfg[d] = 6;
     }
 
     ~Solver() {
@@ -402,7 +414,7 @@ struct Solver {
             for (int s = 0; s < 3; ++s) e[s] = evs->data() + 6 * s;
         stage(U[0], U[1], 1, false, 0, 0.0, e[0]);
         stage(U[1], U[2], 2, false, 0, 0.0, e[1]);
-        hipLaunchKernelGGL(k_zero_rate, dim3(1), dim3(1), 0, stream, sc);
+        hipLaunchKernelGGL(k_zero_rate, dim3(1), dim3(kSlots), 0, stream, sc);
         stage(U[2], U[0], 3, true, 0, 0.0, e[2]);
         hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, stream, sc);
         HIPCHK(hipGetLastError());
@@ -414,7 +426,7 @@ struct Solver {
     }
 
     void compute_rate() {
-        hipLaunchKernelGGL(k_zero_rate, dim3(1), dim3(1), 0, stream, sc);
+        hipLaunchKernelGGL(k_zero_rate, dim3(1), dim3(kSlots), 0, stream, sc);
         long long nb = std::min<long long>((ncell + 255) / 256, 2048);
         hipLaunchKernelGGL(k_rate, dim3((unsigned)nb), dim3(256), 0, stream, U[0], ncell, eos, dx[0], dx[1], dx[2],
                            act[0], act[1], act[2], sc);
@@ -422,20 +434,45 @@ struct Solver {
         rate_valid = true;
     }
 
-    DevScalars read_scalars() {
-        DevScalars h;
-        HIPCHK(hipMemcpyAsync(&h, sc, sizeof h, hipMemcpyDeviceToHost, stream));
+    // Host view of the device scalars with the slots reduced.
+    struct Scalars {
+        double dt, t;
+        int active, err;
+        long long steps;
+        uint64_t rate_bits;
+        uint64_t counters[3];
+    };
+    Scalars read_scalars() {
+        std::vector<DevScalars> h(1);
+        HIPCHK(hipMemcpyAsync(h.data(), sc, sizeof(DevScalars), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        Scalars r{h[0].dt, h[0].t, h[0].active, h[0].err, h[0].steps, 0, {0, 0, 0}};
+        for (int i = 0; i < kSlots; ++i) {
+            r.rate_bits = std::max<uint64_t>(r.rate_bits, h[0].rate[i][0]);
+            for (int q = 0; q < 3; ++q) r.counters[q] += h[0].counters[q][i][0];
+        }
+        return r;
+    }
+    // Restores the counters (every slot) saved before a timing-only run.
+    void restore_counters(const std::vector<DevScalars>& saved) {
+        HIPCHK(hipMemcpyAsync(&sc->counters[0][0][0], &saved[0].counters[0][0][0], sizeof(saved[0].counters),
+                              hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+    std::vector<DevScalars> save_scalars() {
+        std::vector<DevScalars> h(1);
+        HIPCHK(hipMemcpyAsync(h.data(), sc, sizeof(DevScalars), hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
         return h;
     }
 
     void rhs(double stage_dt, double* out, uint64_t stats[4]) {
-        DevScalars before = read_scalars();
+        Scalars before = read_scalars();
         reset_run(0.0);
         stage(U[0], nullptr, 0, false, 1, stage_dt, nullptr);
         std::vector<double> h(5 * ncell);
         HIPCHK(hipMemcpyAsync(h.data(), R, sizeof(double) * 5 * ncell, hipMemcpyDeviceToHost, stream));
-        DevScalars after = read_scalars();
+        Scalars after = read_scalars();
         const long long L = len();
         std::fill(out, out + 5 * L, 0.0);
         const int m1 = m(1), m2 = m(2);
@@ -468,8 +505,8 @@ struct Solver {
 
     double dt_cfl() {
         compute_rate();
-        DevScalars h = read_scalars();
-        const double rate = __builtin_bit_cast(double, h.rate);
+        Scalars h = read_scalars();
+        const double rate = __builtin_bit_cast(double, h.rate_bits);
         return cfl / rate;  // solver.rs:517
     }
 
@@ -505,7 +542,7 @@ struct Solver {
         const int chunk = 16;
         for (;;) {
             for (int q = 0; q < chunk; ++q) HIPCHK(hipGraphLaunch(gx, stream));
-            DevScalars h = read_scalars();
+            Scalars h = read_scalars();
             if (h.err & 1) throw Error(EM_ERR_COLLAPSE, "time step collapsed: the solution lost admissibility");
             if (!h.active) return h.steps;
         }
@@ -542,18 +579,20 @@ struct Solver {
             }
             for (auto& x : ev) (void)hipEventDestroy(x);
         }
-        DevScalars h = read_scalars();
+        Scalars h = read_scalars();
         if (h.err & 1) throw Error(EM_ERR_COLLAPSE, "time step collapsed: the solution lost admissibility");
     }
 
-    // Times every tile shape of every active axis on residual-only passes
-    // of the current state and keeps the fastest; the state is untouched and
-    // the counters are restored.
+    // Times every tile shape of every active axis on the residual pass of
+    // the current state (limiter on, with the state's CFL dt) and keeps the
+    // fastest; the state is untouched and the counters are restored.
     void autotune(int repeats, int32_t out[3]) {
-        DevScalars before = read_scalars();
+        std::vector<DevScalars> saved = save_scalars();
+        const double dt = dt_cfl();
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
+        reset_run(0.0);
         for (int d = 0; d < 3; ++d) {
             if (!active(d)) continue;
             double best = 1e300;
@@ -569,7 +608,7 @@ struct Solver {
                 a.first = 1;
                 a.rk_stage = 0;
                 a.stage_dt_host = 1;
-                a.stage_dt = 0.0;
+                a.stage_dt = std::isfinite(dt) ? dt : 0.0;
                 HIPCHK(launch(d, a));  // warm-up
                 HIPCHK(hipEventRecord(e0, stream));
                 for (int r = 0; r < repeats; ++r) HIPCHK(launch(d, a));
@@ -586,9 +625,7 @@ struct Solver {
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        HIPCHK(hipMemcpyAsync(&sc->counters[0], &before.counters[0], sizeof before.counters, hipMemcpyHostToDevice,
-                              stream));
-        HIPCHK(hipStreamSynchronize(stream));
+        restore_counters(saved);
         if (out)
             for (int d = 0; d < 3; ++d) out[d] = cfg[d];
     }
@@ -804,7 +841,7 @@ int em_solver_run_steps(em_solver* h, int64_t nsteps, double pass_ms[3]) {
 int em_solver_counters(em_solver* h, uint64_t out[3]) {
     NEED(h);
     return guarded([&] {
-        em::DevScalars d = h->s->read_scalars();
+        auto d = h->s->read_scalars();
         for (int q = 0; q < 3; ++q) out[q] = d.counters[q];
     });
 }

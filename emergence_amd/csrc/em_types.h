@@ -9,16 +9,22 @@ namespace em {
 enum BcKind : int { BC_PERIODIC = 0, BC_OUTFLOW = 1, BC_REFLECT = 2, BC_DIRICHLET = 3, BC_CHARACTERISTIC = 4 };
 enum Mode : int { MODE_HYBRID = 0, MODE_WENO = 1, MODE_CENTRAL = 2 };
 
-// Device scalars of one solver (one allocation, 256-byte aligned slots).
-struct DevScalars {
+// Device scalars of one solver.  Workgroup-level accumulators (counters and
+// the CFL rate) are spread over kSlots slots on separate 128-byte lines so
+// that tens of thousands of workgroups per launch do not serialise on one
+// address; readers reduce over the slots.
+constexpr int kSlots = 64;
+struct alignas(256) DevScalars {
     double dt;                    // step size used by every kernel of a step
     double t;                     // solver time
     double t_end;                 // run_to target (run mode 1)
-    unsigned long long rate;      // max CFL rate of the state, as IEEE bits (>= 0)
-    unsigned long long counters[4];  // shock-path faces, fallbacks, limited faces, spare
     int active;                   // 0 once run_to reached t_end or failed
-    int err;                      // bit 0: dt collapsed, bit 1: ghost recursion bound
+    int err;                      // bit 0: dt collapsed
     long long steps;              // steps taken by device-driven runs
+    // max CFL rate of the state as IEEE bits of a double >= 0, per slot
+    unsigned long long rate[kSlots][16];
+    // [shock-path faces, fallbacks, limited faces] per slot
+    unsigned long long counters[3][kSlots][16];
 };
 
 // One axis pass over interior-only SoA state arrays (variable plane stride
