@@ -145,3 +145,152 @@ class OracleSolver:
     def interior(self, a):
         g = [self.ng if v > 1 else 0 for v in self.n]
         return a[g[0]:g[0] + self.n[0], g[1]:g[1] + self.n[1], g[2]:g[2] + self.n[2]]
+
+
+def _load_amr(lib):
+    if getattr(lib, "_amr_ready", False):
+        return lib
+    P = C.c_void_p
+    D = C.POINTER(C.c_double)
+    U = C.POINTER(C.c_uint64)
+    I64 = C.POINTER(C.c_int64)
+    I32 = C.POINTER(C.c_int32)
+    U8 = C.POINTER(C.c_uint8)
+    lib.emref_amr_create.restype = P
+    lib.emref_amr_create.argtypes = [C.c_int, D, I64, C.c_int64, C.c_int64, C.c_double, C.c_int64, C.c_int64, D, D,
+                                     D, I32, D, C.c_int, C.c_double, C.c_int, C.c_double]
+    lib.emref_amr_destroy.argtypes = [P]
+    for n in ("emref_amr_nblocks", "emref_amr_bcells", "emref_amr_ng"):
+        getattr(lib, n).restype = C.c_int64
+        getattr(lib, n).argtypes = [P]
+    lib.emref_amr_blocks.argtypes = [P, I32, I64, U8]
+    lib.emref_amr_get_u.argtypes = [P, D]
+    lib.emref_amr_set_u.argtypes = [P, D]
+    lib.emref_amr_step.argtypes = [P, C.c_double]
+    lib.emref_amr_dt_cfl.restype = C.c_double
+    lib.emref_amr_dt_cfl.argtypes = [P]
+    lib.emref_amr_time.restype = C.c_double
+    lib.emref_amr_time.argtypes = [P]
+    lib.emref_amr_restrict_all.argtypes = [P]
+    lib.emref_amr_regrid.argtypes = [P]
+    lib.emref_amr_set_next_regrid.argtypes = [P, C.c_int64]
+    lib.emref_amr_compute_rhs.argtypes = [P, C.c_double, D, U]
+    lib.emref_amr_fill_scratch.argtypes = [P, C.c_int64, D]
+    lib.emref_amr_details.argtypes = [P, D]
+    lib.emref_amr_totals.argtypes = [P, D]
+    lib.emref_amr_stats.argtypes = [P, U]
+    lib._amr_ready = True
+    return lib
+
+
+AMR_CASE = {"riemann2d": 0, "smooth": 1, "shock_bubble": 2, "vortex": 3, "poly": 4}
+
+
+class OracleAmr:
+    """Reference CPU block AMR (restated), global SSP-RK3 step."""
+
+    def __init__(self, case, params, base, b, levels, lo, hi, eps=1e-3, regrid_every=4, ghosts=5, bc=None,
+                 mode="hybrid", cfl=0.4, positivity=True, tol=1e-3, eos=(1.4, 0.0, 0.0, 0.0)):
+        self.lib = _load_amr(load())
+        prm = np.zeros(16)
+        if params is not None:
+            prm[:len(params)] = params
+        kinds = (C.c_int32 * 6)()
+        states = (C.c_double * 30)()
+        bc = bc or [["periodic", "periodic"]] * 3
+        for d in range(3):
+            for s in range(2):
+                bb = bc[d][s]
+                name, st = (bb, None) if isinstance(bb, str) else (bb[0], bb[1])
+                kinds[2 * d + s] = BC[name]
+                if st is not None:
+                    for q in range(5):
+                        states[(2 * d + s) * 5 + q] = st[q]
+        self.bc = bc
+        self.h = self.lib.emref_amr_create(AMR_CASE[case], _dp(prm), (C.c_int64 * 3)(*base), b, levels, eps,
+                                           regrid_every, ghosts, (C.c_double * 3)(*lo), (C.c_double * 3)(*hi),
+                                           (C.c_double * 4)(*eos), kinds, states, MODE[mode], cfl, int(positivity),
+                                           tol)
+        self.base, self.b, self.levels, self.lo, self.hi = tuple(base), b, levels, tuple(lo), tuple(hi)
+        self.bd = tuple(b if v > 0 else 1 for v in base)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.emref_amr_destroy(self.h)
+            self.h = None
+
+    @property
+    def nblocks(self):
+        return self.lib.emref_amr_nblocks(self.h)
+
+    @property
+    def ng(self):
+        return self.lib.emref_amr_ng(self.h)
+
+    def blocks(self):
+        n = self.nblocks
+        lv = np.empty(n, np.int32)
+        idx = np.empty((n, 3), np.int64)
+        leaf = np.empty(n, np.uint8)
+        self.lib.emref_amr_blocks(self.h, lv.ctypes.data_as(C.POINTER(C.c_int32)),
+                                  idx.ctypes.data_as(C.POINTER(C.c_int64)), leaf.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return lv, idx, leaf.astype(bool)
+
+    def get_u(self):
+        u = np.empty((self.nblocks,) + self.bd + (5,))
+        self.lib.emref_amr_get_u(self.h, _dp(u))
+        return u
+
+    def set_u(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        self.lib.emref_amr_set_u(self.h, _dp(u))
+
+    def step(self, dt):
+        self.lib.emref_amr_step(self.h, dt)
+
+    def dt_cfl(self):
+        return self.lib.emref_amr_dt_cfl(self.h)
+
+    @property
+    def t(self):
+        return self.lib.emref_amr_time(self.h)
+
+    def restrict_all(self):
+        self.lib.emref_amr_restrict_all(self.h)
+
+    def regrid(self):
+        self.lib.emref_amr_regrid(self.h)
+
+    def set_next_regrid(self, n):
+        self.lib.emref_amr_set_next_regrid(self.h, n)
+
+    def compute_rhs(self, stage_dt):
+        out = np.empty((self.nblocks,) + self.bd + (5,))
+        st = (C.c_uint64 * 4)()
+        self.lib.emref_amr_compute_rhs(self.h, stage_dt, _dp(out), st)
+        return out, list(st)
+
+    def fill_scratch(self, bid):
+        ext = tuple(v + 2 * self.ng if v > 1 else 1 for v in self.bd)
+        out = np.empty(ext + (5,))
+        self.lib.emref_amr_fill_scratch(self.h, bid, _dp(out))
+        return out
+
+    def details(self):
+        out = np.empty(self.nblocks)
+        self.lib.emref_amr_details(self.h, _dp(out))
+        return out
+
+    def totals(self):
+        out = (C.c_double * 5)()
+        self.lib.emref_amr_totals(self.h, out)
+        return np.array(list(out))
+
+    def stats(self):
+        st = (C.c_uint64 * 4)()
+        self.lib.emref_amr_stats(self.h, st)
+        return list(st)
+
+    def dx(self, level):
+        return tuple((h - l) / (nb * self.b * 2 ** level) if nb > 0 else (h - l)
+                     for l, h, nb in zip(self.lo, self.hi, self.base))
