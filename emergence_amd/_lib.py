@@ -29,6 +29,26 @@ EXPORTS = (
     "em_solver_time",
     "em_solver_synchronize",
     "em_case_init",
+    "em_amr_last_error",
+    "em_amr_create",
+    "em_amr_create_case",
+    "em_amr_destroy",
+    "em_amr_set_variant",
+    "em_amr_set_batched",
+    "em_amr_set_next_regrid",
+    "em_amr_nblocks",
+    "em_amr_download",
+    "em_amr_upload",
+    "em_amr_rhs",
+    "em_amr_step",
+    "em_amr_run_steps",
+    "em_amr_run_to",
+    "em_amr_dt_cfl",
+    "em_amr_details",
+    "em_amr_regrid",
+    "em_amr_counters",
+    "em_amr_time",
+    "em_amr_synchronize",
 )
 
 _lib = None
@@ -75,8 +95,31 @@ def load():
     lib.em_solver_time.argtypes = [P, D]
     lib.em_solver_synchronize.argtypes = [P]
     lib.em_case_init.argtypes = [C.c_int32, I64, D, D, C.c_int32, D, D]
+    U8 = C.POINTER(C.c_uint8)
+    amr_cfg = [C.POINTER(P), I64, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32, D, D, D, I32, D,
+               C.c_int32, C.c_int32, C.c_double, C.c_double]
+    lib.em_amr_last_error.argtypes = []
+    lib.em_amr_create.argtypes = amr_cfg + [C.c_int64, I32, I64, U8, D]
+    lib.em_amr_create_case.argtypes = amr_cfg + [C.c_int32, D]
+    lib.em_amr_destroy.argtypes = [P]
+    lib.em_amr_set_variant.argtypes = [P, C.c_int32]
+    lib.em_amr_set_batched.argtypes = [P, C.c_int32]
+    lib.em_amr_set_next_regrid.argtypes = [P, C.c_int64]
+    lib.em_amr_nblocks.argtypes = [P, I64, I64]
+    lib.em_amr_download.argtypes = [P, I32, I64, U8, D]
+    lib.em_amr_upload.argtypes = [P, D]
+    lib.em_amr_rhs.argtypes = [P, C.c_double, D, U64]
+    lib.em_amr_step.argtypes = [P, C.c_double]
+    lib.em_amr_run_steps.argtypes = [P, C.c_int64]
+    lib.em_amr_run_to.argtypes = [P, C.c_double, I64]
+    lib.em_amr_dt_cfl.argtypes = [P, D]
+    lib.em_amr_details.argtypes = [P, D]
+    lib.em_amr_regrid.argtypes = [P]
+    lib.em_amr_counters.argtypes = [P, U64]
+    lib.em_amr_time.argtypes = [P, D]
+    lib.em_amr_synchronize.argtypes = [P]
     for name in EXPORTS:
-        getattr(lib, name).restype = C.c_char_p if name == "em_last_error" else C.c_int
+        getattr(lib, name).restype = C.c_char_p if name.endswith("last_error") else C.c_int
     _lib = lib
     return lib
 
@@ -84,4 +127,10 @@ def load():
 def check(rc):
     if rc != 0:
         raise EmError(rc, load().em_last_error().decode())
+    return rc
+
+
+def check_amr(rc):
+    if rc != 0:
+        raise EmError(rc, load().em_amr_last_error().decode())
     return rc

@@ -1,0 +1,363 @@
+// Device kernels of the block-AMR pool other than the flux pass:
+// restriction of ancestors, ghost slabs (copy / boundary / prediction),
+// prediction of refined children, refluxing, SSP-RK3 update, CFL rate,
+// wavelet details and slot moves.  Compiled without FMA contraction so the
+// operators reproduce the reference's arithmetic (amr.rs).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "em_amr_types.h"
+
+namespace em {
+
+// Weights of the separable 6th-order operators (amr.rs:41-47, 305-308,
+// 675-676): prediction by fine-cell parity, centred and shifted restriction.
+struct AmrWeights {
+    double wp[2][6];
+    double wr[6];
+    double wts[5][6];  // restriction stencils for shifts 0, -1, 1, -2, 2
+};
+
+__device__ __forceinline__ long long div_e(long long a, long long b) {
+    long long q = a / b;
+    if (a % b < 0) q = b > 0 ? q - 1 : q + 1;
+    return q;
+}
+__device__ __forceinline__ long long rem_e(long long a, long long b) {
+    long long r = a % b;
+    return r < 0 ? r + (b < 0 ? -b : b) : r;
+}
+
+__device__ __forceinline__ long long amr_n_level(const AmrDev& a, int l, int d) {
+    return a.act[d] ? (long long)a.nb0[d] * a.bd[d] * (1ll << l) : 1;
+}
+
+// Block id holding global cell g (already inside the domain) at level l, -1
+// if absent; *cell receives the cell index inside the block.
+__device__ __forceinline__ int amr_find(const AmrDev& a, int l, const long long g[3], long long* cell) {
+    long long s = 0;
+    int c[3];
+    for (int d = 0; d < 3; ++d) {
+        const long long nb = a.act[d] ? (long long)a.nb0[d] << l : 1;
+        const long long k = a.act[d] ? div_e(g[d], a.bd[d]) : 0;
+        if (k < 0 || k >= nb) return -1;
+        s = s * nb + k;
+        c[d] = a.act[d] ? (int)rem_e(g[d], a.bd[d]) : 0;
+    }
+    *cell = ((long long)c[0] * a.bd[1] + c[1]) * a.bd[2] + c[2];
+    return a.map[a.map_off[l] + s];
+}
+
+// bc_box of amr.rs:450-504 on a box held in scratch (values val[q*cap+i],
+// validity ok[i]): cells outside a non-periodic boundary are filled from
+// cells of the same box, one axis sweep at a time; called by one workgroup.
+__device__ void box_bc(const AmrDev& a, int l, const long long g0[3], const int sz[3], double* val, double* ok,
+                       long long cap) {
+    const long long n3[3] = {amr_n_level(a, l, 0), amr_n_level(a, l, 1), amr_n_level(a, l, 2)};
+    const long long tot = (long long)sz[0] * sz[1] * sz[2];
+    auto outside = [&](const long long g[3], int d) {
+        return a.act[d] && (g[d] < 0 || g[d] >= n3[d]) && a.bc_kind[d][g[d] >= n3[d] ? 1 : 0] != BC_PERIODIC;
+    };
+    for (int d = 0; d < 3; ++d) {
+        if (!a.act[d]) continue;
+        for (long long i = threadIdx.x; i < tot; i += blockDim.x) {
+            int c[3] = {(int)(i / ((long long)sz[1] * sz[2])), (int)((i / sz[2]) % sz[1]), (int)(i % sz[2])};
+            long long g[3] = {g0[0] + c[0], g0[1] + c[1], g0[2] + c[2]};
+            if (!outside(g, d)) continue;
+            bool later = false;
+            for (int e = d + 1; e < 3; ++e) later = later || outside(g, e);
+            if (later) continue;
+            const int side = g[d] >= n3[d] ? 1 : 0;
+            auto local = [&](long long gg) -> long long {
+                long long cc = gg - g0[d];
+                if (cc < 0 || cc >= sz[d]) return -1;
+                int s3[3] = {c[0], c[1], c[2]};
+                s3[d] = (int)cc;
+                const long long p = ((long long)s3[0] * sz[1] + s3[1]) * sz[2] + s3[2];
+                return ok[p] != 0.0 ? p : -1;
+            };
+            const long long gi = g[d] < 0 ? 0 : (g[d] > n3[d] - 1 ? n3[d] - 1 : g[d]);
+            const int kind = a.bc_kind[d][side];
+            double v[5];
+            bool have = false;
+            if (kind == BC_DIRICHLET) {
+                for (int q = 0; q < 5; ++q) v[q] = a.bc_state[d][side][q];
+                have = true;
+            } else if (kind == BC_OUTFLOW) {
+                long long p = local(gi);
+                if (p >= 0) {
+                    for (int q = 0; q < 5; ++q) v[q] = val[q * cap + p];
+                    have = true;
+                }
+            } else if (kind == BC_REFLECT) {
+                long long p = local(side == 0 ? -1 - g[d] : 2 * n3[d] - 1 - g[d]);
+                if (p >= 0) {
+                    for (int q = 0; q < 5; ++q) v[q] = val[q * cap + p];
+                    v[1 + d] *= -1.0;
+                    have = true;
+                }
+            } else if (kind == BC_CHARACTERISTIC) {
+                long long pb = local(gi), p1 = local(gi + (side == 1 ? -1 : 1));
+                if (pb >= 0 && p1 >= 0) {
+                    double ub[5], ub1[5];
+                    for (int q = 0; q < 5; ++q) {
+                        ub[q] = val[q * cap + pb];
+                        ub1[q] = val[q * cap + p1];
+                    }
+                    const double m = (double)(g[d] > gi ? g[d] - gi : gi - g[d]);
+                    if (d == 0)
+                        characteristic_ghost<0>(a.eos, ub, ub1, a.bc_state[d][side], side, m, v);
+                    else if (d == 1)
+                        characteristic_ghost<1>(a.eos, ub, ub1, a.bc_state[d][side], side, m, v);
+                    else
+                        characteristic_ghost<2>(a.eos, ub, ub1, a.bc_state[d][side], side, m, v);
+                    have = true;
+                }
+            }
+            if (have) {
+                for (int q = 0; q < 5; ++q) val[q * cap + i] = v[q];
+                ok[i] = 1.0;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// predict_box of amr.rs:548-586 by one workgroup: the level-(lf-1) patch is
+// gathered into scratch (gather_box: blocks of that level, periodic wrap,
+// bc_box for cells outside the domain), then one 6-point pass per active
+// axis maps coarse to fine indices.  The result (box fmin..fmax of level lf)
+// is left in scratch; returns the buffer holding it.
+__device__ double* predict_region(const AmrDev& a, const AmrWeights& W, int lf, const long long fmin[3],
+                                  const long long fmax[3], double* scr, long long cap) {
+    double* A = scr;
+    double* Bf = scr + 5 * cap;
+    double* ok = scr + 10 * cap;
+    __shared__ int any_out;
+    const int lc = lf - 1;
+    long long cmin[3];
+    int sz[3];
+    for (int d = 0; d < 3; ++d) {
+        cmin[d] = a.act[d] ? div_e(fmin[d], 2) - 3 : 0;
+        const long long cmax = a.act[d] ? div_e(fmax[d], 2) + 3 : 0;
+        sz[d] = (int)(cmax - cmin[d] + 1);
+    }
+    const long long n3[3] = {amr_n_level(a, lc, 0), amr_n_level(a, lc, 1), amr_n_level(a, lc, 2)};
+    if (threadIdx.x == 0) any_out = 0;
+    __syncthreads();
+    const long long tot = (long long)sz[0] * sz[1] * sz[2];
+    for (long long i = threadIdx.x; i < tot; i += blockDim.x) {
+        const int c[3] = {(int)(i / ((long long)sz[1] * sz[2])), (int)((i / sz[2]) % sz[1]), (int)(i % sz[2])};
+        long long g[3] = {cmin[0] + c[0], cmin[1] + c[1], cmin[2] + c[2]};
+        bool out = false;
+        for (int d = 0; d < 3; ++d) {
+            if (!a.act[d]) {
+                g[d] = 0;
+                continue;
+            }
+            if (g[d] < 0 || g[d] >= n3[d]) {
+                if (a.bc_kind[d][g[d] >= n3[d] ? 1 : 0] == BC_PERIODIC)
+                    g[d] = rem_e(g[d], n3[d]);
+                else
+                    out = true;
+            }
+        }
+        if (out) {
+            ok[i] = 0.0;
+            any_out = 1;
+            for (int q = 0; q < 5; ++q) A[q * cap + i] = 0.0;
+            continue;
+        }
+        long long cell;
+        const int id = amr_find(a, lc, g, &cell);
+        if (id < 0) {
+            // the hierarchy is 2:1 balanced, so the level below a leaf's
+            // neighbourhood exists; report otherwise
+            atomicOr(&a.sc->err, 2);
+            for (int q = 0; q < 5; ++q) A[q * cap + i] = 0.0;
+        } else {
+            for (int q = 0; q < 5; ++q) A[q * cap + i] = a.U[amr_u(a, q, id, cell)];
+        }
+        ok[i] = 1.0;
+    }
+    __syncthreads();
+    if (any_out) {
+        box_bc(a, lc, cmin, sz, A, ok, cap);
+        for (long long i = threadIdx.x; i < tot; i += blockDim.x)
+            if (ok[i] == 0.0) atomicOr(&a.sc->err, 4);
+        __syncthreads();
+    }
+    double* in = A;
+    double* out = Bf;
+    for (int d = 0; d < 3; ++d) {
+        if (!a.act[d]) continue;
+        int nsz[3] = {sz[0], sz[1], sz[2]};
+        nsz[d] = (int)(fmax[d] - fmin[d] + 1);
+        const long long ntot = (long long)nsz[0] * nsz[1] * nsz[2];
+        for (long long i = threadIdx.x; i < ntot; i += blockDim.x) {
+            int c[3] = {(int)(i / ((long long)nsz[1] * nsz[2])), (int)((i / nsz[2]) % nsz[1]), (int)(i % nsz[2])};
+            const long long f = fmin[d] + c[d];
+            const int par = (int)rem_e(f, 2);
+            const long long base = div_e(f, 2) + (par == 1 ? -2 : -3) - cmin[d];
+            double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            for (int kk = 0; kk < 6; ++kk) {
+                int s3[3] = {c[0], c[1], c[2]};
+                s3[d] = (int)(base + kk);
+                const long long src = ((long long)s3[0] * sz[1] + s3[1]) * sz[2] + s3[2];
+                const double wgt = W.wp[par][kk];
+#pragma unroll
+                for (int q = 0; q < 5; ++q) acc[q] += wgt * in[q * cap + src];
+            }
+#pragma unroll
+            for (int q = 0; q < 5; ++q) out[q * cap + i] = acc[q];
+        }
+        __syncthreads();
+        double* tmp = in;
+        in = out;
+        out = tmp;
+        for (int e = 0; e < 3; ++e) sz[e] = nsz[e];
+    }
+    return in;
+}
+
+// restrict_box of amr.rs:622-730 for a whole ancestor block (level l) by
+// one workgroup: the (2b+8)^D fine patch with its validity mask (level l+1
+// blocks, periodic wrap, bc_box), then one pass per axis with the centred
+// 6-point stencil, shifted where the fine data ends, the child average as a
+// last resort.  Cells left invalid are written as zero, as the reference does.
+// Only cells selected by `mask` are stored (see k_amr_restrict); a coarse
+// cell's stencils stay inside its own box's fine patch, so its value is the
+// same whichever box the reference restricts it in.
+__device__ void restrict_block(const AmrDev& a, const AmrWeights& W, int l, int slot, const int idx[3], int mask,
+                               int w, double* scr, long long cap) {
+    double* A = scr;
+    double* Bf = scr + 5 * cap;
+    double* okA = scr + 10 * cap;
+    double* okB = scr + 11 * cap;
+    __shared__ int any_out;
+    long long f0[3];
+    int sz[3];
+    const int lf = l + 1;
+    const long long nf[3] = {amr_n_level(a, lf, 0), amr_n_level(a, lf, 1), amr_n_level(a, lf, 2)};
+    for (int d = 0; d < 3; ++d) {
+        const long long c0 = (long long)idx[d] * a.bd[d];
+        f0[d] = a.act[d] ? 2 * c0 - 4 : 0;
+        sz[d] = a.act[d] ? 2 * a.bd[d] + 8 : 1;
+    }
+    if (threadIdx.x == 0) any_out = 0;
+    __syncthreads();
+    const long long tot = (long long)sz[0] * sz[1] * sz[2];
+    for (long long i = threadIdx.x; i < tot; i += blockDim.x) {
+        const int c[3] = {(int)(i / ((long long)sz[1] * sz[2])), (int)((i / sz[2]) % sz[1]), (int)(i % sz[2])};
+        long long g[3] = {f0[0] + c[0], f0[1] + c[1], f0[2] + c[2]};
+        bool outside = false;
+        for (int d = 0; d < 3; ++d)
+            if (a.act[d] && (g[d] < 0 || g[d] >= nf[d])) {
+                if (a.bc_kind[d][g[d] >= nf[d] ? 1 : 0] == BC_PERIODIC)
+                    g[d] = rem_e(g[d], nf[d]);
+                else
+                    outside = true;
+            }
+        double okv = 0.0;
+        if (outside) {
+            any_out = 1;
+            for (int q = 0; q < 5; ++q) A[q * cap + i] = 0.0;
+        } else {
+            long long cell;
+            const int id = amr_find(a, lf, g, &cell);
+            if (id >= 0) {
+                for (int q = 0; q < 5; ++q) A[q * cap + i] = a.U[amr_u(a, q, id, cell)];
+                okv = 1.0;
+            } else {
+                for (int q = 0; q < 5; ++q) A[q * cap + i] = 0.0;
+            }
+        }
+        okA[i] = okv;
+    }
+    __syncthreads();
+    if (any_out) box_bc(a, lf, f0, sz, A, okA, cap);
+    const int shifts[5] = {0, -1, 1, -2, 2};
+    double* in = A;
+    double* out = Bf;
+    double* iok = okA;
+    double* ook = okB;
+    for (int d = 0; d < 3; ++d) {
+        if (!a.act[d]) continue;
+        int nsz[3] = {sz[0], sz[1], sz[2]};
+        nsz[d] = a.bd[d];
+        const long long ntot = (long long)nsz[0] * nsz[1] * nsz[2];
+        for (long long i = threadIdx.x; i < ntot; i += blockDim.x) {
+            int c[3] = {(int)(i / ((long long)nsz[1] * nsz[2])), (int)((i / nsz[2]) % nsz[1]), (int)(i % nsz[2])};
+            auto at_src = [&](int pos) {
+                int s3[3] = {c[0], c[1], c[2]};
+                s3[d] = pos;
+                return ((long long)s3[0] * sz[1] + s3[1]) * sz[2] + s3[2];
+            };
+            const int base = 2 * c[d] + 2;
+            bool done = false;
+            double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            for (int m = 0; m < 5 && !done; ++m) {
+                const int st = base + shifts[m];
+                if (st < 0 || st + 5 >= sz[d]) continue;
+                bool all = true;
+                for (int kk = 0; kk < 6; ++kk) all = all && iok[at_src(st + kk)] != 0.0;
+                if (all) {
+                    for (int kk = 0; kk < 6; ++kk) {
+                        const long long sidx = at_src(st + kk);
+#pragma unroll
+                        for (int q = 0; q < 5; ++q) acc[q] += W.wts[m][kk] * in[q * cap + sidx];
+                    }
+                    done = true;
+                }
+            }
+            double okv = done ? 1.0 : 0.0;
+            if (!done) {
+                const long long pa = at_src(base + 2), pb = at_src(base + 3);
+                if (iok[pa] != 0.0 && iok[pb] != 0.0) {
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) acc[q] = 0.5 * (in[q * cap + pa] + in[q * cap + pb]);
+                    okv = 1.0;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 5; ++q) out[q * cap + i] = acc[q];
+            ook[i] = okv;
+        }
+        __syncthreads();
+        double* t = in;
+        in = out;
+        out = t;
+        t = iok;
+        iok = ook;
+        ook = t;
+        for (int e = 0; e < 3; ++e) sz[e] = nsz[e];
+    }
+    for (long long i = threadIdx.x; i < a.bcells; i += blockDim.x) {
+        bool wr = (mask >> 27) & 1;
+        if (!wr) {
+            const int c[3] = {(int)(i / ((long long)a.bd[1] * a.bd[2])), (int)((i / a.bd[2]) % a.bd[1]),
+                              (int)(i % a.bd[2])};
+            // offsets whose edge box holds the cell: per axis -1 (c < w),
+            // 0 (always), 1 (c >= b - w)
+            for (int o0 = -1; o0 <= 1 && !wr; ++o0)
+                for (int o1 = -1; o1 <= 1 && !wr; ++o1)
+                    for (int o2 = -1; o2 <= 1 && !wr; ++o2) {
+                        if (!((mask >> ((o0 + 1) * 9 + (o1 + 1) * 3 + (o2 + 1))) & 1)) continue;
+                        const int o[3] = {o0, o1, o2};
+                        bool in_box = true;
+                        for (int d = 0; d < 3; ++d) {
+                            const int lo = o[d] == 1 ? (a.bd[d] >= w ? a.bd[d] - w : 0) : 0;
+                            const int hi = o[d] == -1 ? (w < a.bd[d] ? w : a.bd[d]) : a.bd[d];
+                            in_box = in_box && c[d] >= lo && c[d] < hi;
+                        }
+                        wr = in_box;
+                    }
+        }
+        if (wr)
+#pragma unroll
+            for (int q = 0; q < 5; ++q) a.U[amr_u(a, q, slot, i)] = in[q * cap + i];
+    }
+    __syncthreads();
+}
+
+}  // namespace em

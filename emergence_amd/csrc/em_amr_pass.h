@@ -1,0 +1,162 @@
+// Batched block-AMR axis pass (one launch covers every leaf of every level).
+//
+// A work item is (leaf, group of P pencils along the pass axis D); the whole
+// pencil of B cells plus the 5 ghost layers on each side is staged in LDS
+// (interior cells from the pool, ghosts from the leaf's face slabs), the face
+// fluxes come from tile_faces (the uniform pass's arithmetic), and the
+// residual is accumulated as in block_rhs (amr.rs:1224-1299):
+//   rhs -= (F_{i+1/2} - F_{i-1/2}) / dx_level
+// in the reference's axis order; the two boundary face fluxes of every pencil
+// are stored for refluxing.  The grid is persistent: workgroups stride over
+// the work items, so one launch serves any number of leaves.
+#pragma once
+#include "em_amr_types.h"
+#include "em_faces.h"
+
+namespace em {
+
+namespace EM_VARIANT {
+
+template <int D, int MODE, int B, int P, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_WAVES_PER_EU))) void amr_pass(AmrPassArgs pa) {
+    constexpr int C = B + 10;
+    constexpr int M = B + 5;
+    constexpr bool HYB = MODE == MODE_HYBRID;
+    constexpr int CS = HYB ? C : 1;
+    constexpr int FS = HYB ? B + 1 : 1;
+    constexpr int E0 = D == 0 ? 1 : 0;
+    constexpr int E1 = D == 2 ? 1 : 2;
+    __shared__ double sw[5][C][P];
+    __shared__ double sm[5][M][P];
+    __shared__ double ss[2][CS][P];
+    __shared__ double sth[FS][P];
+    __shared__ double sred[NT / 64 + 1];
+    __shared__ unsigned int scnt[3];
+
+    const AmrDev& a = pa.a;
+    if (a.sc->active == 0) return;
+    const int tid = threadIdx.x;
+    if (tid < 3) scnt[tid] = 0;
+    const int be1 = a.bd[E1];
+    const int nt = a.bcells / B;  // pencils of a block along D
+    const long long nwork = (long long)pa.nleaves * pa.ngroup;
+    const int ngh = a.ng;
+    const Eos e = a.eos;
+    unsigned int n_shock = 0, n_fb = 0, n_lim = 0;
+    const double dtv = pa.stage_dt_host ? pa.stage_dt : a.sc->dt;
+    for (long long w = blockIdx.x; w < nwork; w += gridDim.x) {
+        const int li = pa.leaf0 + (int)(w / pa.ngroup);
+        const int gp = (int)(w % pa.ngroup);
+        const long long slot = pa.leaves[li];
+        const int lvl = pa.leaf_level[li];
+        const double dx = a.dx[lvl][D];
+        const int t0 = gp * P;
+        const int Pv = min(P, nt - t0);
+        // ---- load the pencils (interior from the pool, ghosts from the slabs)
+        for (int it = tid; it < C * P; it += NT) {
+            int c, p;
+            if (pa.dfast) {
+                c = it % C;
+                p = it / C;
+            } else {
+                p = it % P;
+                c = it / P;
+            }
+            if (p >= Pv) continue;
+            const int t = t0 + p;
+            const int x = c - 5;
+            double u[5], wv[5];
+            if (x >= 0 && x < B) {
+                int cc[3];
+                cc[D] = x;
+                cc[E0] = t / be1;
+                cc[E1] = t % be1;
+                const long long cell = ((long long)cc[0] * a.bd[1] + cc[1]) * a.bd[2] + cc[2];
+#pragma unroll
+                for (int q = 0; q < 5; ++q) u[q] = a.U[amr_u(a, q, slot, cell)];
+            } else {
+                const int s = 2 * D + (x < 0 ? 0 : 1);
+                const int layer = x < 0 ? x + ngh : x - B;
+#pragma unroll
+                for (int q = 0; q < 5; ++q) u[q] = a.G[amr_g(a, q, slot, s, layer, t)];
+            }
+            to_prim(e, u, wv);
+#pragma unroll
+            for (int q = 0; q < 5; ++q) sw[q][c][p] = wv[q];
+            if constexpr (HYB) {
+                ss[0][c][p] = log(fmax(wv[0], 1e-300));
+                ss[1][c][p] = log(fmax(wv[4] + e.p_inf, 1e-300));
+            }
+        }
+        __syncthreads();
+        const double lam = pa.ndim_pos > 0.0 ? pa.ndim_pos * dtv / dx : 0.0;  // amr.rs:1254
+        tile_faces<D, MODE, B, P, NT>(sw, sm, ss, sth, B, Pv, 0, B, e, lam, pa.tol, n_shock, n_fb, n_lim);
+        // ---- residual and boundary face fluxes
+        for (int it = tid; it < B * P; it += NT) {
+            int c, p;
+            if (pa.dfast) {
+                c = it % B;
+                p = it / B;
+            } else {
+                p = it % P;
+                c = it / P;
+            }
+            if (p >= Pv) continue;
+            const int t = t0 + p;
+            int cc[3];
+            cc[D] = c;
+            cc[E0] = t / be1;
+            cc[E1] = t % be1;
+            const long long cell = ((long long)cc[0] * a.bd[1] + cc[1]) * a.bd[2] + cc[2];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) {
+                const long long ix = amr_u(a, q, slot, cell);
+                const double prev = pa.first ? 0.0 : a.R[ix];
+                a.R[ix] = prev - (sm[q][c + 1][p] - sm[q][c][p]) / dx;
+            }
+        }
+        for (int it = tid; it < 2 * P; it += NT) {
+            const int p = it % P, side = it / P;
+            if (p >= Pv) continue;
+            const int t = t0 + p;
+#pragma unroll
+            for (int q = 0; q < 5; ++q) a.F[amr_f(a, q, slot, 2 * D + side, t)] = sm[q][side ? B : 0][p];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        n_shock += __shfl_xor(n_shock, off);
+        n_fb += __shfl_xor(n_fb, off);
+        n_lim += __shfl_xor(n_lim, off);
+    }
+    if ((tid & 63) == 0) {
+        if (n_shock) atomicAdd(&scnt[0], n_shock);
+        if (n_fb) atomicAdd(&scnt[1], n_fb);
+        if (n_lim) atomicAdd(&scnt[2], n_lim);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const int slotc = blockIdx.x % kSlots;
+        if (scnt[0]) atomicAdd(&a.sc->counters[0][slotc][0], (unsigned long long)scnt[0]);
+        if (scnt[1]) atomicAdd(&a.sc->counters[1][slotc][0], (unsigned long long)scnt[1]);
+        if (scnt[2]) atomicAdd(&a.sc->counters[2][slotc][0], (unsigned long long)scnt[2]);
+    }
+    (void)sred;
+}
+
+// Block sizes with a compiled pass: B = 8, 16, 32 (P pencils, NT threads).
+template <int D, int MODE>
+static hipError_t launch_amr_b(int b, const AmrPassArgs& pa, unsigned grid, hipStream_t s) {
+    switch (b) {
+        case 8: hipLaunchKernelGGL((amr_pass<D, MODE, 8, 8, 128>), dim3(grid), dim3(128), 0, s, pa); break;
+        case 16: hipLaunchKernelGGL((amr_pass<D, MODE, 16, 16, 384>), dim3(grid), dim3(384), 0, s, pa); break;
+        case 32: hipLaunchKernelGGL((amr_pass<D, MODE, 32, 8, 320>), dim3(grid), dim3(320), 0, s, pa); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace EM_VARIANT
+
+}  // namespace em

@@ -1,0 +1,91 @@
+// Device descriptors of the block-AMR pool (host/device shared).
+//
+// All blocks (leaves and ancestors) live in one pool indexed by block id
+// (the host hierarchy keeps ids in the reference's order).  Per variable q:
+//   U[q][slot][cell]          conserved state, cell = (i*b1 + j)*b2 + k
+//   U0, R                     step-start state and residual (leaves)
+//   G[q][slot][s][layer][t]   ghost slab s = 2*axis + side; on the low
+//                             side layer l is the cell at axis coordinate
+//                             l - ng, on the high side at b + l
+//   F[q][slot][s][t]          boundary face fluxes (for refluxing)
+// where t is the transverse cell index of a face, (t0*bd[e1] + t1) over the
+// two other axes e0 < e1.
+#pragma once
+#include <cstdint>
+
+#include "em_device.h"
+#include "em_types.h"
+
+namespace em {
+
+constexpr int kMaxLevels = 8;
+
+struct AmrDev {
+    double* U;
+    double* U0;
+    double* R;
+    double* G;
+    double* F;
+    long long cap;        // pool capacity (blocks)
+    long long vs;         // variable stride of U/U0/R = cap * bcells
+    long long gs;         // variable stride of G = cap * 6 * ng * tmax
+    long long fs;         // variable stride of F = cap * 6 * tmax
+    int bd[3];            // block extent per axis (1 on inactive axes)
+    int act[3];
+    int ndim;
+    int bcells;
+    int tmax;             // largest face cell count, bcells / min active bd
+    int ng;               // ghost layers held in the slabs (>= 5)
+    int levels;
+    int nb0[3];           // level-0 blocks per axis (1 on inactive axes)
+    long long map_off[kMaxLevels];  // dense per-level block index offsets
+    const int* map;       // block id at (level, block coords) or -1
+    double dx[kMaxLevels][3];
+    Eos eos;
+    int bc_kind[3][2];
+    double bc_state[3][2][5];
+    DevScalars* sc;
+};
+
+// One ghost slab of one leaf: kind 0 same-level copy from block `src`,
+// 1 physical boundary condition, 2 6th-order prediction from level-1.
+struct GhostTask {
+    int slot, axis, side, kind, src, level;
+    int idx[3];
+};
+
+// One coarse boundary face corrected by refluxing (amr.rs:1327-1406):
+// coarse leaf slot, cell and face index, the covering fine faces.
+struct RefluxTask {
+    int slot, cell, face, nfine, level;
+    int fslot[4], fface[4];
+};
+
+struct AmrPassArgs {
+    AmrDev a;
+    const int* leaves;      // leaf slots (block ids)
+    const int* leaf_level;  // level of each leaf
+    int leaf0, nleaves;     // leaves [leaf0, leaf0 + nleaves) of the list
+    int ngroup;             // pencil groups per leaf along the axis
+    int first;              // residual input is zero (first active axis)
+    int dfast;              // the pass axis is the contiguous one in a block
+    double ndim_pos;        // active dimensions if the limiter is on, else 0
+    double tol;
+    int stage_dt_host;
+    double stage_dt;
+};
+
+// pencils per work item of the batched pass for block size b (launch_amr_b)
+inline int amr_pass_pencils(int b) { return b == 16 ? 16 : 8; }
+
+__host__ __device__ inline long long amr_u(const AmrDev& a, int q, long long slot, long long cell) {
+    return q * a.vs + slot * a.bcells + cell;
+}
+__host__ __device__ inline long long amr_g(const AmrDev& a, int q, long long slot, int s, int layer, int t) {
+    return q * a.gs + ((slot * 6 + s) * a.ng + layer) * (long long)a.tmax + t;
+}
+__host__ __device__ inline long long amr_f(const AmrDev& a, int q, long long slot, int s, int t) {
+    return q * a.fs + (slot * 6 + s) * (long long)a.tmax + t;
+}
+
+}  // namespace em

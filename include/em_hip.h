@@ -126,6 +126,109 @@ int em_solver_synchronize(em_solver* s);
 int em_case_init(int32_t case_id, const int64_t n[3], const double lo[3], const double hi[3], int32_t ng,
                  const double* params, double* u);
 
+/* ------------------------------------------------------------------------
+ * Block-structured AMR (global SSP-RK3 step), drop-in for
+ *   GpuAmr<f64, 5, Euler> (fluid/crates/compressible-cuda/src/amr.rs:95-858)
+ * with the arithmetic of the CPU `Amr` (fluid/crates/compressible/src/amr.rs),
+ * which is its parity target.  Blocks are b^D cells (b = 8, 16 or 32; 1 on
+ * axes with base[d] == 0), ids follow the reference's order (children
+ * appended, coarsened children swap-removed).  Host block data is AoS float64
+ * per block: index ((id*b0 + i)*b1 + j)*b2 + k, five conserved values.
+ * Every block (leaves and ancestors) lives on the device; a regrid moves only
+ * the per-leaf details to the host. */
+typedef struct em_amr em_amr;
+
+/* Message of the calling thread's last failed em_amr_* call. */
+const char* em_amr_last_error(void);
+
+/* GpuAmr::new(Amr, ..) (amr.rs:132-184 of compressible-cuda): a hierarchy
+ * handed over block by block, as the reference's Amr holds it (level, block
+ * coordinates idx[3*i..], leaf flag, data).  Configuration as AmrConfig
+ * (compressible/src/amr.rs:49-63): base blocks per axis (0 = inactive axis),
+ * block size b, levels, ghost depth (>= 5 used), refinement threshold eps,
+ * regrid interval; domain lo/hi; eos {gamma, p_inf, b, q}; boundaries and
+ * flux mode as em_solver_create; positivity limiter; cfl; hybrid tolerance. */
+int em_amr_create(em_amr** out, const int64_t base[3], int32_t b, int32_t levels, int32_t ng, double eps,
+                  int32_t regrid_every, const double lo[3], const double hi[3], const double eos[4],
+                  const int32_t bc_kind[6], const double bc_state[30], int32_t mode, int32_t positivity, double cfl,
+                  double tol, int64_t nblocks, const int32_t* level, const int64_t* idx, const uint8_t* leaf,
+                  const double* u);
+
+/* Amr::new (compressible/src/amr.rs:289-343) on the device: level-0 blocks
+ * from a built-in initial condition, refined where the wavelet details exceed
+ * eps (children re-evaluated from the initial condition).  Cases (primitive
+ * states, params[0..14]):
+ *   0 four-quadrant 2-D Riemann problem split at (params[0], params[1])
+ *   1 smooth wave rho = 1 + 0.1 sin(2 pi x) cos(2 pi y), u = (0.3, 0.2, 0.1)
+ *   2 shock-bubble: Mach params[0] shock at x = params[1] moving into
+ *     (rho 1, p 1), bubble centre params[2..4], radius params[5], density
+ *     params[6] (gamma 1.4)
+ *   3 isentropic vortex (compressible/examples/amr_validate.rs:44-51)
+ *   4 state at rest rho = 1 + sum c_d x_d^5 / 5, p = 1 + sum c_d x_d^4 / 4,
+ *     c = params[0..2] */
+int em_amr_create_case(em_amr** out, const int64_t base[3], int32_t b, int32_t levels, int32_t ng, double eps,
+                       int32_t regrid_every, const double lo[3], const double hi[3], const double eos[4],
+                       const int32_t bc_kind[6], const double bc_state[30], int32_t mode, int32_t positivity,
+                       double cfl, double tol, int32_t case_id, const double* params);
+
+int em_amr_destroy(em_amr* h);
+
+/* 1 (default): no FMA contraction, the CPU Amr's arithmetic bit for bit;
+ * 0: contracted flux pass. */
+int em_amr_set_variant(em_amr* h, int32_t precise);
+
+/* 1 (default): one persistent launch per stage phase over every leaf of
+ * every level; 0: one launch per leaf (the unbatched comparison). */
+int em_amr_set_batched(em_amr* h, int32_t batched);
+
+/* Step count of the next regrid check (Amr::next_regrid). */
+int em_amr_set_next_regrid(em_amr* h, int64_t step);
+
+/* Block and leaf counts (GpuAmr::leaf_count, amr.rs:855-857). */
+int em_amr_nblocks(em_amr* h, int64_t* nblocks, int64_t* leaves);
+
+/* GpuAmr::download (amr.rs:821-843): hierarchy and data of every block in id
+ * order; any pointer may be NULL. */
+int em_amr_download(em_amr* h, int32_t* level, int64_t* idx, uint8_t* leaf, double* u);
+
+/* Replace the data of every block (same layout as download). */
+int em_amr_upload(em_amr* h, const double* u);
+
+/* Residual of every leaf for one stage (restriction of the edge ancestors,
+ * ghost slabs, fluxes, refluxing; Amr::compute_rhs, amr.rs:1302-1323) with
+ * the limiter's lambda from stage_dt; ancestors are written as zero.
+ * stats = {faces, shock-path faces, fallbacks, limited faces} of this call. */
+int em_amr_rhs(em_amr* h, double stage_dt, double* rhs, uint64_t stats[4]);
+
+/* GpuAmr::step (amr.rs:683-700): SSP-RK3 step of size dt, regrid when due. */
+int em_amr_step(em_amr* h, double dt);
+
+/* nsteps CFL steps with dt = cfl / rate computed on the device each step
+ * (no host round trip between steps except regrids). */
+int em_amr_run_steps(em_amr* h, int64_t nsteps);
+
+/* GpuAmr::run_to (amr.rs:807-819). */
+int em_amr_run_to(em_amr* h, double t_end, int64_t* steps);
+
+/* GpuAmr::dt_cfl (amr.rs:769-805): cfl / max over leaf cells of
+ * sum_d (|u_d| + c) / dx_d(level). */
+int em_amr_dt_cfl(em_amr* h, double* dt);
+
+/* Wavelet details of every block in id order (0 for ancestors), after
+ * restricting the edge ancestors (GpuAmr::details, amr.rs:702-723). */
+int em_amr_details(em_amr* h, double* det);
+
+/* Amr::regrid (compressible/src/amr.rs:1039-1046) on the device data. */
+int em_amr_regrid(em_amr* h);
+
+/* GpuAmr::counters (amr.rs:850-853). */
+int em_amr_counters(em_amr* h, uint64_t out[3]);
+
+/* GpuAmr::t (amr.rs:845-847). */
+int em_amr_time(em_amr* h, double* t);
+
+int em_amr_synchronize(em_amr* h);
+
 #ifdef __cplusplus
 }
 #endif
