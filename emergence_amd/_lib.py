@@ -35,6 +35,9 @@ EXPORTS = (
     "em_amr_destroy",
     "em_amr_set_variant",
     "em_amr_set_batched",
+    "em_amr_set_graph",
+    "em_amr_set_profile",
+    "em_amr_profile",
     "em_amr_set_next_regrid",
     "em_amr_nblocks",
     "em_amr_download",
@@ -104,6 +107,9 @@ def load():
     lib.em_amr_destroy.argtypes = [P]
     lib.em_amr_set_variant.argtypes = [P, C.c_int32]
     lib.em_amr_set_batched.argtypes = [P, C.c_int32]
+    lib.em_amr_set_graph.argtypes = [P, C.c_int32]
+    lib.em_amr_set_profile.argtypes = [P, C.c_int32]
+    lib.em_amr_profile.argtypes = [P, D]
     lib.em_amr_set_next_regrid.argtypes = [P, C.c_int64]
     lib.em_amr_nblocks.argtypes = [P, I64, I64]
     lib.em_amr_download.argtypes = [P, I32, I64, U8, D]

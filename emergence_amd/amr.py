@@ -88,6 +88,18 @@ class HipAmr:
     def set_batched(self, batched):
         _lib.check_amr(self.lib.em_amr_set_batched(self.h, int(bool(batched))))
 
+    def set_graph(self, on):
+        _lib.check_amr(self.lib.em_amr_set_graph(self.h, int(bool(on))))
+
+    def set_profile(self, on):
+        _lib.check_amr(self.lib.em_amr_set_profile(self.h, int(bool(on))))
+
+    def profile(self):
+        """Accumulated ms of [restriction, ghosts, fluxes, reflux + update, regrid]."""
+        out = (C.c_double * 5)()
+        _lib.check_amr(self.lib.em_amr_profile(self.h, out))
+        return list(out)
+
     def set_next_regrid(self, step):
         _lib.check_amr(self.lib.em_amr_set_next_regrid(self.h, int(step)))
 

@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <functional>
 #include <tuple>
@@ -560,6 +561,11 @@ struct Amr {
             for (auto p : r) f(p);
         f(d_tmp);
         f(d_det);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        for (auto& e : prof_ev) {
+            (void)hipEventDestroy(std::get<1>(e));
+            (void)hipEventDestroy(std::get<2>(e));
+        }
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -584,6 +590,7 @@ struct Amr {
             (void)hipFree(dev.G);
             (void)hipFree(dev.F);
         }
+        ++version;
         dev.U = U;
         dev.U0 = U0;
         dev.R = R;
@@ -1047,6 +1054,7 @@ struct Amr {
             for (int s = 0; s < 2; ++s) d_reflux[d][s] = dev_vec(reflux[d][s], d_reflux[d][s]);
         HCHK(hipStreamSynchronize(stream));
         tables_dirty = false;
+        ++version;
     }
 
     void tables() {
@@ -1103,13 +1111,19 @@ struct Amr {
     }
 
     // -------------------------------------------------------------- stages
-    void restrict_all() {  // amr.rs:737-816, finest level first (whole blocks)
+    void restrict_all() {  // amr.rs:737-816, finest level first
         for (int l = levels - 2; l >= 0; --l) {
             const int n = (int)edge[l].size();
             if (!n) continue;
-            const unsigned g = (unsigned)std::min(n, big_wg);
-            hipLaunchKernelGGL(k_amr_restrict, dim3(g), dim3(256), 0, stream, dev, W, d_edge_ids[l], d_edge_idx[l], n, l,
-                               box_w(), big_scr, big_cap);
+            if (batched) {
+                const unsigned g = (unsigned)std::min(n, big_wg);
+                hipLaunchKernelGGL(k_amr_restrict, dim3(g), dim3(256), 0, stream, dev, W, d_edge_ids[l], d_edge_idx[l],
+                                   n, l, box_w(), big_scr, big_cap);
+            } else {
+                for (int i = 0; i < n; ++i)
+                    hipLaunchKernelGGL(k_amr_restrict, dim3(1), dim3(256), 0, stream, dev, W, d_edge_ids[l] + i,
+                                       d_edge_idx[l] + 4 * i, 1, l, box_w(), big_scr, big_cap);
+            }
             HCHK(hipGetLastError());
         }
     }
@@ -1195,20 +1209,76 @@ struct Amr {
         refluxes();
     }
 
+    // stage == 0: copy U to U0; 1..3: SSP-RK3 update
     void rk(int stage) {
         const int n = (int)leaves.size();
-        const unsigned g = (unsigned)std::min<long long>(((long long)n * bcells + 255) / 256, 8192);
-        hipLaunchKernelGGL(k_amr_rk, dim3(g), dim3(256), 0, stream, dev, d_leaves, n, stage);
+        if (batched) {
+            const unsigned g = (unsigned)std::min<long long>(((long long)n * bcells + 255) / 256, 8192);
+            if (stage == 0)
+                hipLaunchKernelGGL(k_amr_copy_u0, dim3(g), dim3(256), 0, stream, dev, d_leaves, n);
+            else
+                hipLaunchKernelGGL(k_amr_rk, dim3(g), dim3(256), 0, stream, dev, d_leaves, n, stage);
+        } else {
+            const unsigned g = (unsigned)std::max(1, (bcells + 255) / 256);
+            for (int li = 0; li < n; ++li) {
+                if (stage == 0)
+                    hipLaunchKernelGGL(k_amr_copy_u0, dim3(g), dim3(256), 0, stream, dev, d_leaves + li, 1);
+                else
+                    hipLaunchKernelGGL(k_amr_rk, dim3(g), dim3(256), 0, stream, dev, d_leaves + li, 1, stage);
+            }
+        }
         HCHK(hipGetLastError());
     }
 
+    // Per-phase time (ms) as GpuAmr::prof (compressible-cuda/src/amr.rs:
+    // 110-113): device time of restriction, ghost slabs, flux passes,
+    // refluxing + RK update (events recorded only while profiling), and host
+    // wall time of regrids.
+    bool profile = false;
+    double prof_ms[5] = {0, 0, 0, 0, 0};
+    std::vector<std::tuple<int, hipEvent_t, hipEvent_t>> prof_ev;
+    size_t prof_used = 0;
+
+    template <class F>
+    void phase(int k, F&& f) {
+        if (!profile) {
+            f();
+            return;
+        }
+        if (prof_used == prof_ev.size()) {
+            hipEvent_t a, b;
+            HCHK(hipEventCreate(&a));
+            HCHK(hipEventCreate(&b));
+            prof_ev.emplace_back(k, a, b);
+        }
+        auto& e = prof_ev[prof_used++];
+        std::get<0>(e) = k;
+        HCHK(hipEventRecord(std::get<1>(e), stream));
+        f();
+        HCHK(hipEventRecord(std::get<2>(e), stream));
+    }
+
+    void collect_profile() {
+        if (!prof_used) return;
+        HCHK(hipStreamSynchronize(stream));
+        for (size_t i = 0; i < prof_used; ++i) {
+            float ms = 0.f;
+            HCHK(hipEventElapsedTime(&ms, std::get<1>(prof_ev[i]), std::get<2>(prof_ev[i])));
+            prof_ms[std::get<0>(prof_ev[i])] += ms;
+        }
+        prof_used = 0;
+    }
+
     void step_kernels() {
-        const int n = (int)leaves.size();
-        const unsigned g = (unsigned)std::min<long long>(((long long)n * bcells + 255) / 256, 8192);
-        hipLaunchKernelGGL(k_amr_copy_u0, dim3(g), dim3(256), 0, stream, dev, d_leaves, n);
+        phase(3, [&] { rk(0); });
         for (int s = 1; s <= 3; ++s) {
-            compute_rhs(0, 0.0);
-            rk(s);
+            phase(0, [&] { restrict_all(); });
+            phase(1, [&] { fill_ghosts(); });
+            phase(2, [&] { passes(0, 0.0); });
+            phase(3, [&] {
+                refluxes();
+                rk(s);
+            });
         }
         hipLaunchKernelGGL(k_amr_advance, dim3(1), dim3(1), 0, stream, sc);
         HCHK(hipGetLastError());
@@ -1220,6 +1290,7 @@ struct Amr {
         hipLaunchKernelGGL(k_amr_set_run, dim3(1), dim3(1), 0, stream, sc);
         hipLaunchKernelGGL(k_amr_set_dt, dim3(1), dim3(1), 0, stream, sc, dt);
         step_kernels();
+        collect_profile();
         t += dt;
         steps += 1;
         check_err();
@@ -1235,13 +1306,49 @@ struct Amr {
         if (h[0].err & 1) throw AmrError(EM_ERR_COLLAPSE, "time step collapsed");
     }
 
+    // One CFL step with dt computed on the device (rate, dt, RK stages).
+    void device_step() {
+        rate_kernels();
+        hipLaunchKernelGGL(k_amr_dt_from_rate, dim3(1), dim3(1), 0, stream, sc, cfl);
+        step_kernels();
+    }
+
+    // The launch sequence of a device step is fixed between hierarchy edits,
+    // so batched stepping replays it as one HIP graph, captured again when
+    // the tables or the pool change (kernel arguments are captured by value).
+    hipGraphExec_t gexec = nullptr;
+    long long graph_version = -1;
+    long long version = 0;
+
+    void graph_step() {
+        if (!gexec || graph_version != version) {
+            if (gexec) {
+                HCHK(hipGraphExecDestroy(gexec));
+                gexec = nullptr;
+            }
+            hipGraph_t g;
+            HCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            device_step();
+            HCHK(hipStreamEndCapture(stream, &g));
+            const hipError_t e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HCHK(e);
+            graph_version = version;
+        }
+        HCHK(hipGraphLaunch(gexec, stream));
+    }
+
+    bool use_graph = true;
+
     void run_steps(int64_t nsteps) {
         hipLaunchKernelGGL(k_amr_set_run, dim3(1), dim3(1), 0, stream, sc);
         for (int64_t q = 0; q < nsteps; ++q) {
             tables();
-            compute_rate();
-            hipLaunchKernelGGL(k_amr_dt_from_rate, dim3(1), dim3(1), 0, stream, sc, cfl);
-            step_kernels();
+            if (batched && use_graph && !profile)
+                graph_step();
+            else
+                device_step();
+            collect_profile();
             steps += 1;
             if (steps >= next_regrid) {
                 t = read_t();
@@ -1274,6 +1381,10 @@ struct Amr {
 
     void compute_rate() {
         tables();
+        rate_kernels();
+    }
+
+    void rate_kernels() {
         hipLaunchKernelGGL(k_amr_zero_rate, dim3(1), dim3(kSlots), 0, stream, sc);
         const int n = (int)leaves.size();
         const unsigned g = (unsigned)std::min<long long>(((long long)n * bcells + 255) / 256, 2048);
@@ -1326,6 +1437,7 @@ struct Amr {
 
     bool regrid_if_due() {  // amr.rs:1525-1537
         if (steps < next_regrid) return false;
+        const auto t0 = std::chrono::steady_clock::now();
         std::vector<std::tuple<int, int64_t, int64_t, int64_t, bool>> before, after;
         for (auto& b : blocks) before.emplace_back(b.level, b.idx[0], b.idx[1], b.idx[2], b.leaf);
         regrid();
@@ -1333,6 +1445,10 @@ struct Amr {
         const bool changed = before != after;
         regrid_interval = changed ? regrid_every : std::min<int64_t>(2 * regrid_interval, 2 * (int64_t)regrid_every);
         next_regrid = steps + regrid_interval;
+        if (profile) {
+            HCHK(hipStreamSynchronize(stream));
+            prof_ms[4] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        }
         return true;
     }
 
@@ -1605,12 +1721,32 @@ int em_amr_destroy(em_amr* h) {
 int em_amr_set_variant(em_amr* h, int32_t precise) {
     ANEED(h);
     h->a->precise = precise ? 1 : 0;
+    ++h->a->version;
     return EM_OK;
 }
 
 int em_amr_set_batched(em_amr* h, int32_t batched) {
     ANEED(h);
     h->a->batched = batched != 0;
+    ++h->a->version;
+    return EM_OK;
+}
+
+int em_amr_set_graph(em_amr* h, int32_t on) {
+    ANEED(h);
+    h->a->use_graph = on != 0;
+    return EM_OK;
+}
+
+int em_amr_set_profile(em_amr* h, int32_t on) {
+    ANEED(h);
+    h->a->profile = on != 0;
+    return EM_OK;
+}
+
+int em_amr_profile(em_amr* h, double out[5]) {
+    ANEED(h);
+    for (int i = 0; i < 5; ++i) out[i] = h->a->prof_ms[i];
     return EM_OK;
 }
 

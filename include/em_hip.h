@@ -181,6 +181,17 @@ int em_amr_set_variant(em_amr* h, int32_t precise);
  * every level; 0: one launch per leaf (the unbatched comparison). */
 int em_amr_set_batched(em_amr* h, int32_t batched);
 
+/* 1 (default): batched em_amr_run_steps replays each step as one captured
+ * HIP graph (captured again after every hierarchy edit); 0: eager launches. */
+int em_amr_set_graph(em_amr* h, int32_t on);
+
+/* GpuAmr::profile / GpuAmr::prof (amr.rs:110-115): when on, record device
+ * time per phase; em_amr_profile returns the accumulated milliseconds of
+ * [restriction, ghost slabs, flux passes, refluxing + RK update, regrid
+ * (host wall time)].  Profiled steps are launched eagerly. */
+int em_amr_set_profile(em_amr* h, int32_t on);
+int em_amr_profile(em_amr* h, double out[5]);
+
 /* Step count of the next regrid check (Amr::next_regrid). */
 int em_amr_set_next_regrid(em_amr* h, int64_t step);
 
