@@ -15,6 +15,15 @@ axis pass), timed with HIP events on the solver's stream inside the timed
 region.  The CPU baseline is the oracle (bit-faithful restatement of the
 reference CPU solver) timed for one step of the same 256^3 workload on the
 host cores.
+
+The "amr" entry is the batched block-AMR workload: 3-D shock-bubble on
+[0,2]x[0,1]x[0,1] (Mach 1.22 shock at x = 0.5 into rho = 1, p = 1; sphere of
+radius 0.25 at (1, 0.5, 0.5) with rho = 0.138), 8^3-cell blocks, 3 levels,
+wavelet threshold 1e-2, regrid every 4 steps, Hybrid flux mode, CFL 0.4,
+outflow x / reflecting y, z boundaries (about 2,900 blocks).  value = leaf
+cells * steps * 3 / seconds, regrids included, for the batched path (one
+persistent launch per phase, graph-replayed steps) and for per-block launches
+of the same kernels, whose states are checked bit-identical.
 """
 import argparse
 import json
@@ -49,7 +58,83 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shapes (short profiled runs)")
+    ap.add_argument("--no-amr", action="store_true", help="skip the block-AMR workload")
+    ap.add_argument("--amr-only", action="store_true", help="only the block-AMR workload (profiling)")
+    ap.add_argument("--amr-steps", type=int, default=20)
+    ap.add_argument("--amr-per-block-steps", type=int, default=3)
     return ap.parse_args()
+
+
+AMR_BC = [["outflow", "outflow"], ["reflect", "reflect"], ["reflect", "reflect"]]
+AMR_CASE = dict(base=(8, 4, 4), b=8, levels=3, lo=(0.0, 0.0, 0.0), hi=(2.0, 1.0, 1.0),
+                params=[1.22, 0.5, 1.0, 0.5, 0.5, 0.25, 0.138], eps=1e-2, regrid_every=4, mode="hybrid", cfl=0.4)
+
+
+def amr_bench(em, args):
+    """Batched vs per-block block-AMR stepping on the 3-D shock-bubble."""
+    c = AMR_CASE
+    kw = dict(eps=c["eps"], regrid_every=c["regrid_every"], bc=AMR_BC, mode=c["mode"], cfl=c["cfl"])
+    root = em.HipAmr(c["base"], c["b"], c["levels"], c["lo"], c["hi"], case="shock_bubble", params=c["params"], **kw)
+    hier = root.blocks() + (root.download(),)
+    nblocks, nleaves = root.counts()
+    del root
+    bat = em.HipAmr(c["base"], c["b"], c["levels"], c["lo"], c["hi"], blocks=hier, **kw)
+    per = em.HipAmr(c["base"], c["b"], c["levels"], c["lo"], c["hi"], blocks=hier, batched=False, **kw)
+    k = args.amr_per_block_steps
+    bat.run_steps(k)
+    per.synchronize()
+    t0 = time.perf_counter()
+    per.run_steps(k)
+    per.synchronize()
+    el_per = time.perf_counter() - t0
+    same = bool(np.array_equal(bat.download(), per.download()))
+    cells_per = per.leaf_count() * c["b"] ** 3
+    del per
+    bat.synchronize()
+    leaves0 = bat.leaf_count()
+    t0 = time.perf_counter()
+    bat.run_steps(args.amr_steps)
+    bat.synchronize()
+    el = time.perf_counter() - t0
+    leaves1 = bat.leaf_count()
+    cells = 0.5 * (leaves0 + leaves1) * c["b"] ** 3
+    bat.set_profile(True)
+    bat.run_steps(4)
+    phase = [v / 4 for v in bat.profile()]
+    value = cells * args.amr_steps * 3 / el
+    value_per = cells_per * k * 3 / el_per
+    cpu = None if args.no_cpu_baseline else amr_cpu_baseline(nleaves * c["b"] ** 3)
+    return {
+        "workload": "3-D shock-bubble block AMR, 8^3 blocks, 3 levels, Hybrid, CFL 0.4, regrid every 4",
+        "blocks": nblocks,
+        "leaves": nleaves,
+        "value": value,
+        "unit": "cell-updates/s per RK stage",
+        "ms_per_step": el / args.amr_steps * 1e3,
+        "steps": args.amr_steps,
+        "per_block": {"value": value_per, "ms_per_step": el_per / k * 1e3, "steps": k},
+        "speedup_vs_per_block": value / value_per,
+        "identical_to_per_block": same,
+        "phase_ms_per_step": dict(zip(["restrict", "ghosts", "fluxes", "reflux_update", "regrid"], phase)),
+        "cpu_baseline": cpu,
+    }
+
+
+def amr_cpu_baseline(cells):
+    """Oracle block AMR (restated reference CPU Amr) on the host cores: one
+    global SSP-RK3 step of the same hierarchy."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle  # noqa: E402  (CPU-baseline leg only)
+
+    c = AMR_CASE
+    threads = _oracle.load().emref_max_threads()
+    o = _oracle.OracleAmr("shock_bubble", c["params"], c["base"], c["b"], c["levels"], c["lo"], c["hi"], eps=c["eps"],
+                          regrid_every=c["regrid_every"], bc=AMR_BC, mode=c["mode"], cfl=c["cfl"])
+    t0 = time.perf_counter()
+    o.step(o.dt_cfl())
+    el = time.perf_counter() - t0
+    return {"value": cells * 3 / el, "unit": "cell-updates/s per RK stage", "cores": int(threads), "kind": "port",
+            "sample": f"1 global SSP-RK3 step of the same hierarchy, OpenMP over leaves, {el:.1f} s wall"}
 
 
 def profiled_traffic(variant, d):
@@ -102,6 +187,9 @@ def main():
         sys.exit(2)
     import emergence_amd as em
 
+    if args.amr_only:
+        print(json.dumps({"amr": amr_bench(em, args)}), flush=True)
+        return
     n = args.n
     lo, hi = (0.0, 0.0, 0.0), (2 * np.pi,) * 3
     u = em.case_init(1, (n, n, n), lo, hi, 6, [0.3, float(0x5EED)])
@@ -156,6 +244,9 @@ def main():
     }
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline(n, u, args.cpu_steps)
+    if not args.no_amr:
+        del g
+        rec["amr"] = amr_bench(em, args)
     print(json.dumps(rec), flush=True)
 
 

@@ -199,15 +199,15 @@ __global__ __launch_bounds__(256) void k_amr_ghost(AmrDev a, AmrWeights W, const
                 for (int q = 0; q < 5; ++q) a.G[amr_g(a, q, tk.slot, s, layer, t)] = u[q];
             }
         } else {
-            long long fmin[3], fmax[3];
+            int fmin[3], fmax[3];
             for (int e = 0; e < 3; ++e) {
                 if (!a.act[e]) {
                     fmin[e] = fmax[e] = 0;
                 } else if (e == d) {
-                    fmin[e] = side == 0 ? (long long)tk.idx[e] * B - ng : (long long)tk.idx[e] * B + B;
+                    fmin[e] = side == 0 ? tk.idx[e] * B - ng : tk.idx[e] * B + B;
                     fmax[e] = fmin[e] + ng - 1;
                 } else {
-                    fmin[e] = (long long)tk.idx[e] * a.bd[e];
+                    fmin[e] = tk.idx[e] * a.bd[e];
                     fmax[e] = fmin[e] + a.bd[e] - 1;
                 }
             }
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(256) void k_amr_ghost(AmrDev a, AmrWeights W, const
                 cc[E0] = t / a.bd[E1];
                 cc[E1] = t % a.bd[E1];
                 cc[d] = layer;
-                const long long li = ((long long)cc[0] * rsz[1] + cc[1]) * rsz[2] + cc[2];
+                const int li = (cc[0] * rsz[1] + cc[1]) * rsz[2] + cc[2];
                 for (int q = 0; q < 5; ++q) a.G[amr_g(a, q, tk.slot, s, layer, t)] = res[q * cap + li];
             }
         }
@@ -252,14 +252,14 @@ __global__ __launch_bounds__(256) void k_amr_predict_blocks(AmrDev a, AmrWeights
                                                            double* scratch, long long cap) {
     double* scr = scratch + (long long)blockIdx.x * 12 * cap;
     for (int i = blockIdx.x; i < n; i += gridDim.x) {
-        long long fmin[3], fmax[3];
+        int fmin[3], fmax[3];
         for (int e = 0; e < 3; ++e) {
-            fmin[e] = (long long)idx[3 * i + e] * a.bd[e];
+            fmin[e] = idx[3 * i + e] * a.bd[e];
             fmax[e] = fmin[e] + a.bd[e] - 1;
             if (!a.act[e]) fmin[e] = fmax[e] = 0;
         }
         const double* res = predict_region(a, W, level, fmin, fmax, scr, cap);
-        for (long long c = threadIdx.x; c < a.bcells; c += blockDim.x)
+        for (int c = threadIdx.x; c < a.bcells; c += blockDim.x)
             for (int q = 0; q < 5; ++q) a.U[amr_u(a, q, slots[i], c)] = res[q * cap + c];
         __syncthreads();
     }
@@ -294,15 +294,15 @@ __global__ __launch_bounds__(256) void k_amr_details(AmrDev a, AmrWeights W, con
     for (int li = blockIdx.x; li < n; li += gridDim.x) {
         const long long slot = leaves[li];
         double mx = 0.0;
-        for (long long c = threadIdx.x; c < a.bcells; c += blockDim.x) {
-            const int ci[3] = {(int)(c / ((long long)a.bd[1] * a.bd[2])), (int)((c / a.bd[2]) % a.bd[1]),
-                               (int)(c % a.bd[2])};
+        for (int c = threadIdx.x; c < a.bcells; c += blockDim.x) {
+            int ci[3];
+            unflat(c, a.bd, ci);
             auto sens = [&](int d, int x, double s[2]) {
                 double u[5], w[5];
                 if (x >= 0 && x < a.bd[d]) {
                     int c3[3] = {ci[0], ci[1], ci[2]};
                     c3[d] = x;
-                    const long long cell = ((long long)c3[0] * a.bd[1] + c3[1]) * a.bd[2] + c3[2];
+                    const int cell = (c3[0] * a.bd[1] + c3[1]) * a.bd[2] + c3[2];
                     for (int q = 0; q < 5; ++q) u[q] = a.U[amr_u(a, q, slot, cell)];
                 } else {
                     const int E0 = d == 0 ? 1 : 0, E1 = d == 2 ? 1 : 2;
@@ -513,12 +513,16 @@ struct Amr {
         }
         big_cap = std::max<long long>(rp, bcells);
         small_cap = sp;
+        // workgroups of the restriction / block-prediction kernels: up to
+        // 1024, with their scratch held under 2 GiB
+        big_wg = (int)std::max<long long>(16, std::min<long long>(1024, (2ll << 30) / (96ll * big_cap)));
         HCHK(hipMalloc(&big_scr, sizeof(double) * 12 * big_cap * big_wg));
         HCHK(hipMalloc(&small_scr, sizeof(double) * 12 * small_cap * small_wg));
         dev.map = d_map;
         dev.sc = sc;
         for (int d = 0; d < 3; ++d) {
             dev.bd[d] = bd[d];
+            dev.lb[d] = bd[d] == 32 ? 5 : bd[d] == 16 ? 4 : bd[d] == 8 ? 3 : 0;
             dev.act[d] = act[d] ? 1 : 0;
             dev.nb0[d] = (int)nbl(0, d);
             for (int s = 0; s < 2; ++s) {

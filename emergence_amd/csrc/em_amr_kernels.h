@@ -18,65 +18,69 @@ struct AmrWeights {
     double wts[5][6];  // restriction stencils for shifts 0, -1, 1, -2, 2
 };
 
-__device__ __forceinline__ long long div_e(long long a, long long b) {
-    long long q = a / b;
-    if (a % b < 0) q = b > 0 ? q - 1 : q + 1;
-    return q;
-}
-__device__ __forceinline__ long long rem_e(long long a, long long b) {
-    long long r = a % b;
+// Index arithmetic is 32-bit throughout: global cell indices of one level
+// stay below 2^31 for any hierarchy that fits in device memory.
+__device__ __forceinline__ int rem_e(int a, int b) {
+    const int r = a % b;
     return r < 0 ? r + (b < 0 ? -b : b) : r;
 }
 
-__device__ __forceinline__ long long amr_n_level(const AmrDev& a, int l, int d) {
-    return a.act[d] ? (long long)a.nb0[d] * a.bd[d] * (1ll << l) : 1;
+__device__ __forceinline__ int amr_n_level(const AmrDev& a, int l, int d) {
+    return a.act[d] ? (a.nb0[d] * a.bd[d]) << l : 1;
 }
 
-// Block id holding global cell g (already inside the domain) at level l, -1
-// if absent; *cell receives the cell index inside the block.
-__device__ __forceinline__ int amr_find(const AmrDev& a, int l, const long long g[3], long long* cell) {
-    long long s = 0;
-    int c[3];
+// Block id holding global cell g (inside the domain) at level l, -1 if
+// absent; *cell receives the cell index inside the block.
+__device__ __forceinline__ int amr_find(const AmrDev& a, int l, const int g[3], int* cell) {
+    int s = 0;
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const long long nb = a.act[d] ? (long long)a.nb0[d] << l : 1;
-        const long long k = a.act[d] ? div_e(g[d], a.bd[d]) : 0;
+        const int nb = a.act[d] ? a.nb0[d] << l : 1;
+        const int k = g[d] >> a.lb[d];
         if (k < 0 || k >= nb) return -1;
         s = s * nb + k;
-        c[d] = a.act[d] ? (int)rem_e(g[d], a.bd[d]) : 0;
     }
-    *cell = ((long long)c[0] * a.bd[1] + c[1]) * a.bd[2] + c[2];
+    *cell = ((g[0] & (a.bd[0] - 1)) * a.bd[1] + (g[1] & (a.bd[1] - 1))) * a.bd[2] + (g[2] & (a.bd[2] - 1));
     return a.map[a.map_off[l] + s];
+}
+
+__device__ __forceinline__ void unflat(int i, const int sz[3], int c[3]) {
+    const int q = i / sz[2];
+    c[2] = i - q * sz[2];
+    c[0] = q / sz[1];
+    c[1] = q - c[0] * sz[1];
 }
 
 // bc_box of amr.rs:450-504 on a box held in scratch (values val[q*cap+i],
 // validity ok[i]): cells outside a non-periodic boundary are filled from
 // cells of the same box, one axis sweep at a time; called by one workgroup.
-__device__ void box_bc(const AmrDev& a, int l, const long long g0[3], const int sz[3], double* val, double* ok,
+__device__ void box_bc(const AmrDev& a, int l, const int g0[3], const int sz[3], double* val, double* ok,
                        long long cap) {
-    const long long n3[3] = {amr_n_level(a, l, 0), amr_n_level(a, l, 1), amr_n_level(a, l, 2)};
-    const long long tot = (long long)sz[0] * sz[1] * sz[2];
-    auto outside = [&](const long long g[3], int d) {
+    const int n3[3] = {amr_n_level(a, l, 0), amr_n_level(a, l, 1), amr_n_level(a, l, 2)};
+    const int tot = sz[0] * sz[1] * sz[2];
+    auto outside = [&](const int g[3], int d) {
         return a.act[d] && (g[d] < 0 || g[d] >= n3[d]) && a.bc_kind[d][g[d] >= n3[d] ? 1 : 0] != BC_PERIODIC;
     };
     for (int d = 0; d < 3; ++d) {
         if (!a.act[d]) continue;
-        for (long long i = threadIdx.x; i < tot; i += blockDim.x) {
-            int c[3] = {(int)(i / ((long long)sz[1] * sz[2])), (int)((i / sz[2]) % sz[1]), (int)(i % sz[2])};
-            long long g[3] = {g0[0] + c[0], g0[1] + c[1], g0[2] + c[2]};
+        for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+            int c[3];
+            unflat(i, sz, c);
+            const int g[3] = {g0[0] + c[0], g0[1] + c[1], g0[2] + c[2]};
             if (!outside(g, d)) continue;
             bool later = false;
             for (int e = d + 1; e < 3; ++e) later = later || outside(g, e);
             if (later) continue;
             const int side = g[d] >= n3[d] ? 1 : 0;
-            auto local = [&](long long gg) -> long long {
-                long long cc = gg - g0[d];
+            auto local = [&](int gg) -> int {
+                const int cc = gg - g0[d];
                 if (cc < 0 || cc >= sz[d]) return -1;
                 int s3[3] = {c[0], c[1], c[2]};
-                s3[d] = (int)cc;
-                const long long p = ((long long)s3[0] * sz[1] + s3[1]) * sz[2] + s3[2];
+                s3[d] = cc;
+                const int p = (s3[0] * sz[1] + s3[1]) * sz[2] + s3[2];
                 return ok[p] != 0.0 ? p : -1;
             };
-            const long long gi = g[d] < 0 ? 0 : (g[d] > n3[d] - 1 ? n3[d] - 1 : g[d]);
+            const int gi = g[d] < 0 ? 0 : (g[d] > n3[d] - 1 ? n3[d] - 1 : g[d]);
             const int kind = a.bc_kind[d][side];
             double v[5];
             bool have = false;
@@ -84,20 +88,20 @@ __device__ void box_bc(const AmrDev& a, int l, const long long g0[3], const int 
                 for (int q = 0; q < 5; ++q) v[q] = a.bc_state[d][side][q];
                 have = true;
             } else if (kind == BC_OUTFLOW) {
-                long long p = local(gi);
+                const int p = local(gi);
                 if (p >= 0) {
                     for (int q = 0; q < 5; ++q) v[q] = val[q * cap + p];
                     have = true;
                 }
             } else if (kind == BC_REFLECT) {
-                long long p = local(side == 0 ? -1 - g[d] : 2 * n3[d] - 1 - g[d]);
+                const int p = local(side == 0 ? -1 - g[d] : 2 * n3[d] - 1 - g[d]);
                 if (p >= 0) {
                     for (int q = 0; q < 5; ++q) v[q] = val[q * cap + p];
                     v[1 + d] *= -1.0;
                     have = true;
                 }
             } else if (kind == BC_CHARACTERISTIC) {
-                long long pb = local(gi), p1 = local(gi + (side == 1 ? -1 : 1));
+                const int pb = local(gi), p1 = local(gi + (side == 1 ? -1 : 1));
                 if (pb >= 0 && p1 >= 0) {
                     double ub[5], ub1[5];
                     for (int q = 0; q < 5; ++q) {
@@ -128,27 +132,27 @@ __device__ void box_bc(const AmrDev& a, int l, const long long g0[3], const int 
 // bc_box for cells outside the domain), then one 6-point pass per active
 // axis maps coarse to fine indices.  The result (box fmin..fmax of level lf)
 // is left in scratch; returns the buffer holding it.
-__device__ double* predict_region(const AmrDev& a, const AmrWeights& W, int lf, const long long fmin[3],
-                                  const long long fmax[3], double* scr, long long cap) {
+__device__ double* predict_region(const AmrDev& a, const AmrWeights& W, int lf, const int fmin[3], const int fmax[3],
+                                  double* scr, long long cap) {
     double* A = scr;
     double* Bf = scr + 5 * cap;
     double* ok = scr + 10 * cap;
     __shared__ int any_out;
     const int lc = lf - 1;
-    long long cmin[3];
-    int sz[3];
+    int cmin[3], sz[3];
     for (int d = 0; d < 3; ++d) {
-        cmin[d] = a.act[d] ? div_e(fmin[d], 2) - 3 : 0;
-        const long long cmax = a.act[d] ? div_e(fmax[d], 2) + 3 : 0;
-        sz[d] = (int)(cmax - cmin[d] + 1);
+        cmin[d] = a.act[d] ? (fmin[d] >> 1) - 3 : 0;  // floor division by 2
+        const int cmax = a.act[d] ? (fmax[d] >> 1) + 3 : 0;
+        sz[d] = cmax - cmin[d] + 1;
     }
-    const long long n3[3] = {amr_n_level(a, lc, 0), amr_n_level(a, lc, 1), amr_n_level(a, lc, 2)};
+    const int n3[3] = {amr_n_level(a, lc, 0), amr_n_level(a, lc, 1), amr_n_level(a, lc, 2)};
     if (threadIdx.x == 0) any_out = 0;
     __syncthreads();
-    const long long tot = (long long)sz[0] * sz[1] * sz[2];
-    for (long long i = threadIdx.x; i < tot; i += blockDim.x) {
-        const int c[3] = {(int)(i / ((long long)sz[1] * sz[2])), (int)((i / sz[2]) % sz[1]), (int)(i % sz[2])};
-        long long g[3] = {cmin[0] + c[0], cmin[1] + c[1], cmin[2] + c[2]};
+    const int tot = sz[0] * sz[1] * sz[2];
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+        int c[3];
+        unflat(i, sz, c);
+        int g[3] = {cmin[0] + c[0], cmin[1] + c[1], cmin[2] + c[2]};
         bool out = false;
         for (int d = 0; d < 3; ++d) {
             if (!a.act[d]) {
@@ -168,7 +172,7 @@ __device__ double* predict_region(const AmrDev& a, const AmrWeights& W, int lf, 
             for (int q = 0; q < 5; ++q) A[q * cap + i] = 0.0;
             continue;
         }
-        long long cell;
+        int cell;
         const int id = amr_find(a, lc, g, &cell);
         if (id < 0) {
             // the hierarchy is 2:1 balanced, so the level below a leaf's
@@ -183,7 +187,7 @@ __device__ double* predict_region(const AmrDev& a, const AmrWeights& W, int lf, 
     __syncthreads();
     if (any_out) {
         box_bc(a, lc, cmin, sz, A, ok, cap);
-        for (long long i = threadIdx.x; i < tot; i += blockDim.x)
+        for (int i = threadIdx.x; i < tot; i += blockDim.x)
             if (ok[i] == 0.0) atomicOr(&a.sc->err, 4);
         __syncthreads();
     }
@@ -192,18 +196,19 @@ __device__ double* predict_region(const AmrDev& a, const AmrWeights& W, int lf, 
     for (int d = 0; d < 3; ++d) {
         if (!a.act[d]) continue;
         int nsz[3] = {sz[0], sz[1], sz[2]};
-        nsz[d] = (int)(fmax[d] - fmin[d] + 1);
-        const long long ntot = (long long)nsz[0] * nsz[1] * nsz[2];
-        for (long long i = threadIdx.x; i < ntot; i += blockDim.x) {
-            int c[3] = {(int)(i / ((long long)nsz[1] * nsz[2])), (int)((i / nsz[2]) % nsz[1]), (int)(i % nsz[2])};
-            const long long f = fmin[d] + c[d];
-            const int par = (int)rem_e(f, 2);
-            const long long base = div_e(f, 2) + (par == 1 ? -2 : -3) - cmin[d];
+        nsz[d] = fmax[d] - fmin[d] + 1;
+        const int ntot = nsz[0] * nsz[1] * nsz[2];
+        for (int i = threadIdx.x; i < ntot; i += blockDim.x) {
+            int c[3];
+            unflat(i, nsz, c);
+            const int f = fmin[d] + c[d];
+            const int par = f & 1;
+            const int base = (f >> 1) + (par == 1 ? -2 : -3) - cmin[d];
             double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            int s3[3] = {c[0], c[1], c[2]};
             for (int kk = 0; kk < 6; ++kk) {
-                int s3[3] = {c[0], c[1], c[2]};
-                s3[d] = (int)(base + kk);
-                const long long src = ((long long)s3[0] * sz[1] + s3[1]) * sz[2] + s3[2];
+                s3[d] = base + kk;
+                const int src = (s3[0] * sz[1] + s3[1]) * sz[2] + s3[2];
                 const double wgt = W.wp[par][kk];
 #pragma unroll
                 for (int q = 0; q < 5; ++q) acc[q] += wgt * in[q * cap + src];
@@ -235,21 +240,20 @@ __device__ void restrict_block(const AmrDev& a, const AmrWeights& W, int l, int 
     double* okA = scr + 10 * cap;
     double* okB = scr + 11 * cap;
     __shared__ int any_out;
-    long long f0[3];
-    int sz[3];
+    int f0[3], sz[3];
     const int lf = l + 1;
-    const long long nf[3] = {amr_n_level(a, lf, 0), amr_n_level(a, lf, 1), amr_n_level(a, lf, 2)};
+    const int nf[3] = {amr_n_level(a, lf, 0), amr_n_level(a, lf, 1), amr_n_level(a, lf, 2)};
     for (int d = 0; d < 3; ++d) {
-        const long long c0 = (long long)idx[d] * a.bd[d];
-        f0[d] = a.act[d] ? 2 * c0 - 4 : 0;
+        f0[d] = a.act[d] ? 2 * idx[d] * a.bd[d] - 4 : 0;
         sz[d] = a.act[d] ? 2 * a.bd[d] + 8 : 1;
     }
     if (threadIdx.x == 0) any_out = 0;
     __syncthreads();
-    const long long tot = (long long)sz[0] * sz[1] * sz[2];
-    for (long long i = threadIdx.x; i < tot; i += blockDim.x) {
-        const int c[3] = {(int)(i / ((long long)sz[1] * sz[2])), (int)((i / sz[2]) % sz[1]), (int)(i % sz[2])};
-        long long g[3] = {f0[0] + c[0], f0[1] + c[1], f0[2] + c[2]};
+    const int tot = sz[0] * sz[1] * sz[2];
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+        int c[3];
+        unflat(i, sz, c);
+        int g[3] = {f0[0] + c[0], f0[1] + c[1], f0[2] + c[2]};
         bool outside = false;
         for (int d = 0; d < 3; ++d)
             if (a.act[d] && (g[d] < 0 || g[d] >= nf[d])) {
@@ -263,7 +267,7 @@ __device__ void restrict_block(const AmrDev& a, const AmrWeights& W, int l, int 
             any_out = 1;
             for (int q = 0; q < 5; ++q) A[q * cap + i] = 0.0;
         } else {
-            long long cell;
+            int cell;
             const int id = amr_find(a, lf, g, &cell);
             if (id >= 0) {
                 for (int q = 0; q < 5; ++q) A[q * cap + i] = a.U[amr_u(a, q, id, cell)];
@@ -285,13 +289,14 @@ __device__ void restrict_block(const AmrDev& a, const AmrWeights& W, int l, int 
         if (!a.act[d]) continue;
         int nsz[3] = {sz[0], sz[1], sz[2]};
         nsz[d] = a.bd[d];
-        const long long ntot = (long long)nsz[0] * nsz[1] * nsz[2];
-        for (long long i = threadIdx.x; i < ntot; i += blockDim.x) {
-            int c[3] = {(int)(i / ((long long)nsz[1] * nsz[2])), (int)((i / nsz[2]) % nsz[1]), (int)(i % nsz[2])};
+        const int ntot = nsz[0] * nsz[1] * nsz[2];
+        for (int i = threadIdx.x; i < ntot; i += blockDim.x) {
+            int c[3];
+            unflat(i, nsz, c);
+            int s3[3] = {c[0], c[1], c[2]};
             auto at_src = [&](int pos) {
-                int s3[3] = {c[0], c[1], c[2]};
                 s3[d] = pos;
-                return ((long long)s3[0] * sz[1] + s3[1]) * sz[2] + s3[2];
+                return (s3[0] * sz[1] + s3[1]) * sz[2] + s3[2];
             };
             const int base = 2 * c[d] + 2;
             bool done = false;
@@ -303,7 +308,7 @@ __device__ void restrict_block(const AmrDev& a, const AmrWeights& W, int l, int 
                 for (int kk = 0; kk < 6; ++kk) all = all && iok[at_src(st + kk)] != 0.0;
                 if (all) {
                     for (int kk = 0; kk < 6; ++kk) {
-                        const long long sidx = at_src(st + kk);
+                        const int sidx = at_src(st + kk);
 #pragma unroll
                         for (int q = 0; q < 5; ++q) acc[q] += W.wts[m][kk] * in[q * cap + sidx];
                     }
@@ -312,7 +317,7 @@ __device__ void restrict_block(const AmrDev& a, const AmrWeights& W, int l, int 
             }
             double okv = done ? 1.0 : 0.0;
             if (!done) {
-                const long long pa = at_src(base + 2), pb = at_src(base + 3);
+                const int pa = at_src(base + 2), pb = at_src(base + 3);
                 if (iok[pa] != 0.0 && iok[pb] != 0.0) {
 #pragma unroll
                     for (int q = 0; q < 5; ++q) acc[q] = 0.5 * (in[q * cap + pa] + in[q * cap + pb]);
@@ -332,11 +337,11 @@ __device__ void restrict_block(const AmrDev& a, const AmrWeights& W, int l, int 
         ook = t;
         for (int e = 0; e < 3; ++e) sz[e] = nsz[e];
     }
-    for (long long i = threadIdx.x; i < a.bcells; i += blockDim.x) {
+    for (int i = threadIdx.x; i < a.bcells; i += blockDim.x) {
         bool wr = (mask >> 27) & 1;
         if (!wr) {
-            const int c[3] = {(int)(i / ((long long)a.bd[1] * a.bd[2])), (int)((i / a.bd[2]) % a.bd[1]),
-                              (int)(i % a.bd[2])};
+            int c[3];
+            unflat(i, a.bd, c);
             // offsets whose edge box holds the cell: per axis -1 (c < w),
             // 0 (always), 1 (c >= b - w)
             for (int o0 = -1; o0 <= 1 && !wr; ++o0)

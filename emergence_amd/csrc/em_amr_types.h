@@ -31,6 +31,7 @@ struct AmrDev {
     long long gs;         // variable stride of G = cap * 6 * ng * tmax
     long long fs;         // variable stride of F = cap * 6 * tmax
     int bd[3];            // block extent per axis (1 on inactive axes)
+    int lb[3];            // log2 of bd
     int act[3];
     int ndim;
     int bcells;
