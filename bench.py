@@ -99,8 +99,8 @@ def amr_bench(em, args):
     leaves1 = bat.leaf_count()
     cells = 0.5 * (leaves0 + leaves1) * c["b"] ** 3
     bat.set_profile(True)
-    bat.run_steps(4)
-    phase = [v / 4 for v in bat.profile()]
+    bat.run_steps(16)
+    phase = [v / 16 for v in bat.profile()]
     value = cells * args.amr_steps * 3 / el
     value_per = cells_per * k * 3 / el_per
     cpu = None if args.no_cpu_baseline else amr_cpu_baseline(nleaves * c["b"] ** 3)

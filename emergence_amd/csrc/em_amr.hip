@@ -19,6 +19,7 @@
 #include <cmath>
 #include <functional>
 #include <tuple>
+#include <unordered_map>
 #include <cstring>
 #include <set>
 #include <stdexcept>
@@ -245,6 +246,13 @@ __global__ __launch_bounds__(256) void k_amr_restrict(AmrDev a, AmrWeights W, co
     }
 }
 
+// Whole-block restriction of one ancestor (coarsening, amr.rs:1014-1032).
+__global__ __launch_bounds__(256) void k_amr_restrict_one(AmrDev a, AmrWeights W, int slot, int i0, int i1, int i2,
+                                                         int level, int w, double* scratch, long long cap) {
+    const int bi[3] = {i0, i1, i2};
+    restrict_block(a, W, level, slot, bi, 1 << 27, w, scratch, cap);
+}
+
 // Prediction of whole refined children from their parent level
 // (refine_block, amr.rs:989-996).
 __global__ __launch_bounds__(256) void k_amr_predict_blocks(AmrDev a, AmrWeights W, const int* __restrict__ slots,
@@ -407,7 +415,7 @@ struct Amr {
     int big_wg = 64, small_wg = 512;
     // tables
     std::vector<int> leaves, leaf_level;
-    std::vector<std::vector<int>> edge;
+    std::vector<std::vector<int>> edge, edge_ent;
     std::vector<GhostTask> ghost;
     std::vector<RefluxTask> reflux[3][2];
     int* d_leaves = nullptr;
@@ -609,13 +617,26 @@ struct Amr {
 
     template <class T>
     T* dev_vec(const std::vector<T>& v, T* old) {
-        if (old) HCHK(hipFree(old));
-        if (v.empty()) return nullptr;
+        // device copy of a table, reusing the old buffer when it is large
+        // enough (an empty table keeps the buffer; its count guards use)
+        const size_t need = sizeof(T) * v.size();
+        if (v.empty()) return old;
+        if (old && dev_caps[old] >= need) {
+            HCHK(hipMemcpyAsync(old, v.data(), need, hipMemcpyHostToDevice, stream));
+            return old;
+        }
+        if (old) {
+            HCHK(hipFree(old));
+            dev_caps.erase(old);
+        }
+        const size_t cap = need + need / 2 + 256;
         T* p;
-        HCHK(hipMalloc(&p, sizeof(T) * v.size()));
-        HCHK(hipMemcpyAsync(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, stream));
+        HCHK(hipMalloc(&p, cap));
+        dev_caps[p] = cap;
+        HCHK(hipMemcpyAsync(p, v.data(), need, hipMemcpyHostToDevice, stream));
         return p;
     }
+    std::unordered_map<const void*, size_t> dev_caps;
 
     // -------------------------------------------------------------- hierarchy
     long long slot(int l, const int64_t idx[3]) const {
@@ -791,21 +812,52 @@ struct Amr {
             for (size_t q = 0; q < kids.size(); ++q) init_data(l + 1, kids[q].data(), init, prm, data.data() + q * 5 * bcells);
             upload_blocks(slots, data.data());
         } else {
-            // predictions read level l only: run them before the children enter the index
-            ensure_tmp((long long)slots.size() * 4);
-            HCHK(hipMemcpyAsync(d_tmp, slots.data(), sizeof(int) * slots.size(), hipMemcpyHostToDevice, stream));
-            HCHK(hipMemcpyAsync(d_tmp + slots.size(), kidx.data(), sizeof(int) * kidx.size(), hipMemcpyHostToDevice,
-                                stream));
-            const unsigned g = (unsigned)std::min<size_t>(slots.size(), (size_t)big_wg);
-            hipLaunchKernelGGL(k_amr_predict_blocks, dim3(g), dim3(256), 0, stream, dev, W, d_tmp,
-                               d_tmp + slots.size(), (int)slots.size(), l + 1, big_scr, big_cap);
-            HCHK(hipGetLastError());
-            HCHK(hipStreamSynchronize(stream));
+            // Predicted from level l, which a refinement never changes and
+            // whose blocks around this one exist (balanced above): deferred
+            // to one launch per level (flush_predictions).
+            for (size_t q = 0; q < slots.size(); ++q)
+                pending.push_back({l + 1, slots[q], kidx[3 * q], kidx[3 * q + 1], kidx[3 * q + 2]});
         }
         blocks[id].leaf = false;
         for (size_t q = 0; q < kids.size(); ++q) add_block(l + 1, kids[q].data(), ed);
         flush_edits(ed);
         tables_dirty = true;
+    }
+
+    // Children awaiting prediction: {level, slot, idx[3]}.  Levels run in
+    // increasing order, so a child predicted from a block refined in the
+    // same regrid reads that block's predicted data.
+    std::vector<std::array<int, 5>> pending;
+    std::vector<std::vector<int>> pending_host;
+
+    void flush_predictions() {
+        if (pending.empty()) return;
+        std::stable_sort(pending.begin(), pending.end(),
+                         [](const std::array<int, 5>& a, const std::array<int, 5>& b) { return a[0] < b[0]; });
+        ensure_tmp((long long)pending.size() * 4);
+        size_t off = 0;
+        pending_host.clear();
+        for (size_t i = 0; i < pending.size();) {
+            size_t j = i;
+            while (j < pending.size() && pending[j][0] == pending[i][0]) ++j;
+            const int n = (int)(j - i);
+            std::vector<int> h(4 * n);
+            for (int k = 0; k < n; ++k) {
+                h[k] = pending[i + k][1];
+                for (int d = 0; d < 3; ++d) h[n + 3 * k + d] = pending[i + k][2 + d];
+            }
+            HCHK(hipMemcpyAsync(d_tmp + off, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, stream));
+            const unsigned g = (unsigned)std::min(n, big_wg);
+            hipLaunchKernelGGL(k_amr_predict_blocks, dim3(g), dim3(256), 0, stream, dev, W, d_tmp + off,
+                               d_tmp + off + n, n, pending[i][0], big_scr, big_cap);
+            HCHK(hipGetLastError());
+            pending_host.push_back(std::move(h));
+            off += 4 * n;
+            i = j;
+        }
+        HCHK(hipStreamSynchronize(stream));
+        pending.clear();
+        pending_host.clear();
     }
 
     void ensure_tmp(long long n) {
@@ -820,13 +872,10 @@ struct Amr {
     void coarsen(int l, const int64_t idx[3]) {
         auto kids = children(idx);
         const int pid = get(l, idx);
-        ensure_tmp(5);
-        const int hd[5] = {pid, (int)idx[0], (int)idx[1], (int)idx[2], 1 << 27};
-        HCHK(hipMemcpyAsync(d_tmp, hd, sizeof(hd), hipMemcpyHostToDevice, stream));
-        hipLaunchKernelGGL(k_amr_restrict, dim3(1), dim3(256), 0, stream, dev, W, d_tmp, d_tmp + 1, 1, l, box_w(),
-                           big_scr, big_cap);
+        // stream-ordered after the previous coarsening's index edits and moves
+        hipLaunchKernelGGL(k_amr_restrict_one, dim3(1), dim3(256), 0, stream, dev, W, pid, (int)idx[0], (int)idx[1],
+                           (int)idx[2], l, box_w(), big_scr, big_cap);
         HCHK(hipGetLastError());
-        HCHK(hipStreamSynchronize(stream));
         blocks[pid].leaf = true;
         std::vector<int> ids;
         for (auto& c : kids) ids.push_back(get(l + 1, c.data()));
@@ -895,6 +944,7 @@ struct Amr {
             const int id = get(k.l, k.i);
             if (id >= 0 && k.l + 1 < levels) refine_block(id, nullptr, nullptr);
         }
+        flush_predictions();
         return prot;
     }
 
@@ -1045,8 +1095,9 @@ struct Amr {
         d_ghost = dev_vec(ghost, d_ghost);
         d_edge_ids.resize(levels, nullptr);
         d_edge_idx.resize(levels, nullptr);
+        edge_ent.assign(levels, {});
         for (int l = 0; l < levels; ++l) {
-            std::vector<int> ix;
+            std::vector<int>& ix = edge_ent[l];
             for (int id : edge[l]) {
                 for (int d = 0; d < 3; ++d) ix.push_back((int)blocks[id].idx[d]);
                 ix.push_back(box_mask(id));
