@@ -294,3 +294,79 @@ class OracleAmr:
     def dx(self, level):
         return tuple((h - l) / (nb * self.b * 2 ** level) if nb > 0 else (h - l)
                      for l, h, nb in zip(self.lo, self.hi, self.base))
+
+
+def _load_adj(lib):
+    if getattr(lib, "_adj_ready", False):
+        return lib
+    P = C.c_void_p
+    D = C.POINTER(C.c_double)
+    lib.emadj_create.restype = P
+    lib.emadj_create.argtypes = [C.POINTER(C.c_int64), D, D, C.c_int64, D, C.POINTER(C.c_int32), D, C.c_int32]
+    lib.emadj_destroy.argtypes = [P]
+    lib.emadj_rhs.argtypes = [P, C.c_double, D, D]
+    lib.emadj_steps.argtypes = [P, D, C.c_int64, D]
+    lib.emadj_vjp_rhs.argtypes = [P, C.c_double, D, D, D]
+    lib.emadj_gradient.argtypes = [P, D, C.c_int64, D, D]
+    lib._adj_ready = True
+    return lib
+
+
+class OracleAdj:
+    """Weno-mode residual and SSP-RK3 step of the restated reference solver,
+    written over a scalar type: evaluated in double (bit-identical to
+    OracleSolver) or on a reverse-mode tape (reference gradients).  Arrays are
+    ghost-padded AoS float64 of shape (m0, m1, m2, 5)."""
+
+    def __init__(self, n, lo, hi, ng=6, eos=(1.4, 0.0, 0.0, 0.0), bc=None, positivity=True):
+        self.lib = _load_adj(load())
+        self.n = tuple(int(v) for v in n)
+        self.ng = int(ng)
+        self.m = tuple(v + 2 * ng if v > 1 else 1 for v in self.n)
+        kinds = (C.c_int32 * 6)()
+        states = (C.c_double * 30)()
+        bc = bc or [["periodic", "periodic"]] * 3
+        for d in range(3):
+            for s in range(2):
+                b = bc[d][s]
+                name, st = (b, None) if isinstance(b, str) else (b[0], b[1])
+                kinds[2 * d + s] = BC[name]
+                if st is not None:
+                    for q in range(5):
+                        states[(2 * d + s) * 5 + q] = st[q]
+        self.h = self.lib.emadj_create((C.c_int64 * 3)(*self.n), (C.c_double * 3)(*lo), (C.c_double * 3)(*hi), ng,
+                                       (C.c_double * 4)(*eos), kinds, states, int(positivity))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.emadj_destroy(self.h)
+            self.h = None
+
+    def _arr(self, a):
+        return np.ascontiguousarray(a, dtype=np.float64).reshape(self.m + (5,))
+
+    def rhs(self, stage_dt, u):
+        out = np.empty(self.m + (5,))
+        self.lib.emadj_rhs(self.h, stage_dt, _dp(self._arr(u)), _dp(out))
+        return out
+
+    def steps(self, u, dts):
+        u = self._arr(u).copy()
+        dts = np.ascontiguousarray(dts, dtype=np.float64)
+        self.lib.emadj_steps(self.h, _dp(u), len(dts), _dp(dts))
+        return u
+
+    def vjp_rhs(self, stage_dt, u, lam):
+        out = np.empty(self.m + (5,))
+        self.lib.emadj_vjp_rhs(self.h, stage_dt, _dp(self._arr(u)), _dp(self._arr(lam)), _dp(out))
+        return out
+
+    def gradient(self, u0, dts, dj_du_final):
+        g = self._arr(dj_du_final).copy()
+        dts = np.ascontiguousarray(dts, dtype=np.float64)
+        self.lib.emadj_gradient(self.h, _dp(self._arr(u0)), len(dts), _dp(dts), _dp(g))
+        return g
+
+    def interior(self, a):
+        g = [self.ng if v > 1 else 0 for v in self.n]
+        return a[g[0]:g[0] + self.n[0], g[1]:g[1] + self.n[1], g[2]:g[2] + self.n[2]]
