@@ -28,6 +28,9 @@ EXPORTS = (
     "em_solver_counters",
     "em_solver_time",
     "em_solver_synchronize",
+    "em_solver_adjoint_rhs",
+    "em_solver_forward",
+    "em_solver_backward",
     "em_case_init",
     "em_amr_last_error",
     "em_amr_create",
@@ -97,6 +100,9 @@ def load():
     lib.em_solver_counters.argtypes = [P, U64]
     lib.em_solver_time.argtypes = [P, D]
     lib.em_solver_synchronize.argtypes = [P]
+    lib.em_solver_adjoint_rhs.argtypes = [P, C.c_double, D, D]
+    lib.em_solver_forward.argtypes = [P, C.c_int64, C.c_int32, D, D]
+    lib.em_solver_backward.argtypes = [P, D, D]
     lib.em_case_init.argtypes = [C.c_int32, I64, D, D, C.c_int32, D, D]
     U8 = C.POINTER(C.c_uint8)
     amr_cfg = [C.POINTER(P), I64, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32, D, D, D, I32, D,

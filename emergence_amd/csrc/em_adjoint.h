@@ -1,0 +1,36 @@
+// Discrete adjoint of the Weno-mode residual (host-side declarations).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstddef>
+
+#include "em_device.h"
+
+namespace em {
+
+// One axis of MU (+)= (d R_axis / d U)^T LAM at the state U (SoA interior
+// arrays of 5 * ncell doubles).
+struct AdjArgs {
+    const double* U;
+    const double* LAM;
+    double* MU;
+    int n[3];
+    long long stride[3];
+    long long ncell;
+    Eos eos;
+    int bc_kind[2];          // boundary kinds of the pass axis (low, high)
+    double bc_state[2][5];
+    double inv_dx;
+    double lam_lim;          // limiter lambda D * dt / dx (0: limiter off)
+    int first;               // MU = contribution (else MU += contribution)
+    int P;                   // pencils per workgroup
+};
+
+int adj_pencils_per_group(int n);
+size_t adj_lds_bytes(int n, int P);
+hipError_t launch_adj_pass(int d, const AdjArgs& a, hipStream_t s);
+hipError_t launch_adj_combine(int stage, long long nv, double dt, const double* A, const double* T, double* B,
+                              double* Z, hipStream_t s);
+
+}  // namespace em

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/em_hip.h"
+#include "em_adjoint.h"
 #include "em_pass.h"
 #include "em_types.h"
 
@@ -265,6 +266,7 @@ struct Solver {
     }
 
     ~Solver() {
+        free_adjoint();
         for (auto& gx : graph)
             if (gx) (void)hipGraphExecDestroy(gx);
         for (auto* p : U)
@@ -629,6 +631,196 @@ struct Solver {
         if (out)
             for (int d = 0; d < 3; ++d) out[d] = cfg[d];
     }
+
+    // ---------------------------------------------------------- adjoint
+    // Checkpoints of the last forward run (every ck_every steps), its dt
+    // sequence, and the device buffers of the backward sweep.
+    std::vector<double*> ckpt;
+    std::vector<double> fwd_dts;
+    int ck_every = 0;
+    double* adjbuf = nullptr;  // A, B, Z, T: 4 x 5 ncell
+    double* store = nullptr;   // u_n, u1, u2 of every step of one segment
+    long long store_steps = 0;
+    double* final_state = nullptr;
+
+    void free_adjoint() {
+        for (auto* p : ckpt) (void)hipFree(p);
+        ckpt.clear();
+        if (adjbuf) (void)hipFree(adjbuf);
+        if (store) (void)hipFree(store);
+        if (final_state) (void)hipFree(final_state);
+        adjbuf = store = final_state = nullptr;
+        store_steps = 0;
+    }
+
+    void adj_check() {
+        if (mode != MODE_WENO) throw Error(EM_ERR_ARG, "the adjoint is implemented for FluxMode::Weno (mode 1)");
+        int dev = 0;
+        HIPCHK(hipGetDevice(&dev));
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, dev));
+        for (int d = 0; d < 3; ++d) {
+            if (!active(d)) continue;
+            for (int sd = 0; sd < 2; ++sd) {
+                if (bc_kind[d][sd] == BC_CHARACTERISTIC)
+                    throw Error(EM_ERR_ARG, "the adjoint does not support characteristic boundaries");
+                if (bc_kind[d][sd] == BC_REFLECT && n[d] < 5)
+                    throw Error(EM_ERR_ARG, "the adjoint needs at least 5 cells on a reflecting axis");
+            }
+            if (adj_lds_bytes((int)n[d], 1) > (size_t)prop.sharedMemPerBlock)
+                throw Error(EM_ERR_ARG, "the adjoint keeps whole pencils in LDS: axis too long");
+        }
+    }
+
+    // out = (dL/du at state)^T lam (all active axes, limiter lambda of stage_dt)
+    void adj_apply(const double* state, const double* lam, double* out, double stage_dt) {
+        bool first = true;
+        for (int d = 0; d < 3; ++d) {
+            if (!active(d)) continue;
+            AdjArgs a{};
+            a.U = state;
+            a.LAM = lam;
+            a.MU = out;
+            for (int e = 0; e < 3; ++e) {
+                a.n[e] = (int)n[e];
+                a.stride[e] = stride[e];
+            }
+            a.ncell = ncell;
+            a.eos = eos;
+            for (int sd = 0; sd < 2; ++sd) {
+                a.bc_kind[sd] = bc_kind[d][sd];
+                for (int q = 0; q < 5; ++q) a.bc_state[sd][q] = bc_state[d][sd][q];
+            }
+            a.inv_dx = 1.0 / dx[d];
+            a.lam_lim = positivity ? (double)nactive * stage_dt / dx[d] : 0.0;  // as the forward pass
+            a.first = first ? 1 : 0;
+            a.P = adj_pencils_per_group((int)n[d]);
+            HIPCHK(launch_adj_pass(d, a, stream));
+            first = false;
+        }
+    }
+
+    void host_to_soa(const double* u, double* dev) {
+        std::vector<double> h(5 * ncell);
+        const int m1 = m(1), m2 = m(2);
+        for (long long i = 0; i < n[0]; ++i)
+            for (long long j = 0; j < n[1]; ++j)
+                for (long long k = 0; k < n[2]; ++k) {
+                    const long long q = ((i + g(0)) * m1 + (j + g(1))) * m2 + (k + g(2));
+                    const long long c = (i * n[1] + j) * n[2] + k;
+                    for (int v = 0; v < 5; ++v) h[v * ncell + c] = u[q * 5 + v];
+                }
+        HIPCHK(hipMemcpyAsync(dev, h.data(), sizeof(double) * 5 * ncell, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    void soa_to_host(const double* dev, double* u) {
+        std::vector<double> h(5 * ncell);
+        HIPCHK(hipMemcpyAsync(h.data(), dev, sizeof(double) * 5 * ncell, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        std::fill(u, u + 5 * len(), 0.0);
+        const int m1 = m(1), m2 = m(2);
+        for (long long i = 0; i < n[0]; ++i)
+            for (long long j = 0; j < n[1]; ++j)
+                for (long long k = 0; k < n[2]; ++k) {
+                    const long long q = ((i + g(0)) * m1 + (j + g(1))) * m2 + (k + g(2));
+                    const long long c = (i * n[1] + j) * n[2] + k;
+                    for (int v = 0; v < 5; ++v) u[q * 5 + v] = h[v * ncell + c];
+                }
+    }
+
+    void ensure_adjbuf() {
+        if (!adjbuf) HIPCHK(hipMalloc(&adjbuf, sizeof(double) * 20 * ncell));
+    }
+
+    // (dR/du)^T lam at the current state (one residual evaluation)
+    void adjoint_rhs(double stage_dt, const double* lam, double* out) {
+        adj_check();
+        ensure_adjbuf();
+        double* A = adjbuf;
+        double* T = adjbuf + 15 * ncell;
+        host_to_soa(lam, A);
+        adj_apply(U[0], A, T, stage_dt);
+        soa_to_host(T, out);
+    }
+
+    // Forward run of nsteps SSP-RK3 steps with checkpoints every k steps;
+    // dt from dts or, if null, the CFL dt of each step (recorded).
+    void forward(int64_t nsteps, int k, const double* dts, double* dts_out) {
+        if (nsteps < 1 || k < 1) throw Error(EM_ERR_ARG, "nsteps and checkpoint_every must be >= 1");
+        adj_check();
+        free_adjoint();
+        ck_every = k;
+        fwd_dts.assign((size_t)nsteps, 0.0);
+        const int64_t nck = (nsteps + k - 1) / k;
+        for (int64_t c = 0; c < nck; ++c) {
+            double* p;
+            HIPCHK(hipMalloc(&p, sizeof(double) * 5 * ncell));
+            ckpt.push_back(p);
+        }
+        for (int64_t s = 0; s < nsteps; ++s) {
+            if (s % k == 0)
+                HIPCHK(hipMemcpyAsync(ckpt[s / k], U[0], sizeof(double) * 5 * ncell, hipMemcpyDeviceToDevice, stream));
+            const double dt = dts ? dts[s] : dt_cfl();
+            if (!(std::isfinite(dt) && dt > 0.0)) throw Error(EM_ERR_COLLAPSE, "time step collapsed");
+            fwd_dts[s] = dt;
+            step(dt);
+        }
+        HIPCHK(hipMalloc(&final_state, sizeof(double) * 5 * ncell));
+        HIPCHK(hipMemcpyAsync(final_state, U[0], sizeof(double) * 5 * ncell, hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        rate_valid = false;
+        if (dts_out) std::copy(fwd_dts.begin(), fwd_dts.end(), dts_out);
+    }
+
+    // dJ/du_0 from dJ/du_N of the last forward run: per segment, the states
+    // are recomputed from its checkpoint (u_n and both RK stage states of
+    // every step kept), then the steps are reversed.
+    void backward(const double* dj_final, double* dj_initial) {
+        if (ckpt.empty()) throw Error(EM_ERR_STATE, "backward needs a forward run with checkpoints first");
+        ensure_adjbuf();
+        const long long nv = 5 * ncell;
+        const size_t sz = sizeof(double) * nv;
+        if (store_steps < ck_every) {
+            if (store) HIPCHK(hipFree(store));
+            HIPCHK(hipMalloc(&store, sz * 3 * ck_every));
+            store_steps = ck_every;
+        }
+        std::vector<DevScalars> saved = save_scalars();
+        double* A = adjbuf;
+        double* B = adjbuf + nv;
+        double* Zb = adjbuf + 2 * nv;
+        double* T = adjbuf + 3 * nv;
+        host_to_soa(dj_final, A);
+        const int64_t nsteps = (int64_t)fwd_dts.size();
+        for (int64_t seg = (int64_t)ckpt.size() - 1; seg >= 0; --seg) {
+            const int64_t s0 = seg * ck_every, s1 = std::min<int64_t>(s0 + ck_every, nsteps);
+            HIPCHK(hipMemcpyAsync(U[0], ckpt[seg], sz, hipMemcpyDeviceToDevice, stream));
+            for (int64_t s = s0; s < s1; ++s) {
+                double* st = store + 3 * (s - s0) * nv;
+                HIPCHK(hipMemcpyAsync(st, U[0], sz, hipMemcpyDeviceToDevice, stream));
+                step(fwd_dts[s]);
+                HIPCHK(hipMemcpyAsync(st + nv, U[1], sz, hipMemcpyDeviceToDevice, stream));
+                HIPCHK(hipMemcpyAsync(st + 2 * nv, U[2], sz, hipMemcpyDeviceToDevice, stream));
+            }
+            for (int64_t s = s1 - 1; s >= s0; --s) {
+                const double dt = fwd_dts[s];
+                const double* st = store + 3 * (s - s0) * nv;
+                adj_apply(st + 2 * nv, A, T, dt);
+                HIPCHK(launch_adj_combine(3, nv, dt, A, T, B, Zb, stream));
+                adj_apply(st + nv, B, T, dt);
+                HIPCHK(launch_adj_combine(2, nv, dt, A, T, B, Zb, stream));
+                adj_apply(st, B, T, dt);
+                HIPCHK(launch_adj_combine(1, nv, dt, A, T, B, Zb, stream));
+                std::swap(A, Zb);
+            }
+        }
+        soa_to_host(A, dj_initial);
+        HIPCHK(hipMemcpyAsync(U[0], final_state, sz, hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipMemcpyAsync(sc, saved.data(), sizeof(DevScalars), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        rate_valid = false;
+    }
 };
 
 // ------------------------------------------------------------------ cases
@@ -854,6 +1046,29 @@ int em_solver_time(em_solver* h, double* t) {
 int em_solver_synchronize(em_solver* h) {
     NEED(h);
     return guarded([&] { em::hip_check(hipStreamSynchronize(h->s->stream), "hipStreamSynchronize"); });
+}
+
+int em_solver_adjoint_rhs(em_solver* h, double stage_dt, const double* lam, double* out) {
+    NEED(h);
+    if (!lam || !out) {
+        em::g_last_error = "null argument";
+        return EM_ERR_ARG;
+    }
+    return guarded([&] { h->s->adjoint_rhs(stage_dt, lam, out); });
+}
+
+int em_solver_forward(em_solver* h, int64_t nsteps, int32_t checkpoint_every, const double* dts, double* dts_out) {
+    NEED(h);
+    return guarded([&] { h->s->forward(nsteps, checkpoint_every, dts, dts_out); });
+}
+
+int em_solver_backward(em_solver* h, const double* dj_final, double* dj_initial) {
+    NEED(h);
+    if (!dj_final || !dj_initial) {
+        em::g_last_error = "null argument";
+        return EM_ERR_ARG;
+    }
+    return guarded([&] { h->s->backward(dj_final, dj_initial); });
 }
 
 int em_case_init(int32_t case_id, const int64_t n[3], const double lo[3], const double hi[3], int32_t ng,

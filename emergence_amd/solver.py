@@ -134,6 +134,29 @@ class HipSolver:
     def synchronize(self):
         _lib.check(self.lib.em_solver_synchronize(self.h))
 
+    # ---- discrete adjoint (FluxMode::Weno)
+    def adjoint_rhs(self, stage_dt, lam):
+        """(dR/du)^T lam of one residual evaluation at the current state."""
+        lam = np.ascontiguousarray(lam, dtype=np.float64).reshape(self.shape)
+        out = np.empty(self.shape, dtype=np.float64)
+        _lib.check(self.lib.em_solver_adjoint_rhs(self.h, float(stage_dt), _dp(lam), _dp(out)))
+        return out
+
+    def forward(self, nsteps, checkpoint_every, dts=None):
+        """nsteps steps with checkpoints; dts frozen (or CFL if None).
+        Returns the dt sequence used."""
+        out = np.empty(int(nsteps), dtype=np.float64)
+        d = None if dts is None else _dp(np.ascontiguousarray(dts, dtype=np.float64))
+        _lib.check(self.lib.em_solver_forward(self.h, int(nsteps), int(checkpoint_every), d, _dp(out)))
+        return out
+
+    def backward(self, dj_final):
+        """dJ/du_0 of the last forward run from dJ/du_N (CPU layout)."""
+        dj = np.ascontiguousarray(dj_final, dtype=np.float64).reshape(self.shape)
+        out = np.empty(self.shape, dtype=np.float64)
+        _lib.check(self.lib.em_solver_backward(self.h, _dp(dj), _dp(out)))
+        return out
+
     def interior(self, a):
         g = [self.ng if v > 1 else 0 for v in self.n]
         return a[g[0]:g[0] + self.n[0], g[1]:g[1] + self.n[1], g[2]:g[2] + self.n[2]]

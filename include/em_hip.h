@@ -114,6 +114,33 @@ int em_solver_time(em_solver* s, double* t);
 /* GpuSolver::synchronize (lib.rs:469-472). */
 int em_solver_synchronize(em_solver* s);
 
+/* ------------------------------------------------------------------------
+ * Discrete adjoint (no counterpart in the reference, which has none; the
+ * forward/backward pair the survey of the path asks for).  FluxMode::Weno
+ * only, periodic / outflow / reflect (n >= 5) / Dirichlet boundaries, axes up
+ * to the LDS limit (about 1,300 cells).  Conventions: branches follow the
+ * forward run, min/max differentiate the returned operand (the first on a
+ * tie), d|x|/dx = sign(x) with sign(0) = 0, the positivity limiter's blend
+ * weight and Lax-Friedrichs speed are constants, and the dt sequence is
+ * frozen (taken from the forward run). */
+
+/* (dR/du)^T lam of one residual evaluation at the current state, with the
+ * limiter's lambda from stage_dt; lam and out in the CPU layout (ghosts
+ * ignored / written as zero). */
+int em_solver_adjoint_rhs(em_solver* s, double stage_dt, const double* lam, double* out);
+
+/* Forward run of nsteps SSP-RK3 steps from the current state, keeping a
+ * checkpoint every checkpoint_every steps.  dt per step from dts (frozen
+ * sequence) or, with dts == NULL, the CFL dt of each step; dts_out (may be
+ * NULL) receives the sequence used. */
+int em_solver_forward(em_solver* s, int64_t nsteps, int32_t checkpoint_every, const double* dts, double* dts_out);
+
+/* Backward sweep of the last forward run: dJ/du_0 from dJ/du_N (CPU
+ * layout).  Each checkpoint segment is recomputed forward keeping u_n and
+ * both RK stage states of every step, then the steps are reversed.  The
+ * state and time are left as after the forward run. */
+int em_solver_backward(em_solver* s, const double* dj_final, double* dj_initial);
+
 /* Workload initialisers (host, ghost-padded AoS output of the given mesh),
  * mirroring the reference's example set-ups:
  *   case 0  2-D four-quadrant Riemann problem, params {x_split, y_split}
