@@ -62,7 +62,127 @@ def parse():
     ap.add_argument("--amr-only", action="store_true", help="only the block-AMR workload (profiling)")
     ap.add_argument("--amr-steps", type=int, default=20)
     ap.add_argument("--amr-per-block-steps", type=int, default=3)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload")
+    ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint workload (profiling)")
+    ap.add_argument("--adj-n", type=int, default=512)
+    ap.add_argument("--adj-steps", type=int, default=400)
+    ap.add_argument("--adj-checkpoint", type=int, default=20)
+    ap.add_argument("--adj-fd-cells", type=int, default=16, help="central-difference checks (seeded cells)")
     return ap.parse_args()
+
+
+def _splitmix64(state):
+    state = (state + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    z = state
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return state, z ^ (z >> 31)
+
+
+def adjoint_case(n, seed=0x5EED + 1):
+    """2-D four-quadrant Riemann problem (split (0.8, 0.8), outflow) on [0,1]^2
+    with rho scaled by 1 + 0.01 * sum of 16 seeded Fourier modes (|k| <= 4,
+    SplitMix64); CPU layout with 6 ghost layers."""
+    s = seed
+    modes = []
+    while len(modes) < 16:
+        s, a = _splitmix64(s)
+        s, b = _splitmix64(s)
+        k = (int(a % 9) - 4, int(b % 9) - 4)
+        if 0 < k[0] ** 2 + k[1] ** 2 <= 16:
+            s, c = _splitmix64(s)
+            modes.append((k, 2 * np.pi * (c >> 11) * 2.0 ** -53))
+    x = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    pert = sum(np.cos(2 * np.pi * (k[0] * X + k[1] * Y) + ph) for k, ph in modes)
+    e, nn = X > 0.8, Y > 0.8
+    quad = [(1.5, 0.0, 0.0, 1.5), (0.5323, 1.206, 0.0, 0.3), (0.138, 1.206, 1.206, 0.029), (0.5323, 0.0, 1.206, 0.3)]
+    sel = np.where(e & nn, 0, np.where(nn, 1, np.where(~e, 2, 3)))
+    w = np.array(quad)[sel]
+    rho = w[..., 0] * (1.0 + 0.01 * pert)
+    u = np.zeros((n + 12, n + 12, 1, 5))
+    ui = u[6:6 + n, 6:6 + n, 0]
+    ui[..., 0] = rho
+    ui[..., 1] = rho * w[..., 1]
+    ui[..., 2] = rho * w[..., 2]
+    ui[..., 4] = w[..., 3] / 0.4 + 0.5 * rho * (w[..., 1] ** 2 + w[..., 2] ** 2)
+    return u
+
+
+def pressure_window(u, n, lo=0.6, hi=0.8):
+    """J = sum of p dx dy over cells with centre in [lo, hi]^2, and dJ/du."""
+    x = (np.arange(n) + 0.5) / n
+    m = (x >= lo) & (x <= hi)
+    mask = m[:, None] & m[None, :]
+    q = u[6:6 + n, 6:6 + n, 0]
+    vel = q[..., 1:4] / q[..., :1]
+    ke = 0.5 * np.sum(vel * vel, axis=-1)
+    p = 0.4 * (q[..., 4] - q[..., 0] * ke)
+    dxdy = 1.0 / (n * n)
+    J = float(np.sum(p[mask]) * dxdy)
+    g = np.zeros_like(u)
+    gi = g[6:6 + n, 6:6 + n, 0]
+    gi[..., 0] = 0.4 * ke
+    gi[..., 1:4] = -0.4 * vel
+    gi[..., 4] = 0.4
+    gi *= (mask * dxdy)[..., None]
+    return J, g
+
+
+def adjoint_bench(em, args):
+    """Adjoint/forward cost of dJ/drho_0 (J = integral of p over [0.6,0.8]^2 at
+    the last step) through args.adj_steps SSP-RK3 steps with checkpoints, and
+    central finite differences of the forward run at seeded cells."""
+    n = args.adj_n
+    u0 = adjoint_case(n)
+    bc = [["outflow", "outflow"], ["outflow", "outflow"], ["periodic", "periodic"]]
+    mk = lambda u: em.HipSolver((n, n, 1), (0, 0, 0), (1, 1, 1), u, bc=bc, mode="weno", cfl=0.5)
+    g = mk(u0)
+    g.forward(2, 2)  # warm-up (kernels, buffers)
+    g.upload(u0)
+    g.synchronize()
+    t0 = time.perf_counter()
+    dts = g.forward(args.adj_steps, args.adj_checkpoint)
+    g.synchronize()
+    t_fwd = time.perf_counter() - t0
+    J, dj = pressure_window(g.download(), n)
+    t0 = time.perf_counter()
+    du0 = g.backward(dj)
+    t_bwd = time.perf_counter() - t0
+    q0 = u0[6:6 + n, 6:6 + n, 0]
+    vel = q0[..., 1:4] / q0[..., :1]
+    dirn = np.concatenate([np.ones_like(q0[..., :1]), vel, 0.5 * np.sum(vel * vel, axis=-1, keepdims=True)], -1)
+    grad_rho = np.sum(du0[6:6 + n, 6:6 + n, 0] * dirn, axis=-1)
+    rng = np.random.default_rng(0x5EED)
+    scale = float(np.abs(grad_rho).max())
+    errs = []
+    f = mk(u0)
+    for _ in range(args.adj_fd_cells):
+        i, j = int(rng.integers(0, n)), int(rng.integers(0, n))
+        h = 1e-7 * q0[i, j, 0]
+        Js = []
+        for sg in (1.0, -1.0):
+            up = u0.copy()
+            up[6 + i, 6 + j, 0] += sg * h * dirn[i, j]
+            f.upload(up)
+            f.forward(args.adj_steps, args.adj_steps, dts)
+            Js.append(pressure_window(f.download(), n)[0])
+        fd = (Js[0] - Js[1]) / (2 * h)
+        ad = float(grad_rho[i, j])
+        errs.append(abs(fd - ad) / max(abs(ad), 1e-2 * scale))
+    return {
+        "workload": f"2-D {n}^2 four-quadrant Riemann + 0.01 x 16 seeded Fourier modes on rho, Weno, CFL 0.5, "
+                    f"{args.adj_steps} steps, J = integral of p over [0.6,0.8]^2, dJ/drho_0",
+        "forward_s": t_fwd,
+        "backward_s": t_bwd,
+        "cost_ratio": t_bwd / t_fwd,
+        "checkpoint_every": args.adj_checkpoint,
+        "J": J,
+        "grad_max_abs": scale,
+        "fd": {"cells": args.adj_fd_cells, "h": "1e-7 rho", "max_rel_err": max(errs) if errs else None,
+               "median_rel_err": float(np.median(errs)) if errs else None,
+               "rel_err_floor": "1e-2 x max |dJ/drho_0|"},
+    }
 
 
 AMR_BC = [["outflow", "outflow"], ["reflect", "reflect"], ["reflect", "reflect"]]
@@ -190,6 +310,9 @@ def main():
     if args.amr_only:
         print(json.dumps({"amr": amr_bench(em, args)}), flush=True)
         return
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": adjoint_bench(em, args)}), flush=True)
+        return
     n = args.n
     lo, hi = (0.0, 0.0, 0.0), (2 * np.pi,) * 3
     u = em.case_init(1, (n, n, n), lo, hi, 6, [0.3, float(0x5EED)])
@@ -244,9 +367,12 @@ def main():
     }
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline(n, u, args.cpu_steps)
+    del g
     if not args.no_amr:
-        del g
         rec["amr"] = amr_bench(em, args)
+    if not args.no_adjoint:
+        rec["adjoint"] = adjoint_bench(em, args)
+        rec["adjoint_forward_cost_ratio"] = rec["adjoint"]["cost_ratio"]
     print(json.dumps(rec), flush=True)
 
 

@@ -8,7 +8,8 @@
 //     trigger), min/max differentiate the operand they return (the first on
 //     a tie), d|x|/dx = sign(x) with sign(0) = 0;
 //   * the limiter's blend weight and Lax-Friedrichs speed are constants;
-//   * dt is given (frozen), so the CFL reduction is not differentiated.
+//   * dt is given (frozen), so the CFL reduction is not differentiated;
+//   * a WENO-Z stencil flat to rounding takes the linear weights' derivative.
 // Forward values are recomputed with the forward pass's own device
 // functions and operation order (no FMA contraction), so every branch
 // decision matches the forward run bit for bit.
@@ -98,7 +99,11 @@ __device__ __forceinline__ void to_prim_vjp(const Eos& e, const double w[5], con
     ub[0] = rhob;
 }
 
-// out = weno5z_interp(v0..v4)
+// out = weno5z_interp(v0..v4).  On a stencil that is flat to rounding (all
+// smoothness indicators below 1e-26 max v^2) the Z-weights are not
+// differentiable: tau / beta is a ratio of rounding errors.  There the
+// derivative is taken with the linear weights (the smooth limit of WENO-Z),
+// the convention the tape oracle follows.
 __device__ __forceinline__ void weno_vjp(const double v[5], double ob, double vb[5]) {
     const double q0 = (3.0 * v[0] - 10.0 * v[1] + 15.0 * v[2]) * 0.125;
     const double q1 = (-v[1] + 6.0 * v[2] + 3.0 * v[3]) * 0.125;
@@ -110,6 +115,17 @@ __device__ __forceinline__ void weno_vjp(const double v[5], double ob, double vb
     const double b0 = c13 * (t0 * t0) + 0.25 * (s0 * s0);
     const double b1 = c13 * (t1 * t1) + 0.25 * (s1 * s1);
     const double b2 = c13 * (t2 * t2) + 0.25 * (s2 * s2);
+    const double mv = fmax(fmax(fmax(fabs(v[0]), fabs(v[1])), fmax(fabs(v[2]), fabs(v[3]))), fabs(v[4]));
+    const double flat = 1e-26 * mv * mv;
+    if (b0 <= flat && b1 <= flat && b2 <= flat) {
+        const double q0b = 0.0625 * ob, q1b = 0.625 * ob, q2b = 0.3125 * ob;
+        vb[0] += 0.375 * q0b;
+        vb[1] += -1.25 * q0b - 0.125 * q1b;
+        vb[2] += 1.875 * q0b + 0.75 * q1b + 0.375 * q2b;
+        vb[3] += 0.375 * q1b + 0.75 * q2b;
+        vb[4] += -0.125 * q2b;
+        return;
+    }
     const double tau = fabs(b0 - b2);
     const double e0 = b0 + 1e-40, e1 = b1 + 1e-40, e2 = b2 + 1e-40;
     const double r0 = tau / e0, r1 = tau / e1, r2 = tau / e2;

@@ -111,3 +111,27 @@ def test_tape_gradient_through_steps_matches_central_differences():
         ad = grad[g + i, g + j, 0, q]
         # relative to the entry, or to 1% of the largest entry where FD rounding dominates
         assert abs(fd - ad) <= 1e-6 * max(abs(ad), 1e-2 * np.abs(grad).max()), (i, j, q, fd, ad)
+
+
+def test_tape_vjp_on_flat_stencils_takes_the_linear_weights():
+    # A uniform state at rest makes every WENO-Z stencil flat to rounding.
+    # With the linear-weight convention the adjoint residual inherits the
+    # scheme's mirror symmetry: the response to a unit energy cotangent at one
+    # cell is the same on both sides along each axis, with the normal
+    # momentum negated.  Z-weight derivatives of rounding noise (the
+    # unregularised tape) break this by O(1).
+    n = 12
+    o = OracleAdj((n, n, 1), (0, 0, 0), (1, 1, 1))
+    u = _fill(o, lambda i, j, k: [1.0, 0.0, 0.0, 0.0, 1.0])
+    g, c = o.ng, 5
+    lam = np.zeros_like(u)
+    lam[g + c, g + c, 0, 4] = 1.0
+    v = o.interior(o.vjp_rhs(1e-3, u, lam))[:, :, 0]
+    scale = np.abs(v).max()
+    assert scale > 1.0
+    for d in range(2):
+        for off in (1, 2, 3):
+            a = v[c + off, c] if d == 0 else v[c, c + off]
+            b = (v[c - off, c] if d == 0 else v[c, c - off]).copy()
+            b[1 + d] *= -1.0
+            np.testing.assert_allclose(a, b, rtol=0, atol=1e-12 * scale)
