@@ -296,7 +296,9 @@ __device__ void hllc_vjp(const Eos& e, const double wl[5], const double wr[5], c
     sound_speed_vjp(e, wr[0], wr[4], crb, wrb[0], wrb[4]);
 }
 
-// fbar of the WCNS midpoint flux of the six states s[0..5] -> sb (added)
+// fbar of the WCNS midpoint flux of the six states s[0..5] -> sb (added).
+// The characteristic values are recomputed per field instead of being kept
+// (they are cheap and registers are not).
 template <int D>
 __device__ void midpoint_vjp(const Eos& e, const double (&s)[6][5], const double fbar[5], double (&sb)[6][5]) {
     constexpr int un = 1 + D;
@@ -310,20 +312,21 @@ __device__ void midpoint_vjp(const Eos& e, const double (&s)[6][5], const double
     const double l14 = -1.0 / (c * c);
     const double l4un = rho / (2.0 * c);
     const double l44 = 1.0 / (2.0 * c * c);
-    double ch[6][5];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        ch[k][0] = l0un * s[k][un] + l04 * s[k][4];
-        ch[k][1] = s[k][0] + l14 * s[k][4];
-        ch[k][2] = s[k][t1];
-        ch[k][3] = s[k][t2];
-        ch[k][4] = l4un * s[k][un] + l44 * s[k][4];
-    }
+    // characteristic field a of state k (rows of L, sparse)
+    auto chv = [&](int a, int k) -> double {
+        switch (a) {
+            case 0: return l0un * s[k][un] + l04 * s[k][4];
+            case 1: return s[k][0] + l14 * s[k][4];
+            case 2: return s[k][t1];
+            case 3: return s[k][t2];
+            default: return l4un * s[k][un] + l44 * s[k][4];
+        }
+    };
     double cl[5], cr[5];
-#pragma unroll
+#pragma unroll 1
     for (int a = 0; a < 5; ++a) {
-        cl[a] = weno5z_interp(ch[0][a], ch[1][a], ch[2][a], ch[3][a], ch[4][a]);
-        cr[a] = weno5z_interp(ch[5][a], ch[4][a], ch[3][a], ch[2][a], ch[1][a]);
+        cl[a] = weno5z_interp(chv(a, 0), chv(a, 1), chv(a, 2), chv(a, 3), chv(a, 4));
+        cr[a] = weno5z_interp(chv(a, 5), chv(a, 4), chv(a, 3), chv(a, 2), chv(a, 1));
     }
     const double run0 = -c / rho, run4 = c / rho, r40 = c * c;
     double wl[5], wr[5];
@@ -370,39 +373,42 @@ __device__ void midpoint_vjp(const Eos& e, const double (&s)[6][5], const double
     const double run0b = wlb[un] * cl[0] + wrb[un] * cr[0];
     const double run4b = wlb[un] * cl[4] + wrb[un] * cr[4];
     const double r40b = wlb[4] * (cl[0] + cl[4]) + wrb[4] * (cr[0] + cr[4]);
-    // WENO5-Z of each characteristic field (right side mirrored)
-    double chb[6][5];
-#pragma unroll
-    for (int k = 0; k < 6; ++k)
-#pragma unroll
-        for (int a = 0; a < 5; ++a) chb[k][a] = 0.0;
-#pragma unroll
+    // per characteristic field: WENO5-Z (right side mirrored), then the
+    // projection ch = L s
+    double l0unb = 0.0, l04b = 0.0, l14b = 0.0, l4unb = 0.0, l44b = 0.0;
+#pragma unroll 1
     for (int a = 0; a < 5; ++a) {
-        double v[5] = {ch[0][a], ch[1][a], ch[2][a], ch[3][a], ch[4][a]};
-        double vb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        weno_vjp(v, clb[a], vb);
-#pragma unroll
-        for (int k = 0; k < 5; ++k) chb[k][a] += vb[k];
-        double vr[5] = {ch[5][a], ch[4][a], ch[3][a], ch[2][a], ch[1][a]};
+        double v[5] = {chv(a, 0), chv(a, 1), chv(a, 2), chv(a, 3), chv(a, 4)};
+        double chb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        weno_vjp(v, clb[a], chb);
+        double vr[5] = {chv(a, 5), v[4], v[3], v[2], v[1]};
         double vrb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         weno_vjp(vr, crb[a], vrb);
 #pragma unroll
-        for (int k = 0; k < 5; ++k) chb[5 - k][a] += vrb[k];
-    }
-    // projection ch = L s
-    double l0unb = 0.0, l04b = 0.0, l14b = 0.0, l4unb = 0.0, l44b = 0.0;
+        for (int k = 0; k < 5; ++k) chb[5 - k] += vrb[k];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        sb[k][un] += l0un * chb[k][0] + l4un * chb[k][4];
-        sb[k][4] += l04 * chb[k][0] + l14 * chb[k][1] + l44 * chb[k][4];
-        sb[k][0] += chb[k][1];
-        sb[k][t1] += chb[k][2];
-        sb[k][t2] += chb[k][3];
-        l0unb += chb[k][0] * s[k][un];
-        l04b += chb[k][0] * s[k][4];
-        l14b += chb[k][1] * s[k][4];
-        l4unb += chb[k][4] * s[k][un];
-        l44b += chb[k][4] * s[k][4];
+        for (int k = 0; k < 6; ++k) {
+            const double g = chb[k];
+            if (a == 0) {
+                sb[k][un] += l0un * g;
+                sb[k][4] += l04 * g;
+                l0unb += g * s[k][un];
+                l04b += g * s[k][4];
+            } else if (a == 1) {
+                sb[k][0] += g;
+                sb[k][4] += l14 * g;
+                l14b += g * s[k][4];
+            } else if (a == 2) {
+                sb[k][t1] += g;
+            } else if (a == 3) {
+                sb[k][t2] += g;
+            } else {
+                sb[k][un] += l4un * g;
+                sb[k][4] += l44 * g;
+                l4unb += g * s[k][un];
+                l44b += g * s[k][4];
+            }
+        }
     }
     // eigenvector entries as functions of the average density and c
     double rhob = -l0unb / (2.0 * c) + l4unb / (2.0 * c) + run0b * c / (rho * rho) - run4b * c / (rho * rho);
