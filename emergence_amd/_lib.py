@@ -32,6 +32,16 @@ EXPORTS = (
     "em_solver_forward",
     "em_solver_backward",
     "em_case_init",
+    "em_ensemble_create",
+    "em_ensemble_destroy",
+    "em_ensemble_set_batched",
+    "em_ensemble_set_tiles",
+    "em_ensemble_run_steps",
+    "em_ensemble_download",
+    "em_ensemble_time",
+    "em_ensemble_forward",
+    "em_ensemble_dts",
+    "em_ensemble_backward",
     "em_amr_last_error",
     "em_amr_create",
     "em_amr_create_case",
@@ -105,6 +115,18 @@ def load():
     lib.em_solver_backward.argtypes = [P, D, D]
     lib.em_case_init.argtypes = [C.c_int32, I64, D, D, C.c_int32, D, D]
     U8 = C.POINTER(C.c_uint8)
+    DP = C.POINTER(D)
+    lib.em_ensemble_create.argtypes = [C.POINTER(P), C.c_int32, I64, D, D, C.c_int32, D, I32, D, C.c_int32,
+                                       C.c_int32, C.c_double, C.c_double, DP]
+    lib.em_ensemble_destroy.argtypes = [P]
+    lib.em_ensemble_set_batched.argtypes = [P, C.c_int32]
+    lib.em_ensemble_set_tiles.argtypes = [P, C.c_int32]
+    lib.em_ensemble_run_steps.argtypes = [P, C.c_int64]
+    lib.em_ensemble_download.argtypes = [P, C.c_int32, D]
+    lib.em_ensemble_time.argtypes = [P, C.c_int32, D]
+    lib.em_ensemble_forward.argtypes = [P, C.c_int64, C.c_int32]
+    lib.em_ensemble_dts.argtypes = [P, C.c_int32, D]
+    lib.em_ensemble_backward.argtypes = [P, DP, DP]
     amr_cfg = [C.POINTER(P), I64, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_int32, D, D, D, I32, D,
                C.c_int32, C.c_int32, C.c_double, C.c_double]
     lib.em_amr_last_error.argtypes = []

@@ -27,7 +27,30 @@ struct AdjArgs {
     int P;                   // pencils per workgroup
 };
 
+// Ensemble VJP: per-instance arguments and prefix sums of their pencil
+// groups (ninst + 1 entries; total = prefix[ninst]).
+struct EnsAdjArgs {
+    const AdjArgs* args;
+    const long long* prefix;
+    int ninst;
+    long long total;
+};
+
+// One instance of the batched RK adjoint combination (dts[dt_index] is the
+// instance's dt of the step being reversed).
+struct AdjCombineTask {
+    double* A;
+    const double* T;
+    double* B;
+    double* Z;
+    long long nv;
+    long long dt_index;
+};
+
 int adj_pencils_per_group(int n);
+hipError_t launch_ens_adj_pass(int d, const EnsAdjArgs& e, size_t lds, unsigned grid, hipStream_t s);
+hipError_t launch_ens_adj_combine(int stage, const AdjCombineTask* tasks, int ninst, const double* dts,
+                                  long long maxnv, hipStream_t s);
 size_t adj_lds_bytes(int n, int P);
 hipError_t launch_adj_pass(int d, const AdjArgs& a, hipStream_t s);
 hipError_t launch_adj_combine(int stage, long long nv, double dt, const double* A, const double* T, double* B,

@@ -425,9 +425,10 @@ __device__ void midpoint_vjp(const Eos& e, const double (&s)[6][5], const double
 
 // ------------------------------------------------------------------ kernel
 
+// Pencil group `grp` (P pencils) of one axis VJP described by `a`; every
+// thread of the workgroup calls it and it returns uniformly.
 template <int D>
-__global__ __launch_bounds__(256) void adj_pass(AdjArgs a) {
-    extern __shared__ double lds[];
+__device__ __forceinline__ void adj_group(const AdjArgs& a, long long grp, double* lds) {
     constexpr int un = 1 + D;
     constexpr int E0 = D == 0 ? 1 : 0;
     constexpr int E1 = D == 2 ? 1 : 2;
@@ -441,7 +442,7 @@ __global__ __launch_bounds__(256) void adj_pass(AdjArgs a) {
     double* smf = sacc + 5 * C * P;   // [5][M][P] midpoint fluxes, then face cotangents
     auto W = [&](double* arr, int q, int c, int p, int ext) -> double& { return arr[(q * ext + c) * P + p]; };
     const long long npen = a.ncell / n;
-    const long long pen0 = (long long)blockIdx.x * P;
+    const long long pen0 = grp * P;
     const int Pv = (int)min((long long)P, npen - pen0);
     if (Pv <= 0) return;
     const Eos e = a.eos;
@@ -706,7 +707,42 @@ __global__ __launch_bounds__(256) void adj_pass(AdjArgs a) {
     }
 }
 
+template <int D>
+__global__ __launch_bounds__(256) void adj_pass(AdjArgs a) {
+    extern __shared__ double lds[];
+    adj_group<D>(a, blockIdx.x, lds);
+}
+
+// Ensemble VJP: the pencil groups of every instance in one persistent
+// launch (instance i owns work items prefix[i] .. prefix[i+1]-1).
+template <int D>
+__global__ __launch_bounds__(256) void ens_adj_pass(EnsAdjArgs e) {
+    extern __shared__ double lds[];
+    for (long long w = blockIdx.x; w < e.total; w += gridDim.x) {
+        int lo = 0, hi = e.ninst;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (e.prefix[mid] <= w)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        adj_group<D>(e.args[lo], w - e.prefix[lo], lds);
+        __syncthreads();
+    }
+}
+
 }  // namespace adj
+
+hipError_t launch_ens_adj_pass(int d, const EnsAdjArgs& e, size_t lds, unsigned grid, hipStream_t s) {
+    if (d == 0)
+        hipLaunchKernelGGL(adj::ens_adj_pass<0>, dim3(grid), dim3(256), lds, s, e);
+    else if (d == 1)
+        hipLaunchKernelGGL(adj::ens_adj_pass<1>, dim3(grid), dim3(256), lds, s, e);
+    else
+        hipLaunchKernelGGL(adj::ens_adj_pass<2>, dim3(grid), dim3(256), lds, s, e);
+    return hipGetLastError();
+}
 
 int adj_pencils_per_group(int n) {
     const long long per = 8ll * (15ll * (n + 10));  // bytes of LDS per pencil (upper bound)
@@ -750,6 +786,33 @@ __global__ void k_adj_combine(int stage, long long nv, double dt, const double* 
             Z[i] = Z[i] + B[i] + dt * T[i];
         }
     }
+}
+
+// Ensemble form: instance blockIdx.y with its own arrays and the dt of the
+// step from the device dt history; stage 1 leaves the step's input
+// cotangent in A (A is free once stage 3 has read it).
+__global__ void k_ens_adj_combine(int stage, const AdjCombineTask* __restrict__ tasks, const double* __restrict__ dts) {
+    const AdjCombineTask t = tasks[blockIdx.y];
+    const double dt = dts[t.dt_index];
+    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < t.nv; i += (long long)gridDim.x * 256ll) {
+        if (stage == 3) {
+            t.B[i] = (2.0 / 3.0) * t.A[i] + (2.0 / 3.0) * dt * t.T[i];
+            t.Z[i] = t.A[i] / 3.0;
+        } else if (stage == 2) {
+            const double b = t.B[i];
+            t.Z[i] = t.Z[i] + 0.75 * b;
+            t.B[i] = 0.25 * b + 0.25 * dt * t.T[i];
+        } else {
+            t.A[i] = t.Z[i] + t.B[i] + dt * t.T[i];  // the cotangent of the step's input
+        }
+    }
+}
+
+hipError_t launch_ens_adj_combine(int stage, const AdjCombineTask* tasks, int ninst, const double* dts,
+                                  long long maxnv, hipStream_t s) {
+    const unsigned gx = (unsigned)std::min<long long>((maxnv + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_ens_adj_combine, dim3(gx, (unsigned)ninst), dim3(256), 0, s, stage, tasks, dts);
+    return hipGetLastError();
 }
 
 hipError_t launch_adj_combine(int stage, long long nv, double dt, const double* A, const double* T, double* B,

@@ -29,6 +29,18 @@ constexpr int kNumTileCfgs = 9;
             default: return hipErrorInvalidValue;                                          \
         }                                                                                  \
     }                                                                                      \
+    hipError_t launch_ens_hybrid(int d, int cfg, const EnsPassArgs& e, unsigned grid, hipStream_t s); \
+    hipError_t launch_ens_weno(int d, int cfg, const EnsPassArgs& e, unsigned grid, hipStream_t s);   \
+    hipError_t launch_ens_central(int d, int cfg, const EnsPassArgs& e, unsigned grid, hipStream_t s); \
+    inline hipError_t launch_ens_pass(int d, int mode, int cfg, const EnsPassArgs& e, unsigned grid,  \
+                                      hipStream_t s) {                                               \
+        switch (mode) {                                                                    \
+            case MODE_HYBRID: return launch_ens_hybrid(d, cfg, e, grid, s);                \
+            case MODE_WENO: return launch_ens_weno(d, cfg, e, grid, s);                    \
+            case MODE_CENTRAL: return launch_ens_central(d, cfg, e, grid, s);              \
+            default: return hipErrorInvalidValue;                                          \
+        }                                                                                  \
+    }                                                                                      \
     inline hipError_t launch_pass(int d, int mode, int cfg, const PassArgs& a, hipStream_t s) { \
         switch (mode) {                                                                    \
             case MODE_HYBRID: return launch_pass_hybrid(d, cfg, a, s);                     \

@@ -37,12 +37,15 @@
 #if EM_MODE == 0
 #define EM_LAUNCHER launch_pass_hybrid
 #define EM_AMR_LAUNCHER launch_amr_hybrid
+#define EM_ENS_LAUNCHER launch_ens_hybrid
 #elif EM_MODE == 1
 #define EM_LAUNCHER launch_pass_weno
 #define EM_AMR_LAUNCHER launch_amr_weno
+#define EM_ENS_LAUNCHER launch_ens_weno
 #else
 #define EM_LAUNCHER launch_pass_central
 #define EM_AMR_LAUNCHER launch_amr_central
+#define EM_ENS_LAUNCHER launch_ens_central
 #endif
 #ifndef EM_WAVES_PER_EU
 #define EM_WAVES_PER_EU 4
@@ -66,23 +69,33 @@ __device__ __forceinline__ void ghost_state(const PassArgs& a, long long pen, in
     bc_ghost<D>(a.eos, a.n[D], g, a.bc_kind, a.bc_state, ld, u);
 }
 
+// LDS of one tile (declared by the kernels, passed to axis_tile)
+template <int MODE, int T, int P, int NT>
+struct TileLds {
+    static constexpr bool HYB = MODE == MODE_HYBRID;
+    double sw[5][T + 10][P];
+    double sm[5][T + 5][P];
+    double ss[2][HYB ? T + 10 : 1][P];
+    double sth[HYB ? T + 1 : 1][P];
+    double sred[NT / 64 + 1];
+    unsigned int scnt[3];
+};
+
+// Tile `bid` of one axis pass described by `a` (every thread of the
+// workgroup calls it; it returns uniformly).
 template <int D, int MODE, int T, int P, int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_WAVES_PER_EU))) void axis_pass(PassArgs a) {
+__device__ __forceinline__ void axis_tile(const PassArgs& a, int bid, TileLds<MODE, T, P, NT>& L) {
     constexpr int C = T + 10;  // cells held in LDS
-    constexpr int M = T + 5;   // midpoints
     constexpr bool HYB = MODE == MODE_HYBRID;
-    constexpr int CS = HYB ? C : 1;
-    constexpr int FS = HYB ? T + 1 : 1;
-    __shared__ double sw[5][C][P];
-    __shared__ double sm[5][M][P];
-    __shared__ double ss[2][CS][P];
-    __shared__ double sth[FS][P];
-    __shared__ double sred[NT / 64 + 1];
-    __shared__ unsigned int scnt[3];
+    auto& sw = L.sw;
+    auto& sm = L.sm;
+    auto& ss = L.ss;
+    auto& sth = L.sth;
+    auto& sred = L.sred;
+    auto& scnt = L.scnt;
 
     if (a.sc->active == 0) return;
     const int tid = threadIdx.x;
-    const int bid = blockIdx.x;
     const int tile = bid % a.ntile;
     const int rest = bid / a.ntile;
     const int gF = rest % a.ngroup;
@@ -223,9 +236,43 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_WAVES_PER
 }
 
 template <int D, int MODE, int T, int P, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_WAVES_PER_EU))) void axis_pass(PassArgs a) {
+    __shared__ TileLds<MODE, T, P, NT> L;
+    axis_tile<D, MODE, T, P, NT>(a, blockIdx.x, L);
+}
+
+// Ensemble pass: the tiles of every instance in one persistent launch.  Work
+// item w belongs to the instance i with prefix[i] <= w < prefix[i+1] and is
+// its tile w - prefix[i]; each instance brings its own arrays, extents,
+// boundary conditions and device scalars (dt, CFL rate, counters).
+template <int D, int MODE, int T, int P, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_WAVES_PER_EU))) void ens_pass(EnsPassArgs e) {
+    __shared__ TileLds<MODE, T, P, NT> L;
+    for (long long w = blockIdx.x; w < e.total; w += gridDim.x) {
+        int lo = 0, hi = e.ninst;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (e.prefix[mid] <= w)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const PassArgs& a = e.args[lo];  // read in place (global memory, cached)
+        axis_tile<D, MODE, T, P, NT>(a, (int)(w - e.prefix[lo]), L);
+        __syncthreads();
+    }
+}
+
+template <int D, int MODE, int T, int P, int NT>
 static hipError_t launch_one(const PassArgs& a, hipStream_t s) {
     const long long nblk = (long long)a.ntile * a.ngroup * a.n[a.slow_axis];
     hipLaunchKernelGGL((axis_pass<D, MODE, T, P, NT>), dim3((unsigned)nblk), dim3(NT), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int D, int MODE, int T, int P, int NT>
+static hipError_t launch_ens_one(const EnsPassArgs& e, unsigned grid, hipStream_t s) {
+    hipLaunchKernelGGL((ens_pass<D, MODE, T, P, NT>), dim3(grid), dim3(NT), 0, s, e);
     return hipGetLastError();
 }
 
@@ -246,12 +293,37 @@ static hipError_t launch_cfg(int cfg, const PassArgs& a, hipStream_t s) {
     }
 }
 
+template <int D, int MODE>
+static hipError_t launch_ens_cfg(int cfg, const EnsPassArgs& e, unsigned grid, hipStream_t s) {
+    switch (cfg) {
+        case 0: return launch_ens_one<D, MODE, 32, 16, 640>(e, grid, s);
+        case 1: return launch_ens_one<D, MODE, 64, 8, 576>(e, grid, s);
+        case 2: return launch_ens_one<D, MODE, 128, 4, 576>(e, grid, s);
+        case 3: return launch_ens_one<D, MODE, 256, 2, 576>(e, grid, s);
+        case 4: return launch_ens_one<D, MODE, 27, 8, 256>(e, grid, s);
+        case 5: return launch_ens_one<D, MODE, 59, 4, 256>(e, grid, s);
+        case 6: return launch_ens_one<D, MODE, 59, 8, 512>(e, grid, s);
+        case 7: return launch_ens_one<D, MODE, 27, 16, 512>(e, grid, s);
+        case 8: return launch_ens_one<D, MODE, 123, 4, 512>(e, grid, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 // This translation unit instantiates one flux mode (EM_MODE).
 hipError_t EM_LAUNCHER(int d, int cfg, const PassArgs& a, hipStream_t s) {
     switch (d) {
         case 0: return launch_cfg<0, EM_MODE>(cfg, a, s);
         case 1: return launch_cfg<1, EM_MODE>(cfg, a, s);
         case 2: return launch_cfg<2, EM_MODE>(cfg, a, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t EM_ENS_LAUNCHER(int d, int cfg, const EnsPassArgs& e, unsigned grid, hipStream_t s) {
+    switch (d) {
+        case 0: return launch_ens_cfg<0, EM_MODE>(cfg, e, grid, s);
+        case 1: return launch_ens_cfg<1, EM_MODE>(cfg, e, grid, s);
+        case 2: return launch_ens_cfg<2, EM_MODE>(cfg, e, grid, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -823,6 +823,461 @@ struct Solver {
     }
 };
 
+// ------------------------------------------------------------------ ensemble
+
+// Small batched kernels over the instances of an ensemble (thread i or
+// workgroup i serves instance i).
+__global__ void k_ens_set_run(DevScalars* const* scs, int n) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= n) return;
+    scs[i]->active = 1;
+    scs[i]->err = 0;
+    scs[i]->steps = 0;
+}
+__global__ void k_ens_zero_rate(DevScalars* const* scs) {
+    if (threadIdx.x < kSlots) scs[blockIdx.x]->rate[threadIdx.x][0] = 0ull;
+}
+// dt = cfl / rate per instance, recorded in hist[step * n + i]; the step
+// index lives on the device so a captured step can be replayed.
+__global__ void k_ens_dt(DevScalars* const* scs, int n, double cfl, double* hist, const long long* step) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= n) return;
+    DevScalars* sc = scs[i];
+    if (sc->active == 0) return;
+    const double dt = cfl / __longlong_as_double((long long)rate_bits(sc));
+    if (!(isfinite(dt) && dt > 0.0)) {
+        sc->err |= 1;
+        sc->active = 0;
+        return;
+    }
+    sc->dt = dt;
+    if (hist) hist[*step * n + i] = dt;
+}
+__global__ void k_ens_set_dt(DevScalars* const* scs, int n, const double* hist, long long step) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i < n) scs[i]->dt = hist[step * n + i];
+}
+__global__ void k_ens_advance(DevScalars* const* scs, int n, long long* step) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i == 0 && step) *step += 1;
+    if (i >= n) return;
+    DevScalars* sc = scs[i];
+    if (sc->active == 0) return;
+    sc->t = sc->t + sc->dt;
+    sc->steps += 1;
+}
+struct CopyTask {
+    const double* src;
+    double* dst;
+    long long nv;
+};
+__global__ void k_ens_copy(const CopyTask* __restrict__ t) {
+    const CopyTask c = t[blockIdx.y];
+    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < c.nv; i += (long long)gridDim.x * 256ll) c.dst[i] = c.src[i];
+}
+
+// An ensemble of independent uniform-grid solvers (own mesh, domain, state,
+// dt and CFL rate each) advanced together: every axis pass of every
+// instance runs in one persistent launch (ens_pass), a step is replayed as
+// one HIP graph; the adjoint runs its VJPs the same way (ens_adj_pass).
+struct Ensemble {
+    std::vector<Solver*> in;
+    int n = 0;
+    int mode, cfg = 6, precise = 1;
+    bool batched = true;
+    double cfl;
+    hipStream_t stream = nullptr;
+    DevScalars** d_scs = nullptr;
+    PassArgs* d_args[3][3] = {};
+    long long* d_prefix[3][3] = {};
+    long long total[3][3] = {};
+    long long* d_step = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int graph_key = -1;
+    std::vector<long long> off;  // instance offsets in concatenated 5 * ncell layouts
+    long long tot_nv = 0, max_nv = 0;
+    // adjoint
+    int ck_every = 0;
+    long long fwd_steps = 0;
+    std::vector<double*> ckpt;
+    double* dt_hist = nullptr;
+    std::vector<double> h_dts;
+    double* work = nullptr;   // A, B, Z, T
+    double* store = nullptr;  // 3 states per step of a segment
+    double* fin = nullptr;    // final states
+
+    Ensemble(int ninst, const int64_t* nn, const double* lo, const double* hi, int ng, const double e[4],
+             const int32_t bk[6], const double bs[30], int mode_, int pos, double cfl_, double tol,
+             const double* const* u) {
+        if (ninst < 1) throw Error(EM_ERR_ARG, "an ensemble needs at least one instance");
+        n = ninst;
+        mode = mode_;
+        cfl = cfl_;
+        HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        try {
+            for (int i = 0; i < n; ++i) {
+                in.push_back(new Solver(nn + 3 * i, lo + 3 * i, hi + 3 * i, ng, e, bk, bs, mode_, pos, cfl_, tol, 0.0));
+                in.back()->upload(u[i]);
+            }
+        } catch (...) {
+            release();
+            throw;
+        }
+        off.resize(n + 1);
+        off[0] = 0;
+        for (int i = 0; i < n; ++i) {
+            const long long nv = 5 * in[i]->ncell;
+            off[i + 1] = off[i] + nv;
+            max_nv = std::max(max_nv, nv);
+        }
+        tot_nv = off[n];
+        std::vector<DevScalars*> h(n);
+        for (int i = 0; i < n; ++i) h[i] = in[i]->sc;
+        HIPCHK(hipMalloc(&d_scs, sizeof(DevScalars*) * n));
+        HIPCHK(hipMemcpyAsync(d_scs, h.data(), sizeof(DevScalars*) * n, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMalloc(&d_step, sizeof(long long)));
+        build_args();
+    }
+
+    void release() {
+        for (auto* p : in) delete p;
+        in.clear();
+        auto f = [](void* p) {
+            if (p) (void)hipFree(p);
+        };
+        for (int sidx = 0; sidx < 3; ++sidx)
+            for (int d = 0; d < 3; ++d) {
+                f(d_args[sidx][d]);
+                f(d_prefix[sidx][d]);
+                d_args[sidx][d] = nullptr;
+                d_prefix[sidx][d] = nullptr;
+            }
+        f(d_scs);
+        f(d_step);
+        f(d_copy);
+        d_copy = nullptr;
+        free_adjoint();
+        d_scs = nullptr;
+        d_step = nullptr;
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+    ~Ensemble() { release(); }
+
+    void free_adjoint() {
+        for (auto* p : ckpt) (void)hipFree(p);
+        ckpt.clear();
+        for (double** p : {&dt_hist, &work, &store, &fin})
+            if (*p) {
+                (void)hipFree(*p);
+                *p = nullptr;
+            }
+    }
+
+    // pass arguments of every instance for stage s (1..3) and axis d, as
+    // Solver::stage builds them
+    void build_args() {
+        for (int i = 0; i < n; ++i)
+            for (int d = 0; d < 3; ++d) in[i]->cfg[d] = cfg;
+        for (int st = 1; st <= 3; ++st)
+            for (int d = 0; d < 3; ++d) {
+                std::vector<PassArgs> a;
+                std::vector<long long> pre(1, 0);
+                for (int i = 0; i < n; ++i) {
+                    Solver& S = *in[i];
+                    if (!S.active(d)) {
+                        pre.push_back(pre.back());
+                        a.push_back(PassArgs{});
+                        continue;
+                    }
+                    int first_ax = -1, last = -1;
+                    for (int e = 0; e < 3; ++e)
+                        if (S.active(e)) {
+                            if (first_ax < 0) first_ax = e;
+                            last = e;
+                        }
+                    PassArgs p = S.pass_args(d);
+                    const double* Uin = st == 1 ? S.U[0] : st == 2 ? S.U[1] : S.U[2];
+                    double* Uout = st == 1 ? S.U[1] : st == 2 ? S.U[2] : S.U[0];
+                    p.U = Uin;
+                    p.R = S.R;
+                    p.U0 = S.U[0];
+                    p.Uout = Uout;
+                    p.first = d == first_ax ? 1 : 0;
+                    p.rk_stage = d == last ? st : 0;
+                    p.want_rate = (d == last && st == 3) ? 1 : 0;
+                    p.stage_dt_host = 0;
+                    p.stage_dt = 0.0;
+                    a.push_back(p);
+                    pre.push_back(pre.back() + (long long)p.ntile * p.ngroup * p.n[p.slow_axis]);
+                }
+                total[st - 1][d] = pre.back();
+                if (d_args[st - 1][d]) HIPCHK(hipFree(d_args[st - 1][d]));
+                if (d_prefix[st - 1][d]) HIPCHK(hipFree(d_prefix[st - 1][d]));
+                HIPCHK(hipMalloc(&d_args[st - 1][d], sizeof(PassArgs) * n));
+                HIPCHK(hipMalloc(&d_prefix[st - 1][d], sizeof(long long) * (n + 1)));
+                HIPCHK(hipMemcpyAsync(d_args[st - 1][d], a.data(), sizeof(PassArgs) * n, hipMemcpyHostToDevice, stream));
+                HIPCHK(hipMemcpyAsync(d_prefix[st - 1][d], pre.data(), sizeof(long long) * (n + 1),
+                                      hipMemcpyHostToDevice, stream));
+                HIPCHK(hipStreamSynchronize(stream));
+            }
+        graph_key = -1;
+    }
+
+    unsigned small_grid() const { return (unsigned)((n + 63) / 64); }
+
+    void stage_launches(int st) {
+        for (int d = 0; d < 3; ++d) {
+            if (total[st - 1][d] == 0) continue;
+            if (batched) {
+                EnsPassArgs e{d_args[st - 1][d], d_prefix[st - 1][d], n, total[st - 1][d]};
+                const unsigned grid = (unsigned)std::min<long long>(total[st - 1][d], 4096);
+                HIPCHK(precise ? precise::launch_ens_pass(d, mode, cfg, e, grid, stream)
+                               : fast::launch_ens_pass(d, mode, cfg, e, grid, stream));
+            } else {
+                for (int i = 0; i < n; ++i) {
+                    Solver& S = *in[i];
+                    if (!S.active(d)) continue;
+                    PassArgs p = S.pass_args(d);
+                    int first_ax = -1, last = -1;
+                    for (int e2 = 0; e2 < 3; ++e2)
+                        if (S.active(e2)) {
+                            if (first_ax < 0) first_ax = e2;
+                            last = e2;
+                        }
+                    p.U = st == 1 ? S.U[0] : st == 2 ? S.U[1] : S.U[2];
+                    p.R = S.R;
+                    p.U0 = S.U[0];
+                    p.Uout = st == 1 ? S.U[1] : st == 2 ? S.U[2] : S.U[0];
+                    p.first = d == first_ax ? 1 : 0;
+                    p.rk_stage = d == last ? st : 0;
+                    p.want_rate = (d == last && st == 3) ? 1 : 0;
+                    HIPCHK(precise ? precise::launch_pass(d, mode, cfg, p, stream)
+                                   : fast::launch_pass(d, mode, cfg, p, stream));
+                }
+            }
+        }
+    }
+
+    // one step: dt (CFL, recorded) or given dt from the history, 3 stages
+    void step_kernels(bool cfl_dt, long long given_step) {
+        if (cfl_dt)
+            hipLaunchKernelGGL(k_ens_dt, dim3(small_grid()), dim3(64), 0, stream, d_scs, n, cfl, dt_hist, d_step);
+        else
+            hipLaunchKernelGGL(k_ens_set_dt, dim3(small_grid()), dim3(64), 0, stream, d_scs, n, dt_hist, given_step);
+        stage_launches(1);
+        stage_launches(2);
+        hipLaunchKernelGGL(k_ens_zero_rate, dim3(n), dim3(kSlots), 0, stream, d_scs);
+        stage_launches(3);
+        hipLaunchKernelGGL(k_ens_advance, dim3(small_grid()), dim3(64), 0, stream, d_scs, n, d_step);
+        HIPCHK(hipGetLastError());
+    }
+
+    void start_run() {
+        hipLaunchKernelGGL(k_ens_set_run, dim3(small_grid()), dim3(64), 0, stream, d_scs, n);
+        const long long zero = 0;
+        HIPCHK(hipMemcpyAsync(d_step, &zero, sizeof(long long), hipMemcpyHostToDevice, stream));
+        // CFL rate of the initial states (each solver on its own stream)
+        HIPCHK(hipStreamSynchronize(stream));
+        for (auto* S : in) S->compute_rate();
+        for (auto* S : in) HIPCHK(hipStreamSynchronize(S->stream));
+    }
+
+    void check_err() {
+        HIPCHK(hipStreamSynchronize(stream));
+        for (int i = 0; i < n; ++i) {
+            Solver::Scalars h = in[i]->read_scalars();
+            if (h.err & 1) throw Error(EM_ERR_COLLAPSE, "time step collapsed in instance " + std::to_string(i));
+        }
+    }
+
+    void run_steps(int64_t nsteps) {
+        start_run();
+        const int key = (batched ? 1 : 0) + 2 * precise + 4 * cfg;
+        if (!graph || graph_key != key) {
+            if (graph) HIPCHK(hipGraphExecDestroy(graph));
+            graph = nullptr;
+            hipGraph_t g;
+            HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            step_kernels(true, 0);
+            HIPCHK(hipStreamEndCapture(stream, &g));
+            const hipError_t er = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIPCHK(er);
+            graph_key = key;
+        }
+        for (int64_t q = 0; q < nsteps; ++q) HIPCHK(hipGraphLaunch(graph, stream));
+        check_err();
+    }
+
+    CopyTask* d_copy = nullptr;
+    void copy_all(const std::vector<CopyTask>& tasks) {
+        if (tasks.empty()) return;
+        if (!d_copy) HIPCHK(hipMalloc(&d_copy, sizeof(CopyTask) * (n + 1)));
+        HIPCHK(hipMemcpyAsync(d_copy, tasks.data(), sizeof(CopyTask) * tasks.size(), hipMemcpyHostToDevice, stream));
+        const unsigned gx = (unsigned)std::min<long long>((max_nv + 255) / 256, 1024);
+        hipLaunchKernelGGL(k_ens_copy, dim3(gx, (unsigned)tasks.size()), dim3(256), 0, stream, d_copy);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream));  // the task table is rewritten next call
+    }
+    // state buffer b (0: U^n, 1, 2: stage states) of every instance <-> a
+    // concatenated buffer
+    std::vector<CopyTask> gather_tasks(int b, double* dst) {
+        std::vector<CopyTask> t;
+        for (int i = 0; i < n; ++i) t.push_back(CopyTask{in[i]->U[b], dst + off[i], off[i + 1] - off[i]});
+        return t;
+    }
+    std::vector<CopyTask> scatter_tasks(const double* src, int b) {
+        std::vector<CopyTask> t;
+        for (int i = 0; i < n; ++i) t.push_back(CopyTask{src + off[i], in[i]->U[b], off[i + 1] - off[i]});
+        return t;
+    }
+
+    // forward with checkpoints every k steps; CFL dt per instance recorded
+    void forward(int64_t nsteps, int k) {
+        if (nsteps < 1 || k < 1) throw Error(EM_ERR_ARG, "nsteps and checkpoint_every must be >= 1");
+        for (auto* S : in) S->adj_check();
+        free_adjoint();
+        ck_every = k;
+        fwd_steps = nsteps;
+        HIPCHK(hipMalloc(&dt_hist, sizeof(double) * nsteps * n));
+        const int64_t nck = (nsteps + k - 1) / k;
+        for (int64_t c = 0; c < nck; ++c) {
+            double* p;
+            HIPCHK(hipMalloc(&p, sizeof(double) * tot_nv));
+            ckpt.push_back(p);
+        }
+        start_run();
+        for (int64_t s0 = 0; s0 < nsteps; s0 += k) {
+            copy_all(gather_tasks(0, ckpt[s0 / k]));
+            const int64_t s1 = std::min<int64_t>(s0 + k, nsteps);
+            for (int64_t q = s0; q < s1; ++q) step_kernels(true, 0);
+        }
+        check_err();
+        HIPCHK(hipMalloc(&fin, sizeof(double) * tot_nv));
+        copy_all(gather_tasks(0, fin));
+        h_dts.resize((size_t)nsteps * n);
+        HIPCHK(hipMemcpyAsync(h_dts.data(), dt_hist, sizeof(double) * nsteps * n, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    // dJ/du_0 of every instance from dJ/du_N (host arrays in each
+    // instance's CPU layout)
+    void backward(const double* const* dj_final, double* const* dj_initial) {
+        if (ckpt.empty()) throw Error(EM_ERR_STATE, "backward needs a forward run first");
+        if (!work) HIPCHK(hipMalloc(&work, sizeof(double) * 4 * tot_nv));
+        if (!store) HIPCHK(hipMalloc(&store, sizeof(double) * 3 * tot_nv * ck_every));
+        std::vector<std::vector<DevScalars>> saved;
+        for (auto* S : in) saved.push_back(S->save_scalars());
+        double* A = work;
+        double* B = work + tot_nv;
+        double* Zb = work + 2 * tot_nv;
+        double* T = work + 3 * tot_nv;
+        for (int i = 0; i < n; ++i) in[i]->host_to_soa(dj_final[i], A + off[i]);
+        // per-instance pencils per group and LDS: the largest axis decides
+        int maxn = 1;
+        for (auto* S : in)
+            for (int d = 0; d < 3; ++d)
+                if (S->active(d)) maxn = std::max(maxn, (int)S->n[d]);
+        const int P = adj_pencils_per_group(maxn);
+        const size_t lds = adj_lds_bytes(maxn, P);
+        AdjArgs* d_aa = nullptr;
+        long long* d_pre = nullptr;
+        AdjCombineTask* d_ct = nullptr;
+        const int64_t nsteps = fwd_steps;
+        // argument tables of one segment: [step][stage 0..2][axis]
+        const size_t per_step = (size_t)9 * n;
+        HIPCHK(hipMalloc(&d_aa, sizeof(AdjArgs) * per_step * ck_every));
+        HIPCHK(hipMalloc(&d_pre, sizeof(long long) * 3 * (n + 1)));
+        HIPCHK(hipMalloc(&d_ct, sizeof(AdjCombineTask) * 3 * n));
+        std::vector<long long> pre(3 * (n + 1), 0);
+        for (int d = 0; d < 3; ++d)
+            for (int i = 0; i < n; ++i) {
+                const Solver& S = *in[i];
+                const long long groups = S.active(d) ? (S.ncell / S.n[d] + P - 1) / P : 0;
+                pre[d * (n + 1) + i + 1] = pre[d * (n + 1) + i] + groups;
+            }
+        HIPCHK(hipMemcpyAsync(d_pre, pre.data(), sizeof(long long) * pre.size(), hipMemcpyHostToDevice, stream));
+        for (int64_t seg = (int64_t)ckpt.size() - 1; seg >= 0; --seg) {
+            const int64_t s0 = seg * ck_every, s1 = std::min<int64_t>(s0 + ck_every, nsteps);
+            copy_all(scatter_tasks(ckpt[seg], 0));
+            for (int64_t q = s0; q < s1; ++q) {
+                double* st = store + 3 * (q - s0) * tot_nv;
+                copy_all(gather_tasks(0, st));
+                step_kernels(false, q);
+                copy_all(gather_tasks(1, st + tot_nv));
+                copy_all(gather_tasks(2, st + 2 * tot_nv));
+            }
+            // VJP arguments of the segment
+            std::vector<AdjArgs> aa(per_step * (s1 - s0));
+            for (int64_t q = s0; q < s1; ++q)
+                for (int stg = 0; stg < 3; ++stg)
+                    for (int d = 0; d < 3; ++d)
+                        for (int i = 0; i < n; ++i) {
+                            const Solver& S = *in[i];
+                            AdjArgs& a = aa[(size_t)(q - s0) * per_step + (size_t)(stg * 3 + d) * n + i];
+                            a = AdjArgs{};
+                            if (!S.active(d)) continue;
+                            const double dt = h_dts[(size_t)q * n + i];
+                            double* st = store + 3 * (q - s0) * tot_nv;
+                            // stage 3 linearises at u2, stage 2 at u1, stage 1 at u^n
+                            a.U = st + (2 - stg) * tot_nv + off[i];
+                            a.LAM = (stg == 0 ? A : B) + off[i];
+                            a.MU = T + off[i];
+                            for (int e = 0; e < 3; ++e) {
+                                a.n[e] = (int)S.n[e];
+                                a.stride[e] = S.stride[e];
+                            }
+                            a.ncell = S.ncell;
+                            a.eos = S.eos;
+                            for (int sd = 0; sd < 2; ++sd) {
+                                a.bc_kind[sd] = S.bc_kind[d][sd];
+                                for (int v = 0; v < 5; ++v) a.bc_state[sd][v] = S.bc_state[d][sd][v];
+                            }
+                            a.inv_dx = 1.0 / S.dx[d];
+                            a.lam_lim = S.positivity ? (double)S.nactive * dt / S.dx[d] : 0.0;
+                            int first_ax = -1;
+                            for (int e = 0; e < 3; ++e)
+                                if (S.active(e) && first_ax < 0) first_ax = e;
+                            a.first = d == first_ax ? 1 : 0;
+                            a.P = P;
+                        }
+            HIPCHK(hipMemcpyAsync(d_aa, aa.data(), sizeof(AdjArgs) * aa.size(), hipMemcpyHostToDevice, stream));
+            for (int64_t q = s1 - 1; q >= s0; --q) {
+                std::vector<AdjCombineTask> ct(3 * n);
+                for (int stg = 0; stg < 3; ++stg)
+                    for (int i = 0; i < n; ++i)
+                        ct[stg * n + i] = AdjCombineTask{A + off[i], T + off[i], B + off[i], Zb + off[i],
+                                                         off[i + 1] - off[i], q * n + i};
+                HIPCHK(hipMemcpyAsync(d_ct, ct.data(), sizeof(AdjCombineTask) * ct.size(), hipMemcpyHostToDevice,
+                                      stream));
+                for (int stg = 0; stg < 3; ++stg) {
+                    for (int d = 0; d < 3; ++d) {
+                        const long long tot = pre[d * (n + 1) + n];
+                        if (tot == 0) continue;
+                        EnsAdjArgs e{d_aa + (size_t)(q - s0) * per_step + (size_t)(stg * 3 + d) * n,
+                                     d_pre + d * (n + 1), n, tot};
+                        HIPCHK(launch_ens_adj_pass(d, e, lds, (unsigned)std::min<long long>(tot, 2048), stream));
+                    }
+                    HIPCHK(launch_ens_adj_combine(3 - stg, d_ct + stg * n, n, dt_hist, max_nv, stream));
+                }
+                HIPCHK(hipStreamSynchronize(stream));  // ct is rewritten next step
+            }
+        }
+        for (int i = 0; i < n; ++i) in[i]->soa_to_host(A + off[i], dj_initial[i]);
+        copy_all(scatter_tasks(fin, 0));
+        for (int i = 0; i < n; ++i) {
+            HIPCHK(hipMemcpyAsync(in[i]->sc, saved[i].data(), sizeof(DevScalars), hipMemcpyHostToDevice, stream));
+        }
+        HIPCHK(hipStreamSynchronize(stream));
+        (void)hipFree(d_aa);
+        (void)hipFree(d_pre);
+        (void)hipFree(d_ct);
+    }
+};
+
 // ------------------------------------------------------------------ cases
 
 static uint64_t splitmix64(uint64_t& s) {  // Steele, Lea & Flood 2014
@@ -917,6 +1372,10 @@ static void case_init(int id, const int64_t n[3], const double lo[3], const doub
 
 struct em_solver {
     em::Solver* s;
+};
+
+struct em_ensemble {
+    em::Ensemble* e;
 };
 
 template <class F>
@@ -1069,6 +1528,101 @@ int em_solver_backward(em_solver* h, const double* dj_final, double* dj_initial)
         return EM_ERR_ARG;
     }
     return guarded([&] { h->s->backward(dj_final, dj_initial); });
+}
+
+// ---------------------------------------------------------------- ensemble
+
+int em_ensemble_create(em_ensemble** out, int32_t ninst, const int64_t* n, const double* lo, const double* hi,
+                       int32_t ng, const double eos[4], const int32_t bc_kind[6], const double bc_state[30],
+                       int32_t mode, int32_t positivity, double cfl, double tol, const double* const* u) {
+    if (!out || !n || !lo || !hi || !eos || !bc_kind || !u) {
+        em::g_last_error = "null argument";
+        return EM_ERR_ARG;
+    }
+    return guarded([&] {
+        auto* e = new em::Ensemble(ninst, n, lo, hi, ng, eos, bc_kind, bc_state, mode, positivity, cfl, tol, u);
+        *out = new em_ensemble{e};
+    });
+}
+
+int em_ensemble_destroy(em_ensemble* h) {
+    if (!h) return EM_OK;
+    delete h->e;
+    delete h;
+    return EM_OK;
+}
+
+#define ENEED(h)                                                             \
+    if (!(h) || !(h)->e) {                                                   \
+        em::g_last_error = "null ensemble handle";                           \
+        return EM_ERR_ARG;                                                   \
+    }
+
+int em_ensemble_set_batched(em_ensemble* h, int32_t batched) {
+    ENEED(h);
+    h->e->batched = batched != 0;
+    return EM_OK;
+}
+
+int em_ensemble_set_tiles(em_ensemble* h, int32_t cfg) {
+    ENEED(h);
+    if (cfg < 0 || cfg >= em::kNumTileCfgs) {
+        em::g_last_error = "tile configuration out of range";
+        return EM_ERR_ARG;
+    }
+    return guarded([&] {
+        h->e->cfg = cfg;
+        h->e->build_args();
+    });
+}
+
+int em_ensemble_run_steps(em_ensemble* h, int64_t nsteps) {
+    ENEED(h);
+    return guarded([&] { h->e->run_steps(nsteps); });
+}
+
+int em_ensemble_download(em_ensemble* h, int32_t i, double* u) {
+    ENEED(h);
+    if (i < 0 || i >= h->e->n || !u) {
+        em::g_last_error = "instance index out of range or null buffer";
+        return EM_ERR_ARG;
+    }
+    return guarded([&] { h->e->in[i]->download(u); });
+}
+
+int em_ensemble_time(em_ensemble* h, int32_t i, double* t) {
+    ENEED(h);
+    if (i < 0 || i >= h->e->n || !t) {
+        em::g_last_error = "instance index out of range or null pointer";
+        return EM_ERR_ARG;
+    }
+    return guarded([&] { *t = h->e->in[i]->read_scalars().t; });
+}
+
+int em_ensemble_forward(em_ensemble* h, int64_t nsteps, int32_t checkpoint_every) {
+    ENEED(h);
+    return guarded([&] { h->e->forward(nsteps, checkpoint_every); });
+}
+
+int em_ensemble_dts(em_ensemble* h, int32_t i, double* dts) {
+    ENEED(h);
+    if (i < 0 || i >= h->e->n || !dts) {
+        em::g_last_error = "instance index out of range or null buffer";
+        return EM_ERR_ARG;
+    }
+    return guarded([&] {
+        if (h->e->h_dts.empty()) throw em::Error(EM_ERR_STATE, "no forward run");
+        for (int64_t q = 0; q < h->e->fwd_steps; ++q) dts[q] = h->e->h_dts[(size_t)q * h->e->n + i];
+    });
+}
+
+int em_ensemble_backward(em_ensemble* h, const double* const* dj_final, double* const* dj_initial) {
+    ENEED(h);
+    if (!dj_final || !dj_initial) {
+        em::g_last_error = "null argument";
+        return EM_ERR_ARG;
+    }
+    return guarded([&] { h->e->backward(dj_final, dj_initial); });
 }
 
 int em_case_init(int32_t case_id, const int64_t n[3], const double lo[3], const double hi[3], int32_t ng,

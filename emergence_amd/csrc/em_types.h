@@ -57,6 +57,15 @@ struct PassArgs {
     double stage_dt;       // used when stage_dt_host != 0
 };
 
+// Ensemble pass: per-instance pass arguments and the prefix sums of the
+// instances' tile counts (ninst + 1 entries; total = prefix[ninst]).
+struct EnsPassArgs {
+    const PassArgs* args;
+    const long long* prefix;
+    int ninst;
+    long long total;
+};
+
 struct TileCfg {
     int T, P, NT;
 };

@@ -141,6 +141,42 @@ int em_solver_forward(em_solver* s, int64_t nsteps, int32_t checkpoint_every, co
  * state and time are left as after the forward run. */
 int em_solver_backward(em_solver* s, const double* dj_final, double* dj_initial);
 
+/* ------------------------------------------------------------------------
+ * Ensemble: independent uniform-grid solvers (own mesh extents, domain,
+ * state, dt and CFL rate each; shared EOS, boundary kinds and options)
+ * advanced together.  Every axis pass of every instance runs in one
+ * persistent launch and a step is replayed as one HIP graph; the adjoint
+ * batches its VJPs the same way.  No counterpart in the reference (its
+ * GpuSolver holds one mesh).  Host arrays are per instance, in that
+ * instance's CPU layout; u / dj arrays are arrays of per-instance pointers. */
+typedef struct em_ensemble em_ensemble;
+
+int em_ensemble_create(em_ensemble** out, int32_t ninst, const int64_t* n /* [ninst][3] */,
+                       const double* lo /* [ninst][3] */, const double* hi /* [ninst][3] */, int32_t ng,
+                       const double eos[4], const int32_t bc_kind[6], const double bc_state[30], int32_t mode,
+                       int32_t positivity, double cfl, double tol, const double* const* u);
+int em_ensemble_destroy(em_ensemble* h);
+
+/* 1 (default): one persistent launch per pass over all instances; 0: one
+ * launch per instance (same results). */
+int em_ensemble_set_batched(em_ensemble* h, int32_t batched);
+
+/* Tile shape of every pass (index into the library's tile table). */
+int em_ensemble_set_tiles(em_ensemble* h, int32_t cfg);
+
+/* nsteps CFL steps of every instance (each its own dt). */
+int em_ensemble_run_steps(em_ensemble* h, int64_t nsteps);
+
+int em_ensemble_download(em_ensemble* h, int32_t i, double* u);
+int em_ensemble_time(em_ensemble* h, int32_t i, double* t);
+
+/* Forward/backward pair of the ensemble (FluxMode::Weno, conventions as
+ * em_solver_forward/backward): CFL dt per instance recorded and frozen,
+ * checkpoints every checkpoint_every steps. */
+int em_ensemble_forward(em_ensemble* h, int64_t nsteps, int32_t checkpoint_every);
+int em_ensemble_dts(em_ensemble* h, int32_t i, double* dts);
+int em_ensemble_backward(em_ensemble* h, const double* const* dj_final, double* const* dj_initial);
+
 /* Workload initialisers (host, ghost-padded AoS output of the given mesh),
  * mirroring the reference's example set-ups:
  *   case 0  2-D four-quadrant Riemann problem, params {x_split, y_split}
