@@ -68,7 +68,99 @@ def parse():
     ap.add_argument("--adj-steps", type=int, default=400)
     ap.add_argument("--adj-checkpoint", type=int, default=20)
     ap.add_argument("--adj-fd-cells", type=int, default=16, help="central-difference checks (seeded cells)")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble workload")
+    ap.add_argument("--ensemble-only", action="store_true", help="only the ensemble workload (profiling)")
+    ap.add_argument("--ens-n", type=int, default=256, help="instances")
+    ap.add_argument("--ens-steps", type=int, default=300)
+    ap.add_argument("--ens-checkpoint", type=int, default=20)
+    ap.add_argument("--ens-fd", type=int, default=2, help="instances checked by central differences in Mach")
     return ap.parse_args()
+
+
+def ensemble_bench(em, args):
+    """256 2-D shock-bubble instances (seeded Mach U[1.1,2], radius U[0.1,0.3],
+    density ratio U[0.1,0.5], side uniform in {96, 128, ..., 384}) advanced
+    together; forward with checkpoints and the adjoint of the bubble-region
+    mean pressure with respect to Mach."""
+    rng = np.random.default_rng(0x5EED + 2)
+    bc = [["outflow", "outflow"], ["outflow", "outflow"], ["periodic", "periodic"]]
+    inst = []
+    for _ in range(args.ens_n):
+        n = int(rng.choice(np.arange(96, 385, 32)))
+        mach, r, dens = rng.uniform(1.1, 2.0), rng.uniform(0.1, 0.3), rng.uniform(0.1, 0.5)
+        u, du = em.shock_bubble_2d(n, mach, r, dens)
+        inst.append((n, mach, r, dens, u, du))
+    cells = sum(n * n for n, *_ in inst)
+    sizes = [(n, n, 1) for n, *_ in inst]
+    states = [x[4] for x in inst]
+
+    def bubble(u, n, r):
+        x = (np.arange(n) + 0.5) / n
+        X, Y = np.meshgrid(x, x, indexing="ij")
+        m = (X - 0.5) ** 2 + (Y - 0.5) ** 2 < r * r
+        q = u[6:6 + n, 6:6 + n, 0]
+        vel = q[..., 1:4] / q[..., :1]
+        ke = 0.5 * np.sum(vel * vel, -1)
+        p = 0.4 * (q[..., 4] - q[..., 0] * ke)
+        g = np.zeros_like(u)
+        gi = g[6:6 + n, 6:6 + n, 0]
+        gi[..., 0] = 0.4 * ke
+        gi[..., 1:4] = -0.4 * vel
+        gi[..., 4] = 0.4
+        gi *= (m / m.sum())[..., None]
+        return float(p[m].mean()), g
+
+    # plain batched stepping vs one launch per instance (same kernels)
+    timing = {}
+    for batched in (True, False):
+        ens = em.HipEnsemble(sizes, states, bc=bc, batched=batched)
+        ens.run_steps(2)
+        k = 20 if batched else 5
+        t0 = time.perf_counter()
+        ens.run_steps(k)
+        timing[batched] = (time.perf_counter() - t0) / k
+        del ens
+    ens = em.HipEnsemble(sizes, states, bc=bc)
+    t0 = time.perf_counter()
+    ens.forward(args.ens_steps, args.ens_checkpoint)
+    t_fwd = time.perf_counter() - t0
+    djs, js = [], []
+    for i, (n, _, r, *_) in enumerate(inst):
+        j, g = bubble(ens.download(i), n, r)
+        js.append(j)
+        djs.append(g)
+    t0 = time.perf_counter()
+    du0 = ens.backward(djs)
+    t_bwd = time.perf_counter() - t0
+    grads = [float(np.sum(du0[i] * x[5])) for i, x in enumerate(inst)]
+    errs = []
+    for i in range(min(args.ens_fd, len(inst))):
+        n, mach, r, dens, _, _ = inst[i]
+        dts = ens.dts(i)
+        h = 1e-6
+        jj = []
+        for sg in (1.0, -1.0):
+            up, _ = em.shock_bubble_2d(n, mach + sg * h, r, dens)
+            g1 = em.HipSolver((n, n, 1), (0, 0, 0), (1, 1, 1), up, bc=bc, mode="weno", cfl=0.5)
+            g1.forward(args.ens_steps, args.ens_steps, dts)
+            jj.append(bubble(g1.download(), n, r)[0])
+        fd = (jj[0] - jj[1]) / (2 * h)
+        errs.append(abs(fd - grads[i]) / max(abs(grads[i]), 1e-12))
+    return {
+        "workload": f"{args.ens_n} 2-D shock-bubble instances, sides 96..384, Weno, CFL 0.5, {args.ens_steps} steps, "
+                    "J = bubble-region mean p, dJ/dMach",
+        "cells": cells,
+        "value": cells * 3 / timing[True],
+        "unit": "cell-updates/s per RK stage",
+        "per_instance_launches": {"value": cells * 3 / timing[False]},
+        "speedup_vs_per_instance": timing[False] / timing[True],
+        "forward_s": t_fwd,
+        "backward_s": t_bwd,
+        "cost_ratio": t_bwd / t_fwd,
+        "checkpoint_every": args.ens_checkpoint,
+        "dJ_dMach_first": grads[:4],
+        "fd": {"instances": len(errs), "h": 1e-6, "max_rel_err": max(errs) if errs else None},
+    }
 
 
 def _splitmix64(state):
@@ -313,6 +405,9 @@ def main():
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_bench(em, args)}), flush=True)
         return
+    if args.ensemble_only:
+        print(json.dumps({"ensemble": ensemble_bench(em, args)}), flush=True)
+        return
     n = args.n
     lo, hi = (0.0, 0.0, 0.0), (2 * np.pi,) * 3
     u = em.case_init(1, (n, n, n), lo, hi, 6, [0.3, float(0x5EED)])
@@ -373,6 +468,8 @@ def main():
     if not args.no_adjoint:
         rec["adjoint"] = adjoint_bench(em, args)
         rec["adjoint_forward_cost_ratio"] = rec["adjoint"]["cost_ratio"]
+    if not args.no_ensemble:
+        rec["ensemble"] = ensemble_bench(em, args)
     print(json.dumps(rec), flush=True)
 
 

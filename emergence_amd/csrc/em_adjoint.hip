@@ -99,11 +99,13 @@ __device__ __forceinline__ void to_prim_vjp(const Eos& e, const double w[5], con
     ub[0] = rhob;
 }
 
-// out = weno5z_interp(v0..v4).  On a stencil that is flat to rounding (all
-// smoothness indicators below 1e-26 max v^2) the Z-weights are not
-// differentiable: tau / beta is a ratio of rounding errors.  There the
-// derivative is taken with the linear weights (the smooth limit of WENO-Z),
-// the convention the tape oracle follows.
+// out = weno5z_interp(v0..v4).  Where a smoothness indicator is at the
+// rounding floor (beta_k <= 1e-26 max v^2) the Z-weights are not
+// differentiable: tau / (beta_k + eps) is a ratio involving rounding errors
+// and its derivative is noise amplified by up to 1e40.  Convention (shared
+// with the tape oracle): all three at the floor -> the linear weights'
+// derivative (the smooth limit of WENO-Z); some at the floor -> the weights
+// held at their computed values.
 __device__ __forceinline__ void weno_vjp(const double v[5], double ob, double vb[5]) {
     const double q0 = (3.0 * v[0] - 10.0 * v[1] + 15.0 * v[2]) * 0.125;
     const double q1 = (-v[1] + 6.0 * v[2] + 3.0 * v[3]) * 0.125;
@@ -137,6 +139,14 @@ __device__ __forceinline__ void weno_vjp(const double v[5], double ob, double vb
     const double Nb = ob / S;
     const double Sb = -ob * out / S;
     const double q0b = a0 * Nb, q1b = a1 * Nb, q2b = a2 * Nb;
+    if (b0 <= flat || b1 <= flat || b2 <= flat) {
+        vb[0] += 0.375 * q0b;
+        vb[1] += -1.25 * q0b - 0.125 * q1b;
+        vb[2] += 1.875 * q0b + 0.75 * q1b + 0.375 * q2b;
+        vb[3] += 0.375 * q1b + 0.75 * q2b;
+        vb[4] += -0.125 * q2b;
+        return;
+    }
     const double r0b = (q0 * Nb + Sb) * 0.0625 * 2.0 * r0;
     const double r1b = (q1 * Nb + Sb) * 0.625 * 2.0 * r1;
     const double r2b = (q2 * Nb + Sb) * 0.3125 * 2.0 * r2;

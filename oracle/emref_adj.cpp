@@ -17,9 +17,10 @@
 //     held constant;
 //   * the time-step sequence is given (frozen), not recomputed from the CFL
 //     condition;
-//   * a WENO-Z stencil flat to rounding (all smoothness indicators below
-//     1e-26 max v^2) takes the derivative of the linear weights: its
-//     Z-weights are a ratio of rounding errors and have no derivative.
+//   * WENO-Z smoothness indicators at the rounding floor (below 1e-26
+//     max v^2) make the Z-weights non-differentiable: all three at the
+//     floor -> the derivative of the linear weights; some -> the weights
+//     held at their computed values.
 // Restated from fluid/crates/compressible/src/{eos.rs, system.rs,
 // scheme.rs, solver.rs} (Weno mode, Euler; boundaries periodic, outflow,
 // reflect with n >= ng, Dirichlet).
@@ -297,17 +298,25 @@ static T weno5z_interp(const T& v0, const T& v1, const T& v2, const T& v3, const
     T a2 = dw[2] * (1.0 + r2 * r2);
     T out = (a0 * q[0] + a1 * q[1] + a2 * q[2]) / (a0 + a1 + a2);
     if constexpr (std::is_same<T, ad>::value) {
-        // Flat stencil (all smoothness indicators below 1e-26 max v^2): the
-        // Z-weights are a ratio of rounding errors and not differentiable;
-        // the derivative is that of the linear weights (the smooth limit),
-        // the value stays the forward one.
+        // A smoothness indicator at the rounding floor (below 1e-26 max v^2)
+        // makes the Z-weights non-differentiable (tau / (beta + eps) is a
+        // ratio involving rounding errors).  All three at the floor: the
+        // derivative of the linear weights (the smooth limit); some: the
+        // weights held at their computed values.  The value stays the
+        // forward one.
         const double mv = std::fmax(std::fmax(std::fmax(std::fabs(v0.v), std::fabs(v1.v)),
                                               std::fmax(std::fabs(v2.v), std::fabs(v3.v))),
                                     std::fabs(v4.v));
         const double flat = 1e-26 * mv * mv;
-        if (b0.v <= flat && b1.v <= flat && b2.v <= flat) {
+        const int nflat = (b0.v <= flat) + (b1.v <= flat) + (b2.v <= flat);
+        if (nflat == 3) {
             ad t = mk(out.v, q[0].i, dw[0], q[1].i, dw[1]);
             return mk(out.v, t.i, 1.0, q[2].i, dw[2]);
+        }
+        if (nflat > 0) {
+            const double S = (a0 + a1 + a2).v;
+            ad t = mk(out.v, q[0].i, a0.v / S, q[1].i, a1.v / S);
+            return mk(out.v, t.i, 1.0, q[2].i, a2.v / S);
         }
     }
     return out;

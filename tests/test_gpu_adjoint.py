@@ -2,10 +2,11 @@
 tape oracle, and with central finite differences of the HIP forward run.
 
 * (dR/du)^T lam of one residual: 1e-11 relative L-inf against the tape on
-  generic smooth data (limiter idle and acting), 1e-9 on the piecewise
-  constant Riemann data, where WENO-Z derivatives are taken at rounding-level
-  differences (both follow the same forward values, so both see the same
-  ones).
+  generic smooth data (limiter idle and acting) and on the piecewise constant
+  Riemann data, whose WENO-Z stencils are flat or partly flat to rounding
+  (both sides take the same convention there: linear weights when all
+  sub-stencils are flat, the weights held when some are; both recompute the
+  same forward values, so both take the same branches).
 * dJ/du_0 through SSP-RK3 steps with checkpoint segments recomputed: 1e-10
   relative L-inf against the tape gradient with the same frozen dt sequence.
 * dJ/drho_0 against central finite differences (h = 1e-6 relative) of the HIP
@@ -77,7 +78,7 @@ def test_adjoint_rhs_on_riemann_data(em):
     for stage_dt in (1e-3, 5e-2):
         ref = o.vjp_rhs(stage_dt, u, lam)
         got = g.adjoint_rhs(stage_dt, lam)
-        assert _rel(o.interior(got), o.interior(ref)) < 1e-9
+        assert _rel(o.interior(got), o.interior(ref)) < 1e-11, stage_dt
 
 
 def test_adjoint_rhs_3d(em):

@@ -135,3 +135,45 @@ def test_tape_vjp_on_flat_stencils_takes_the_linear_weights():
             b = (v[c - off, c] if d == 0 else v[c, c - off]).copy()
             b[1 + d] *= -1.0
             np.testing.assert_allclose(a, b, rtol=0, atol=1e-12 * scale)
+
+
+def test_tape_vjp_on_partly_flat_stencils_holds_the_weights():
+    # A 3x3 plateau of a second state at rest: stencils crossing its edges
+    # have one or two sub-stencils flat to rounding (beta = 0 or a rounding
+    # residue such as 3b - 4b + b) next to O(1) ones.  The Z-weight
+    # derivative through a residue beta is rounding noise amplified by
+    # ~1e-16 tau / sqrt(beta), which moves by O(1) when the plateau values
+    # move by a few ulps (the raw tape changes by 107% here).  With the
+    # weights held on such stencils the adjoint residual is as stable as the
+    # residual itself.  Perturbations are kept to those under which the
+    # forward residual does not move either: WENO-Z's own value jumps with
+    # rounding on symmetric stencils (tau = |beta0 - beta2| a residue), and
+    # there no derivative is defined.
+    n, c = 16, 7
+    o = OracleAdj((n, n, 1), (0, 0, 0), (1, 1, 1))
+
+    def state(k):
+        e = k * 2.0 ** -52
+        b = [0.125 * (1 + e), 0.0, 0.0, 0.0, 0.1 * (1 + e)]
+
+        def f(i, j, _):
+            di, dj = min(abs(i - c), n - abs(i - c)), min(abs(j - c), n - abs(j - c))
+            return b if max(di, dj) <= 1 else [1.0, 0.0, 0.0, 0.0, 1.0]
+
+        return _fill(o, f)
+
+    rng = np.random.default_rng(3)
+    u0 = state(0)
+    lam = np.zeros_like(u0)
+    o.interior(lam)[...] = rng.standard_normal(o.interior(lam).shape)
+    r0 = o.interior(o.rhs(1e-3, u0))
+    v0 = o.interior(o.vjp_rhs(1e-3, u0, lam))
+    stable = 0
+    for k in range(1, 8):
+        u = state(k)
+        if np.abs(o.interior(o.rhs(1e-3, u)) - r0).max() > 1e-13 * np.abs(r0).max():
+            continue
+        stable += 1
+        v = o.interior(o.vjp_rhs(1e-3, u, lam))
+        assert np.abs(v - v0).max() <= 1e-12 * np.abs(v0).max(), k
+    assert stable >= 2
