@@ -7,9 +7,13 @@
 //   * branches follow the forward evaluation (HLLC case, fallback, limiter
 //     trigger), min/max differentiate the operand they return (the first on
 //     a tie), d|x|/dx = sign(x) with sign(0) = 0;
-//   * the limiter's blend weight and Lax-Friedrichs speed are constants;
+//   * the limiter's blend weight theta follows the admissibility boundary
+//     its bisection resolves (implicit derivative at the constraint that
+//     bounds it), its Lax-Friedrichs speed the max of the two cells';
 //   * dt is given (frozen), so the CFL reduction is not differentiated;
-//   * a WENO-Z stencil flat to rounding takes the linear weights' derivative.
+//   * a WENO-Z stencil flat to rounding takes the linear weights' derivative,
+//     a partly flat one holds its weights, and the weight derivative is
+//     formed without cancellation (see weno_vjp).
 // Forward values are recomputed with the forward pass's own device
 // functions and operation order (no FMA contraction), so every branch
 // decision matches the forward run bit for bit.
@@ -25,7 +29,8 @@
 //      the six stencil cells in six conflict-free sub-phases;
 //   E  primitive -> conserved cotangents, ghost cotangents routed back to
 //      the interior cells they were copied from, accumulated into MU.
-// Every accumulation has a fixed order, so results are bit-reproducible.
+// Every accumulation has a fixed order that does not depend on the pencils
+// per workgroup, so results are bit-reproducible across launch shapes.
 #include <hip/hip_runtime.h>
 
 #include "em_adjoint.h"
@@ -135,9 +140,7 @@ __device__ __forceinline__ void weno_vjp(const double v[5], double ob, double vb
     const double a1 = 0.625 * (1.0 + r1 * r1);
     const double a2 = 0.3125 * (1.0 + r2 * r2);
     const double S = a0 + a1 + a2;
-    const double out = (a0 * q0 + a1 * q1 + a2 * q2) / S;
     const double Nb = ob / S;
-    const double Sb = -ob * out / S;
     const double q0b = a0 * Nb, q1b = a1 * Nb, q2b = a2 * Nb;
     if (b0 <= flat || b1 <= flat || b2 <= flat) {
         vb[0] += 0.375 * q0b;
@@ -147,9 +150,16 @@ __device__ __forceinline__ void weno_vjp(const double v[5], double ob, double vb
         vb[4] += -0.125 * q2b;
         return;
     }
-    const double r0b = (q0 * Nb + Sb) * 0.0625 * 2.0 * r0;
-    const double r1b = (q1 * Nb + Sb) * 0.625 * 2.0 * r1;
-    const double r2b = (q2 * Nb + Sb) * 0.3125 * 2.0 * r2;
+    // d out / d a_k = (q_k - out) / S, with q_k - out = sum_j w_j (q_k - q_j)
+    // formed without cancellation: on a nearly flat sub-stencil (w_k -> 1)
+    // the direct form q_k / S - out / S loses every digit and leaves
+    // eps |q| / sqrt(beta_k) of noise in the beta_k derivative.
+    const double d0 = (a1 * (q0 - q1) + a2 * (q0 - q2)) / S;
+    const double d1 = (a0 * (q1 - q0) + a2 * (q1 - q2)) / S;
+    const double d2 = (a0 * (q2 - q0) + a1 * (q2 - q1)) / S;
+    const double r0b = d0 * Nb * 0.0625 * 2.0 * r0;
+    const double r1b = d1 * Nb * 0.625 * 2.0 * r1;
+    const double r2b = d2 * Nb * 0.3125 * 2.0 * r2;
     const double taub = r0b / e0 + r1b / e1 + r2b / e2;
     const double sg = sgn(b0 - b2);
     const double b0b = -r0b * r0 / e0 + sg * taub;
@@ -304,6 +314,30 @@ __device__ void hllc_vjp(const Eos& e, const double wl[5], const double wr[5], c
     }
     sound_speed_vjp(e, wl[0], wl[4], clb, wlb[0], wlb[4]);
     sound_speed_vjp(e, wr[0], wr[4], crb, wrb[0], wrb[4]);
+}
+
+// The limiter's admissibility constraint that fails at the bisection's
+// upper bound hi, for the test states ul - tl (fx - fl) (0..2) and
+// ur + tl (fx - fr) (3..5): rho > 0, 1 / rho > b, p + p_inf > 0, the first
+// failing in that order; -1 when a test state is not finite or none fails.
+__device__ __forceinline__ int limiter_constraint(const Eos& e, const double ul[5], const double ur[5],
+                                                  const double fl[5], const double fr[5], const double fo[5],
+                                                  const double flf[5], double tl, double hi) {
+#pragma unroll 1
+    for (int side = 0; side < 2; ++side) {
+        double st[5], ws[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            const double fx = isfinite(fo[k]) ? hi * fo[k] + (1.0 - hi) * flf[k] : flf[k];
+            st[k] = side == 0 ? ul[k] - tl * (fx - fl[k]) : ur[k] + tl * (fx - fr[k]);
+        }
+        to_prim(e, st, ws);
+        if (!isfinite(ws[0]) || !isfinite(ws[4])) return -1;
+        if (!(ws[0] > 0.0)) return 3 * side;
+        if (!(1.0 / ws[0] > e.b)) return 3 * side + 1;
+        if (!(ws[4] + e.p_inf > 0.0)) return 3 * side + 2;
+    }
+    return -1;
 }
 
 // fbar of the WCNS midpoint flux of the six states s[0..5] -> sb (added).
@@ -538,8 +572,8 @@ __device__ __forceinline__ void adj_group(const AdjArgs& a, long long grp, doubl
                 Fb[k] = (lr - ll) * a.inv_dx;
             }
             bool limited = false;
-            double lo = 1.0, al = 0.0;
-            double wl[5], wr[5], ul[5], ur[5], fl[5], fr[5];
+            double lo = 1.0, hi = 1.0, al = 0.0;
+            double wl[5], wr[5], ul[5], ur[5], fl[5], fr[5], flf[5];
             if (a.lam_lim > 0.0) {
 #pragma unroll
                 for (int k = 0; k < 5; ++k) {
@@ -568,7 +602,6 @@ __device__ __forceinline__ void adj_group(const AdjArgs& a, long long grp, doubl
                 if (!(fin && ok(fo))) {
                     limited = true;
                     al = fmax(fabs(wl[un]) + sound_speed(e, wl[0], wl[4]), fabs(wr[un]) + sound_speed(e, wr[0], wr[4]));
-                    double flf[5];
 #pragma unroll
                     for (int k = 0; k < 5; ++k) flf[k] = 0.5 * (fl[k] + fr[k]) - 0.5 * al * (ur[k] - ul[k]);
                     double l0 = 0.0, h0 = 1.0;
@@ -583,24 +616,94 @@ __device__ __forceinline__ void adj_group(const AdjArgs& a, long long grp, doubl
                             h0 = mid;
                     }
                     lo = l0;
+                    hi = h0;
                 }
             }
             if (limited) {
                 // out = lo fh + (1 - lo) flf per finite component, flf otherwise
-                double flfb[5];
+                const double tl = 2.0 * a.lam_lim;
+                double flfb[5], flb[5], frb[5], ulb[5], urb[5];
+                double thb = 0.0;
 #pragma unroll
                 for (int k = 0; k < 5; ++k) {
                     const bool fk = isfinite(fo[k]);
                     fhb[k] = fk ? lo * Fb[k] : 0.0;
                     flfb[k] = fk ? (1.0 - lo) * Fb[k] : Fb[k];
+                    if (fk) thb += Fb[k] * (fo[k] - flf[k]);
+                    flb[k] = 0.0;
+                    frb[k] = 0.0;
+                    ulb[k] = 0.0;
+                    urb[k] = 0.0;
                 }
-                double flb[5], frb[5], ulb[5] = {0, 0, 0, 0, 0}, urb[5] = {0, 0, 0, 0, 0};
+                // theta: the bisection resolves theta* (the admissibility
+                // boundary) to 2^-30, so theta follows theta*(u) and is
+                // differentiated implicitly at the constraint g that fails
+                // at the upper bound: d theta = -dg / (dg/dtheta).
+                // theta = 0 (nothing admissible above it) stays constant.
+                const int which = lo > 0.0 ? limiter_constraint(e, ul, ur, fl, fr, fo, flf, tl, hi) : -1;
+                if (which >= 0) {
+                    const bool left = which < 3;  // test state ul - tl (fx - fl), else ur + tl (fx - fr)
+                    const double sg = left ? -tl : tl;
+                    double st[5], ws[5], gw[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, gs[5];
+#pragma unroll
+                    for (int k = 0; k < 5; ++k) {
+                        const double fx = isfinite(fo[k]) ? lo * fo[k] + (1.0 - lo) * flf[k] : flf[k];
+                        st[k] = left ? ul[k] - tl * (fx - fl[k]) : ur[k] + tl * (fx - fr[k]);
+                    }
+                    to_prim(e, st, ws);
+                    const int c = which % 3;
+                    if (c == 0)
+                        gw[0] = 1.0;  // rho
+                    else if (c == 1)
+                        gw[0] = -1.0 / (ws[0] * ws[0]);  // 1 / rho - b
+                    else
+                        gw[4] = 1.0;  // p + p_inf
+                    to_prim_vjp(e, ws, gw, gs);
+                    double gth = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 5; ++k)
+                        if (isfinite(fo[k])) gth += gs[k] * (sg * (fo[k] - flf[k]));
+                    if (isfinite(gth) && gth != 0.0) {
+                        const double gb = -thb / gth;
+#pragma unroll
+                        for (int k = 0; k < 5; ++k) {
+                            const double sb = gb * gs[k];  // cotangent of the test state
+                            const double fxb = sg * sb;
+                            if (isfinite(fo[k])) {
+                                fhb[k] += lo * fxb;
+                                flfb[k] += (1.0 - lo) * fxb;
+                            } else {
+                                flfb[k] += fxb;
+                            }
+                            if (left) {
+                                ulb[k] += sb;
+                                flb[k] += tl * sb;
+                            } else {
+                                urb[k] += sb;
+                                frb[k] -= tl * sb;
+                            }
+                        }
+                    }
+                }
+                // flf = (fl + fr) / 2 - al (ur - ul) / 2; al = max of the two
+                // cells' |u_n| + c (the operand max returns, the first on a tie)
+                double alb = 0.0;
 #pragma unroll
                 for (int k = 0; k < 5; ++k) {
-                    flb[k] = 0.5 * flfb[k];
-                    frb[k] = 0.5 * flfb[k];
-                    urb[k] = -0.5 * al * flfb[k];
-                    ulb[k] = 0.5 * al * flfb[k];
+                    flb[k] += 0.5 * flfb[k];
+                    frb[k] += 0.5 * flfb[k];
+                    urb[k] += -0.5 * al * flfb[k];
+                    ulb[k] += 0.5 * al * flfb[k];
+                    alb -= 0.5 * flfb[k] * (ur[k] - ul[k]);
+                }
+                const double spl = fabs(wl[un]) + sound_speed(e, wl[0], wl[4]);
+                const double spr = fabs(wr[un]) + sound_speed(e, wr[0], wr[4]);
+                if (spr > spl) {
+                    wrb[un] += alb * sgn(wr[un]);
+                    sound_speed_vjp(e, wr[0], wr[4], alb, wrb[0], wrb[4]);
+                } else {
+                    wlb[un] += alb * sgn(wl[un]);
+                    sound_speed_vjp(e, wl[0], wl[4], alb, wlb[0], wlb[4]);
                 }
                 phys_flux_vjp<D>(wl, ul, flb, wlb, ulb);
                 phys_flux_vjp<D>(wr, ur, frb, wrb, urb);
@@ -611,17 +714,20 @@ __device__ __forceinline__ void adj_group(const AdjArgs& a, long long grp, doubl
                 for (int k = 0; k < 5; ++k) fhb[k] = Fb[k];
             }
         }
+        // a cell takes the right state of face c-5 before the left state of
+        // face c-4: increasing face order within a chunk as across chunks,
+        // so the sums do not depend on the chunking (on P)
         __syncthreads();
         if (live) {
 #pragma unroll
             for (int k = 0; k < 5; ++k) W(smf, k, f, p, M) = fhb[k];
 #pragma unroll
-            for (int k = 0; k < 5; ++k) W(sacc, k, f + 4, p, C) += wlb[k];
+            for (int k = 0; k < 5; ++k) W(sacc, k, f + 5, p, C) += wrb[k];
         }
         __syncthreads();
         if (live) {
 #pragma unroll
-            for (int k = 0; k < 5; ++k) W(sacc, k, f + 5, p, C) += wrb[k];
+            for (int k = 0; k < 5; ++k) W(sacc, k, f + 4, p, C) += wlb[k];
         }
         __syncthreads();
     }
@@ -653,7 +759,9 @@ __device__ __forceinline__ void adj_group(const AdjArgs& a, long long grp, doubl
                 for (int q = 0; q < 5; ++q) s[j][q] = W(sw, q, m + j, p, C);
             midpoint_vjp<D>(e, s, mb, sb);
         }
-        for (int j = 0; j < 6; ++j) {
+        // cell c takes midpoints c-5 .. c in increasing order (j from 5 down
+        // to 0), within a chunk as across chunks: independent of P
+        for (int j = 5; j >= 0; --j) {
             if (live) {
 #pragma unroll
                 for (int q = 0; q < 5; ++q) W(sacc, q, m + j, p, C) += sb[j][q];

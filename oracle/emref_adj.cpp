@@ -13,14 +13,16 @@
 //   * min/max take the derivative of the operand they return, the first on
 //     a tie (as f64::min / f64::max evaluate);
 //   * d|x|/dx = sign(x) with sign(0) = 0;
-//   * the limiter's blend weight theta and Lax-Friedrichs speed alpha are
-//     held constant;
+//   * the limiter's blend weight theta is differentiated implicitly at the
+//     admissibility constraint that bounds it (the bisection resolves it to
+//     2^-30), its Lax-Friedrichs speed alpha through the max;
 //   * the time-step sequence is given (frozen), not recomputed from the CFL
 //     condition;
 //   * WENO-Z smoothness indicators at the rounding floor (below 1e-26
 //     max v^2) make the Z-weights non-differentiable: all three at the
 //     floor -> the derivative of the linear weights; some -> the weights
-//     held at their computed values.
+//     held at their computed values.  Elsewhere the weight derivative
+//     (q_k - out) / S is formed without cancellation.
 // Restated from fluid/crates/compressible/src/{eos.rs, system.rs,
 // scheme.rs, solver.rs} (Weno mode, Euler; boundaries periodic, outflow,
 // reflect with n >= ng, Dirichlet).
@@ -313,11 +315,26 @@ static T weno5z_interp(const T& v0, const T& v1, const T& v2, const T& v3, const
             ad t = mk(out.v, q[0].i, dw[0], q[1].i, dw[1]);
             return mk(out.v, t.i, 1.0, q[2].i, dw[2]);
         }
-        if (nflat > 0) {
-            const double S = (a0 + a1 + a2).v;
-            ad t = mk(out.v, q[0].i, a0.v / S, q[1].i, a1.v / S);
-            return mk(out.v, t.i, 1.0, q[2].i, a2.v / S);
+        const double S = (a0 + a1 + a2).v;
+        ad t = mk(out.v, q[0].i, a0.v / S, q[1].i, a1.v / S);
+        ad wq = mk(out.v, t.i, 1.0, q[2].i, a2.v / S);
+        if (nflat > 0) return wq;
+        // out = sum_k a_k q_k / S with d out / d a_k = (q_k - out) / S, where
+        // q_k - out = sum_j a_j (q_k - q_j) / S is formed without
+        // cancellation (the chain rule through N / S would subtract two
+        // nearly equal terms and leave eps |q| / sqrt(beta_k) of noise on a
+        // nearly flat sub-stencil).
+        const double qv[3] = {q[0].v, q[1].v, q[2].v}, av[3] = {a0.v, a1.v, a2.v};
+        double dk[3];
+        for (int k = 0; k < 3; ++k) {
+            double acc = 0.0;
+            for (int j = 0; j < 3; ++j)
+                if (j != k) acc += av[j] * (qv[k] - qv[j]);
+            dk[k] = acc / S / S;
         }
+        ad u = mk(out.v, wq.i, 1.0, a0.i, dk[0]);
+        u = mk(out.v, u.i, 1.0, a1.i, dk[1]);
+        return mk(out.v, u.i, 1.0, a2.i, dk[2]);
     }
     return out;
 }
@@ -363,7 +380,68 @@ static bool midpoint_flux(const emref::Nasg& e, const VT<T>* w, size_t j, int d,
     return fb;
 }
 
-// scheme.rs:395-464, theta and alpha constant
+// Derivative of the limiter's blend weight.  The bisection returns theta
+// within 2^-30 below the largest admissible theta*, so on any scale a
+// perturbation can resolve theta follows theta*(data): it is differentiated
+// implicitly at the admissibility constraint g (rho, covolume 1/rho - b or
+// p + p_inf of one of the two test states) that fails at the bisection's
+// upper bound, d theta = -dg / (dg/dtheta).  theta = 0 (nothing admissible
+// above it) stays constant.
+static double tape_local_derivative(int out, int in) {
+    // d out / d in over the nodes in..out (created after `in`)
+    const Tape& t = *g_tape;
+    std::vector<double> adj((size_t)(out - in + 1), 0.0);
+    adj[(size_t)(out - in)] = 1.0;
+    for (int i = out; i > in; --i) {
+        const double g = adj[(size_t)(i - in)];
+        if (g == 0.0) continue;
+        if (t.a[i] >= in) adj[(size_t)(t.a[i] - in)] += t.da[i] * g;
+        if (t.b[i] >= in) adj[(size_t)(t.b[i] - in)] += t.db[i] * g;
+    }
+    return adj[0];
+}
+
+template <class T>
+static T limiter_theta(const emref::Nasg& e, double lambda, const VT<T>& ul, const VT<T>& ur, const VT<T>& fl,
+                       const VT<T>& fr, const VT<T>& fh, const VT<T>& flf, double lo, double hi) {
+    if constexpr (!std::is_same<T, ad>::value) {
+        return lo;
+    } else {
+        if (!(lo > 0.0)) return ad(lo);
+        // the constraint that fails at hi (first in a fixed order)
+        int which = -1;
+        for (int side = 0; side < 2 && which < 0; ++side) {
+            VT<double> st;
+            for (int k = 0; k < 5; ++k) {
+                const double fx = std::isfinite(fh[k].v) ? hi * fh[k].v + (1.0 - hi) * flf[k].v : flf[k].v;
+                st[k] = side == 0 ? ul[k].v - 2.0 * lambda * (fx - fl[k].v) : ur[k].v + 2.0 * lambda * (fx - fr[k].v);
+            }
+            const VT<double> ws = to_prim(e, st);
+            if (!std::isfinite(ws[0]) || !std::isfinite(ws[4])) return ad(lo);
+            if (!(ws[0] > 0.0))
+                which = 3 * side;
+            else if (!(1.0 / ws[0] > e.b))
+                which = 3 * side + 1;
+            else if (!(ws[4] + e.p_inf > 0.0))
+                which = 3 * side + 2;
+        }
+        if (which < 0) return ad(lo);
+        const ad tin = ad::input(lo);
+        VT<ad> st;
+        for (int k = 0; k < 5; ++k) {
+            const ad fx = std::isfinite(fh[k].v) ? tin * fh[k] + (1.0 - tin) * flf[k] : flf[k];
+            st[k] = which < 3 ? ul[k] - 2.0 * lambda * (fx - fl[k]) : ur[k] + 2.0 * lambda * (fx - fr[k]);
+        }
+        const VT<ad> ws = to_prim(e, st);
+        const ad g = which % 3 == 0 ? ws[0] : which % 3 == 1 ? 1.0 / ws[0] - e.b : ws[4] + e.p_inf;
+        const double gth = g.i > tin.i ? tape_local_derivative(g.i, tin.i) : 0.0;
+        if (!std::isfinite(gth) || gth == 0.0) return ad(lo);
+        return mk(lo, g.i, -1.0 / gth, -1, 0.0);
+    }
+}
+
+// scheme.rs:395-464; theta differentiated implicitly (limiter_theta), alpha
+// through its max (the operand it returns)
 template <class T>
 static void positivity_limit(const emref::Nasg& e, const VT<T>* w, size_t len, int d, size_t ng, double lambda,
                              VT<T>* out) {
@@ -389,7 +467,7 @@ static void positivity_limit(const emref::Nasg& e, const VT<T>* w, size_t len, i
         bool finite = true;
         for (int k = 0; k < 5; ++k) finite = finite && std::isfinite(val(fh[k]));
         if (finite && ok(fh)) continue;
-        const double alpha = std::fmax(val(max_speed(e, w[i], d)), val(max_speed(e, w[i + 1], d)));
+        const T alpha = rmax<T>(max_speed(e, w[i], d), max_speed(e, w[i + 1], d));
         VT<T> flf;
         for (int k = 0; k < 5; ++k) flf[k] = 0.5 * (fl_phys[k] + fr_phys[k]) - 0.5 * alpha * (ur[k] - ul[k]);
         auto blend = [&](double t) {
@@ -405,7 +483,8 @@ static void positivity_limit(const emref::Nasg& e, const VT<T>* w, size_t len, i
             else
                 hi = mid;
         }
-        out[f] = blend(lo);
+        const T th = limiter_theta<T>(e, lambda, ul, ur, fl_phys, fr_phys, fh, flf, lo, hi);
+        for (int k = 0; k < 5; ++k) out[f][k] = std::isfinite(val(fh[k])) ? th * fh[k] + (1.0 - th) * flf[k] : flf[k];
     }
 }
 
@@ -601,6 +680,20 @@ void emadj_steps(void* h, double* u, int64_t nsteps, const double* dts) {
     load_u(cx, u, uu);
     for (int64_t s = 0; s < nsteps; ++s) step(cx, uu, dts[s]);
     std::memcpy(u, uu.data(), uu.size() * 5 * sizeof(double));
+}
+
+// tape: vb = (d weno5z_interp / d v)^T ob for one stencil v[0..4]
+void emadj_weno_vjp(const double* v, double ob, double* vb) {
+    Tape tape;
+    g_tape = &tape;
+    ad x[5];
+    for (int k = 0; k < 5; ++k) x[k] = ad::input(v[k]);
+    ad o = weno5z_interp<ad>(x[0], x[1], x[2], x[3], x[4]);
+    std::vector<double> adj(tape.a.size(), 0.0);
+    if (o.i >= 0) adj[o.i] = ob;
+    tape.reverse(adj);
+    for (int k = 0; k < 5; ++k) vb[k] = adj[x[k].i];
+    g_tape = nullptr;
 }
 
 // tape: (d rhs / d u)^T lam at u (interior cells; ghosts of u are recomputed)

@@ -308,8 +308,18 @@ def _load_adj(lib):
     lib.emadj_steps.argtypes = [P, D, C.c_int64, D]
     lib.emadj_vjp_rhs.argtypes = [P, C.c_double, D, D, D]
     lib.emadj_gradient.argtypes = [P, D, C.c_int64, D, D]
+    lib.emadj_weno_vjp.argtypes = [D, C.c_double, D]
     lib._adj_ready = True
     return lib
+
+
+def weno_vjp(v, ob=1.0):
+    """Tape VJP of one WENO5-Z interpolation: (d out / d v)^T ob."""
+    lib = _load_adj(load())
+    vv = np.ascontiguousarray(v, dtype=np.float64)
+    out = np.empty(5)
+    lib.emadj_weno_vjp(_dp(vv), float(ob), _dp(out))
+    return out
 
 
 class OracleAdj:

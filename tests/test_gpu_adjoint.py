@@ -2,7 +2,8 @@
 tape oracle, and with central finite differences of the HIP forward run.
 
 * (dR/du)^T lam of one residual: 1e-11 relative L-inf against the tape on
-  generic smooth data (limiter idle and acting) and on the piecewise constant
+  generic smooth data (limiter idle, and blending on many faces with theta
+  differentiated at its admissibility constraint) and on the piecewise constant
   Riemann data, whose WENO-Z stencils are flat or partly flat to rounding
   (both sides take the same convention there: linear weights when all
   sub-stencils are flat, the weights held when some are; both recompute the
@@ -10,7 +11,8 @@ tape oracle, and with central finite differences of the HIP forward run.
 * dJ/du_0 through SSP-RK3 steps with checkpoint segments recomputed: 1e-10
   relative L-inf against the tape gradient with the same frozen dt sequence.
 * dJ/drho_0 against central finite differences (h = 1e-6 relative) of the HIP
-  forward run with the frozen dt sequence: 1e-6 relative, on smooth data
+  forward run with the frozen dt sequence: 1e-6 relative plus the FD
+  rounding floor (100 eps |J| / h), on smooth data
   where every characteristic field varies (WENO-Z with eps = 1e-40 is not
   differentiable where a field is exactly constant along a stencil).
 * Repeated backward sweeps are bit-identical.
@@ -50,7 +52,7 @@ def _pair(em, n, bc, u, positivity=True):
 
 
 @pytest.mark.parametrize("bci", range(len(BCS)))
-@pytest.mark.parametrize("stage_dt", [1e-3, 3e-2])
+@pytest.mark.parametrize("stage_dt", [1e-3, 3e-2, 0.8])  # 0.8: the limiter blends on many faces
 def test_adjoint_rhs_matches_the_tape(em, bci, stage_dt):
     print(f"adjoint rhs smooth 2-D bc={bci} stage_dt={stage_dt}", flush=True)
     n = (24, 20, 1)
@@ -145,6 +147,8 @@ def test_backward_matches_central_differences(em):
             Js.append(pressure_window(o0, f.download(), 0.4, 0.7)[0])
         fd = (Js[0] - Js[1]) / (2 * h)
         scale = np.abs(grad).max()
-        assert abs(fd - ad) <= 1e-6 * max(abs(ad), 1e-2 * scale), (i, j, fd, ad)
+        # FD rounding: the two runs' own rounding errors, ~100 eps |J| / h
+        noise = 100 * np.finfo(float).eps * abs(J0) / h
+        assert abs(fd - ad) <= 1e-6 * max(abs(ad), 1e-2 * scale) + noise, (i, j, fd, ad)
         checks += 1
     assert checks == 8

@@ -177,3 +177,105 @@ def test_tape_vjp_on_partly_flat_stencils_holds_the_weights():
         v = o.interior(o.vjp_rhs(1e-3, u, lam))
         assert np.abs(v - v0).max() <= 1e-12 * np.abs(v0).max(), k
     assert stable >= 2
+
+
+def _weno_grad_exact(v):
+    """d weno5z_interp / d v of the exact function (rational arithmetic on the
+    given doubles, forward-mode duals; eps = 1e-40 as its double value)."""
+    from fractions import Fraction as Fr
+
+    n = 5
+
+    class Du:
+        def __init__(self, a, d):
+            self.a, self.d = a, d
+
+        def __add__(s, o):
+            o = o if isinstance(o, Du) else Du(Fr(o), [Fr(0)] * n)
+            return Du(s.a + o.a, [x + y for x, y in zip(s.d, o.d)])
+
+        __radd__ = __add__
+
+        def __sub__(s, o):
+            o = o if isinstance(o, Du) else Du(Fr(o), [Fr(0)] * n)
+            return Du(s.a - o.a, [x - y for x, y in zip(s.d, o.d)])
+
+        def __rsub__(s, o):
+            return Du(Fr(o), [Fr(0)] * n) - s
+
+        def __mul__(s, o):
+            if not isinstance(o, Du):
+                o = Fr(o)
+                return Du(s.a * o, [x * o for x in s.d])
+            return Du(s.a * o.a, [x * o.a + s.a * y for x, y in zip(s.d, o.d)])
+
+        __rmul__ = __mul__
+
+        def __truediv__(s, o):
+            if not isinstance(o, Du):
+                o = Fr(o)
+                return Du(s.a / o, [x / o for x in s.d])
+            return Du(s.a / o.a, [(x * o.a - s.a * y) / (o.a * o.a) for x, y in zip(s.d, o.d)])
+
+    x = [Du(Fr(float(v[k])), [Fr(int(j == k)) for j in range(n)]) for k in range(n)]
+    q = [(3 * x[0] - 10 * x[1] + 15 * x[2]) / 8, (-1 * x[1] + 6 * x[2] + 3 * x[3]) / 8,
+         (3 * x[2] + 6 * x[3] - 1 * x[4]) / 8]
+    t = [x[0] - 2 * x[1] + x[2], x[1] - 2 * x[2] + x[3], x[2] - 2 * x[3] + x[4]]
+    s = [x[0] - 4 * x[1] + 3 * x[2], x[1] - x[3], 3 * x[2] - 4 * x[3] + x[4]]
+    b = [Fr(13, 12) * (t[k] * t[k]) + Fr(1, 4) * (s[k] * s[k]) for k in range(3)]
+    d = b[0] - b[2]
+    tau = d if d.a >= 0 else -1 * d
+    eps = Fr(1e-40)
+    dw = [Fr(1, 16), Fr(10, 16), Fr(5, 16)]
+    a = [dw[k] * (1 + (tau / (b[k] + eps)) * (tau / (b[k] + eps))) for k in range(3)]
+    out = (a[0] * q[0] + a[1] * q[1] + a[2] * q[2]) / (a[0] + a[1] + a[2])
+    return np.array([float(g) for g in out.d])
+
+
+def test_tape_weno_vjp_matches_exact_derivatives():
+    # Exact derivatives of WENO5-Z (rational arithmetic on the same double
+    # inputs) pin the tape's weight derivative.  On a nearly flat sub-stencil
+    # (beta above the rounding floor but small) the chain rule through
+    # out = N / S subtracts two nearly equal terms and leaves up to
+    # eps |q| / sqrt(beta) of noise; the cancellation-free form is accurate.
+    from _oracle import weno_vjp
+
+    rng = np.random.default_rng(11)
+    cases = [rng.uniform(-1, 2, 5) for _ in range(6)]
+    # one sub-stencil varying by e (beta ~ e^2) next to O(1) jumps; whether
+    # the direct form is noisy depends on how out rounds against q_k, and
+    # several of these (e = 3e-11 .. 1e-9) leave it 1e-7 .. 3e-5 off
+    for e in (1e-12, 3e-11, 1e-10, 3e-10, 1e-9, 3e-9):
+        cases += [np.array([1.0, 1.0 + e, 1.0 + 3.3 * e, 1.2, 0.7]),
+                  np.array([0.7, 1.2, 1.0 + 3.3 * e, 1.0 + e, 1.0]),
+                  np.array([0.5, 0.8, 1.0, 1.0 + 2.1 * e, 1.0 + 3.9 * e])]
+    for v in cases:
+        got, ref = weno_vjp(v), _weno_grad_exact(v)
+        assert np.abs(got - ref).max() <= 1e-10 * np.abs(ref).max(), (v, got, ref)
+
+
+@pytest.mark.parametrize("stage_dt", [0.8, 6.0])
+def test_tape_vjp_with_the_limiter_acting_matches_central_differences(stage_dt):
+    # Large stage dt makes the positivity limiter blend on 27 / 213 faces of
+    # smooth data.  Its bisection resolves theta to 2^-30, so the residual
+    # follows the admissibility boundary theta*(u): the tape differentiates
+    # theta implicitly at the constraint that bounds it, and the
+    # Lax-Friedrichs speed through its max.  Holding both constant is off by
+    # 4e-3 .. 6e-2 here.
+    n = 12
+    o = OracleAdj((n, n, 1), (0, 0, 0), (1, 1, 1))
+    u = _smooth_state(o, 3)
+    s = OracleSolver((n, n, 1), (0, 0, 0), (1, 1, 1))
+    s.set_opts(mode="weno")
+    s.set_u(u)
+    assert s.rhs(stage_dt)[1][3] > 20, "the limiter should act"
+    rng = np.random.default_rng(7)
+    lam = np.zeros_like(u)
+    dlt = np.zeros_like(u)
+    o.interior(lam)[...] = rng.standard_normal(o.interior(lam).shape)
+    o.interior(dlt)[...] = rng.standard_normal(o.interior(dlt).shape) * np.array([1, 1, 1, 0, 2])
+    b = np.sum(o.interior(o.vjp_rhs(stage_dt, u, lam)) * o.interior(dlt))
+    h = 1e-6  # small enough that no face changes branch
+    jvp = (o.rhs(stage_dt, u + h * dlt) - o.rhs(stage_dt, u - h * dlt)) / (2 * h)
+    a = np.sum(o.interior(jvp) * o.interior(lam))
+    assert abs(a - b) <= 2e-6 * abs(b)
