@@ -47,6 +47,7 @@ struct AdjCombineTask {
     long long dt_index;
 };
 
+constexpr int kAdjThreads = 768;  // threads per adjoint workgroup (12 waves: 3 per SIMD)
 int adj_pencils_per_group(int n);
 hipError_t launch_ens_adj_pass(int d, const EnsAdjArgs& e, size_t lds, unsigned grid, hipStream_t s);
 hipError_t launch_ens_adj_combine(int stage, const AdjCombineTask* tasks, int ninst, const double* dts,

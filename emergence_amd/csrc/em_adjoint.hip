@@ -18,17 +18,21 @@
 // functions and operation order (no FMA contraction), so every branch
 // decision matches the forward run bit for bit.
 //
-// One workgroup owns whole pencils along the pass axis (interior cells plus
-// the 5 virtual ghost layers on each side) in LDS:
-//   A  load U, primitives (boundary ghosts through bc_ghost);
-//   B  forward midpoint fluxes;
-//   C  face fluxes and limiter state, face cotangents
-//      Fbar_f = (lam_f - lam_{f-1}) / dx, the limiter's Lax-Friedrichs part
-//      scattered to its two cells;
-//   D  midpoint cotangents (S6 weights) and the midpoint VJP, scattered to
-//      the six stencil cells in six conflict-free sub-phases;
-//   E  primitive -> conserved cotangents, ghost cotangents routed back to
-//      the interior cells they were copied from, accumulated into MU.
+// One workgroup (768 threads) owns whole pencils along the pass axis
+// (interior cells plus the 5 virtual ghost layers on each side) in LDS:
+//   A   load U, primitives (boundary ghosts through bc_ghost);
+//   B1  WENO5-Z reconstructions per (midpoint, characteristic field);
+//   B2  back-projection, fallback and HLLC per midpoint: forward fluxes;
+//   C   face fluxes and limiter state, face cotangents
+//       Fbar_f = (lam_f - lam_{f-1}) / dx, the limiter's Lax-Friedrichs part
+//       and blend weight scattered to its two cells;
+//   D2  per midpoint: cotangent from its faces (S6 weights), HLLC and
+//       back-projection VJPs (the reconstructions of B1 are reused);
+//   D3  per (midpoint, field): WENO5-Z VJPs of both sides, projection VJP,
+//       scattered to the six stencil cells in six conflict-free sub-phases;
+//   D4  per midpoint: the face-average state's cotangent (or the fallback's);
+//   E   primitive -> conserved cotangents, ghost cotangents routed back to
+//       the interior cells they were copied from, accumulated into MU.
 // Every accumulation has a fixed order that does not depend on the pencils
 // per workgroup, so results are bit-reproducible across launch shapes.
 #include <hip/hip_runtime.h>
@@ -133,14 +137,16 @@ __device__ __forceinline__ void weno_vjp(const double v[5], double ob, double vb
         vb[4] += -0.125 * q2b;
         return;
     }
+    // derivative values only (no branch depends on them): reciprocals
+    // instead of divisions
     const double tau = fabs(b0 - b2);
-    const double e0 = b0 + 1e-40, e1 = b1 + 1e-40, e2 = b2 + 1e-40;
-    const double r0 = tau / e0, r1 = tau / e1, r2 = tau / e2;
+    const double i0 = 1.0 / (b0 + 1e-40), i1 = 1.0 / (b1 + 1e-40), i2 = 1.0 / (b2 + 1e-40);
+    const double r0 = tau * i0, r1 = tau * i1, r2 = tau * i2;
     const double a0 = 0.0625 * (1.0 + r0 * r0);
     const double a1 = 0.625 * (1.0 + r1 * r1);
     const double a2 = 0.3125 * (1.0 + r2 * r2);
-    const double S = a0 + a1 + a2;
-    const double Nb = ob / S;
+    const double iS = 1.0 / (a0 + a1 + a2);
+    const double Nb = ob * iS;
     const double q0b = a0 * Nb, q1b = a1 * Nb, q2b = a2 * Nb;
     if (b0 <= flat || b1 <= flat || b2 <= flat) {
         vb[0] += 0.375 * q0b;
@@ -154,17 +160,17 @@ __device__ __forceinline__ void weno_vjp(const double v[5], double ob, double vb
     // formed without cancellation: on a nearly flat sub-stencil (w_k -> 1)
     // the direct form q_k / S - out / S loses every digit and leaves
     // eps |q| / sqrt(beta_k) of noise in the beta_k derivative.
-    const double d0 = (a1 * (q0 - q1) + a2 * (q0 - q2)) / S;
-    const double d1 = (a0 * (q1 - q0) + a2 * (q1 - q2)) / S;
-    const double d2 = (a0 * (q2 - q0) + a1 * (q2 - q1)) / S;
+    const double d0 = (a1 * (q0 - q1) + a2 * (q0 - q2)) * iS;
+    const double d1 = (a0 * (q1 - q0) + a2 * (q1 - q2)) * iS;
+    const double d2 = (a0 * (q2 - q0) + a1 * (q2 - q1)) * iS;
     const double r0b = d0 * Nb * 0.0625 * 2.0 * r0;
     const double r1b = d1 * Nb * 0.625 * 2.0 * r1;
     const double r2b = d2 * Nb * 0.3125 * 2.0 * r2;
-    const double taub = r0b / e0 + r1b / e1 + r2b / e2;
+    const double taub = r0b * i0 + r1b * i1 + r2b * i2;
     const double sg = sgn(b0 - b2);
-    const double b0b = -r0b * r0 / e0 + sg * taub;
-    const double b1b = -r1b * r1 / e1;
-    const double b2b = -r2b * r2 / e2 - sg * taub;
+    const double b0b = -r0b * r0 * i0 + sg * taub;
+    const double b1b = -r1b * r1 * i1;
+    const double b2b = -r2b * r2 * i2 - sg * taub;
     const double t0b = b0b * 2.0 * c13 * t0, s0b = b0b * 0.5 * s0;
     const double t1b = b1b * 2.0 * c13 * t1, s1b = b1b * 0.5 * s1;
     const double t2b = b2b * 2.0 * c13 * t2, s2b = b2b * 0.5 * s2;
@@ -340,133 +346,24 @@ __device__ __forceinline__ int limiter_constraint(const Eos& e, const double ul[
     return -1;
 }
 
-// fbar of the WCNS midpoint flux of the six states s[0..5] -> sb (added).
-// The characteristic values are recomputed per field instead of being kept
-// (they are cheap and registers are not).
-template <int D>
-__device__ void midpoint_vjp(const Eos& e, const double (&s)[6][5], const double fbar[5], double (&sb)[6][5]) {
-    constexpr int un = 1 + D;
-    constexpr int t1 = 1 + (D + 1) % 3;
-    constexpr int t2 = 1 + (D + 2) % 3;
-    const double rho = 0.5 * (s[2][0] + s[3][0]);
-    const double avp = 0.5 * (s[2][4] + s[3][4]);
-    const double c = sound_speed(e, rho, avp);
-    const double l0un = -rho / (2.0 * c);
-    const double l04 = 1.0 / (2.0 * c * c);
-    const double l14 = -1.0 / (c * c);
-    const double l4un = rho / (2.0 * c);
-    const double l44 = 1.0 / (2.0 * c * c);
-    // characteristic field a of state k (rows of L, sparse)
-    auto chv = [&](int a, int k) -> double {
-        switch (a) {
-            case 0: return l0un * s[k][un] + l04 * s[k][4];
-            case 1: return s[k][0] + l14 * s[k][4];
-            case 2: return s[k][t1];
-            case 3: return s[k][t2];
-            default: return l4un * s[k][un] + l44 * s[k][4];
-        }
-    };
-    double cl[5], cr[5];
-#pragma unroll 1
-    for (int a = 0; a < 5; ++a) {
-        cl[a] = weno5z_interp(chv(a, 0), chv(a, 1), chv(a, 2), chv(a, 3), chv(a, 4));
-        cr[a] = weno5z_interp(chv(a, 5), chv(a, 4), chv(a, 3), chv(a, 2), chv(a, 1));
-    }
-    const double run0 = -c / rho, run4 = c / rho, r40 = c * c;
-    double wl[5], wr[5];
-    wl[0] = cl[0] + cl[1] + cl[4];
-    wr[0] = cr[0] + cr[1] + cr[4];
-    wl[un] = run0 * cl[0] + run4 * cl[4];
-    wr[un] = run0 * cr[0] + run4 * cr[4];
-    wl[t1] = cl[2];
-    wr[t1] = cr[2];
-    wl[t2] = cl[3];
-    wr[t2] = cr[3];
-    wl[4] = r40 * cl[0] + r40 * cl[4];
-    wr[4] = r40 * cr[0] + r40 * cr[4];
-    const bool fb = !admissible(e, wl[0], wl[4]) || !admissible(e, wr[0], wr[4]);
-    if (fb) {
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            wl[k] = s[2][k];
-            wr[k] = s[3][k];
-        }
-    }
-    double wlb[5], wrb[5];
-    hllc_vjp<D>(e, wl, wr, fbar, wlb, wrb);
-    if (fb) {
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            sb[2][k] += wlb[k];
-            sb[3][k] += wrb[k];
-        }
-        return;
-    }
-    // back-projection w = R c
-    double clb[5], crb[5];
-    clb[0] = wlb[0] + run0 * wlb[un] + r40 * wlb[4];
-    crb[0] = wrb[0] + run0 * wrb[un] + r40 * wrb[4];
-    clb[1] = wlb[0];
-    crb[1] = wrb[0];
-    clb[2] = wlb[t1];
-    crb[2] = wrb[t1];
-    clb[3] = wlb[t2];
-    crb[3] = wrb[t2];
-    clb[4] = wlb[0] + run4 * wlb[un] + r40 * wlb[4];
-    crb[4] = wrb[0] + run4 * wrb[un] + r40 * wrb[4];
-    const double run0b = wlb[un] * cl[0] + wrb[un] * cr[0];
-    const double run4b = wlb[un] * cl[4] + wrb[un] * cr[4];
-    const double r40b = wlb[4] * (cl[0] + cl[4]) + wrb[4] * (cr[0] + cr[4]);
-    // per characteristic field: WENO5-Z (right side mirrored), then the
-    // projection ch = L s
-    double l0unb = 0.0, l04b = 0.0, l14b = 0.0, l4unb = 0.0, l44b = 0.0;
-#pragma unroll 1
-    for (int a = 0; a < 5; ++a) {
-        double v[5] = {chv(a, 0), chv(a, 1), chv(a, 2), chv(a, 3), chv(a, 4)};
-        double chb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        weno_vjp(v, clb[a], chb);
-        double vr[5] = {chv(a, 5), v[4], v[3], v[2], v[1]};
-        double vrb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        weno_vjp(vr, crb[a], vrb);
-#pragma unroll
-        for (int k = 0; k < 5; ++k) chb[5 - k] += vrb[k];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) {
-            const double g = chb[k];
-            if (a == 0) {
-                sb[k][un] += l0un * g;
-                sb[k][4] += l04 * g;
-                l0unb += g * s[k][un];
-                l04b += g * s[k][4];
-            } else if (a == 1) {
-                sb[k][0] += g;
-                sb[k][4] += l14 * g;
-                l14b += g * s[k][4];
-            } else if (a == 2) {
-                sb[k][t1] += g;
-            } else if (a == 3) {
-                sb[k][t2] += g;
-            } else {
-                sb[k][un] += l4un * g;
-                sb[k][4] += l44 * g;
-                l4unb += g * s[k][un];
-                l44b += g * s[k][4];
-            }
-        }
-    }
-    // eigenvector entries as functions of the average density and c
-    double rhob = -l0unb / (2.0 * c) + l4unb / (2.0 * c) + run0b * c / (rho * rho) - run4b * c / (rho * rho);
-    const double c3 = c * c * c;
-    const double cb = l0unb * rho / (2.0 * c * c) - l4unb * rho / (2.0 * c * c) - (l04b + l44b) / c3 +
-                      2.0 * l14b / c3 - run0b / rho + run4b / rho + 2.0 * c * r40b;
-    double pb = 0.0;
-    sound_speed_vjp(e, rho, avp, cb, rhob, pb);
-    sb[2][0] += 0.5 * rhob;
-    sb[3][0] += 0.5 * rhob;
-    sb[2][4] += 0.5 * pb;
-    sb[3][4] += 0.5 * pb;
+// Characteristic projection at a midpoint (rows of L at the face-average
+// state, sparse): the coefficients of system.rs:232-258 as the forward
+// midpoint flux computes them.
+struct Eig {
+    double rho, avp, c, l0un, l04, l14, l4un, l44;
+};
+__device__ __forceinline__ Eig eig_at(const Eos& e, double rho2, double rho3, double p2, double p3) {
+    Eig g;
+    g.rho = 0.5 * (rho2 + rho3);
+    g.avp = 0.5 * (p2 + p3);
+    g.c = sound_speed(e, g.rho, g.avp);
+    g.l0un = -g.rho / (2.0 * g.c);
+    g.l04 = 1.0 / (2.0 * g.c * g.c);
+    g.l14 = -1.0 / (g.c * g.c);
+    g.l4un = g.rho / (2.0 * g.c);
+    g.l44 = 1.0 / (2.0 * g.c * g.c);
+    return g;
 }
-
 // ------------------------------------------------------------------ kernel
 
 // Pencil group `grp` (P pencils) of one axis VJP described by `a`; every
@@ -483,7 +380,10 @@ __device__ __forceinline__ void adj_group(const AdjArgs& a, long long grp, doubl
     const int tid = threadIdx.x;
     double* sw = lds;                 // [5][C][P] primitives
     double* sacc = sw + 5 * C * P;    // [5][C][P] cotangents (primitive, then conserved)
-    double* smf = sacc + 5 * C * P;   // [5][M][P] midpoint fluxes, then face cotangents
+    double* sx = sacc + 5 * C * P;    // [3][C][P] cotangent parts of fields 1 and 4 (see D3)
+    double* smf = sx + 3 * C * P;     // [5][M][P] midpoint fluxes, then face cotangents
+    double* scc = smf + 5 * M * P;    // [18][M][P] cl, cr (B1); clb, crb, partials, fallback flag (D2, D3);
+                                      // 13..17: rows of L (B0)
     auto W = [&](double* arr, int q, int c, int p, int ext) -> double& { return arr[(q * ext + c) * P + p]; };
     const long long npen = a.ncell / n;
     const long long pen0 = grp * P;
@@ -525,19 +425,98 @@ __device__ __forceinline__ void adj_group(const AdjArgs& a, long long grp, doubl
             W(sw, q, c, p, C) = w[q];
             W(sacc, q, c, p, C) = 0.0;
         }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) W(sx, q, c, p, C) = 0.0;
     }
     __syncthreads();
-    // ---- B. forward midpoint fluxes (midpoint m between cells m+2 and m+3)
+    constexpr int t1 = 1 + (D + 1) % 3;
+    constexpr int t2 = 1 + (D + 2) % 3;
+    // characteristic field a of cell c: row a of L (sparse) at the midpoint's
+    // face-average state, whose nonzero coefficients are (la, lb)
+    //   a = 0: l0un u_n + l04 p,  1: rho + l14 p,  2: u_t1,  3: u_t2,
+    //   a = 4: l4un u_n + l44 p
+    auto chv = [&](int a, double la, double lb, int c, int p) -> double {
+        switch (a) {
+            case 0: return la * W(sw, un, c, p, C) + lb * W(sw, 4, c, p, C);
+            case 1: return W(sw, 0, c, p, C) + lb * W(sw, 4, c, p, C);
+            case 2: return W(sw, t1, c, p, C);
+            case 3: return W(sw, t2, c, p, C);
+            default: return la * W(sw, un, c, p, C) + lb * W(sw, 4, c, p, C);
+        }
+    };
+    // (la, lb) of field a at midpoint m, from the rows stored in B0
+    auto lrow = [&](int a, int m, int p, double& la, double& lb) {
+        la = a == 0 ? W(scc, 13, m, p, M) : a == 4 ? W(scc, 16, m, p, M) : 0.0;
+        lb = a == 0 ? W(scc, 14, m, p, M) : a == 1 ? W(scc, 15, m, p, M) : a == 4 ? W(scc, 17, m, p, M) : 0.0;
+    };
+    auto eig_of = [&](int m, int p) -> Eig {
+        return eig_at(e, W(sw, 0, m + 2, p, C), W(sw, 0, m + 3, p, C), W(sw, 4, m + 2, p, C), W(sw, 4, m + 3, p, C));
+    };
+    // left / right reconstructions of midpoint m: back-projection w = R c,
+    // first-order fallback on an inadmissible state (scheme.rs:310-331)
+    auto recon = [&](const Eig& g, int m, int p, double* wl, double* wr) -> bool {
+        const double run0 = -g.c / g.rho, run4 = g.c / g.rho, r40 = g.c * g.c;
+        double cl[5], cr[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            cl[q] = W(scc, q, m, p, M);
+            cr[q] = W(scc, 5 + q, m, p, M);
+        }
+        wl[0] = cl[0] + cl[1] + cl[4];
+        wr[0] = cr[0] + cr[1] + cr[4];
+        wl[un] = run0 * cl[0] + run4 * cl[4];
+        wr[un] = run0 * cr[0] + run4 * cr[4];
+        wl[t1] = cl[2];
+        wr[t1] = cr[2];
+        wl[t2] = cl[3];
+        wr[t2] = cr[3];
+        wl[4] = r40 * cl[0] + r40 * cl[4];
+        wr[4] = r40 * cr[0] + r40 * cr[4];
+        if (!admissible(e, wl[0], wl[4]) || !admissible(e, wr[0], wr[4])) {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) {
+                wl[q] = W(sw, q, m + 2, p, C);
+                wr[q] = W(sw, q, m + 3, p, C);
+            }
+            return true;
+        }
+        return false;
+    };
+    // ---- B. forward midpoint fluxes (midpoint m between cells m+2 and m+3),
+    // the same operations as the forward pass (midpoint_flux).
+    // B0: the rows of L per midpoint (once: divisions and a square root)
     for (int it = tid; it < M * P; it += NT) {
         const int p = it % P, m = it / P;
         if (p >= Pv) continue;
-        double s[6][5];
-#pragma unroll
-        for (int j = 0; j < 6; ++j)
-#pragma unroll
-            for (int q = 0; q < 5; ++q) s[j][q] = W(sw, q, m + j, p, C);
-        double f[5];
-        midpoint_flux<D>(e, s, f);
+        const Eig g = eig_of(m, p);
+        W(scc, 13, m, p, M) = g.l0un;
+        W(scc, 14, m, p, M) = g.l04;
+        W(scc, 15, m, p, M) = g.l14;
+        W(scc, 16, m, p, M) = g.l4un;
+        W(scc, 17, m, p, M) = g.l44;
+    }
+    __syncthreads();
+    // B1: WENO5-Z reconstructions, one work item per (midpoint, field)
+    for (int it = tid; it < 5 * M * P; it += NT) {
+        const int p = it % P, r = it / P, fa = r % 5, m = r / 5;
+        if (p >= Pv) continue;
+        double la, lb;
+        lrow(fa, m, p, la, lb);
+        const double v0 = chv(fa, la, lb, m, p), v1 = chv(fa, la, lb, m + 1, p), v2 = chv(fa, la, lb, m + 2, p);
+        const double v3 = chv(fa, la, lb, m + 3, p), v4 = chv(fa, la, lb, m + 4, p);
+        const double v5 = chv(fa, la, lb, m + 5, p);
+        W(scc, fa, m, p, M) = weno5z_interp(v0, v1, v2, v3, v4);
+        W(scc, 5 + fa, m, p, M) = weno5z_interp(v5, v4, v3, v2, v1);
+    }
+    __syncthreads();
+    // B2: back-projection, fallback, HLLC per midpoint
+    for (int it = tid; it < M * P; it += NT) {
+        const int p = it % P, m = it / P;
+        if (p >= Pv) continue;
+        const Eig g = eig_of(m, p);
+        double wl[5], wr[5], f[5];
+        recon(g, m, p, wl, wr);
+        hllc<D>(e, wl, wr, f);
 #pragma unroll
         for (int q = 0; q < 5; ++q) W(smf, q, m, p, M) = f[q];
     }
@@ -731,43 +710,167 @@ __device__ __forceinline__ void adj_group(const AdjArgs& a, long long grp, doubl
         }
         __syncthreads();
     }
-    // ---- D. midpoint cotangents and VJPs (chunks; six scatter sub-phases)
-    for (int m0 = 0; m0 < M * P; m0 += NT) {
-        const int it = m0 + tid;
+    // ---- D2. per midpoint: cotangent from its five faces (S6 weights), the
+    // HLLC VJP and the back-projection VJP -> clb, crb (scc 0..9) and the
+    // eigenvector partials (scc 10, 11); on the fallback the state
+    // cotangents of cells m+2, m+3 instead (flag scc 12)
+    for (int it = tid; it < M * P; it += NT) {
         const int p = it % P, m = it / P;
-        const bool live = it < M * P && p < Pv;
-        double sb[6][5];
+        if (p >= Pv) continue;
+        double mb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        const double w6[5] = {S6_2, S6_1, S6_0, S6_1, S6_2};
 #pragma unroll
-        for (int j = 0; j < 6; ++j)
+        for (int j = 0; j < 5; ++j) {
+            const int f = m - 4 + j;  // faces f = m-4 .. m use midpoint m with weight S6[|m - f - 2|]
+            if (f < 0 || f > n) continue;
 #pragma unroll
-            for (int q = 0; q < 5; ++q) sb[j][q] = 0.0;
-        if (live) {
-            // faces f = m-4 .. m use midpoint m with weight S6[|m - f - 2|]
-            double mb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-            const double w6[5] = {S6_2, S6_1, S6_0, S6_1, S6_2};
-#pragma unroll
-            for (int j = 0; j < 5; ++j) {
-                const int f = m - 4 + j;
-                if (f < 0 || f > n) continue;
-#pragma unroll
-                for (int q = 0; q < 5; ++q) mb[q] += w6[4 - j] * W(smf, q, f, p, M);
-            }
-            double s[6][5];
-#pragma unroll
-            for (int j = 0; j < 6; ++j)
-#pragma unroll
-                for (int q = 0; q < 5; ++q) s[j][q] = W(sw, q, m + j, p, C);
-            midpoint_vjp<D>(e, s, mb, sb);
+            for (int q = 0; q < 5; ++q) mb[q] += w6[4 - j] * W(smf, q, f, p, M);
         }
-        // cell c takes midpoints c-5 .. c in increasing order (j from 5 down
-        // to 0), within a chunk as across chunks: independent of P
+        const Eig g = eig_of(m, p);
+        double wl[5], wr[5], wlb[5], wrb[5];
+        const bool fb = recon(g, m, p, wl, wr);
+        hllc_vjp<D>(e, wl, wr, mb, wlb, wrb);
+        if (fb) {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) {
+                W(scc, q, m, p, M) = wlb[q];
+                W(scc, 5 + q, m, p, M) = wrb[q];
+            }
+            W(scc, 10, m, p, M) = 0.0;
+            W(scc, 11, m, p, M) = 0.0;
+            W(scc, 12, m, p, M) = 1.0;
+            continue;
+        }
+        double cl[5], cr[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            cl[q] = W(scc, q, m, p, M);
+            cr[q] = W(scc, 5 + q, m, p, M);
+        }
+        const double run0 = -g.c / g.rho, run4 = g.c / g.rho, r40 = g.c * g.c;
+        const double run0b = wlb[un] * cl[0] + wrb[un] * cr[0];
+        const double run4b = wlb[un] * cl[4] + wrb[un] * cr[4];
+        const double r40b = wlb[4] * (cl[0] + cl[4]) + wrb[4] * (cr[0] + cr[4]);
+        W(scc, 0, m, p, M) = wlb[0] + run0 * wlb[un] + r40 * wlb[4];
+        W(scc, 5, m, p, M) = wrb[0] + run0 * wrb[un] + r40 * wrb[4];
+        W(scc, 1, m, p, M) = wlb[0];
+        W(scc, 6, m, p, M) = wrb[0];
+        W(scc, 2, m, p, M) = wlb[t1];
+        W(scc, 7, m, p, M) = wrb[t1];
+        W(scc, 3, m, p, M) = wlb[t2];
+        W(scc, 8, m, p, M) = wrb[t2];
+        W(scc, 4, m, p, M) = wlb[0] + run4 * wlb[un] + r40 * wlb[4];
+        W(scc, 9, m, p, M) = wrb[0] + run4 * wrb[un] + r40 * wrb[4];
+        // run0 = -c / rho, run4 = c / rho, r40 = c^2 as functions of rho, c
+        W(scc, 10, m, p, M) = run0b * g.c / (g.rho * g.rho) - run4b * g.c / (g.rho * g.rho);
+        W(scc, 11, m, p, M) = -run0b / g.rho + run4b / g.rho + 2.0 * g.c * r40b;
+        W(scc, 12, m, p, M) = 0.0;
+    }
+    __syncthreads();
+    // ---- D3. per (midpoint, field): the WENO5-Z VJPs of both sides and the
+    // projection ch = L s (the coefficient sums left in scc 0..9 for D4),
+    // scattered to the six stencil cells in six
+    // conflict-free sub-phases.  Fields 1 and 4 keep their shares of the
+    // pressure (and field 4 of the normal velocity) cotangent apart in sx, so
+    // that no two fields add to one slot in a sub-phase.  A cell takes
+    // midpoints c-5 .. c in increasing order, within a chunk as across
+    // chunks, so the sums do not depend on P.
+    // Work items run field-major, so that a wave mostly holds one field of
+    // neighbouring midpoints: flat stencils (e.g. every stencil of a
+    // velocity component that is zero) then take weno_vjp's short path for
+    // whole waves.
+    for (int i0 = 0; i0 < 5 * M * P; i0 += NT) {
+        const int it = i0 + tid;
+        const int p = it % P, r = it / P, m = r % M, fa = r / M;
+        const bool live = it < 5 * M * P && p < Pv && W(scc, 12, m, p, M) == 0.0;
+        double chb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        double la = 0.0, lb = 0.0;
+        if (live) {
+            lrow(fa, m, p, la, lb);
+            double v[5], vb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int k = 0; k < 5; ++k) v[k] = chv(fa, la, lb, m + k, p);
+            weno_vjp(v, W(scc, fa, m, p, M), chb);
+            const double v5 = chv(fa, la, lb, m + 5, p);
+            double vr[5] = {v5, v[4], v[3], v[2], v[1]};
+            weno_vjp(vr, W(scc, 5 + fa, m, p, M), vb);
+#pragma unroll
+            for (int k = 0; k < 5; ++k) chb[5 - k] += vb[k];
+            // sums for the coefficient cotangents: d la = sum chb u_n,
+            // d lb = sum chb p (fields 0, 1, 4)
+            double sa = 0.0, sbp = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                sa += chb[k] * W(sw, un, m + k, p, C);
+                sbp += chb[k] * W(sw, 4, m + k, p, C);
+            }
+            W(scc, fa, m, p, M) = sa;
+            W(scc, 5 + fa, m, p, M) = sbp;
+        }
         for (int j = 5; j >= 0; --j) {
             if (live) {
-#pragma unroll
-                for (int q = 0; q < 5; ++q) W(sacc, q, m + j, p, C) += sb[j][q];
+                const double gj = chb[j];
+                const int c = m + j;
+                if (fa == 0) {
+                    W(sacc, un, c, p, C) += la * gj;
+                    W(sacc, 4, c, p, C) += lb * gj;
+                } else if (fa == 1) {
+                    W(sacc, 0, c, p, C) += gj;
+                    W(sx, 0, c, p, C) += lb * gj;
+                } else if (fa == 2) {
+                    W(sacc, t1, c, p, C) += gj;
+                } else if (fa == 3) {
+                    W(sacc, t2, c, p, C) += gj;
+                } else {
+                    W(sx, 1, c, p, C) += la * gj;
+                    W(sx, 2, c, p, C) += lb * gj;
+                }
             }
             __syncthreads();
         }
+    }
+    // ---- D4. per midpoint: the average state's cotangent (rho and p of
+    // cells m+2, m+3 through rho, c), or the fallback's state cotangents;
+    // a cell takes midpoint c-3 (as m+3) before c-2 (as m+2)
+    for (int i0 = 0; i0 < M * P; i0 += NT) {
+        const int it = i0 + tid;
+        const int p = it % P, m = it / P;
+        const bool live = it < M * P && p < Pv;
+        double b2[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, b3[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        if (live) {
+            if (W(scc, 12, m, p, M) != 0.0) {
+#pragma unroll
+                for (int q = 0; q < 5; ++q) {
+                    b2[q] = W(scc, q, m, p, M);
+                    b3[q] = W(scc, 5 + q, m, p, M);
+                }
+            } else {
+                const Eig g = eig_of(m, p);
+                // L coefficients as functions of rho and c: l0un = -rho/(2c),
+                // l04 = l44 = 1/(2c^2), l14 = -1/c^2, l4un = rho/(2c)
+                const double l0unb = W(scc, 0, m, p, M), l04b = W(scc, 5, m, p, M);
+                const double l14b = W(scc, 6, m, p, M);
+                const double l4unb = W(scc, 4, m, p, M), l44b = W(scc, 9, m, p, M);
+                const double c3 = g.c * g.c * g.c;
+                double rhob = W(scc, 10, m, p, M) - l0unb / (2.0 * g.c) + l4unb / (2.0 * g.c);
+                double cb = W(scc, 11, m, p, M) + l0unb * g.rho / (2.0 * g.c * g.c) -
+                            l4unb * g.rho / (2.0 * g.c * g.c) - (l04b + l44b) / c3 + 2.0 * l14b / c3;
+                double pb = 0.0;
+                sound_speed_vjp(e, g.rho, g.avp, cb, rhob, pb);
+                b2[0] = b3[0] = 0.5 * rhob;
+                b2[4] = b3[4] = 0.5 * pb;
+            }
+        }
+        if (live) {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) W(sacc, q, m + 3, p, C) += b3[q];
+        }
+        __syncthreads();
+        if (live) {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) W(sacc, q, m + 2, p, C) += b2[q];
+        }
+        __syncthreads();
     }
     // ---- E. conserved cotangents of every loaded cell
     for (int it = tid; it < C * P; it += NT) {
@@ -779,6 +882,8 @@ __device__ __forceinline__ void adj_group(const AdjArgs& a, long long grp, doubl
             w[q] = W(sw, q, c, p, C);
             wb[q] = W(sacc, q, c, p, C);
         }
+        wb[un] += W(sx, 1, c, p, C);
+        wb[4] += W(sx, 0, c, p, C) + W(sx, 2, c, p, C);
         to_prim_vjp(e, w, wb, ub);
 #pragma unroll
         for (int q = 0; q < 5; ++q) W(sacc, q, c, p, C) = ub[q];
@@ -826,7 +931,7 @@ __device__ __forceinline__ void adj_group(const AdjArgs& a, long long grp, doubl
 }
 
 template <int D>
-__global__ __launch_bounds__(256) void adj_pass(AdjArgs a) {
+__global__ __launch_bounds__(kAdjThreads) void adj_pass(AdjArgs a) {
     extern __shared__ double lds[];
     adj_group<D>(a, blockIdx.x, lds);
 }
@@ -834,7 +939,7 @@ __global__ __launch_bounds__(256) void adj_pass(AdjArgs a) {
 // Ensemble VJP: the pencil groups of every instance in one persistent
 // launch (instance i owns work items prefix[i] .. prefix[i+1]-1).
 template <int D>
-__global__ __launch_bounds__(256) void ens_adj_pass(EnsAdjArgs e) {
+__global__ __launch_bounds__(kAdjThreads) void ens_adj_pass(EnsAdjArgs e) {
     extern __shared__ double lds[];
     for (long long w = blockIdx.x; w < e.total; w += gridDim.x) {
         int lo = 0, hi = e.ninst;
@@ -852,35 +957,57 @@ __global__ __launch_bounds__(256) void ens_adj_pass(EnsAdjArgs e) {
 
 }  // namespace adj
 
+template <class K>
+static hipError_t allow_lds(K kernel, size_t lds) {
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds);
+}
+
 hipError_t launch_ens_adj_pass(int d, const EnsAdjArgs& e, size_t lds, unsigned grid, hipStream_t s) {
-    if (d == 0)
-        hipLaunchKernelGGL(adj::ens_adj_pass<0>, dim3(grid), dim3(256), lds, s, e);
-    else if (d == 1)
-        hipLaunchKernelGGL(adj::ens_adj_pass<1>, dim3(grid), dim3(256), lds, s, e);
-    else
-        hipLaunchKernelGGL(adj::ens_adj_pass<2>, dim3(grid), dim3(256), lds, s, e);
+    hipError_t er;
+    if (d == 0) {
+        if ((er = allow_lds(adj::ens_adj_pass<0>, lds)) != hipSuccess) return er;
+        hipLaunchKernelGGL(adj::ens_adj_pass<0>, dim3(grid), dim3(kAdjThreads), lds, s, e);
+    } else if (d == 1) {
+        if ((er = allow_lds(adj::ens_adj_pass<1>, lds)) != hipSuccess) return er;
+        hipLaunchKernelGGL(adj::ens_adj_pass<1>, dim3(grid), dim3(kAdjThreads), lds, s, e);
+    } else {
+        if ((er = allow_lds(adj::ens_adj_pass<2>, lds)) != hipSuccess) return er;
+        hipLaunchKernelGGL(adj::ens_adj_pass<2>, dim3(grid), dim3(kAdjThreads), lds, s, e);
+    }
     return hipGetLastError();
 }
 
-int adj_pencils_per_group(int n) {
-    const long long per = 8ll * (15ll * (n + 10));  // bytes of LDS per pencil (upper bound)
-    int P = (int)std::max(1ll, std::min(8ll, (64ll << 10) / per));
-    return P;
+// LDS per pencil: 13 arrays over the n + 10 loaded cells (primitives,
+// cotangents, field shares) and 23 over the n + 5 midpoints (fluxes / face
+// cotangents, reconstructions / cotangents / partials / flag, rows of L)
+size_t adj_lds_bytes(int n, int P) {
+    return sizeof(double) * (size_t)P * (13 * (size_t)(n + 10) + 23 * (size_t)(n + 5));
 }
 
-size_t adj_lds_bytes(int n, int P) { return sizeof(double) * (size_t)P * (10 * (size_t)(n + 10) + 5 * (size_t)(n + 5)); }
+// Pencils per workgroup: as many as fit 152 KB of LDS (one workgroup per
+// CU), at most 16.
+int adj_pencils_per_group(int n) {
+    const long long per = (long long)adj_lds_bytes(n, 1);
+    return (int)std::max(1ll, std::min(16ll, (152ll << 10) / per));
+}
 
 hipError_t launch_adj_pass(int d, const AdjArgs& a, hipStream_t s) {
     const int n = a.n[d];
     const long long npen = a.ncell / n;
     const unsigned grid = (unsigned)((npen + a.P - 1) / a.P);
     const size_t lds = adj_lds_bytes(n, a.P);
-    if (d == 0)
-        hipLaunchKernelGGL(adj::adj_pass<0>, dim3(grid), dim3(256), lds, s, a);
-    else if (d == 1)
-        hipLaunchKernelGGL(adj::adj_pass<1>, dim3(grid), dim3(256), lds, s, a);
-    else
-        hipLaunchKernelGGL(adj::adj_pass<2>, dim3(grid), dim3(256), lds, s, a);
+    hipError_t er;
+    if (d == 0) {
+        if ((er = allow_lds(adj::adj_pass<0>, lds)) != hipSuccess) return er;
+        hipLaunchKernelGGL(adj::adj_pass<0>, dim3(grid), dim3(kAdjThreads), lds, s, a);
+    } else if (d == 1) {
+        if ((er = allow_lds(adj::adj_pass<1>, lds)) != hipSuccess) return er;
+        hipLaunchKernelGGL(adj::adj_pass<1>, dim3(grid), dim3(kAdjThreads), lds, s, a);
+    } else {
+        if ((er = allow_lds(adj::adj_pass<2>, lds)) != hipSuccess) return er;
+        hipLaunchKernelGGL(adj::adj_pass<2>, dim3(grid), dim3(kAdjThreads), lds, s, a);
+    }
     return hipGetLastError();
 }
 

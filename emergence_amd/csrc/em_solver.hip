@@ -498,10 +498,14 @@ struct Solver {
         return f;
     }
 
-    void step(double dt) {
+    // one step with the given dt, queued on the stream (no host wait)
+    void step_async(double dt) {
         reset_run(0.0);
         hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, stream, sc, dt);
         step_kernels();
+    }
+    void step(double dt) {
+        step_async(dt);
         HIPCHK(hipStreamSynchronize(stream));
     }
 
@@ -667,7 +671,7 @@ struct Solver {
                 if (bc_kind[d][sd] == BC_REFLECT && n[d] < 5)
                     throw Error(EM_ERR_ARG, "the adjoint needs at least 5 cells on a reflecting axis");
             }
-            if (adj_lds_bytes((int)n[d], 1) > (size_t)prop.sharedMemPerBlock)
+            if (adj_lds_bytes((int)n[d], 1) > std::max(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin))
                 throw Error(EM_ERR_ARG, "the adjoint keeps whole pencils in LDS: axis too long");
         }
     }
@@ -764,7 +768,7 @@ struct Solver {
             const double dt = dts ? dts[s] : dt_cfl();
             if (!(std::isfinite(dt) && dt > 0.0)) throw Error(EM_ERR_COLLAPSE, "time step collapsed");
             fwd_dts[s] = dt;
-            step(dt);
+            step_async(dt);  // dt_cfl waits for the stream before the next step
         }
         HIPCHK(hipMalloc(&final_state, sizeof(double) * 5 * ncell));
         HIPCHK(hipMemcpyAsync(final_state, U[0], sizeof(double) * 5 * ncell, hipMemcpyDeviceToDevice, stream));
@@ -799,7 +803,7 @@ struct Solver {
             for (int64_t s = s0; s < s1; ++s) {
                 double* st = store + 3 * (s - s0) * nv;
                 HIPCHK(hipMemcpyAsync(st, U[0], sz, hipMemcpyDeviceToDevice, stream));
-                step(fwd_dts[s]);
+                step_async(fwd_dts[s]);
                 HIPCHK(hipMemcpyAsync(st + nv, U[1], sz, hipMemcpyDeviceToDevice, stream));
                 HIPCHK(hipMemcpyAsync(st + 2 * nv, U[2], sz, hipMemcpyDeviceToDevice, stream));
             }
@@ -1176,13 +1180,15 @@ struct Ensemble {
         double* Zb = work + 2 * tot_nv;
         double* T = work + 3 * tot_nv;
         for (int i = 0; i < n; ++i) in[i]->host_to_soa(dj_final[i], A + off[i]);
-        // per-instance pencils per group and LDS: the largest axis decides
-        int maxn = 1;
+        // pencils per group of each instance and axis (its own launch shape;
+        // the sums do not depend on it); LDS per axis: the largest need
+        size_t lds_ax[3] = {0, 0, 0};
         for (auto* S : in)
             for (int d = 0; d < 3; ++d)
-                if (S->active(d)) maxn = std::max(maxn, (int)S->n[d]);
-        const int P = adj_pencils_per_group(maxn);
-        const size_t lds = adj_lds_bytes(maxn, P);
+                if (S->active(d)) {
+                    const int nd = (int)S->n[d];
+                    lds_ax[d] = std::max(lds_ax[d], adj_lds_bytes(nd, adj_pencils_per_group(nd)));
+                }
         AdjArgs* d_aa = nullptr;
         long long* d_pre = nullptr;
         AdjCombineTask* d_ct = nullptr;
@@ -1196,6 +1202,7 @@ struct Ensemble {
         for (int d = 0; d < 3; ++d)
             for (int i = 0; i < n; ++i) {
                 const Solver& S = *in[i];
+                const int P = S.active(d) ? adj_pencils_per_group((int)S.n[d]) : 1;
                 const long long groups = S.active(d) ? (S.ncell / S.n[d] + P - 1) / P : 0;
                 pre[d * (n + 1) + i + 1] = pre[d * (n + 1) + i] + groups;
             }
@@ -1242,7 +1249,7 @@ struct Ensemble {
                             for (int e = 0; e < 3; ++e)
                                 if (S.active(e) && first_ax < 0) first_ax = e;
                             a.first = d == first_ax ? 1 : 0;
-                            a.P = P;
+                            a.P = adj_pencils_per_group((int)S.n[d]);
                         }
             HIPCHK(hipMemcpyAsync(d_aa, aa.data(), sizeof(AdjArgs) * aa.size(), hipMemcpyHostToDevice, stream));
             for (int64_t q = s1 - 1; q >= s0; --q) {
@@ -1259,7 +1266,7 @@ struct Ensemble {
                         if (tot == 0) continue;
                         EnsAdjArgs e{d_aa + (size_t)(q - s0) * per_step + (size_t)(stg * 3 + d) * n,
                                      d_pre + d * (n + 1), n, tot};
-                        HIPCHK(launch_ens_adj_pass(d, e, lds, (unsigned)std::min<long long>(tot, 2048), stream));
+                        HIPCHK(launch_ens_adj_pass(d, e, lds_ax[d], (unsigned)std::min<long long>(tot, 2048), stream));
                     }
                     HIPCHK(launch_ens_adj_combine(3 - stg, d_ct + stg * n, n, dt_hist, max_nv, stream));
                 }
