@@ -75,6 +75,27 @@ struct MapEdits {
 __global__ void k_amr_map_edit(int* map, MapEdits e) {
     if (threadIdx.x < e.n) map[e.off[threadIdx.x]] = e.val[threadIdx.x];
 }
+// a list of index edits with distinct entries
+__global__ void k_amr_map_set(int* map, const long long* __restrict__ off, const int* __restrict__ val, int n) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) map[off[i]] = val[i];
+}
+
+// pool slots src[i] -> tmp block i, and tmp block i -> slot dst[i] (a
+// permutation of blocks applied through a staging buffer)
+__global__ void k_amr_blocks_out(AmrDev a, const int* __restrict__ src, int n, double* __restrict__ tmp) {
+    const long long tot = (long long)n * a.bcells;
+    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < tot; i += (long long)gridDim.x * 256ll) {
+        const long long b = i / a.bcells, c = i % a.bcells;
+        for (int q = 0; q < 5; ++q) tmp[(b * 5 + q) * a.bcells + c] = a.U[amr_u(a, q, src[b], c)];
+    }
+}
+__global__ void k_amr_blocks_in(AmrDev a, const int* __restrict__ dst, int n, const double* __restrict__ tmp) {
+    const long long tot = (long long)n * a.bcells;
+    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < tot; i += (long long)gridDim.x * 256ll) {
+        const long long b = i / a.bcells, c = i % a.bcells;
+        for (int q = 0; q < 5; ++q) a.U[amr_u(a, q, dst[b], c)] = tmp[(b * 5 + q) * a.bcells + c];
+    }
+}
 
 // staged host data (per block AoS: bcells x 5) -> pool slots
 __global__ void k_amr_scatter(AmrDev a, const double* __restrict__ stage, const int* __restrict__ slots, int n) {
@@ -156,10 +177,18 @@ __global__ __launch_bounds__(256) void k_amr_rate(AmrDev a, const int* __restric
     }
 }
 
+// Threads per workgroup of the patch kernels (restriction, prediction,
+// ghost slabs): their gathers are chains of dependent loads (block map, then
+// data), latency-bound at a few hundred workgroups, so a workgroup keeps
+// sixteen waves of them in flight.
+constexpr int kPatchThreads = 1024;
+constexpr int kGhostThreads = 256;  // the slab prediction needs more registers
+
 // Ghost slabs of the leaves, one task per workgroup (persistent): same-level
 // copy, physical boundary (bc_box on a face slab), or 6th-order prediction
 // (fill_scratch, amr.rs:1113-1221).
-__global__ __launch_bounds__(256) void k_amr_ghost(AmrDev a, AmrWeights W, const GhostTask* __restrict__ tasks, int n,
+
+__global__ __launch_bounds__(kGhostThreads) void k_amr_ghost(AmrDev a, AmrWeights W, const GhostTask* __restrict__ tasks, int n,
                                                   double* scratch, long long cap) {
     double* scr = scratch + (long long)blockIdx.x * 12 * cap;
     for (int ti = blockIdx.x; ti < n; ti += gridDim.x) {
@@ -236,7 +265,7 @@ __global__ __launch_bounds__(256) void k_amr_ghost(AmrDev a, AmrWeights W, const
 // the whole block (restrict_all's boxes, amr.rs:760-800).  Values do not
 // depend on the box they are computed in, so the whole block is computed and
 // only the selected cells are stored.
-__global__ __launch_bounds__(256) void k_amr_restrict(AmrDev a, AmrWeights W, const int* __restrict__ ids,
+__global__ __launch_bounds__(kPatchThreads) void k_amr_restrict(AmrDev a, AmrWeights W, const int* __restrict__ ids,
                                                      const int* __restrict__ ent, int n, int level, int w,
                                                      double* scratch, long long cap) {
     double* scr = scratch + (long long)blockIdx.x * 12 * cap;
@@ -246,16 +275,9 @@ __global__ __launch_bounds__(256) void k_amr_restrict(AmrDev a, AmrWeights W, co
     }
 }
 
-// Whole-block restriction of one ancestor (coarsening, amr.rs:1014-1032).
-__global__ __launch_bounds__(256) void k_amr_restrict_one(AmrDev a, AmrWeights W, int slot, int i0, int i1, int i2,
-                                                         int level, int w, double* scratch, long long cap) {
-    const int bi[3] = {i0, i1, i2};
-    restrict_block(a, W, level, slot, bi, 1 << 27, w, scratch, cap);
-}
-
 // Prediction of whole refined children from their parent level
 // (refine_block, amr.rs:989-996).
-__global__ __launch_bounds__(256) void k_amr_predict_blocks(AmrDev a, AmrWeights W, const int* __restrict__ slots,
+__global__ __launch_bounds__(kPatchThreads) void k_amr_predict_blocks(AmrDev a, AmrWeights W, const int* __restrict__ slots,
                                                            const int* __restrict__ idx, int n, int level,
                                                            double* scratch, long long cap) {
     double* scr = scratch + (long long)blockIdx.x * 12 * cap;
@@ -661,12 +683,46 @@ struct Amr {
         ed.val[ed.n] = id;
         ed.n++;
     }
+    // Index edits are collected on the host and applied in one launch before
+    // the next device use of the index (apply_edits).
+    std::vector<long long> pend_off;
+    std::vector<int> pend_val;
     void flush_edits(MapEdits& ed) {
-        if (ed.n == 0) return;
-        hipLaunchKernelGGL(k_amr_map_edit, dim3(1), dim3(64), 0, stream, d_map, ed);
-        HCHK(hipGetLastError());
+        for (int i = 0; i < ed.n; ++i) {
+            pend_off.push_back(ed.off[i]);
+            pend_val.push_back(ed.val[i]);
+        }
         ed.n = 0;
     }
+    void map_host_set(int l, const int64_t idx[3], int id) {
+        const long long s = slot(l, idx);
+        if (s >= 0) map[l][s] = id;
+    }
+    long long* d_eoff = nullptr;
+    int* d_eval = nullptr;
+    // uploads and applies an edit list; the last edit of an entry wins
+    void apply_edit_list(std::vector<long long>& off, std::vector<int>& val) {
+        if (off.empty()) return;
+        std::unordered_map<long long, size_t> last;
+        for (size_t i = 0; i < off.size(); ++i) last[off[i]] = i;
+        std::vector<long long> o;
+        std::vector<int> v;
+        for (size_t i = 0; i < off.size(); ++i)
+            if (last[off[i]] == i) {
+                o.push_back(off[i]);
+                v.push_back(val[i]);
+            }
+        HCHK(hipStreamSynchronize(stream));  // the edit buffers may be reallocated
+        d_eoff = dev_vec(o, d_eoff);
+        d_eval = dev_vec(v, d_eval);
+        const unsigned g = (unsigned)std::min<size_t>((o.size() + 255) / 256, 1024);
+        hipLaunchKernelGGL(k_amr_map_set, dim3(g), dim3(256), 0, stream, d_map, d_eoff, d_eval, (int)o.size());
+        HCHK(hipGetLastError());
+        HCHK(hipStreamSynchronize(stream));  // o, v are copied from pageable memory
+        off.clear();
+        val.clear();
+    }
+    void apply_edits() { apply_edit_list(pend_off, pend_val); }
 
     std::vector<std::array<int64_t, 3>> neighbours(int l, const int64_t idx[3]) const {  // amr.rs:919-949
         std::vector<std::array<int64_t, 3>> out;
@@ -832,6 +888,7 @@ struct Amr {
 
     void flush_predictions() {
         if (pending.empty()) return;
+        apply_edits();  // the predictions read the index
         std::stable_sort(pending.begin(), pending.end(),
                          [](const std::array<int, 5>& a, const std::array<int, 5>& b) { return a[0] < b[0]; });
         ensure_tmp((long long)pending.size() * 4);
@@ -848,7 +905,7 @@ struct Amr {
             }
             HCHK(hipMemcpyAsync(d_tmp + off, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, stream));
             const unsigned g = (unsigned)std::min(n, big_wg);
-            hipLaunchKernelGGL(k_amr_predict_blocks, dim3(g), dim3(256), 0, stream, dev, W, d_tmp + off,
+            hipLaunchKernelGGL(k_amr_predict_blocks, dim3(g), dim3(kPatchThreads), 0, stream, dev, W, d_tmp + off,
                                d_tmp + off + n, n, pending[i][0], big_scr, big_cap);
             HCHK(hipGetLastError());
             pending_host.push_back(std::move(h));
@@ -867,35 +924,182 @@ struct Amr {
         d_tmp_n = n;
     }
 
-    // coarsen (amr.rs:1014-1032): restrict the parent on the device, then
-    // swap-remove the children (ids descending) moving the last block's data
+    // coarsen (amr.rs:1014-1032), host part: the parent becomes a leaf and
+    // its children are swap-removed (ids descending), exactly as the
+    // reference orders them; the device work (restriction of the parent,
+    // index edits, moves) is recorded and replayed in batches by
+    // replay_coarsening.
+    struct CoOp {
+        int l;
+        int64_t idx[3];
+        int pid0;                       // the parent's pool slot when the coarsening phase began
+        int batch;
+        std::vector<long long> removed;  // index entries of its children
+    };
+    std::vector<CoOp> co_ops;
+    std::vector<int> orig_slot;  // phase-start pool slot of the data that belongs at slot s
+
     void coarsen(int l, const int64_t idx[3]) {
         auto kids = children(idx);
         const int pid = get(l, idx);
-        // stream-ordered after the previous coarsening's index edits and moves
-        hipLaunchKernelGGL(k_amr_restrict_one, dim3(1), dim3(256), 0, stream, dev, W, pid, (int)idx[0], (int)idx[1],
-                           (int)idx[2], l, box_w(), big_scr, big_cap);
-        HCHK(hipGetLastError());
+        CoOp op;
+        op.l = l;
+        for (int d = 0; d < 3; ++d) op.idx[d] = idx[d];
+        op.pid0 = orig_slot[pid];
+        op.batch = 0;
         blocks[pid].leaf = true;
         std::vector<int> ids;
         for (auto& c : kids) ids.push_back(get(l + 1, c.data()));
         std::sort(ids.begin(), ids.end(), [](int a, int b) { return a > b; });
-        MapEdits ed{};
         for (int i : ids) {
-            set_map(blocks[i].level, blocks[i].idx, -1, ed);
+            op.removed.push_back(map_off[blocks[i].level] + slot(blocks[i].level, blocks[i].idx));
+            map_host_set(blocks[i].level, blocks[i].idx, -1);
             const int last = (int)blocks.size() - 1;
             if (i != last) {
-                hipLaunchKernelGGL(k_amr_move, dim3((unsigned)std::max(1, bcells / 256)), dim3(256), 0, stream, dev,
-                                   last, i);
-                HCHK(hipGetLastError());
                 blocks[i] = blocks[last];
+                orig_slot[i] = orig_slot[last];
             }
             blocks.pop_back();
-            if (i < (int)blocks.size()) set_map(blocks[i].level, blocks[i].idx, i, ed);
-            if (ed.n >= 6) flush_edits(ed);
+            orig_slot.pop_back();
+            if (i < (int)blocks.size()) map_host_set(blocks[i].level, blocks[i].idx, i);
         }
-        flush_edits(ed);
+        co_ops.push_back(std::move(op));
         tables_dirty = true;
+    }
+
+    // index v of level L along d inside [lo, hi] (periodic axes wrap)
+    bool in_span(int L, int d, int64_t v, int64_t lo, int64_t hi) const {
+        if (!act[d]) return true;
+        if (v >= lo && v <= hi) return true;
+        if (bck[d][0] != BC_PERIODIC) return false;
+        const int64_t nb = nbl(L, d);
+        return (v - nb >= lo && v - nb <= hi) || (v + nb >= lo && v + nb <= hi);
+    }
+    // the restriction of k reads what coarsening j (earlier) changes: j's
+    // removed children (same level, neighbours) or j's restricted parent
+    // (one level finer, inside k's fine patch of blocks 2i-1 .. 2i+2)
+    bool co_depends(const CoOp& k, const CoOp& j) const {
+        if (j.l == k.l) {
+            for (int d = 0; d < 3; ++d)
+                if (!in_span(k.l, d, j.idx[d], k.idx[d] - 1, k.idx[d] + 1)) return false;
+            return true;
+        }
+        if (j.l == k.l + 1) {
+            for (int d = 0; d < 3; ++d)
+                if (!in_span(j.l, d, j.idx[d], 2 * k.idx[d] - 1, 2 * k.idx[d] + 2)) return false;
+            return true;
+        }
+        return false;
+    }
+
+    // Device side of the recorded coarsenings.  A restriction reads level
+    // l+1 through the index and writes its parent; it sees the same data as
+    // in the reference's sequential loop when every earlier coarsening it
+    // depends on (co_depends) has been applied and no later one it would see
+    // has.  Batch b holds the coarsenings whose dependencies lie in batches
+    // < b; its restrictions run in one launch per level against the pool as
+    // it was when the phase began, with the children of batches < b removed
+    // from the index.  The swap-remove moves are then applied at once as one
+    // block permutation, and the index entries of moved blocks rewritten.
+    double* d_co_tmp = nullptr;
+    long long d_co_tmp_n = 0;
+    int* d_co_int = nullptr;
+
+    void replay_coarsening() {
+        const int n = (int)co_ops.size();
+        if (n == 0) return;
+        int nb = 0;
+        for (int k = 0; k < n; ++k) {
+            int b = 0;
+            for (int j = 0; j < k; ++j)
+                if (co_ops[j].batch + 1 > b && co_depends(co_ops[k], co_ops[j])) b = co_ops[j].batch + 1;
+            co_ops[k].batch = b;
+            nb = std::max(nb, b + 1);
+        }
+        // int tables: per (batch, level) restriction lists (ids, then 4 ints each)
+        struct Launch {
+            int level, ids, ent, n;
+        };
+        std::vector<int> ib;
+        std::vector<std::vector<Launch>> rl(nb);
+        for (int b = 0; b < nb; ++b)
+            for (int l = levels - 1; l >= 0; --l) {
+                std::vector<int> ids, ent;
+                for (const CoOp& op : co_ops)
+                    if (op.batch == b && op.l == l) {
+                        ids.push_back(op.pid0);
+                        for (int d = 0; d < 3; ++d) ent.push_back((int)op.idx[d]);
+                        ent.push_back(1 << 27);  // the whole block
+                    }
+                if (ids.empty()) continue;
+                Launch L{l, (int)ib.size(), 0, (int)ids.size()};
+                ib.insert(ib.end(), ids.begin(), ids.end());
+                L.ent = (int)ib.size();
+                ib.insert(ib.end(), ent.begin(), ent.end());
+                rl[b].push_back(L);
+            }
+        // block permutation of the moves
+        std::vector<int> src, dst;
+        for (size_t sl = 0; sl < orig_slot.size(); ++sl)
+            if (orig_slot[sl] != (int)sl) {
+                src.push_back(orig_slot[sl]);
+                dst.push_back((int)sl);
+            }
+        const int psrc = (int)ib.size();
+        ib.insert(ib.end(), src.begin(), src.end());
+        const int pdst = (int)ib.size();
+        ib.insert(ib.end(), dst.begin(), dst.end());
+        apply_edits();  // earlier edits (refinement) first
+        HCHK(hipStreamSynchronize(stream));
+        d_co_int = dev_vec(ib, d_co_int);
+        const long long need = (long long)src.size() * 5 * bcells;
+        if (need > d_co_tmp_n) {
+            if (d_co_tmp) HCHK(hipFree(d_co_tmp));
+            HCHK(hipMalloc(&d_co_tmp, sizeof(double) * (size_t)need));
+            d_co_tmp_n = need;
+        }
+        for (int b = 0; b < nb; ++b) {
+            if (b > 0) {
+                std::vector<long long> off;
+                std::vector<int> val;
+                for (const CoOp& op : co_ops)
+                    if (op.batch == b - 1)
+                        for (long long e : op.removed) {
+                            off.push_back(e);
+                            val.push_back(-1);
+                        }
+                apply_edit_list(off, val);
+            }
+            for (const Launch& L : rl[b]) {
+                const unsigned g = (unsigned)std::min(L.n, big_wg);
+                hipLaunchKernelGGL(k_amr_restrict, dim3(g), dim3(kPatchThreads), 0, stream, dev, W, d_co_int + L.ids,
+                                   d_co_int + L.ent, L.n, L.level, box_w(), big_scr, big_cap);
+                HCHK(hipGetLastError());
+            }
+        }
+        // the last batch's removals, the moves, and the moved blocks' entries
+        std::vector<long long> off;
+        std::vector<int> val;
+        for (const CoOp& op : co_ops)
+            if (op.batch == nb - 1)
+                for (long long e : op.removed) {
+                    off.push_back(e);
+                    val.push_back(-1);
+                }
+        if (!src.empty()) {
+            const int np = (int)src.size();
+            const unsigned g = (unsigned)std::min<long long>(((long long)np * bcells + 255) / 256, 4096);
+            hipLaunchKernelGGL(k_amr_blocks_out, dim3(g), dim3(256), 0, stream, dev, d_co_int + psrc, np, d_co_tmp);
+            HCHK(hipGetLastError());
+            hipLaunchKernelGGL(k_amr_blocks_in, dim3(g), dim3(256), 0, stream, dev, d_co_int + pdst, np, d_co_tmp);
+            HCHK(hipGetLastError());
+            for (int sl : dst) {
+                off.push_back(map_off[blocks[sl].level] + slot(blocks[sl].level, blocks[sl].idx));
+                val.push_back(sl);
+            }
+        }
+        apply_edit_list(off, val);
+        co_ops.clear();
     }
 
     std::vector<int8_t> flags_from_details(const std::vector<double>& det) const {  // amr.rs:877-892
@@ -964,6 +1168,9 @@ struct Amr {
             if (ok) cand.push_back(Key{b.level, {b.idx[0], b.idx[1], b.idx[2]}});
         }
         std::stable_sort(cand.begin(), cand.end(), [](const Key& a, const Key& b) { return b.l < a.l; });
+        orig_slot.resize(blocks.size());
+        for (size_t i = 0; i < orig_slot.size(); ++i) orig_slot[i] = (int)i;
+        co_ops.clear();
         for (const Key& k : cand) {
             bool ok = true;
             for (auto& nk : neighbours(k.l, k.i))
@@ -973,10 +1180,12 @@ struct Amr {
                 }
             if (ok && get(k.l, k.i) >= 0) coarsen(k.l, k.i);
         }
+        replay_coarsening();
     }
 
     // -------------------------------------------------------------- tables
     void build_tables() {
+        apply_edits();
         HCHK(hipStreamSynchronize(stream));  // tables in use by queued work are freed below
         leaves.clear();
         leaf_level.clear();
@@ -1114,6 +1323,7 @@ struct Amr {
 
     void tables() {
         if (tables_dirty) build_tables();
+        apply_edits();
     }
 
     // width of restrict_all's edge boxes (amr.rs:782)
@@ -1172,11 +1382,11 @@ struct Amr {
             if (!n) continue;
             if (batched) {
                 const unsigned g = (unsigned)std::min(n, big_wg);
-                hipLaunchKernelGGL(k_amr_restrict, dim3(g), dim3(256), 0, stream, dev, W, d_edge_ids[l], d_edge_idx[l],
+                hipLaunchKernelGGL(k_amr_restrict, dim3(g), dim3(kPatchThreads), 0, stream, dev, W, d_edge_ids[l], d_edge_idx[l],
                                    n, l, box_w(), big_scr, big_cap);
             } else {
                 for (int i = 0; i < n; ++i)
-                    hipLaunchKernelGGL(k_amr_restrict, dim3(1), dim3(256), 0, stream, dev, W, d_edge_ids[l] + i,
+                    hipLaunchKernelGGL(k_amr_restrict, dim3(1), dim3(kPatchThreads), 0, stream, dev, W, d_edge_ids[l] + i,
                                        d_edge_idx[l] + 4 * i, 1, l, box_w(), big_scr, big_cap);
             }
             HCHK(hipGetLastError());
@@ -1188,13 +1398,13 @@ struct Amr {
         if (!n) return;
         if (batched) {
             const unsigned g = (unsigned)std::min(n, small_wg);
-            hipLaunchKernelGGL(k_amr_ghost, dim3(g), dim3(256), 0, stream, dev, W, d_ghost, n, small_scr, small_cap);
+            hipLaunchKernelGGL(k_amr_ghost, dim3(g), dim3(kGhostThreads), 0, stream, dev, W, d_ghost, n, small_scr, small_cap);
             HCHK(hipGetLastError());
         } else {
             // one launch per leaf (its 2*D slabs)
             const int per = 2 * ndim;
             for (int i = 0; i < n; i += per) {
-                hipLaunchKernelGGL(k_amr_ghost, dim3((unsigned)per), dim3(256), 0, stream, dev, W, d_ghost + i, per,
+                hipLaunchKernelGGL(k_amr_ghost, dim3((unsigned)per), dim3(kGhostThreads), 0, stream, dev, W, d_ghost + i, per,
                                    small_scr, small_cap);
                 HCHK(hipGetLastError());
             }
