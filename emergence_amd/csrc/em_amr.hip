@@ -1049,27 +1049,44 @@ struct Amr {
         ib.insert(ib.end(), src.begin(), src.end());
         const int pdst = (int)ib.size();
         ib.insert(ib.end(), dst.begin(), dst.end());
-        apply_edits();  // earlier edits (refinement) first
-        HCHK(hipStreamSynchronize(stream));
+        // index edits: the removals of each batch, then the moved blocks'
+        // entries (all entries distinct), uploaded once
+        std::vector<long long> eo;
+        std::vector<int> ev;
+        std::vector<int> eb(nb + 2, 0);  // batch b's removals: eo[eb[b] .. eb[b+1]); final: eb[nb] .. eb[nb+1]
+        for (int b = 0; b < nb; ++b) {
+            eb[b] = (int)eo.size();
+            for (const CoOp& op : co_ops)
+                if (op.batch == b)
+                    for (long long e : op.removed) {
+                        eo.push_back(e);
+                        ev.push_back(-1);
+                    }
+        }
+        eb[nb] = (int)eo.size();
+        for (int sl : dst) {
+            eo.push_back(map_off[blocks[sl].level] + slot(blocks[sl].level, blocks[sl].idx));
+            ev.push_back(sl);
+        }
+        eb[nb + 1] = (int)eo.size();
+        apply_edits();  // earlier edits (refinement) first; synchronises
         d_co_int = dev_vec(ib, d_co_int);
+        d_eoff = dev_vec(eo, d_eoff);
+        d_eval = dev_vec(ev, d_eval);
         const long long need = (long long)src.size() * 5 * bcells;
         if (need > d_co_tmp_n) {
             if (d_co_tmp) HCHK(hipFree(d_co_tmp));
             HCHK(hipMalloc(&d_co_tmp, sizeof(double) * (size_t)need));
             d_co_tmp_n = need;
         }
+        auto edits = [&](int lo, int hi) {
+            if (hi <= lo) return;
+            const unsigned g = (unsigned)std::min((hi - lo + 255) / 256, 1024);
+            hipLaunchKernelGGL(k_amr_map_set, dim3(g), dim3(256), 0, stream, d_map, d_eoff + lo, d_eval + lo, hi - lo);
+            HCHK(hipGetLastError());
+        };
         for (int b = 0; b < nb; ++b) {
-            if (b > 0) {
-                std::vector<long long> off;
-                std::vector<int> val;
-                for (const CoOp& op : co_ops)
-                    if (op.batch == b - 1)
-                        for (long long e : op.removed) {
-                            off.push_back(e);
-                            val.push_back(-1);
-                        }
-                apply_edit_list(off, val);
-            }
+            if (b > 0) edits(eb[b - 1], eb[b]);
             for (const Launch& L : rl[b]) {
                 const unsigned g = (unsigned)std::min(L.n, big_wg);
                 hipLaunchKernelGGL(k_amr_restrict, dim3(g), dim3(kPatchThreads), 0, stream, dev, W, d_co_int + L.ids,
@@ -1077,15 +1094,7 @@ struct Amr {
                 HCHK(hipGetLastError());
             }
         }
-        // the last batch's removals, the moves, and the moved blocks' entries
-        std::vector<long long> off;
-        std::vector<int> val;
-        for (const CoOp& op : co_ops)
-            if (op.batch == nb - 1)
-                for (long long e : op.removed) {
-                    off.push_back(e);
-                    val.push_back(-1);
-                }
+        edits(eb[nb - 1], eb[nb]);  // the last batch's removals
         if (!src.empty()) {
             const int np = (int)src.size();
             const unsigned g = (unsigned)std::min<long long>(((long long)np * bcells + 255) / 256, 4096);
@@ -1093,12 +1102,9 @@ struct Amr {
             HCHK(hipGetLastError());
             hipLaunchKernelGGL(k_amr_blocks_in, dim3(g), dim3(256), 0, stream, dev, d_co_int + pdst, np, d_co_tmp);
             HCHK(hipGetLastError());
-            for (int sl : dst) {
-                off.push_back(map_off[blocks[sl].level] + slot(blocks[sl].level, blocks[sl].idx));
-                val.push_back(sl);
-            }
         }
-        apply_edit_list(off, val);
+        edits(eb[nb], eb[nb + 1]);
+        HCHK(hipStreamSynchronize(stream));  // ib, eo, ev are copied from pageable memory
         co_ops.clear();
     }
 

@@ -51,6 +51,61 @@ __device__ __forceinline__ void unflat(int i, const int sz[3], int c[3]) {
     c[1] = q - c[0] * sz[1];
 }
 
+// Block ids under a box of level-l cells g0 .. g0+sz-1 (per axis, before any
+// periodic wrap), resolved once per workgroup so that a gather reads U with
+// one dependent load instead of two (index, then data).  ids[] gets the id,
+// -1 for an absent block, -2 outside a non-periodic boundary.  Returns false
+// (and resolves nothing) when the box spans more than kBoxBlocks blocks.
+// Every thread of the workgroup calls it; it ends with a barrier.
+constexpr int kBoxBlocks = 512;
+struct BoxBlocks {
+    int b0[3], nbk[3];
+};
+__device__ bool box_blocks(const AmrDev& a, int l, const int g0[3], const int sz[3], int* ids, BoxBlocks& bb) {
+    int total = 1;
+    for (int d = 0; d < 3; ++d) {
+        if (a.act[d]) {
+            bb.b0[d] = g0[d] >> a.lb[d];  // floor division (arithmetic shift)
+            bb.nbk[d] = ((g0[d] + sz[d] - 1) >> a.lb[d]) - bb.b0[d] + 1;
+        } else {
+            bb.b0[d] = 0;
+            bb.nbk[d] = 1;
+        }
+        total *= bb.nbk[d];
+    }
+    if (total > kBoxBlocks) {
+        __syncthreads();
+        return false;
+    }
+    for (int i = threadIdx.x; i < total; i += blockDim.x) {
+        int c[3];
+        unflat(i, bb.nbk, c);
+        int s = 0, id = 0;
+        for (int d = 0; d < 3; ++d) {
+            const int nb = a.act[d] ? a.nb0[d] << l : 1;
+            int k = bb.b0[d] + c[d];
+            if (k < 0 || k >= nb) {
+                if (a.act[d] && a.bc_kind[d][k >= nb ? 1 : 0] == BC_PERIODIC)
+                    k = rem_e(k, nb);
+                else
+                    id = -2;
+            }
+            s = s * nb + k;
+        }
+        ids[i] = id == -2 ? -2 : a.map[a.map_off[l] + s];
+    }
+    __syncthreads();
+    return true;
+}
+// entry of ids[] for the (unwrapped) cell g, and its cell index in the block
+__device__ __forceinline__ int box_block_of(const AmrDev& a, const BoxBlocks& bb, const int g[3], int* cell) {
+    int bi = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) bi = bi * bb.nbk[d] + ((a.act[d] ? g[d] >> a.lb[d] : 0) - bb.b0[d]);
+    *cell = ((g[0] & (a.bd[0] - 1)) * a.bd[1] + (g[1] & (a.bd[1] - 1))) * a.bd[2] + (g[2] & (a.bd[2] - 1));
+    return bi;
+}
+
 // bc_box of amr.rs:450-504 on a box held in scratch (values val[q*cap+i],
 // validity ok[i]): cells outside a non-periodic boundary are filled from
 // cells of the same box, one axis sweep at a time; called by one workgroup.
@@ -147,9 +202,33 @@ __device__ double* predict_region(const AmrDev& a, const AmrWeights& W, int lf, 
     }
     const int n3[3] = {amr_n_level(a, lc, 0), amr_n_level(a, lc, 1), amr_n_level(a, lc, 2)};
     if (threadIdx.x == 0) any_out = 0;
-    __syncthreads();
+    __shared__ int bids[kBoxBlocks];
+    BoxBlocks bb;
+    const bool cached = box_blocks(a, lc, cmin, sz, bids, bb);  // synchronises
     const int tot = sz[0] * sz[1] * sz[2];
-    for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+    for (int i = threadIdx.x; i < tot && cached; i += blockDim.x) {
+        int c[3];
+        unflat(i, sz, c);
+        int g[3] = {cmin[0] + c[0], cmin[1] + c[1], cmin[2] + c[2]};
+        for (int d = 0; d < 3; ++d)
+            if (!a.act[d]) g[d] = 0;
+        int cell;
+        const int id = bids[box_block_of(a, bb, g, &cell)];
+        if (id == -2) {
+            ok[i] = 0.0;
+            any_out = 1;
+            for (int q = 0; q < 5; ++q) A[q * cap + i] = 0.0;
+            continue;
+        }
+        if (id < 0) {
+            atomicOr(&a.sc->err, 2);  // the hierarchy is 2:1 balanced (see below)
+            for (int q = 0; q < 5; ++q) A[q * cap + i] = 0.0;
+        } else {
+            for (int q = 0; q < 5; ++q) A[q * cap + i] = a.U[amr_u(a, q, id, cell)];
+        }
+        ok[i] = 1.0;
+    }
+    for (int i = threadIdx.x; i < tot && !cached; i += blockDim.x) {
         int c[3];
         unflat(i, sz, c);
         int g[3] = {cmin[0] + c[0], cmin[1] + c[1], cmin[2] + c[2]};
@@ -248,9 +327,29 @@ __device__ void restrict_block(const AmrDev& a, const AmrWeights& W, int l, int 
         sz[d] = a.act[d] ? 2 * a.bd[d] + 8 : 1;
     }
     if (threadIdx.x == 0) any_out = 0;
-    __syncthreads();
+    __shared__ int bids[kBoxBlocks];
+    BoxBlocks bb;
+    const bool cached = box_blocks(a, lf, f0, sz, bids, bb);  // synchronises
     const int tot = sz[0] * sz[1] * sz[2];
-    for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+    for (int i = threadIdx.x; i < tot && cached; i += blockDim.x) {
+        int c[3];
+        unflat(i, sz, c);
+        const int g[3] = {f0[0] + c[0], f0[1] + c[1], f0[2] + c[2]};
+        int cell;
+        const int id = bids[box_block_of(a, bb, g, &cell)];
+        double okv = 0.0;
+        if (id == -2) {
+            any_out = 1;
+            for (int q = 0; q < 5; ++q) A[q * cap + i] = 0.0;
+        } else if (id >= 0) {
+            for (int q = 0; q < 5; ++q) A[q * cap + i] = a.U[amr_u(a, q, id, cell)];
+            okv = 1.0;
+        } else {
+            for (int q = 0; q < 5; ++q) A[q * cap + i] = 0.0;
+        }
+        okA[i] = okv;
+    }
+    for (int i = threadIdx.x; i < tot && !cached; i += blockDim.x) {
         int c[3];
         unflat(i, sz, c);
         int g[3] = {f0[0] + c[0], f0[1] + c[1], f0[2] + c[2]};
