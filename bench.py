@@ -111,10 +111,16 @@ def ensemble_bench(em, args):
         gi *= (m / m.sum())[..., None]
         return float(p[m].mean()), g
 
-    # plain batched stepping vs one launch per instance (same kernels)
+    # plain batched stepping vs one launch per instance (same kernels, the
+    # tile shape the autotuner picks for the batched launch)
     timing = {}
+    tiles = None
     for batched in (True, False):
         ens = em.HipEnsemble(sizes, states, bc=bc, batched=batched)
+        if tiles is None:
+            tiles = ens.autotune(3)
+        else:
+            ens.set_tiles(tiles)
         ens.run_steps(2)
         k = 20 if batched else 5
         t0 = time.perf_counter()
@@ -122,6 +128,7 @@ def ensemble_bench(em, args):
         timing[batched] = (time.perf_counter() - t0) / k
         del ens
     ens = em.HipEnsemble(sizes, states, bc=bc)
+    ens.set_tiles(tiles)
     t0 = time.perf_counter()
     ens.forward(args.ens_steps, args.ens_checkpoint)
     t_fwd = time.perf_counter() - t0
@@ -153,6 +160,7 @@ def ensemble_bench(em, args):
         "cells": cells,
         "value": cells * 3 / timing[True],
         "unit": "cell-updates/s per RK stage",
+        "tiles": tiles,
         "per_instance_launches": {"value": cells * 3 / timing[False]},
         "speedup_vs_per_instance": timing[False] / timing[True],
         "forward_s": t_fwd,
@@ -324,7 +332,9 @@ def amr_bench(em, args):
     nblocks, nleaves = root.counts()
     del root
     bat = em.HipAmr(c["base"], c["b"], c["levels"], c["lo"], c["hi"], blocks=hier, **kw)
+    shapes = bat.autotune(3)
     per = em.HipAmr(c["base"], c["b"], c["levels"], c["lo"], c["hi"], blocks=hier, batched=False, **kw)
+    per.set_shapes(shapes)
     k = args.amr_per_block_steps
     bat.run_steps(k)
     per.synchronize()
@@ -358,6 +368,7 @@ def amr_bench(em, args):
         "ms_per_step": el / args.amr_steps * 1e3,
         "steps": args.amr_steps,
         "per_block": {"value": value_per, "ms_per_step": el_per / k * 1e3, "steps": k},
+        "pass_shapes": shapes,
         "speedup_vs_per_block": value / value_per,
         "identical_to_per_block": same,
         "phase_ms_per_step": dict(zip(["restrict", "ghosts", "fluxes", "reflux_update", "regrid"], phase)),

@@ -36,6 +36,7 @@ EXPORTS = (
     "em_ensemble_destroy",
     "em_ensemble_set_batched",
     "em_ensemble_set_tiles",
+    "em_ensemble_autotune",
     "em_ensemble_run_steps",
     "em_ensemble_download",
     "em_ensemble_time",
@@ -49,6 +50,8 @@ EXPORTS = (
     "em_amr_set_variant",
     "em_amr_set_batched",
     "em_amr_set_graph",
+    "em_amr_set_shapes",
+    "em_amr_autotune",
     "em_amr_set_profile",
     "em_amr_profile",
     "em_amr_set_next_regrid",
@@ -121,6 +124,7 @@ def load():
     lib.em_ensemble_destroy.argtypes = [P]
     lib.em_ensemble_set_batched.argtypes = [P, C.c_int32]
     lib.em_ensemble_set_tiles.argtypes = [P, C.c_int32]
+    lib.em_ensemble_autotune.argtypes = [P, C.c_int32, I32]
     lib.em_ensemble_run_steps.argtypes = [P, C.c_int64]
     lib.em_ensemble_download.argtypes = [P, C.c_int32, D]
     lib.em_ensemble_time.argtypes = [P, C.c_int32, D]
@@ -136,6 +140,8 @@ def load():
     lib.em_amr_set_variant.argtypes = [P, C.c_int32]
     lib.em_amr_set_batched.argtypes = [P, C.c_int32]
     lib.em_amr_set_graph.argtypes = [P, C.c_int32]
+    lib.em_amr_set_shapes.argtypes = [P, I32]
+    lib.em_amr_autotune.argtypes = [P, C.c_int32, I32]
     lib.em_amr_set_profile.argtypes = [P, C.c_int32]
     lib.em_amr_profile.argtypes = [P, D]
     lib.em_amr_set_next_regrid.argtypes = [P, C.c_int64]

@@ -91,6 +91,17 @@ class HipAmr:
     def set_graph(self, on):
         _lib.check_amr(self.lib.em_amr_set_graph(self.h, int(bool(on))))
 
+    def set_shapes(self, shapes):
+        """Pass shape per axis (see em_amr_set_shapes)."""
+        arr = (C.c_int32 * 3)(*[int(v) for v in shapes])
+        _lib.check_amr(self.lib.em_amr_set_shapes(self.h, arr))
+
+    def autotune(self, repeats=3):
+        """Times every pass shape per axis and keeps the fastest; returns them."""
+        out = (C.c_int32 * 3)()
+        _lib.check_amr(self.lib.em_amr_autotune(self.h, int(repeats), out))
+        return list(out)
+
     def set_profile(self, on):
         _lib.check_amr(self.lib.em_amr_set_profile(self.h, int(bool(on))))
 

@@ -1417,19 +1417,22 @@ struct Amr {
         }
     }
 
+    int shape[3] = {0, 0, 0};  // pass shape per axis (amr_shape)
+
     hipError_t launch_pass(int d, const AmrPassArgs& pa, unsigned grid) {
-        return precise ? em::precise::launch_amr_pass(d, mode, B, pa, grid, stream)
-                       : em::fast::launch_amr_pass(d, mode, B, pa, grid, stream);
+        return precise ? em::precise::launch_amr_pass(d, mode, B, shape[d], pa, grid, stream)
+                       : em::fast::launch_amr_pass(d, mode, B, shape[d], pa, grid, stream);
     }
 
-    void passes(int stage_dt_host, double stage_dt) {
+    void passes(int stage_dt_host, double stage_dt, int only_axis = -1) {
         int last = -1;
         for (int d = 0; d < 3; ++d)
             if (act[d]) last = d;
         bool first = true;
-        const int P = amr_pass_pencils(B);
         for (int d = 0; d < 3; ++d) {
             if (!act[d]) continue;
+            if (only_axis >= 0 && d != only_axis) continue;
+            const int P = amr_shape(B, shape[d]).P;
             AmrPassArgs pa{};
             pa.a = dev;
             pa.leaves = d_leaves;
@@ -1590,6 +1593,52 @@ struct Amr {
     hipGraphExec_t gexec = nullptr;
     long long graph_version = -1;
     long long version = 0;
+
+    // Times every pass shape of every active axis on the residual of the
+    // current hierarchy (ghosts filled, limiter on with the CFL dt) and keeps
+    // the fastest; the state is untouched and the counters are restored.
+    // Shapes give identical results: a cell's flux differences and
+    // accumulation are the same in any pencil grouping.
+    void autotune(int repeats, int32_t out[3]) {
+        tables();
+        const double dt = dt_cfl();
+        std::vector<DevScalars> saved(1);
+        HCHK(hipMemcpyAsync(saved.data(), sc, sizeof(DevScalars), hipMemcpyDeviceToHost, stream));
+        hipLaunchKernelGGL(k_amr_set_run, dim3(1), dim3(1), 0, stream, sc);
+        HCHK(hipGetLastError());
+        restrict_all();
+        fill_ghosts();
+        hipEvent_t e0, e1;
+        HCHK(hipEventCreate(&e0));
+        HCHK(hipEventCreate(&e1));
+        for (int d = 0; d < 3; ++d) {
+            if (!act[d]) continue;
+            double best = 1e300;
+            int bs = shape[d];
+            for (int c = 0; c < amr_num_shapes(B); ++c) {
+                shape[d] = c;
+                passes(1, std::isfinite(dt) ? dt : 0.0, d);  // warm-up
+                HCHK(hipEventRecord(e0, stream));
+                for (int r = 0; r < repeats; ++r) passes(1, std::isfinite(dt) ? dt : 0.0, d);
+                HCHK(hipEventRecord(e1, stream));
+                HCHK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HCHK(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) {
+                    best = ms;
+                    bs = c;
+                }
+            }
+            shape[d] = bs;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        HCHK(hipMemcpyAsync(sc, saved.data(), sizeof(DevScalars), hipMemcpyHostToDevice, stream));
+        HCHK(hipStreamSynchronize(stream));
+        ++version;
+        if (out)
+            for (int d = 0; d < 3; ++d) out[d] = shape[d];
+    }
 
     void graph_step() {
         if (!gexec || graph_version != version) {
@@ -2001,6 +2050,27 @@ int em_amr_set_batched(em_amr* h, int32_t batched) {
     h->a->batched = batched != 0;
     ++h->a->version;
     return EM_OK;
+}
+
+int em_amr_set_shapes(em_amr* h, const int32_t shape[3]) {
+    ANEED(h);
+    if (!shape) {
+        em::g_last_error_amr = "null argument";
+        return EM_ERR_ARG;
+    }
+    for (int d = 0; d < 3; ++d)
+        if (shape[d] < 0 || shape[d] >= em::amr_num_shapes(h->a->B)) {
+            em::g_last_error_amr = "no such pass shape";
+            return EM_ERR_ARG;
+        }
+    for (int d = 0; d < 3; ++d) h->a->shape[d] = shape[d];
+    ++h->a->version;
+    return EM_OK;
+}
+
+int em_amr_autotune(em_amr* h, int32_t repeats, int32_t shape_out[3]) {
+    ANEED(h);
+    return aguard([&] { h->a->autotune(repeats > 0 ? repeats : 3, shape_out); });
 }
 
 int em_amr_set_graph(em_amr* h, int32_t on) {

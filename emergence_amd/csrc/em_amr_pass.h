@@ -145,15 +145,23 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_WAVES_PER
     (void)sred;
 }
 
-// Block sizes with a compiled pass: B = 8, 16, 32 (P pencils, NT threads).
+// Block sizes with a compiled pass: B = 8, 16, 32, in the shapes of
+// amr_shape (P pencils, NT threads).
 template <int D, int MODE>
-static hipError_t launch_amr_b(int b, const AmrPassArgs& pa, unsigned grid, hipStream_t s) {
-    switch (b) {
-        case 8: hipLaunchKernelGGL((amr_pass<D, MODE, 8, 8, 128>), dim3(grid), dim3(128), 0, s, pa); break;
-        case 16: hipLaunchKernelGGL((amr_pass<D, MODE, 16, 16, 384>), dim3(grid), dim3(384), 0, s, pa); break;
-        case 32: hipLaunchKernelGGL((amr_pass<D, MODE, 32, 8, 320>), dim3(grid), dim3(320), 0, s, pa); break;
+static hipError_t launch_amr_b(int b, int shape, const AmrPassArgs& pa, unsigned grid, hipStream_t s) {
+#define EM_AMR_CASE(B_, P_, NT_) hipLaunchKernelGGL((amr_pass<D, MODE, B_, P_, NT_>), dim3(grid), dim3(NT_), 0, s, pa)
+    switch (b * 16 + shape) {
+        case 8 * 16 + 0: EM_AMR_CASE(8, 8, 128); break;
+        case 8 * 16 + 1: EM_AMR_CASE(8, 16, 256); break;
+        case 8 * 16 + 2: EM_AMR_CASE(8, 32, 512); break;
+        case 8 * 16 + 3: EM_AMR_CASE(8, 64, 512); break;
+        case 16 * 16 + 0: EM_AMR_CASE(16, 16, 384); break;
+        case 16 * 16 + 1: EM_AMR_CASE(16, 32, 512); break;
+        case 32 * 16 + 0: EM_AMR_CASE(32, 8, 320); break;
+        case 32 * 16 + 1: EM_AMR_CASE(32, 16, 512); break;
         default: return hipErrorInvalidValue;
     }
+#undef EM_AMR_CASE
     return hipGetLastError();
 }
 

@@ -76,8 +76,19 @@ struct AmrPassArgs {
     double stage_dt;
 };
 
-// pencils per work item of the batched pass for block size b (launch_amr_b)
-inline int amr_pass_pencils(int b) { return b == 16 ? 16 : 8; }
+// Shapes of the batched pass with a compiled kernel (launch_amr_b): P pencils
+// per work item, NT threads; shape 0 is the default, em_amr_autotune times
+// the others.
+struct AmrShape {
+    int P, NT;
+};
+inline int amr_num_shapes(int b) { return b == 8 ? 4 : 2; }
+inline AmrShape amr_shape(int b, int shape) {
+    static const AmrShape s8[4] = {{8, 128}, {16, 256}, {32, 512}, {64, 512}};
+    static const AmrShape s16[2] = {{16, 384}, {32, 512}};
+    static const AmrShape s32[2] = {{8, 320}, {16, 512}};
+    return b == 8 ? s8[shape] : b == 16 ? s16[shape] : s32[shape];
+}
 
 __host__ __device__ inline long long amr_u(const AmrDev& a, int q, long long slot, long long cell) {
     return q * a.vs + slot * a.bcells + cell;

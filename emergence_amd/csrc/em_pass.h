@@ -17,15 +17,15 @@ constexpr int kNumTileCfgs = 9;
     hipError_t launch_pass_hybrid(int d, int cfg, const PassArgs& a, hipStream_t s);      \
     hipError_t launch_pass_weno(int d, int cfg, const PassArgs& a, hipStream_t s);        \
     hipError_t launch_pass_central(int d, int cfg, const PassArgs& a, hipStream_t s);     \
-    hipError_t launch_amr_hybrid(int d, int b, const AmrPassArgs& pa, unsigned grid, hipStream_t s); \
-    hipError_t launch_amr_weno(int d, int b, const AmrPassArgs& pa, unsigned grid, hipStream_t s);   \
-    hipError_t launch_amr_central(int d, int b, const AmrPassArgs& pa, unsigned grid, hipStream_t s); \
-    inline hipError_t launch_amr_pass(int d, int mode, int b, const AmrPassArgs& pa, unsigned grid,  \
-                                      hipStream_t s) {                                               \
+    hipError_t launch_amr_hybrid(int d, int b, int shape, const AmrPassArgs& pa, unsigned grid, hipStream_t s); \
+    hipError_t launch_amr_weno(int d, int b, int shape, const AmrPassArgs& pa, unsigned grid, hipStream_t s);   \
+    hipError_t launch_amr_central(int d, int b, int shape, const AmrPassArgs& pa, unsigned grid, hipStream_t s); \
+    inline hipError_t launch_amr_pass(int d, int mode, int b, int shape, const AmrPassArgs& pa,       \
+                                      unsigned grid, hipStream_t s) {                                \
         switch (mode) {                                                                    \
-            case MODE_HYBRID: return launch_amr_hybrid(d, b, pa, grid, s);                 \
-            case MODE_WENO: return launch_amr_weno(d, b, pa, grid, s);                     \
-            case MODE_CENTRAL: return launch_amr_central(d, b, pa, grid, s);               \
+            case MODE_HYBRID: return launch_amr_hybrid(d, b, shape, pa, grid, s);          \
+            case MODE_WENO: return launch_amr_weno(d, b, shape, pa, grid, s);              \
+            case MODE_CENTRAL: return launch_amr_central(d, b, shape, pa, grid, s);        \
             default: return hipErrorInvalidValue;                                          \
         }                                                                                  \
     }                                                                                      \

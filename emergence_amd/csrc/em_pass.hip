@@ -328,11 +328,11 @@ hipError_t EM_ENS_LAUNCHER(int d, int cfg, const EnsPassArgs& e, unsigned grid, 
     }
 }
 
-hipError_t EM_AMR_LAUNCHER(int d, int b, const AmrPassArgs& pa, unsigned grid, hipStream_t s) {
+hipError_t EM_AMR_LAUNCHER(int d, int b, int shape, const AmrPassArgs& pa, unsigned grid, hipStream_t s) {
     switch (d) {
-        case 0: return launch_amr_b<0, EM_MODE>(b, pa, grid, s);
-        case 1: return launch_amr_b<1, EM_MODE>(b, pa, grid, s);
-        case 2: return launch_amr_b<2, EM_MODE>(b, pa, grid, s);
+        case 0: return launch_amr_b<0, EM_MODE>(b, shape, pa, grid, s);
+        case 1: return launch_amr_b<1, EM_MODE>(b, shape, pa, grid, s);
+        case 2: return launch_amr_b<2, EM_MODE>(b, shape, pa, grid, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -1032,6 +1032,54 @@ struct Ensemble {
 
     unsigned small_grid() const { return (unsigned)((n + 63) / 64); }
 
+    // Times every tile shape on the stage-1 passes of all instances (a stage
+    // writes only the scratch state U1 and the residual) and keeps the
+    // fastest; the instances' states and counters are untouched.  Tile
+    // shapes give identical results.
+    int autotune(int repeats) {
+        std::vector<std::vector<DevScalars>> saved;
+        for (auto* S : in) saved.push_back(S->save_scalars());
+        int maxn = 1;
+        for (auto* S : in)
+            for (int d = 0; d < 3; ++d)
+                if (S->active(d)) maxn = std::max(maxn, (int)S->n[d]);
+        double* hist = nullptr;
+        HIPCHK(hipMalloc(&hist, sizeof(double) * n));
+        start_run();
+        hipLaunchKernelGGL(k_ens_dt, dim3(small_grid()), dim3(64), 0, stream, d_scs, n, cfl, hist, d_step);
+        HIPCHK(hipGetLastError());
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        double best = 1e300;
+        int bestc = cfg;
+        for (int c = 0; c < kNumTileCfgs; ++c) {
+            if (kTileCfgs[c].T > 2 * maxn && c != 0) continue;
+            cfg = c;
+            build_args();
+            stage_launches(1);  // warm-up
+            HIPCHK(hipEventRecord(e0, stream));
+            for (int r = 0; r < repeats; ++r) stage_launches(1);
+            HIPCHK(hipEventRecord(e1, stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) {
+                best = ms;
+                bestc = c;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        HIPCHK(hipStreamSynchronize(stream));
+        (void)hipFree(hist);
+        for (int i = 0; i < n; ++i)
+            HIPCHK(hipMemcpyAsync(in[i]->sc, saved[i].data(), sizeof(DevScalars), hipMemcpyHostToDevice, stream));
+        cfg = bestc;
+        build_args();
+        return cfg;
+    }
+
     void stage_launches(int st) {
         for (int d = 0; d < 3; ++d) {
             if (total[st - 1][d] == 0) continue;
@@ -1569,6 +1617,14 @@ int em_ensemble_set_batched(em_ensemble* h, int32_t batched) {
     ENEED(h);
     h->e->batched = batched != 0;
     return EM_OK;
+}
+
+int em_ensemble_autotune(em_ensemble* h, int32_t repeats, int32_t* cfg_out) {
+    ENEED(h);
+    return guarded([&] {
+        const int c = h->e->autotune(repeats > 0 ? repeats : 3);
+        if (cfg_out) *cfg_out = c;
+    });
 }
 
 int em_ensemble_set_tiles(em_ensemble* h, int32_t cfg) {

@@ -108,6 +108,12 @@ class HipEnsemble:
     def set_tiles(self, cfg):
         _lib.check(self.lib.em_ensemble_set_tiles(self.h, int(cfg)))
 
+    def autotune(self, repeats=3):
+        """Times every tile shape on all instances and keeps the fastest; returns it."""
+        out = C.c_int32()
+        _lib.check(self.lib.em_ensemble_autotune(self.h, int(repeats), C.byref(out)))
+        return out.value
+
     def run_steps(self, nsteps):
         _lib.check(self.lib.em_ensemble_run_steps(self.h, int(nsteps)))
 

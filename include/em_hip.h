@@ -163,6 +163,11 @@ int em_ensemble_set_batched(em_ensemble* h, int32_t batched);
 
 /* Tile shape of every pass (index into the library's tile table). */
 int em_ensemble_set_tiles(em_ensemble* h, int32_t cfg);
+/* Project autotuner for the ensemble: times every tile shape on the stage
+ * passes of all instances and keeps the fastest (states and counters
+ * untouched); *cfg_out (may be NULL) receives it.  No counterpart in the
+ * reference (one GpuSolver per grid, compressible-cuda/src/lib.rs:155-477). */
+int em_ensemble_autotune(em_ensemble* h, int32_t repeats, int32_t* cfg_out);
 
 /* nsteps CFL steps of every instance (each its own dt). */
 int em_ensemble_run_steps(em_ensemble* h, int64_t nsteps);
@@ -247,6 +252,16 @@ int em_amr_set_batched(em_amr* h, int32_t batched);
 /* 1 (default): batched em_amr_run_steps replays each step as one captured
  * HIP graph (captured again after every hierarchy edit); 0: eager launches. */
 int em_amr_set_graph(em_amr* h, int32_t on);
+
+/* Pass shape per axis (pencils per work item and threads of the batched
+ * flux pass; 4 shapes for 8^3 blocks, 2 for 16^3 and 32^3, 0 the default).
+ * Results do not depend on it.  No counterpart in the reference, whose
+ * kernel shapes are fixed (compressible-cuda/src/amr.rs:572-818). */
+int em_amr_set_shapes(em_amr* h, const int32_t shape[3]);
+/* Project autotuner: times every pass shape of every active axis on the
+ * residual of the current hierarchy and keeps the fastest (state and
+ * counters untouched); shape_out (may be NULL) receives the choice. */
+int em_amr_autotune(em_amr* h, int32_t repeats, int32_t shape_out[3]);
 
 /* GpuAmr::profile / GpuAmr::prof (amr.rs:110-115): when on, record device
  * time per phase; em_amr_profile returns the accumulated milliseconds of

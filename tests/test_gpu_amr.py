@@ -243,3 +243,32 @@ def test_characteristic_and_dirichlet_boundaries(em):
     ro, so = o.compute_rhs(1e-3)
     assert np.array_equal(rg, ro)
     assert sg == so
+
+
+def test_pass_shapes_and_autotune_give_identical_results(em):
+    # Every compiled shape of the batched flux pass (pencils per work item,
+    # threads) gives the same bits: a cell's flux differences and their
+    # accumulation do not depend on the grouping.  The autotuner picks one
+    # of them and leaves the state and counters as they were.
+    print("amr pass shapes and autotune, 3-D shock-bubble", flush=True)
+    bc = [["outflow", "outflow"], ["reflect", "reflect"], ["reflect", "reflect"]]
+    prm = [2.0, 0.3, 0.6, 0.25, 0.25, 0.15, 0.1]
+    o = OracleAmr("shock_bubble", prm, (4, 2, 2), 8, 2, (0, 0, 0), (1, 0.5, 0.5), eps=0.2, bc=bc, mode="weno",
+                  regrid_every=2)
+    ref = None
+    for shape in range(4):
+        g = _from_oracle(em, o, bc=bc, mode="weno", eps=0.2, regrid_every=2)
+        g.set_shapes([shape] * 3)
+        g.run_steps(3)
+        got = (g.download(), g.counters(), g.t)
+        if ref is None:
+            ref = got
+        else:
+            assert np.array_equal(got[0], ref[0]) and got[1:] == ref[1:], shape
+    g = _from_oracle(em, o, bc=bc, mode="weno", eps=0.2, regrid_every=2)
+    before = (g.download(), g.counters())
+    shapes = g.autotune(2)
+    assert all(0 <= s < 4 for s in shapes)
+    assert np.array_equal(g.download(), before[0]) and g.counters() == before[1]
+    g.run_steps(3)
+    assert np.array_equal(g.download(), ref[0]) and g.counters() == ref[1]
