@@ -118,11 +118,14 @@ int em_solver_synchronize(em_solver* s);
  * Discrete adjoint (no counterpart in the reference, which has none; the
  * forward/backward pair the survey of the path asks for).  FluxMode::Weno
  * only, periodic / outflow / reflect (n >= 5) / Dirichlet boundaries, axes up
- * to the LDS limit (about 1,300 cells).  Conventions: branches follow the
- * forward run, min/max differentiate the returned operand (the first on a
- * tie), d|x|/dx = sign(x) with sign(0) = 0, the positivity limiter's blend
- * weight and Lax-Friedrichs speed are constants, and the dt sequence is
- * frozen (taken from the forward run). */
+ * to the LDS limit (whole pencils in LDS: 562 cells).  Conventions: branches
+ * follow the forward run, min/max differentiate the returned operand (the
+ * first on a tie), d|x|/dx = sign(x) with sign(0) = 0; the positivity
+ * limiter's blend weight is differentiated implicitly at the admissibility
+ * constraint that bounds it (its bisection resolves it to 2^-30) and its
+ * Lax-Friedrichs speed through the max; WENO-Z weights take the linear
+ * weights' derivative on stencils flat to rounding and are held on partly
+ * flat ones; the dt sequence is frozen (taken from the forward run). */
 
 /* (dR/du)^T lam of one residual evaluation at the current state, with the
  * limiter's lambda from stage_dt; lam and out in the CPU layout (ghosts

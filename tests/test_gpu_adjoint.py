@@ -152,3 +152,20 @@ def test_backward_matches_central_differences(em):
         assert abs(fd - ad) <= 1e-6 * max(abs(ad), 1e-2 * scale) + noise, (i, j, fd, ad)
         checks += 1
     assert checks == 8
+
+
+def test_adjoint_rhs_long_pencils(em):
+    # 540-cell pencils (one pencil per workgroup, 138 KB of LDS, near the
+    # 562-cell limit) and 4-cell ones (16 pencils per workgroup) in one grid
+    print("adjoint rhs 540 x 4 periodic", flush=True)
+    n = (540, 4, 1)
+    o0 = OracleAdj(n, (0, 0, 0), (1, 1, 1))
+    u = smooth_state(o0, 8)
+    o, g = _pair(em, n, [["periodic", "periodic"]] * 3, u)
+    rng = np.random.default_rng(12)
+    lam = np.zeros_like(u)
+    o.interior(lam)[...] = rng.standard_normal(o.interior(lam).shape)
+    for stage_dt in (1e-3, 0.2):
+        ref = o.vjp_rhs(stage_dt, u, lam)
+        got = g.adjoint_rhs(stage_dt, lam)
+        assert _rel(o.interior(got), o.interior(ref)) < 1e-11, stage_dt
