@@ -24,11 +24,12 @@ struct AdjArgs {
     double inv_dx;
     double lam_lim;          // limiter lambda D * dt / dx (0: limiter off)
     int first;               // MU = contribution (else MU += contribution)
-    int P;                   // pencils per workgroup
+    int ntile;               // tiles per pencil (adj_tiles)
+    int ne;                  // output positions of the longest tile (adj_tile_extent)
 };
 
-// Ensemble VJP: per-instance arguments and prefix sums of their pencil
-// groups (ninst + 1 entries; total = prefix[ninst]).
+// Ensemble VJP: per-instance arguments and prefix sums of their work items
+// (pencils x tiles; ninst + 1 entries; total = prefix[ninst]).
 struct EnsAdjArgs {
     const AdjArgs* args;
     const long long* prefix;
@@ -47,12 +48,14 @@ struct AdjCombineTask {
     long long dt_index;
 };
 
-constexpr int kAdjThreads = 768;  // threads per adjoint workgroup (12 waves: 3 per SIMD)
-int adj_pencils_per_group(int n);
+constexpr int kAdjThreads = 256;     // threads per adjoint workgroup (4 waves; three workgroups per CU)
+constexpr int kAdjTileCells = 128;   // interior cells per tile at most (LDS of a tile <= 45 KB)
+int adj_tiles(int n, int kind_lo, int kind_hi);
+int adj_tile_extent(int n, int ntile, int kind_lo, int kind_hi);
 hipError_t launch_ens_adj_pass(int d, const EnsAdjArgs& e, size_t lds, unsigned grid, hipStream_t s);
 hipError_t launch_ens_adj_combine(int stage, const AdjCombineTask* tasks, int ninst, const double* dts,
                                   long long maxnv, hipStream_t s);
-size_t adj_lds_bytes(int n, int P);
+size_t adj_lds_bytes(int ne);
 hipError_t launch_adj_pass(int d, const AdjArgs& a, hipStream_t s);
 hipError_t launch_adj_combine(int stage, long long nv, double dt, const double* A, const double* T, double* B,
                               double* Z, hipStream_t s);

@@ -18,8 +18,9 @@
 // functions and operation order (no FMA contraction), so every branch
 // decision matches the forward run bit for bit.
 //
-// One workgroup (768 threads) owns whole pencils along the pass axis
-// (interior cells plus the 5 virtual ghost layers on each side) in LDS:
+// One workgroup (256 threads, three per CU) owns one tile of a pencil along
+// the pass axis (at most kAdjTileCells interior cells, with the ghost
+// positions of a non-periodic edge) and the halo its cotangents need, in LDS:
 //   A   load U, primitives (boundary ghosts through bc_ghost);
 //   B1  WENO5-Z reconstructions per (midpoint, characteristic field);
 //   B2  back-projection, fallback and HLLC per midpoint: forward fluxes;
@@ -33,8 +34,8 @@
 //   D4  per midpoint: the face-average state's cotangent (or the fallback's);
 //   E   primitive -> conserved cotangents, ghost cotangents routed back to
 //       the interior cells they were copied from, accumulated into MU.
-// Every accumulation has a fixed order that does not depend on the pencils
-// per workgroup, so results are bit-reproducible across launch shapes.
+// Every accumulation has a fixed order that does not depend on the tiling,
+// so results are bit-reproducible across launch shapes.
 #include <hip/hip_runtime.h>
 
 #include "em_adjoint.h"
@@ -179,6 +180,98 @@ __device__ __forceinline__ void weno_vjp(const double v[5], double ob, double vb
     vb[2] += 1.875 * q0b + 0.75 * q1b + 0.375 * q2b + t0b + 3.0 * s0b - 2.0 * t1b + t2b + 3.0 * s2b;
     vb[3] += 0.375 * q1b + 0.75 * q2b + t1b - s1b - 2.0 * t2b - 4.0 * s2b;
     vb[4] += -0.125 * q2b + t2b + s2b;
+}
+
+// Both WENO5-Z VJPs of a midpoint's characteristic field, left (v0..v4)
+// and right (v5, v4, .., v1), as straight-line code that the compiler can
+// interleave: the same values as weno_vjp on each side (its three cases
+// become selects), chb[k] += the cotangent of v_k.  Only where both sides
+// are flat to rounding (often whole waves, e.g. a velocity component that is
+// zero) is the general form skipped.
+struct WenoTerms {
+    double q0, q1, q2, t0, s0, t1, s1, t2, s2, b0, b1, b2;
+    bool all_flat, some_flat;
+};
+__device__ __forceinline__ WenoTerms weno_terms(double v0, double v1, double v2, double v3, double v4) {
+    WenoTerms w;
+    w.q0 = (3.0 * v0 - 10.0 * v1 + 15.0 * v2) * 0.125;
+    w.q1 = (-v1 + 6.0 * v2 + 3.0 * v3) * 0.125;
+    w.q2 = (3.0 * v2 + 6.0 * v3 - v4) * 0.125;
+    w.t0 = v0 - 2.0 * v1 + v2;
+    w.s0 = v0 - 4.0 * v1 + 3.0 * v2;
+    w.t1 = v1 - 2.0 * v2 + v3;
+    w.s1 = v1 - v3;
+    w.t2 = v2 - 2.0 * v3 + v4;
+    w.s2 = 3.0 * v2 - 4.0 * v3 + v4;
+    const double c13 = 13.0 / 12.0;
+    w.b0 = c13 * (w.t0 * w.t0) + 0.25 * (w.s0 * w.s0);
+    w.b1 = c13 * (w.t1 * w.t1) + 0.25 * (w.s1 * w.s1);
+    w.b2 = c13 * (w.t2 * w.t2) + 0.25 * (w.s2 * w.s2);
+    const double mv = fmax(fmax(fmax(fabs(v0), fabs(v1)), fmax(fabs(v2), fabs(v3))), fabs(v4));
+    const double flat = 1e-26 * mv * mv;
+    w.all_flat = w.b0 <= flat && w.b1 <= flat && w.b2 <= flat;
+    w.some_flat = w.b0 <= flat || w.b1 <= flat || w.b2 <= flat;
+    return w;
+}
+// cotangents of v0..v4 from (q0b, q1b, q2b) and the beta parts (t*b, s*b)
+__device__ __forceinline__ void weno_back(double q0b, double q1b, double q2b, double t0b, double s0b, double t1b,
+                                          double s1b, double t2b, double s2b, double g[5]) {
+    g[0] = 0.375 * q0b + t0b + s0b;
+    g[1] = -1.25 * q0b - 0.125 * q1b - 2.0 * t0b - 4.0 * s0b + t1b + s1b;
+    g[2] = 1.875 * q0b + 0.75 * q1b + 0.375 * q2b + t0b + 3.0 * s0b - 2.0 * t1b + t2b + 3.0 * s2b;
+    g[3] = 0.375 * q1b + 0.75 * q2b + t1b - s1b - 2.0 * t2b - 4.0 * s2b;
+    g[4] = -0.125 * q2b + t2b + s2b;
+}
+// weno_vjp's general and held-weight cases of one side (the caller selects
+// the linear case)
+__device__ __forceinline__ void weno_general(const WenoTerms& w, double ob, double g[5]) {
+    const double c13 = 13.0 / 12.0;
+    const double tau = fabs(w.b0 - w.b2);
+    const double i0 = 1.0 / (w.b0 + 1e-40), i1 = 1.0 / (w.b1 + 1e-40), i2 = 1.0 / (w.b2 + 1e-40);
+    const double r0 = tau * i0, r1 = tau * i1, r2 = tau * i2;
+    const double a0 = 0.0625 * (1.0 + r0 * r0);
+    const double a1 = 0.625 * (1.0 + r1 * r1);
+    const double a2 = 0.3125 * (1.0 + r2 * r2);
+    const double iS = 1.0 / (a0 + a1 + a2);
+    const double Nb = ob * iS;
+    const double q0b = a0 * Nb, q1b = a1 * Nb, q2b = a2 * Nb;
+    const double d0 = (a1 * (w.q0 - w.q1) + a2 * (w.q0 - w.q2)) * iS;
+    const double d1 = (a0 * (w.q1 - w.q0) + a2 * (w.q1 - w.q2)) * iS;
+    const double d2 = (a0 * (w.q2 - w.q0) + a1 * (w.q2 - w.q1)) * iS;
+    const double r0b = d0 * Nb * 0.0625 * 2.0 * r0;
+    const double r1b = d1 * Nb * 0.625 * 2.0 * r1;
+    const double r2b = d2 * Nb * 0.3125 * 2.0 * r2;
+    const double taub = r0b * i0 + r1b * i1 + r2b * i2;
+    const double sg = sgn(w.b0 - w.b2);
+    const double b0b = -r0b * r0 * i0 + sg * taub;
+    const double b1b = -r1b * r1 * i1;
+    const double b2b = -r2b * r2 * i2 - sg * taub;
+    const bool held = w.some_flat;  // weights held: no beta parts
+    const double t0b = held ? 0.0 : b0b * 2.0 * c13 * w.t0, s0b = held ? 0.0 : b0b * 0.5 * w.s0;
+    const double t1b = held ? 0.0 : b1b * 2.0 * c13 * w.t1, s1b = held ? 0.0 : b1b * 0.5 * w.s1;
+    const double t2b = held ? 0.0 : b2b * 2.0 * c13 * w.t2, s2b = held ? 0.0 : b2b * 0.5 * w.s2;
+    weno_back(q0b, q1b, q2b, t0b, s0b, t1b, s1b, t2b, s2b, g);
+}
+__device__ __forceinline__ void weno_vjp_pair(const double v[6], double obl, double obr, double chb[6]) {
+    const WenoTerms L = weno_terms(v[0], v[1], v[2], v[3], v[4]);
+    const WenoTerms R = weno_terms(v[5], v[4], v[3], v[2], v[1]);
+    double gl[5], gr[5];
+    weno_back(0.0625 * obl, 0.625 * obl, 0.3125 * obl, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, gl);
+    weno_back(0.0625 * obr, 0.625 * obr, 0.3125 * obr, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, gr);
+    if (!(L.all_flat && R.all_flat)) {
+        double hl[5], hr[5];
+        weno_general(L, obl, hl);
+        weno_general(R, obr, hr);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            gl[k] = L.all_flat ? gl[k] : hl[k];
+            gr[k] = R.all_flat ? gr[k] : hr[k];
+        }
+    }
+    chb[0] = gl[0];
+#pragma unroll
+    for (int k = 1; k < 5; ++k) chb[k] = gl[k] + gr[5 - k];
+    chb[5] = gr[0];
 }
 
 // f = hllc<D>(wl, wr) with the Davis speeds (system.rs:260-312)
@@ -366,101 +459,128 @@ __device__ __forceinline__ Eig eig_at(const Eos& e, double rho2, double rho3, do
 }
 // ------------------------------------------------------------------ kernel
 
-// Pencil group `grp` (P pencils) of one axis VJP described by `a`; every
-// thread of the workgroup calls it and it returns uniformly.
+// Output range of tile t of a pencil: interior cells [x0, x1) split evenly,
+// plus the ghost positions of a non-periodic edge (their cotangents go back
+// to the interior cells they were copied from, which lie in the same tile).
+struct AdjTile {
+    int x0, x1, e0, e1;
+};
+__device__ __forceinline__ AdjTile adj_tile(int n, int ntile, int t, bool cyc) {
+    AdjTile r;
+    r.x0 = (int)((long long)t * n / ntile);
+    r.x1 = (int)((long long)(t + 1) * n / ntile);
+    r.e0 = (!cyc && t == 0) ? -5 : r.x0;
+    r.e1 = (!cyc && t == ntile - 1) ? n + 5 : r.x1;
+    return r;
+}
+
+__device__ __forceinline__ int wrap(int x, int n) { return ((x % n) + n) % n; }
+
+// Work item w (pencil w / ntile, tile w % ntile) of one axis VJP described
+// by `a`; every thread of the workgroup calls it and it returns uniformly.
+//
+// Positions are global along the pencil (cell X, midpoint j between cells j
+// and j + 1, face f between cells f - 1 and f); the tile's output positions
+// are [e0, e1).  Local slots: cell X at X - e0 + 9, midpoint j at j - e0 + 7,
+// face f at f - e0 + 4, so that midpoint slot m reads cells m .. m + 5 and
+// averages cells m + 2, m + 3; face slot f combines midpoint slots f .. f + 4
+// and its limiter reads cells f + 4, f + 5; midpoint slot m takes the
+// cotangents of face slots m - 4 .. m.  The window: cells [e0 - 9, e1 + 9),
+// forward midpoints [e0 - 7, e1 + 6), faces [e0 - 4, e1 + 5), VJP midpoints
+// [e0 - 3, e1 + 2), clipped to the pencil's positions unless the axis is
+// periodic, where positions wrap (the forward's periodic ghosts are the
+// cyclic neighbours, so a position's values and branches are those of the
+// cell it wraps to).
 template <int D>
-__device__ __forceinline__ void adj_group(const AdjArgs& a, long long grp, double* lds) {
+__device__ __forceinline__ void adj_group(const AdjArgs& a, long long w, double* lds) {
     constexpr int un = 1 + D;
     constexpr int E0 = D == 0 ? 1 : 0;
     constexpr int E1 = D == 2 ? 1 : 2;
     const int n = a.n[D];
-    const int C = n + 10, M = n + 5, NF = n + 1;
-    const int P = a.P;
+    const int NC = a.ne + 18, NM = a.ne + 13;  // slot extents (the launch's largest tile)
     const int NT = blockDim.x;
     const int tid = threadIdx.x;
-    double* sw = lds;                 // [5][C][P] primitives
-    double* sacc = sw + 5 * C * P;    // [5][C][P] cotangents (primitive, then conserved)
-    double* sx = sacc + 5 * C * P;    // [3][C][P] cotangent parts of fields 1 and 4 (see D3)
-    double* smf = sx + 3 * C * P;     // [5][M][P] midpoint fluxes, then face cotangents
-    double* scc = smf + 5 * M * P;    // [18][M][P] cl, cr (B1); clb, crb, partials, fallback flag (D2, D3);
-                                      // 13..17: rows of L (B0)
-    auto W = [&](double* arr, int q, int c, int p, int ext) -> double& { return arr[(q * ext + c) * P + p]; };
+    double* sw = lds;              // [5][NC] primitives
+    double* sacc = sw + 5 * NC;    // [5][NC] cotangents (primitive, then conserved)
+    double* sx = sacc + 5 * NC;    // [3][NC] cotangent parts of fields 1 and 4 (see D3)
+    double* smf = sx + 3 * NC;     // [5][NM] midpoint fluxes, then face cotangents
+    double* scc = smf + 5 * NM;    // [18][NM] cl, cr (B1); clb, crb, partials, fallback flag (D2, D3);
+                                   // 13..17: rows of L (B0)
+    auto WC = [&](double* arr, int q, int c) -> double& { return arr[q * NC + c]; };
+    auto WM = [&](double* arr, int q, int m) -> double& { return arr[q * NM + m]; };
     const long long npen = a.ncell / n;
-    const long long pen0 = grp * P;
-    const int Pv = (int)min((long long)P, npen - pen0);
-    if (Pv <= 0) return;
+    const long long pen = w / a.ntile;
+    const int t = (int)(w - pen * a.ntile);
+    if (pen >= npen) return;
+    const bool cyc = a.bc_kind[0] == 0 && a.bc_kind[1] == 0;
+    const AdjTile tl_ = adj_tile(n, a.ntile, t, cyc);
+    const int e0 = tl_.e0, ne = tl_.e1 - tl_.e0;
+    // slot ranges (non-periodic: cells [-5, n + 5), midpoints [-3, n + 2),
+    // faces [0, n])
+    const int lo = cyc ? 0 : max(0, 4 - e0);
+    const int c_hi = cyc ? ne + 18 : min(ne + 18, n + 14 - e0);
+    const int m_hi = cyc ? ne + 13 : min(ne + 13, n + 9 - e0);
+    const int f_hi = cyc ? ne + 9 : min(ne + 9, n + 5 - e0);
+    const int v_lo = max(4, lo), v_hi = min(ne + 9, m_hi);  // VJP midpoints
     const Eos e = a.eos;
-    // pencil p -> base offset (transverse coordinates t0 along E0, t1 along E1)
-    auto base_of = [&](int p) -> long long {
-        const long long pi = pen0 + p;
-        const long long t0 = pi / a.n[E1], t1 = pi % a.n[E1];
-        return t0 * a.stride[E0] + t1 * a.stride[E1];
-    };
+    const long long base = (pen / a.n[E1]) * a.stride[E0] + (pen % a.n[E1]) * a.stride[E1];
     const long long sd = a.stride[D];
-    // ---- A. load
-    for (int it = tid; it < C * P; it += NT) {
-        int c, p;
-        if (sd == 1) {
-            c = it % C;
-            p = it / C;
-        } else {
-            p = it % P;
-            c = it / P;
-        }
-        if (p >= Pv) continue;
-        const long long base = base_of(p);
+    // ---- A. load (boundary ghosts through bc_ghost, periodic positions wrapped)
+    for (int c = tid; c < NC; c += NT) {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) WC(sacc, q, c) = 0.0;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) WC(sx, q, c) = 0.0;
+        if (c < lo || c >= c_hi) continue;
         auto ld = [&](int x, double* o) {
 #pragma unroll
             for (int q = 0; q < 5; ++q) o[q] = a.U[q * a.ncell + base + (long long)x * sd];
         };
-        const int x = c - 5;
-        double u[5], w[5];
-        if (x >= 0 && x < n)
+        const int x = c + e0 - 9;
+        double u[5], wv[5];
+        if (cyc)
+            ld(wrap(x, n), u);
+        else if (x >= 0 && x < n)
             ld(x, u);
         else
             bc_ghost<D>(e, n, x, a.bc_kind, a.bc_state, ld, u);
-        to_prim(e, u, w);
+        to_prim(e, u, wv);
 #pragma unroll
-        for (int q = 0; q < 5; ++q) {
-            W(sw, q, c, p, C) = w[q];
-            W(sacc, q, c, p, C) = 0.0;
-        }
-#pragma unroll
-        for (int q = 0; q < 3; ++q) W(sx, q, c, p, C) = 0.0;
+        for (int q = 0; q < 5; ++q) WC(sw, q, c) = wv[q];
     }
     __syncthreads();
     constexpr int t1 = 1 + (D + 1) % 3;
     constexpr int t2 = 1 + (D + 2) % 3;
-    // characteristic field a of cell c: row a of L (sparse) at the midpoint's
-    // face-average state, whose nonzero coefficients are (la, lb)
-    //   a = 0: l0un u_n + l04 p,  1: rho + l14 p,  2: u_t1,  3: u_t2,
-    //   a = 4: l4un u_n + l44 p
-    auto chv = [&](int a, double la, double lb, int c, int p) -> double {
-        switch (a) {
-            case 0: return la * W(sw, un, c, p, C) + lb * W(sw, 4, c, p, C);
-            case 1: return W(sw, 0, c, p, C) + lb * W(sw, 4, c, p, C);
-            case 2: return W(sw, t1, c, p, C);
-            case 3: return W(sw, t2, c, p, C);
-            default: return la * W(sw, un, c, p, C) + lb * W(sw, 4, c, p, C);
+    // characteristic field fa of cell c: row fa of L (sparse) at the
+    // midpoint's face-average state, whose nonzero coefficients are (la, lb)
+    //   fa = 0: l0un u_n + l04 p,  1: rho + l14 p,  2: u_t1,  3: u_t2,
+    //   fa = 4: l4un u_n + l44 p
+    auto chv = [&](int fa, double la, double lb, int c) -> double {
+        switch (fa) {
+            case 0: return la * WC(sw, un, c) + lb * WC(sw, 4, c);
+            case 1: return WC(sw, 0, c) + lb * WC(sw, 4, c);
+            case 2: return WC(sw, t1, c);
+            case 3: return WC(sw, t2, c);
+            default: return la * WC(sw, un, c) + lb * WC(sw, 4, c);
         }
     };
-    // (la, lb) of field a at midpoint m, from the rows stored in B0
-    auto lrow = [&](int a, int m, int p, double& la, double& lb) {
-        la = a == 0 ? W(scc, 13, m, p, M) : a == 4 ? W(scc, 16, m, p, M) : 0.0;
-        lb = a == 0 ? W(scc, 14, m, p, M) : a == 1 ? W(scc, 15, m, p, M) : a == 4 ? W(scc, 17, m, p, M) : 0.0;
+    // (la, lb) of field fa at midpoint m, from the rows stored in B0
+    auto lrow = [&](int fa, int m, double& la, double& lb) {
+        la = fa == 0 ? WM(scc, 13, m) : fa == 4 ? WM(scc, 16, m) : 0.0;
+        lb = fa == 0 ? WM(scc, 14, m) : fa == 1 ? WM(scc, 15, m) : fa == 4 ? WM(scc, 17, m) : 0.0;
     };
-    auto eig_of = [&](int m, int p) -> Eig {
-        return eig_at(e, W(sw, 0, m + 2, p, C), W(sw, 0, m + 3, p, C), W(sw, 4, m + 2, p, C), W(sw, 4, m + 3, p, C));
+    auto eig_of = [&](int m) -> Eig {
+        return eig_at(e, WC(sw, 0, m + 2), WC(sw, 0, m + 3), WC(sw, 4, m + 2), WC(sw, 4, m + 3));
     };
     // left / right reconstructions of midpoint m: back-projection w = R c,
     // first-order fallback on an inadmissible state (scheme.rs:310-331)
-    auto recon = [&](const Eig& g, int m, int p, double* wl, double* wr) -> bool {
+    auto recon = [&](const Eig& g, int m, double* wl, double* wr) -> bool {
         const double run0 = -g.c / g.rho, run4 = g.c / g.rho, r40 = g.c * g.c;
         double cl[5], cr[5];
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
-            cl[q] = W(scc, q, m, p, M);
-            cr[q] = W(scc, 5 + q, m, p, M);
+            cl[q] = WM(scc, q, m);
+            cr[q] = WM(scc, 5 + q, m);
         }
         wl[0] = cl[0] + cl[1] + cl[4];
         wr[0] = cr[0] + cr[1] + cr[4];
@@ -475,58 +595,54 @@ __device__ __forceinline__ void adj_group(const AdjArgs& a, long long grp, doubl
         if (!admissible(e, wl[0], wl[4]) || !admissible(e, wr[0], wr[4])) {
 #pragma unroll
             for (int q = 0; q < 5; ++q) {
-                wl[q] = W(sw, q, m + 2, p, C);
-                wr[q] = W(sw, q, m + 3, p, C);
+                wl[q] = WC(sw, q, m + 2);
+                wr[q] = WC(sw, q, m + 3);
             }
             return true;
         }
         return false;
     };
-    // ---- B. forward midpoint fluxes (midpoint m between cells m+2 and m+3),
-    // the same operations as the forward pass (midpoint_flux).
+    // ---- B. forward midpoint fluxes, the same operations as the forward
+    // pass (midpoint_flux).
     // B0: the rows of L per midpoint (once: divisions and a square root)
-    for (int it = tid; it < M * P; it += NT) {
-        const int p = it % P, m = it / P;
-        if (p >= Pv) continue;
-        const Eig g = eig_of(m, p);
-        W(scc, 13, m, p, M) = g.l0un;
-        W(scc, 14, m, p, M) = g.l04;
-        W(scc, 15, m, p, M) = g.l14;
-        W(scc, 16, m, p, M) = g.l4un;
-        W(scc, 17, m, p, M) = g.l44;
+    for (int m = lo + tid; m < m_hi; m += NT) {
+        const Eig g = eig_of(m);
+        WM(scc, 13, m) = g.l0un;
+        WM(scc, 14, m) = g.l04;
+        WM(scc, 15, m) = g.l14;
+        WM(scc, 16, m) = g.l4un;
+        WM(scc, 17, m) = g.l44;
     }
     __syncthreads();
-    // B1: WENO5-Z reconstructions, one work item per (midpoint, field)
-    for (int it = tid; it < 5 * M * P; it += NT) {
-        const int p = it % P, r = it / P, fa = r % 5, m = r / 5;
-        if (p >= Pv) continue;
+    // B1: WENO5-Z reconstructions, one work item per (field, midpoint),
+    // field-major so that a wave mostly holds one field
+    const int nfm = m_hi - lo;
+    for (int it = tid; it < 5 * nfm; it += NT) {
+        const int fa = it / nfm, m = lo + it - fa * nfm;
         double la, lb;
-        lrow(fa, m, p, la, lb);
-        const double v0 = chv(fa, la, lb, m, p), v1 = chv(fa, la, lb, m + 1, p), v2 = chv(fa, la, lb, m + 2, p);
-        const double v3 = chv(fa, la, lb, m + 3, p), v4 = chv(fa, la, lb, m + 4, p);
-        const double v5 = chv(fa, la, lb, m + 5, p);
-        W(scc, fa, m, p, M) = weno5z_interp(v0, v1, v2, v3, v4);
-        W(scc, 5 + fa, m, p, M) = weno5z_interp(v5, v4, v3, v2, v1);
+        lrow(fa, m, la, lb);
+        const double v0 = chv(fa, la, lb, m), v1 = chv(fa, la, lb, m + 1), v2 = chv(fa, la, lb, m + 2);
+        const double v3 = chv(fa, la, lb, m + 3), v4 = chv(fa, la, lb, m + 4);
+        const double v5 = chv(fa, la, lb, m + 5);
+        WM(scc, fa, m) = weno5z_interp(v0, v1, v2, v3, v4);
+        WM(scc, 5 + fa, m) = weno5z_interp(v5, v4, v3, v2, v1);
     }
     __syncthreads();
     // B2: back-projection, fallback, HLLC per midpoint
-    for (int it = tid; it < M * P; it += NT) {
-        const int p = it % P, m = it / P;
-        if (p >= Pv) continue;
-        const Eig g = eig_of(m, p);
-        double wl[5], wr[5], f[5];
-        recon(g, m, p, wl, wr);
-        hllc<D>(e, wl, wr, f);
+    for (int m = lo + tid; m < m_hi; m += NT) {
+        const Eig g = eig_of(m);
+        double wl[5], wr[5], fx[5];
+        recon(g, m, wl, wr);
+        hllc<D>(e, wl, wr, fx);
 #pragma unroll
-        for (int q = 0; q < 5; ++q) W(smf, q, m, p, M) = f[q];
+        for (int q = 0; q < 5; ++q) WM(smf, q, m) = fx[q];
     }
     __syncthreads();
     // ---- C. faces in chunks of increasing f: a chunk only overwrites the
     // midpoint slots of its own faces, which later chunks do not read
-    const int fchunk = NT / P;
-    for (int f0 = 0; f0 < NF; f0 += fchunk) {
-        const int p = tid % P, f = f0 + tid / P;
-        const bool live = tid < fchunk * P && p < Pv && f < NF;
+    for (int f0 = lo; f0 < f_hi; f0 += NT) {
+        const int f = f0 + tid;
+        const bool live = f < f_hi;
         double fhb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         double wlb[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, wrb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         if (live) {
@@ -534,30 +650,33 @@ __device__ __forceinline__ void adj_group(const AdjArgs& a, long long grp, doubl
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
                 double v = 0.0;
-                v += S6_0 * W(smf, k, f + 2, p, M);
-                v += S6_1 * W(smf, k, f + 3, p, M);
-                v += S6_1 * W(smf, k, f + 1, p, M);
-                v += S6_2 * W(smf, k, f + 4, p, M);
-                v += S6_2 * W(smf, k, f, p, M);
+                v += S6_0 * WM(smf, k, f + 2);
+                v += S6_1 * WM(smf, k, f + 3);
+                v += S6_1 * WM(smf, k, f + 1);
+                v += S6_2 * WM(smf, k, f + 4);
+                v += S6_2 * WM(smf, k, f);
                 fo[k] = v;
             }
-            // face cotangent from the divergence
-            const long long base = base_of(p);
+            // face cotangent from the divergence: Fbar = (lam_f - lam_{f-1}) / dx
+            // (lam = 0 outside the pencil, cyclic on a periodic axis)
+            const int fg = f + e0 - 4;
+            const bool has_r = cyc || fg < n, has_l = cyc || fg > 0;
+            const long long xr = cyc ? wrap(fg, n) : fg, xl = cyc ? wrap(fg - 1, n) : fg - 1;
             double Fb[5];
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
-                const double lr = f < n ? a.LAM[k * a.ncell + base + (long long)f * sd] : 0.0;
-                const double ll = f > 0 ? a.LAM[k * a.ncell + base + (long long)(f - 1) * sd] : 0.0;
+                const double lr = has_r ? a.LAM[k * a.ncell + base + xr * sd] : 0.0;
+                const double ll = has_l ? a.LAM[k * a.ncell + base + xl * sd] : 0.0;
                 Fb[k] = (lr - ll) * a.inv_dx;
             }
             bool limited = false;
-            double lo = 1.0, hi = 1.0, al = 0.0;
+            double lo_t = 1.0, hi_t = 1.0, al = 0.0;
             double wl[5], wr[5], ul[5], ur[5], fl[5], fr[5], flf[5];
             if (a.lam_lim > 0.0) {
 #pragma unroll
                 for (int k = 0; k < 5; ++k) {
-                    wl[k] = W(sw, k, f + 4, p, C);
-                    wr[k] = W(sw, k, f + 5, p, C);
+                    wl[k] = WC(sw, k, f + 4);
+                    wr[k] = WC(sw, k, f + 5);
                 }
                 to_cons(e, wl, ul);
                 to_cons(e, wr, ur);
@@ -594,8 +713,8 @@ __device__ __forceinline__ void adj_group(const AdjArgs& a, long long grp, doubl
                         else
                             h0 = mid;
                     }
-                    lo = l0;
-                    hi = h0;
+                    lo_t = l0;
+                    hi_t = h0;
                 }
             }
             if (limited) {
@@ -606,8 +725,8 @@ __device__ __forceinline__ void adj_group(const AdjArgs& a, long long grp, doubl
 #pragma unroll
                 for (int k = 0; k < 5; ++k) {
                     const bool fk = isfinite(fo[k]);
-                    fhb[k] = fk ? lo * Fb[k] : 0.0;
-                    flfb[k] = fk ? (1.0 - lo) * Fb[k] : Fb[k];
+                    fhb[k] = fk ? lo_t * Fb[k] : 0.0;
+                    flfb[k] = fk ? (1.0 - lo_t) * Fb[k] : Fb[k];
                     if (fk) thb += Fb[k] * (fo[k] - flf[k]);
                     flb[k] = 0.0;
                     frb[k] = 0.0;
@@ -619,14 +738,14 @@ __device__ __forceinline__ void adj_group(const AdjArgs& a, long long grp, doubl
                 // differentiated implicitly at the constraint g that fails
                 // at the upper bound: d theta = -dg / (dg/dtheta).
                 // theta = 0 (nothing admissible above it) stays constant.
-                const int which = lo > 0.0 ? limiter_constraint(e, ul, ur, fl, fr, fo, flf, tl, hi) : -1;
+                const int which = lo_t > 0.0 ? limiter_constraint(e, ul, ur, fl, fr, fo, flf, tl, hi_t) : -1;
                 if (which >= 0) {
                     const bool left = which < 3;  // test state ul - tl (fx - fl), else ur + tl (fx - fr)
                     const double sg = left ? -tl : tl;
                     double st[5], ws[5], gw[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, gs[5];
 #pragma unroll
                     for (int k = 0; k < 5; ++k) {
-                        const double fx = isfinite(fo[k]) ? lo * fo[k] + (1.0 - lo) * flf[k] : flf[k];
+                        const double fx = isfinite(fo[k]) ? lo_t * fo[k] + (1.0 - lo_t) * flf[k] : flf[k];
                         st[k] = left ? ul[k] - tl * (fx - fl[k]) : ur[k] + tl * (fx - fr[k]);
                     }
                     to_prim(e, st, ws);
@@ -649,8 +768,8 @@ __device__ __forceinline__ void adj_group(const AdjArgs& a, long long grp, doubl
                             const double sb = gb * gs[k];  // cotangent of the test state
                             const double fxb = sg * sb;
                             if (isfinite(fo[k])) {
-                                fhb[k] += lo * fxb;
-                                flfb[k] += (1.0 - lo) * fxb;
+                                fhb[k] += lo_t * fxb;
+                                flfb[k] += (1.0 - lo_t) * fxb;
                             } else {
                                 flfb[k] += fxb;
                             }
@@ -695,166 +814,159 @@ __device__ __forceinline__ void adj_group(const AdjArgs& a, long long grp, doubl
         }
         // a cell takes the right state of face c-5 before the left state of
         // face c-4: increasing face order within a chunk as across chunks,
-        // so the sums do not depend on the chunking (on P)
+        // so the sums do not depend on the tiling
         __syncthreads();
         if (live) {
 #pragma unroll
-            for (int k = 0; k < 5; ++k) W(smf, k, f, p, M) = fhb[k];
+            for (int k = 0; k < 5; ++k) WM(smf, k, f) = fhb[k];
 #pragma unroll
-            for (int k = 0; k < 5; ++k) W(sacc, k, f + 5, p, C) += wrb[k];
+            for (int k = 0; k < 5; ++k) WC(sacc, k, f + 5) += wrb[k];
         }
         __syncthreads();
         if (live) {
 #pragma unroll
-            for (int k = 0; k < 5; ++k) W(sacc, k, f + 4, p, C) += wlb[k];
+            for (int k = 0; k < 5; ++k) WC(sacc, k, f + 4) += wlb[k];
         }
         __syncthreads();
     }
-    // ---- D2. per midpoint: cotangent from its five faces (S6 weights), the
-    // HLLC VJP and the back-projection VJP -> clb, crb (scc 0..9) and the
+    // ---- D2. per VJP midpoint: cotangent from its five faces (S6 weights),
+    // the HLLC VJP and the back-projection VJP -> clb, crb (scc 0..9) and the
     // eigenvector partials (scc 10, 11); on the fallback the state
     // cotangents of cells m+2, m+3 instead (flag scc 12)
-    for (int it = tid; it < M * P; it += NT) {
-        const int p = it % P, m = it / P;
-        if (p >= Pv) continue;
+    for (int m = v_lo + tid; m < v_hi; m += NT) {
         double mb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         const double w6[5] = {S6_2, S6_1, S6_0, S6_1, S6_2};
 #pragma unroll
         for (int j = 0; j < 5; ++j) {
             const int f = m - 4 + j;  // faces f = m-4 .. m use midpoint m with weight S6[|m - f - 2|]
-            if (f < 0 || f > n) continue;
+            if (f < lo || f >= f_hi) continue;
 #pragma unroll
-            for (int q = 0; q < 5; ++q) mb[q] += w6[4 - j] * W(smf, q, f, p, M);
+            for (int q = 0; q < 5; ++q) mb[q] += w6[4 - j] * WM(smf, q, f);
         }
-        const Eig g = eig_of(m, p);
+        const Eig g = eig_of(m);
         double wl[5], wr[5], wlb[5], wrb[5];
-        const bool fb = recon(g, m, p, wl, wr);
+        const bool fb = recon(g, m, wl, wr);
         hllc_vjp<D>(e, wl, wr, mb, wlb, wrb);
         if (fb) {
 #pragma unroll
             for (int q = 0; q < 5; ++q) {
-                W(scc, q, m, p, M) = wlb[q];
-                W(scc, 5 + q, m, p, M) = wrb[q];
+                WM(scc, q, m) = wlb[q];
+                WM(scc, 5 + q, m) = wrb[q];
             }
-            W(scc, 10, m, p, M) = 0.0;
-            W(scc, 11, m, p, M) = 0.0;
-            W(scc, 12, m, p, M) = 1.0;
+            WM(scc, 10, m) = 0.0;
+            WM(scc, 11, m) = 0.0;
+            WM(scc, 12, m) = 1.0;
             continue;
         }
         double cl[5], cr[5];
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
-            cl[q] = W(scc, q, m, p, M);
-            cr[q] = W(scc, 5 + q, m, p, M);
+            cl[q] = WM(scc, q, m);
+            cr[q] = WM(scc, 5 + q, m);
         }
         const double run0 = -g.c / g.rho, run4 = g.c / g.rho, r40 = g.c * g.c;
         const double run0b = wlb[un] * cl[0] + wrb[un] * cr[0];
         const double run4b = wlb[un] * cl[4] + wrb[un] * cr[4];
         const double r40b = wlb[4] * (cl[0] + cl[4]) + wrb[4] * (cr[0] + cr[4]);
-        W(scc, 0, m, p, M) = wlb[0] + run0 * wlb[un] + r40 * wlb[4];
-        W(scc, 5, m, p, M) = wrb[0] + run0 * wrb[un] + r40 * wrb[4];
-        W(scc, 1, m, p, M) = wlb[0];
-        W(scc, 6, m, p, M) = wrb[0];
-        W(scc, 2, m, p, M) = wlb[t1];
-        W(scc, 7, m, p, M) = wrb[t1];
-        W(scc, 3, m, p, M) = wlb[t2];
-        W(scc, 8, m, p, M) = wrb[t2];
-        W(scc, 4, m, p, M) = wlb[0] + run4 * wlb[un] + r40 * wlb[4];
-        W(scc, 9, m, p, M) = wrb[0] + run4 * wrb[un] + r40 * wrb[4];
+        WM(scc, 0, m) = wlb[0] + run0 * wlb[un] + r40 * wlb[4];
+        WM(scc, 5, m) = wrb[0] + run0 * wrb[un] + r40 * wrb[4];
+        WM(scc, 1, m) = wlb[0];
+        WM(scc, 6, m) = wrb[0];
+        WM(scc, 2, m) = wlb[t1];
+        WM(scc, 7, m) = wrb[t1];
+        WM(scc, 3, m) = wlb[t2];
+        WM(scc, 8, m) = wrb[t2];
+        WM(scc, 4, m) = wlb[0] + run4 * wlb[un] + r40 * wlb[4];
+        WM(scc, 9, m) = wrb[0] + run4 * wrb[un] + r40 * wrb[4];
         // run0 = -c / rho, run4 = c / rho, r40 = c^2 as functions of rho, c
-        W(scc, 10, m, p, M) = run0b * g.c / (g.rho * g.rho) - run4b * g.c / (g.rho * g.rho);
-        W(scc, 11, m, p, M) = -run0b / g.rho + run4b / g.rho + 2.0 * g.c * r40b;
-        W(scc, 12, m, p, M) = 0.0;
+        WM(scc, 10, m) = run0b * g.c / (g.rho * g.rho) - run4b * g.c / (g.rho * g.rho);
+        WM(scc, 11, m) = -run0b / g.rho + run4b / g.rho + 2.0 * g.c * r40b;
+        WM(scc, 12, m) = 0.0;
     }
     __syncthreads();
-    // ---- D3. per (midpoint, field): the WENO5-Z VJPs of both sides and the
-    // projection ch = L s (the coefficient sums left in scc 0..9 for D4),
-    // scattered to the six stencil cells in six
-    // conflict-free sub-phases.  Fields 1 and 4 keep their shares of the
-    // pressure (and field 4 of the normal velocity) cotangent apart in sx, so
-    // that no two fields add to one slot in a sub-phase.  A cell takes
-    // midpoints c-5 .. c in increasing order, within a chunk as across
-    // chunks, so the sums do not depend on P.
-    // Work items run field-major, so that a wave mostly holds one field of
-    // neighbouring midpoints: flat stencils (e.g. every stencil of a
-    // velocity component that is zero) then take weno_vjp's short path for
-    // whole waves.
-    for (int i0 = 0; i0 < 5 * M * P; i0 += NT) {
+    // ---- D3. per (field, VJP midpoint): the WENO5-Z VJPs of both sides and
+    // the projection ch = L s (the coefficient sums left in scc 0..9 for D4),
+    // scattered to the six stencil cells in six conflict-free sub-phases.
+    // Fields 1 and 4 keep their shares of the pressure (and field 4 of the
+    // normal velocity) cotangent apart in sx, so that no two fields add to
+    // one slot in a sub-phase.  A cell takes midpoints c-5 .. c in
+    // increasing order, within a chunk as across chunks, so the sums do not
+    // depend on the tiling.  Work items run field-major, so that a wave
+    // mostly holds one field of neighbouring midpoints: flat stencils (e.g.
+    // every stencil of a velocity component that is zero) then take
+    // weno_vjp's short path for whole waves.
+    const int nvm = v_hi - v_lo;
+    for (int i0 = 0; i0 < 5 * nvm; i0 += NT) {
         const int it = i0 + tid;
-        const int p = it % P, r = it / P, m = r % M, fa = r / M;
-        const bool live = it < 5 * M * P && p < Pv && W(scc, 12, m, p, M) == 0.0;
+        const bool in = it < 5 * nvm;
+        const int fa = in ? it / nvm : 0, m = in ? v_lo + it - fa * nvm : v_lo;
+        const bool live = in && WM(scc, 12, m) == 0.0;
         double chb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         double la = 0.0, lb = 0.0;
         if (live) {
-            lrow(fa, m, p, la, lb);
-            double v[5], vb[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            lrow(fa, m, la, lb);
+            double v[6];
 #pragma unroll
-            for (int k = 0; k < 5; ++k) v[k] = chv(fa, la, lb, m + k, p);
-            weno_vjp(v, W(scc, fa, m, p, M), chb);
-            const double v5 = chv(fa, la, lb, m + 5, p);
-            double vr[5] = {v5, v[4], v[3], v[2], v[1]};
-            weno_vjp(vr, W(scc, 5 + fa, m, p, M), vb);
-#pragma unroll
-            for (int k = 0; k < 5; ++k) chb[5 - k] += vb[k];
+            for (int k = 0; k < 6; ++k) v[k] = chv(fa, la, lb, m + k);
+            weno_vjp_pair(v, WM(scc, fa, m), WM(scc, 5 + fa, m), chb);
             // sums for the coefficient cotangents: d la = sum chb u_n,
             // d lb = sum chb p (fields 0, 1, 4)
             double sa = 0.0, sbp = 0.0;
 #pragma unroll
             for (int k = 0; k < 6; ++k) {
-                sa += chb[k] * W(sw, un, m + k, p, C);
-                sbp += chb[k] * W(sw, 4, m + k, p, C);
+                sa += chb[k] * WC(sw, un, m + k);
+                sbp += chb[k] * WC(sw, 4, m + k);
             }
-            W(scc, fa, m, p, M) = sa;
-            W(scc, 5 + fa, m, p, M) = sbp;
+            WM(scc, fa, m) = sa;
+            WM(scc, 5 + fa, m) = sbp;
         }
         for (int j = 5; j >= 0; --j) {
             if (live) {
                 const double gj = chb[j];
                 const int c = m + j;
                 if (fa == 0) {
-                    W(sacc, un, c, p, C) += la * gj;
-                    W(sacc, 4, c, p, C) += lb * gj;
+                    WC(sacc, un, c) += la * gj;
+                    WC(sacc, 4, c) += lb * gj;
                 } else if (fa == 1) {
-                    W(sacc, 0, c, p, C) += gj;
-                    W(sx, 0, c, p, C) += lb * gj;
+                    WC(sacc, 0, c) += gj;
+                    WC(sx, 0, c) += lb * gj;
                 } else if (fa == 2) {
-                    W(sacc, t1, c, p, C) += gj;
+                    WC(sacc, t1, c) += gj;
                 } else if (fa == 3) {
-                    W(sacc, t2, c, p, C) += gj;
+                    WC(sacc, t2, c) += gj;
                 } else {
-                    W(sx, 1, c, p, C) += la * gj;
-                    W(sx, 2, c, p, C) += lb * gj;
+                    WC(sx, 1, c) += la * gj;
+                    WC(sx, 2, c) += lb * gj;
                 }
             }
             __syncthreads();
         }
     }
-    // ---- D4. per midpoint: the average state's cotangent (rho and p of
+    // ---- D4. per VJP midpoint: the average state's cotangent (rho and p of
     // cells m+2, m+3 through rho, c), or the fallback's state cotangents;
     // a cell takes midpoint c-3 (as m+3) before c-2 (as m+2)
-    for (int i0 = 0; i0 < M * P; i0 += NT) {
-        const int it = i0 + tid;
-        const int p = it % P, m = it / P;
-        const bool live = it < M * P && p < Pv;
+    for (int m0 = v_lo; m0 < v_hi; m0 += NT) {
+        const int m = m0 + tid;
+        const bool live = m < v_hi;
         double b2[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, b3[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         if (live) {
-            if (W(scc, 12, m, p, M) != 0.0) {
+            if (WM(scc, 12, m) != 0.0) {
 #pragma unroll
                 for (int q = 0; q < 5; ++q) {
-                    b2[q] = W(scc, q, m, p, M);
-                    b3[q] = W(scc, 5 + q, m, p, M);
+                    b2[q] = WM(scc, q, m);
+                    b3[q] = WM(scc, 5 + q, m);
                 }
             } else {
-                const Eig g = eig_of(m, p);
+                const Eig g = eig_of(m);
                 // L coefficients as functions of rho and c: l0un = -rho/(2c),
                 // l04 = l44 = 1/(2c^2), l14 = -1/c^2, l4un = rho/(2c)
-                const double l0unb = W(scc, 0, m, p, M), l04b = W(scc, 5, m, p, M);
-                const double l14b = W(scc, 6, m, p, M);
-                const double l4unb = W(scc, 4, m, p, M), l44b = W(scc, 9, m, p, M);
+                const double l0unb = WM(scc, 0, m), l04b = WM(scc, 5, m);
+                const double l14b = WM(scc, 6, m);
+                const double l4unb = WM(scc, 4, m), l44b = WM(scc, 9, m);
                 const double c3 = g.c * g.c * g.c;
-                double rhob = W(scc, 10, m, p, M) - l0unb / (2.0 * g.c) + l4unb / (2.0 * g.c);
-                double cb = W(scc, 11, m, p, M) + l0unb * g.rho / (2.0 * g.c * g.c) -
-                            l4unb * g.rho / (2.0 * g.c * g.c) - (l04b + l44b) / c3 + 2.0 * l14b / c3;
+                double rhob = WM(scc, 10, m) - l0unb / (2.0 * g.c) + l4unb / (2.0 * g.c);
+                double cb = WM(scc, 11, m) + l0unb * g.rho / (2.0 * g.c * g.c) - l4unb * g.rho / (2.0 * g.c * g.c) -
+                            (l04b + l44b) / c3 + 2.0 * l14b / c3;
                 double pb = 0.0;
                 sound_speed_vjp(e, g.rho, g.avp, cb, rhob, pb);
                 b2[0] = b3[0] = 0.5 * rhob;
@@ -863,65 +975,61 @@ __device__ __forceinline__ void adj_group(const AdjArgs& a, long long grp, doubl
         }
         if (live) {
 #pragma unroll
-            for (int q = 0; q < 5; ++q) W(sacc, q, m + 3, p, C) += b3[q];
+            for (int q = 0; q < 5; ++q) WC(sacc, q, m + 3) += b3[q];
         }
         __syncthreads();
         if (live) {
 #pragma unroll
-            for (int q = 0; q < 5; ++q) W(sacc, q, m + 2, p, C) += b2[q];
+            for (int q = 0; q < 5; ++q) WC(sacc, q, m + 2) += b2[q];
         }
         __syncthreads();
     }
-    // ---- E. conserved cotangents of every loaded cell
-    for (int it = tid; it < C * P; it += NT) {
-        const int p = it % P, c = it / P;
-        if (p >= Pv) continue;
-        double w[5], wb[5], ub[5];
+    // ---- E. conserved cotangents of the tile's output positions
+    for (int c = 9 + tid; c < ne + 9; c += NT) {
+        double wv[5], wb[5], ub[5];
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
-            w[q] = W(sw, q, c, p, C);
-            wb[q] = W(sacc, q, c, p, C);
+            wv[q] = WC(sw, q, c);
+            wb[q] = WC(sacc, q, c);
         }
-        wb[un] += W(sx, 1, c, p, C);
-        wb[4] += W(sx, 0, c, p, C) + W(sx, 2, c, p, C);
-        to_prim_vjp(e, w, wb, ub);
+        wb[un] += WC(sx, 1, c);
+        wb[4] += WC(sx, 0, c) + WC(sx, 2, c);
+        to_prim_vjp(e, wv, wb, ub);
 #pragma unroll
-        for (int q = 0; q < 5; ++q) W(sacc, q, c, p, C) = ub[q];
+        for (int q = 0; q < 5; ++q) WC(sacc, q, c) = ub[q];
     }
     __syncthreads();
-    // interior cells: own cotangent plus the ghosts copied from them
-    for (int it = tid; it < n * P; it += NT) {
-        int x, p;
-        if (sd == 1) {
-            x = it % n;
-            p = it / n;
-        } else {
-            p = it % P;
-            x = it / P;
-        }
-        if (p >= Pv) continue;
+    // interior cells: own cotangent plus the ghosts copied from them (a
+    // non-periodic edge tile holds its edge's ghosts)
+    const int x0 = tl_.x0, nx = tl_.x1 - tl_.x0;
+    for (int it = tid; it < nx; it += NT) {
+        const int x = x0 + it;
+        const int c = x - e0 + 9;
         double tot[5];
 #pragma unroll
-        for (int q = 0; q < 5; ++q) tot[q] = W(sacc, q, x + 5, p, C);
-        for (int gi = 0; gi < 10; ++gi) {
-            const int gx = gi < 5 ? gi - 5 : n + gi - 5;
-            const int side = gx < 0 ? 0 : 1;
-            const int kind = a.bc_kind[side];
-            int src = -1;
-            double flip = 1.0;
-            if (kind == 0)
-                src = ((gx % n) + n) % n;
-            else if (kind == 1)
-                src = side == 0 ? 0 : n - 1;
-            else if (kind == 2) {
-                src = side == 0 ? -1 - gx : 2 * n - 1 - gx;
-                flip = -1.0;
-            }
-            if (src != x) continue;
+        for (int q = 0; q < 5; ++q) tot[q] = WC(sacc, q, c);
+        if (!cyc && (x < 5 || x >= n - 5)) {
+            for (int gi = 0; gi < 10; ++gi) {
+                const int gx = gi < 5 ? gi - 5 : n + gi - 5;
+                if (gx < e0 || gx >= tl_.e1) continue;
+                const int side = gx < 0 ? 0 : 1;
+                const int kind = a.bc_kind[side];
+                int src = -1;
+                double flip = 1.0;
+                if (kind == 0)
+                    src = wrap(gx, n);
+                else if (kind == 1)
+                    src = side == 0 ? 0 : n - 1;
+                else if (kind == 2) {
+                    src = side == 0 ? -1 - gx : 2 * n - 1 - gx;
+                    flip = -1.0;
+                }
+                if (src != x) continue;
 #pragma unroll
-            for (int q = 0; q < 5; ++q) tot[q] += (q == un ? flip : 1.0) * W(sacc, q, gx + 5, p, C);
+                for (int q = 0; q < 5; ++q) tot[q] += (q == un ? flip : 1.0) * WC(sacc, q, gx - e0 + 9);
+            }
         }
-        const long long cell = base_of(p) + (long long)x * sd;
+        const long long cell = base + (long long)x * sd;
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
             const long long ix = q * a.ncell + cell;
@@ -930,16 +1038,27 @@ __device__ __forceinline__ void adj_group(const AdjArgs& a, long long grp, doubl
     }
 }
 
+// Three workgroups per CU (LDS of a tile <= 45 KB): at most 168 VGPRs, so
+// that their 12 waves are resident together.  A tile's phases keep only a
+// few of its waves busy (about 140 midpoints for 256 threads), so tiles in
+// flight per CU, not LDS per tile, set the speed: 128-cell tiles with 256
+// threads ran the 512^2 backward 12% faster than whole pencils, and 10-25%
+// faster than 200-256-cell tiles with 384 threads.
 template <int D>
-__global__ __launch_bounds__(kAdjThreads) void adj_pass(AdjArgs a) {
+__global__ __launch_bounds__(kAdjThreads) __attribute__((amdgpu_waves_per_eu(3))) void adj_pass(AdjArgs a) {
     extern __shared__ double lds[];
-    adj_group<D>(a, blockIdx.x, lds);
+    // neighbouring work items (tiles of neighbouring pencils) on one XCD:
+    // workgroups are dealt round-robin to the 8 XCDs, each with its own L2
+    const long long G = gridDim.x, b = blockIdx.x;
+    const long long q = G / 8, r = G % 8, x = b % 8, k = b / 8;
+    const long long w = x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+    adj_group<D>(a, w, lds);
 }
 
-// Ensemble VJP: the pencil groups of every instance in one persistent
-// launch (instance i owns work items prefix[i] .. prefix[i+1]-1).
+// Ensemble VJP: the tiles of every instance in one persistent launch
+// (instance i owns work items prefix[i] .. prefix[i+1]-1).
 template <int D>
-__global__ __launch_bounds__(kAdjThreads) void ens_adj_pass(EnsAdjArgs e) {
+__global__ __launch_bounds__(kAdjThreads) __attribute__((amdgpu_waves_per_eu(3))) void ens_adj_pass(EnsAdjArgs e) {
     extern __shared__ double lds[];
     for (long long w = blockIdx.x; w < e.total; w += gridDim.x) {
         int lo = 0, hi = e.ninst;
@@ -978,25 +1097,34 @@ hipError_t launch_ens_adj_pass(int d, const EnsAdjArgs& e, size_t lds, unsigned 
     return hipGetLastError();
 }
 
-// LDS per pencil: 13 arrays over the n + 10 loaded cells (primitives,
-// cotangents, field shares) and 23 over the n + 5 midpoints (fluxes / face
-// cotangents, reconstructions / cotangents / partials / flag, rows of L)
-size_t adj_lds_bytes(int n, int P) {
-    return sizeof(double) * (size_t)P * (13 * (size_t)(n + 10) + 23 * (size_t)(n + 5));
+// Tiles per pencil: pencils longer than kAdjTileCells cells are split
+// evenly; an axis periodic on one side only keeps whole pencils (its ghosts
+// are copies of the far end).
+int adj_tiles(int n, int kind_lo, int kind_hi) {
+    if ((kind_lo == 0) != (kind_hi == 0)) return 1;
+    return std::max(1, (n + kAdjTileCells - 1) / kAdjTileCells);
 }
 
-// Pencils per workgroup: as many as fit 152 KB of LDS (one workgroup per
-// CU), at most 16.
-int adj_pencils_per_group(int n) {
-    const long long per = (long long)adj_lds_bytes(n, 1);
-    return (int)std::max(1ll, std::min(16ll, (152ll << 10) / per));
+// Output positions of a pencil's longest tile: its interior cells plus the
+// ghost positions of a non-periodic edge.
+int adj_tile_extent(int n, int ntile, int kind_lo, int kind_hi) {
+    const bool cyc = kind_lo == 0 && kind_hi == 0;
+    return (n + ntile - 1) / ntile + (cyc ? 0 : (ntile == 1 ? 10 : 5));
+}
+
+// LDS of a tile with ne output positions: 13 arrays over its ne + 18 cells
+// (primitives, cotangents, field shares) and 23 over its ne + 13 forward
+// midpoints (fluxes / face cotangents, reconstructions / cotangents /
+// partials / flag, rows of L)
+size_t adj_lds_bytes(int ne) {
+    return sizeof(double) * (13 * (size_t)(ne + 18) + 23 * (size_t)(ne + 13));
 }
 
 hipError_t launch_adj_pass(int d, const AdjArgs& a, hipStream_t s) {
     const int n = a.n[d];
     const long long npen = a.ncell / n;
-    const unsigned grid = (unsigned)((npen + a.P - 1) / a.P);
-    const size_t lds = adj_lds_bytes(n, a.P);
+    const unsigned grid = (unsigned)(npen * a.ntile);
+    const size_t lds = adj_lds_bytes(a.ne);
     hipError_t er;
     if (d == 0) {
         if ((er = allow_lds(adj::adj_pass<0>, lds)) != hipSuccess) return er;

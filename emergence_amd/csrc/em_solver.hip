@@ -671,8 +671,11 @@ struct Solver {
                 if (bc_kind[d][sd] == BC_REFLECT && n[d] < 5)
                     throw Error(EM_ERR_ARG, "the adjoint needs at least 5 cells on a reflecting axis");
             }
-            if (adj_lds_bytes((int)n[d], 1) > std::max(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin))
-                throw Error(EM_ERR_ARG, "the adjoint keeps whole pencils in LDS: axis too long");
+            const int nt = adj_tiles((int)n[d], bc_kind[d][0], bc_kind[d][1]);
+            const size_t need = adj_lds_bytes(adj_tile_extent((int)n[d], nt, bc_kind[d][0], bc_kind[d][1]));
+            if (need > std::max(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin))
+                throw Error(EM_ERR_ARG,
+                            "the adjoint keeps whole pencils in LDS on an axis periodic on one side only: axis too long");
         }
     }
 
@@ -698,7 +701,8 @@ struct Solver {
             a.inv_dx = 1.0 / dx[d];
             a.lam_lim = positivity ? (double)nactive * stage_dt / dx[d] : 0.0;  // as the forward pass
             a.first = first ? 1 : 0;
-            a.P = adj_pencils_per_group((int)n[d]);
+            a.ntile = adj_tiles((int)n[d], bc_kind[d][0], bc_kind[d][1]);
+            a.ne = adj_tile_extent((int)n[d], a.ntile, bc_kind[d][0], bc_kind[d][1]);
             HIPCHK(launch_adj_pass(d, a, stream));
             first = false;
         }
@@ -1228,14 +1232,17 @@ struct Ensemble {
         double* Zb = work + 2 * tot_nv;
         double* T = work + 3 * tot_nv;
         for (int i = 0; i < n; ++i) in[i]->host_to_soa(dj_final[i], A + off[i]);
-        // pencils per group of each instance and axis (its own launch shape;
-        // the sums do not depend on it); LDS per axis: the largest need
+        // tiles of each instance and axis (its own, as a single-instance
+        // backward would use; the sums do not depend on the tiling); LDS per
+        // axis: the largest need
         size_t lds_ax[3] = {0, 0, 0};
         for (auto* S : in)
             for (int d = 0; d < 3; ++d)
                 if (S->active(d)) {
                     const int nd = (int)S->n[d];
-                    lds_ax[d] = std::max(lds_ax[d], adj_lds_bytes(nd, adj_pencils_per_group(nd)));
+                    const int nt = adj_tiles(nd, S->bc_kind[d][0], S->bc_kind[d][1]);
+                    lds_ax[d] = std::max(lds_ax[d],
+                                         adj_lds_bytes(adj_tile_extent(nd, nt, S->bc_kind[d][0], S->bc_kind[d][1])));
                 }
         AdjArgs* d_aa = nullptr;
         long long* d_pre = nullptr;
@@ -1250,8 +1257,8 @@ struct Ensemble {
         for (int d = 0; d < 3; ++d)
             for (int i = 0; i < n; ++i) {
                 const Solver& S = *in[i];
-                const int P = S.active(d) ? adj_pencils_per_group((int)S.n[d]) : 1;
-                const long long groups = S.active(d) ? (S.ncell / S.n[d] + P - 1) / P : 0;
+                const long long groups =
+                    S.active(d) ? S.ncell / S.n[d] * adj_tiles((int)S.n[d], S.bc_kind[d][0], S.bc_kind[d][1]) : 0;
                 pre[d * (n + 1) + i + 1] = pre[d * (n + 1) + i] + groups;
             }
         HIPCHK(hipMemcpyAsync(d_pre, pre.data(), sizeof(long long) * pre.size(), hipMemcpyHostToDevice, stream));
@@ -1297,7 +1304,8 @@ struct Ensemble {
                             for (int e = 0; e < 3; ++e)
                                 if (S.active(e) && first_ax < 0) first_ax = e;
                             a.first = d == first_ax ? 1 : 0;
-                            a.P = adj_pencils_per_group((int)S.n[d]);
+                            a.ntile = adj_tiles((int)S.n[d], S.bc_kind[d][0], S.bc_kind[d][1]);
+                            a.ne = adj_tile_extent((int)S.n[d], a.ntile, S.bc_kind[d][0], S.bc_kind[d][1]);
                         }
             HIPCHK(hipMemcpyAsync(d_aa, aa.data(), sizeof(AdjArgs) * aa.size(), hipMemcpyHostToDevice, stream));
             for (int64_t q = s1 - 1; q >= s0; --q) {

@@ -117,8 +117,10 @@ int em_solver_synchronize(em_solver* s);
 /* ------------------------------------------------------------------------
  * Discrete adjoint (no counterpart in the reference, which has none; the
  * forward/backward pair the survey of the path asks for).  FluxMode::Weno
- * only, periodic / outflow / reflect (n >= 5) / Dirichlet boundaries, axes up
- * to the LDS limit (whole pencils in LDS: 562 cells).  Conventions: branches
+ * only, periodic / outflow / reflect (n >= 5) / Dirichlet boundaries, axes of
+ * any length (pencils are cut into tiles of at most 128 cells with their
+ * halo; an axis periodic on one side only keeps whole pencils in LDS, up to
+ * 562 cells).  Conventions: branches
  * follow the forward run, min/max differentiate the returned operand (the
  * first on a tie), d|x|/dx = sign(x) with sign(0) = 0; the positivity
  * limiter's blend weight is differentiated implicitly at the admissibility

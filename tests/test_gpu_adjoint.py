@@ -154,14 +154,24 @@ def test_backward_matches_central_differences(em):
     assert checks == 8
 
 
-def test_adjoint_rhs_long_pencils(em):
-    # 540-cell pencils (one pencil per workgroup, 138 KB of LDS, near the
-    # 562-cell limit) and 4-cell ones (16 pencils per workgroup) in one grid
-    print("adjoint rhs 540 x 4 periodic", flush=True)
-    n = (540, 4, 1)
-    o0 = OracleAdj(n, (0, 0, 0), (1, 1, 1))
+@pytest.mark.parametrize("case", ["periodic-540", "reflect-dirichlet-300", "outflow-263"])
+def test_adjoint_rhs_long_pencils(em, case):
+    # pencils cut into tiles of at most 128 cells: 540 periodic cells (5
+    # cyclic tiles of 108) across 4-cell pencils (one tile each, the window
+    # wrapping several times); 300 cells between a reflecting and a Dirichlet
+    # boundary and 263 outflow cells (edge tiles holding the ghost positions,
+    # uneven splits)
+    print(f"adjoint rhs long pencils {case}", flush=True)
+    if case == "periodic-540":
+        n, bc = (540, 4, 1), [["periodic", "periodic"]] * 3
+    elif case == "reflect-dirichlet-300":
+        n = (300, 4, 1)
+        bc = [["reflect", ("dirichlet", (1.0, 0.1, 0.0, 0.0, 2.6))], ["periodic", "periodic"], ["periodic", "periodic"]]
+    else:
+        n, bc = (263, 3, 1), [["outflow", "outflow"], ["outflow", "reflect"], ["periodic", "periodic"]]
+    o0 = OracleAdj(n, (0, 0, 0), (1, 1, 1), bc=bc)
     u = smooth_state(o0, 8)
-    o, g = _pair(em, n, [["periodic", "periodic"]] * 3, u)
+    o, g = _pair(em, n, bc, u)
     rng = np.random.default_rng(12)
     lam = np.zeros_like(u)
     o.interior(lam)[...] = rng.standard_normal(o.interior(lam).shape)
