@@ -551,23 +551,31 @@ __device__ __forceinline__ void adj_group(const AdjArgs& a, long long w, double*
     __syncthreads();
     constexpr int t1 = 1 + (D + 1) % 3;
     constexpr int t2 = 1 + (D + 2) % 3;
-    // characteristic field fa of cell c: row fa of L (sparse) at the
-    // midpoint's face-average state, whose nonzero coefficients are (la, lb)
+    // characteristic field fa over cells m .. m+5: row fa of L (sparse) at
+    // the midpoint's face-average state, whose nonzero coefficients are
+    // (la, lb) from the rows stored in B0 (13 l0un, 14 l04, 15 l14, 16 l4un,
+    // 17 l44):
     //   fa = 0: l0un u_n + l04 p,  1: rho + l14 p,  2: u_t1,  3: u_t2,
     //   fa = 4: l4un u_n + l44 p
-    auto chv = [&](int fa, double la, double lb, int c) -> double {
-        switch (fa) {
-            case 0: return la * WC(sw, un, c) + lb * WC(sw, 4, c);
-            case 1: return WC(sw, 0, c) + lb * WC(sw, 4, c);
-            case 2: return WC(sw, t1, c);
-            case 3: return WC(sw, t2, c);
-            default: return la * WC(sw, un, c) + lb * WC(sw, 4, c);
+    // Rows and coefficients are selected, not branched on, so that the
+    // twelve loads (rows ra and 4 of the six cells) issue together; a[] and
+    // b[] keep them (rows u_n and p for fields 0 and 4: the coefficient sums
+    // of D3 reuse them).
+    auto stencil = [&](int fa, int m, double v[6], double ar[6], double br[6], double& la, double& lb) {
+        const int ra = (fa == 0 || fa == 4) ? un : fa == 1 ? 0 : fa == 2 ? t1 : t2;
+        const double lav = WM(scc, fa == 0 ? 13 : 16, m);
+        const double lbv = WM(scc, fa == 0 ? 14 : fa == 1 ? 15 : 17, m);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            ar[k] = WC(sw, ra, m + k);
+            br[k] = WC(sw, 4, m + k);
         }
-    };
-    // (la, lb) of field fa at midpoint m, from the rows stored in B0
-    auto lrow = [&](int fa, int m, double& la, double& lb) {
-        la = fa == 0 ? WM(scc, 13, m) : fa == 4 ? WM(scc, 16, m) : 0.0;
-        lb = fa == 0 ? WM(scc, 14, m) : fa == 1 ? WM(scc, 15, m) : fa == 4 ? WM(scc, 17, m) : 0.0;
+        la = (fa == 0 || fa == 4) ? lav : 0.0;
+        lb = (fa == 2 || fa == 3) ? 0.0 : lbv;
+        const double ca = fa == 1 ? 1.0 : la;  // 1.0 * rho is rho exactly
+        const bool single = fa == 2 || fa == 3;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) v[k] = single ? ar[k] : ca * ar[k] + lb * br[k];
     };
     auto eig_of = [&](int m) -> Eig {
         return eig_at(e, WC(sw, 0, m + 2), WC(sw, 0, m + 3), WC(sw, 4, m + 2), WC(sw, 4, m + 3));
@@ -619,13 +627,10 @@ __device__ __forceinline__ void adj_group(const AdjArgs& a, long long w, double*
     const int nfm = m_hi - lo;
     for (int it = tid; it < 5 * nfm; it += NT) {
         const int fa = it / nfm, m = lo + it - fa * nfm;
-        double la, lb;
-        lrow(fa, m, la, lb);
-        const double v0 = chv(fa, la, lb, m), v1 = chv(fa, la, lb, m + 1), v2 = chv(fa, la, lb, m + 2);
-        const double v3 = chv(fa, la, lb, m + 3), v4 = chv(fa, la, lb, m + 4);
-        const double v5 = chv(fa, la, lb, m + 5);
-        WM(scc, fa, m) = weno5z_interp(v0, v1, v2, v3, v4);
-        WM(scc, 5 + fa, m) = weno5z_interp(v5, v4, v3, v2, v1);
+        double v[6], ar[6], br[6], la, lb;
+        stencil(fa, m, v, ar, br, la, lb);
+        WM(scc, fa, m) = weno5z_interp(v[0], v[1], v[2], v[3], v[4]);
+        WM(scc, 5 + fa, m) = weno5z_interp(v[5], v[4], v[3], v[2], v[1]);
     }
     __syncthreads();
     // B2: back-projection, fallback, HLLC per midpoint
@@ -904,40 +909,34 @@ __device__ __forceinline__ void adj_group(const AdjArgs& a, long long w, double*
         double chb[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         double la = 0.0, lb = 0.0;
         if (live) {
-            lrow(fa, m, la, lb);
-            double v[6];
-#pragma unroll
-            for (int k = 0; k < 6; ++k) v[k] = chv(fa, la, lb, m + k);
+            double v[6], ar[6], br[6];
+            stencil(fa, m, v, ar, br, la, lb);
             weno_vjp_pair(v, WM(scc, fa, m), WM(scc, 5 + fa, m), chb);
-            // sums for the coefficient cotangents: d la = sum chb u_n,
-            // d lb = sum chb p (fields 0, 1, 4)
+            // sums for the coefficient cotangents: d la = sum chb u_n (fields
+            // 0, 4: ar is row u_n), d lb = sum chb p (fields 0, 1, 4); the
+            // other fields' sums are not read
             double sa = 0.0, sbp = 0.0;
 #pragma unroll
             for (int k = 0; k < 6; ++k) {
-                sa += chb[k] * WC(sw, un, m + k);
-                sbp += chb[k] * WC(sw, 4, m + k);
+                sa += chb[k] * ar[k];
+                sbp += chb[k] * br[k];
             }
             WM(scc, fa, m) = sa;
             WM(scc, 5 + fa, m) = sbp;
         }
+        // targets as rows of sacc (rows 5, 6, 7 are sx 0, 1, 2):
+        //   fa = 0: u_n (la), p (lb);  1: rho (1), sx 0 (lb);  2: u_t1 (1);
+        //   fa = 3: u_t2 (1);  4: sx 1 (la), sx 2 (lb)
+        const int d1 = fa == 0 ? un : fa == 1 ? 0 : fa == 2 ? t1 : fa == 3 ? t2 : 6;
+        const int d2 = fa == 0 ? 4 : fa == 1 ? 5 : 7;
+        const bool two = fa == 0 || fa == 1 || fa == 4;
+        const double k1 = (fa == 0 || fa == 4) ? la : 1.0;  // 1.0 * g is g exactly
         for (int j = 5; j >= 0; --j) {
             if (live) {
                 const double gj = chb[j];
                 const int c = m + j;
-                if (fa == 0) {
-                    WC(sacc, un, c) += la * gj;
-                    WC(sacc, 4, c) += lb * gj;
-                } else if (fa == 1) {
-                    WC(sacc, 0, c) += gj;
-                    WC(sx, 0, c) += lb * gj;
-                } else if (fa == 2) {
-                    WC(sacc, t1, c) += gj;
-                } else if (fa == 3) {
-                    WC(sacc, t2, c) += gj;
-                } else {
-                    WC(sx, 1, c) += la * gj;
-                    WC(sx, 2, c) += lb * gj;
-                }
+                sacc[d1 * NC + c] += k1 * gj;
+                if (two) sacc[d2 * NC + c] += lb * gj;
             }
             __syncthreads();
         }

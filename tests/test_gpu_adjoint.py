@@ -168,7 +168,7 @@ def test_adjoint_rhs_long_pencils(em, case):
         n = (300, 4, 1)
         bc = [["reflect", ("dirichlet", (1.0, 0.1, 0.0, 0.0, 2.6))], ["periodic", "periodic"], ["periodic", "periodic"]]
     else:
-        n, bc = (263, 3, 1), [["outflow", "outflow"], ["outflow", "reflect"], ["periodic", "periodic"]]
+        n, bc = (263, 5, 1), [["outflow", "outflow"], ["outflow", "reflect"], ["periodic", "periodic"]]
     o0 = OracleAdj(n, (0, 0, 0), (1, 1, 1), bc=bc)
     u = smooth_state(o0, 8)
     o, g = _pair(em, n, bc, u)
