@@ -26,6 +26,17 @@ struct AdjArgs {
     int first;               // MU = contribution (else MU += contribution)
     int ntile;               // tiles per pencil (adj_tiles)
     int ne;                  // output positions of the longest tile (adj_tile_extent)
+    // SSP-RK3 adjoint combination fused into the store of T = MU + this
+    // axis's contribution (the last axis of a stage; k_adj_combine's
+    // updates, T itself not stored): 0 none (MU = T); 3: cBout = 2/3 cA +
+    // 2/3 cdt T, cZ = cA / 3; 2: cZ += 3/4 cB, cBout = 1/4 cB + 1/4 cdt T
+    // (cBout must not be this pass's LAM); 1: cZ = cZ + cB + cdt T
+    int combine;
+    double cdt;
+    const double* cA;
+    const double* cB;
+    double* cZ;
+    double* cBout;
 };
 
 // Ensemble VJP: per-instance arguments and prefix sums of their work items

@@ -1032,7 +1032,20 @@ __device__ __forceinline__ void adj_group(const AdjArgs& a, long long w, double*
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
             const long long ix = q * a.ncell + cell;
-            a.MU[ix] = (a.first ? 0.0 : a.MU[ix]) + tot[q];
+            const double T = (a.first ? 0.0 : a.MU[ix]) + tot[q];
+            if (a.combine == 0) {
+                a.MU[ix] = T;
+            } else if (a.combine == 3) {
+                const double av = a.cA[ix];
+                a.cBout[ix] = (2.0 / 3.0) * av + (2.0 / 3.0) * a.cdt * T;
+                a.cZ[ix] = av / 3.0;
+            } else if (a.combine == 2) {
+                const double b = a.cB[ix];
+                a.cZ[ix] = a.cZ[ix] + 0.75 * b;
+                a.cBout[ix] = 0.25 * b + 0.25 * a.cdt * T;
+            } else {
+                a.cZ[ix] = a.cZ[ix] + a.cB[ix] + a.cdt * T;
+            }
         }
     }
 }

@@ -382,7 +382,7 @@ struct Solver {
     // One stage: residual of Uin over the active axes in order, fused with
     // the SSP-RK3 combination on the last axis (rk_stage 0: residual only).
     void stage(const double* Uin, double* Uout, int rk_stage, bool want_rate, int stage_dt_host, double stage_dt,
-               hipEvent_t* ev) {
+               hipEvent_t* ev, const double* base = nullptr) {
         int last = -1;
         for (int d = 0; d < 3; ++d)
             if (active(d)) last = d;
@@ -392,7 +392,7 @@ struct Solver {
             PassArgs a = pass_args(d);
             a.U = Uin;
             a.R = R;
-            a.U0 = U[0];
+            a.U0 = base ? base : U[0];
             a.Uout = Uout;
             a.first = first ? 1 : 0;
             a.rk_stage = d == last ? rk_stage : 0;
@@ -507,6 +507,17 @@ struct Solver {
     void step(double dt) {
         step_async(dt);
         HIPCHK(hipStreamSynchronize(stream));
+    }
+    // One step of the backward sweep's recompute, dt given: the stages are
+    // written straight into the store (u1, u2; unext = U^{n+1}, or null to
+    // skip the last stage), with no CFL rate and no clock update (the sweep
+    // restores the scalars at its end).
+    void recompute_step(const double* un, double* u1, double* u2, double* unext, double dt) {
+        hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, stream, sc, dt);
+        HIPCHK(hipGetLastError());
+        stage(un, u1, 1, false, 0, 0.0, nullptr, un);
+        stage(u1, u2, 2, false, 0, 0.0, nullptr, un);
+        if (unext) stage(u2, unext, 3, false, 0, 0.0, nullptr, un);
     }
 
     double dt_cfl() {
@@ -679,8 +690,15 @@ struct Solver {
         }
     }
 
-    // out = (dL/du at state)^T lam (all active axes, limiter lambda of stage_dt)
-    void adj_apply(const double* state, const double* lam, double* out, double stage_dt) {
+    // out = (dL/du at state)^T lam (all active axes, limiter lambda of stage_dt);
+    // combine != 0: the RK adjoint update of that stage fused into the last
+    // axis (AdjArgs::combine; out is then scratch between the axes)
+    void adj_apply(const double* state, const double* lam, double* out, double stage_dt, int combine = 0,
+                   const double* cA = nullptr, const double* cB = nullptr, double* cZ = nullptr,
+                   double* cBout = nullptr) {
+        int last = -1;
+        for (int d = 0; d < 3; ++d)
+            if (active(d)) last = d;
         bool first = true;
         for (int d = 0; d < 3; ++d) {
             if (!active(d)) continue;
@@ -703,6 +721,14 @@ struct Solver {
             a.first = first ? 1 : 0;
             a.ntile = adj_tiles((int)n[d], bc_kind[d][0], bc_kind[d][1]);
             a.ne = adj_tile_extent((int)n[d], a.ntile, bc_kind[d][0], bc_kind[d][1]);
+            if (d == last && combine != 0) {
+                a.combine = combine;
+                a.cdt = stage_dt;
+                a.cA = cA;
+                a.cB = cB;
+                a.cZ = cZ;
+                a.cBout = cBout;
+            }
             HIPCHK(launch_adj_pass(d, a, stream));
             first = false;
         }
@@ -800,26 +826,26 @@ struct Solver {
         double* Zb = adjbuf + 2 * nv;
         double* T = adjbuf + 3 * nv;
         host_to_soa(dj_final, A);
+        reset_run(0.0);
         const int64_t nsteps = (int64_t)fwd_dts.size();
         for (int64_t seg = (int64_t)ckpt.size() - 1; seg >= 0; --seg) {
             const int64_t s0 = seg * ck_every, s1 = std::min<int64_t>(s0 + ck_every, nsteps);
-            HIPCHK(hipMemcpyAsync(U[0], ckpt[seg], sz, hipMemcpyDeviceToDevice, stream));
+            // store slots 3 (s - s0) .. +2: U^s and its two stage states
+            HIPCHK(hipMemcpyAsync(store, ckpt[seg], sz, hipMemcpyDeviceToDevice, stream));
             for (int64_t s = s0; s < s1; ++s) {
                 double* st = store + 3 * (s - s0) * nv;
-                HIPCHK(hipMemcpyAsync(st, U[0], sz, hipMemcpyDeviceToDevice, stream));
-                step_async(fwd_dts[s]);
-                HIPCHK(hipMemcpyAsync(st + nv, U[1], sz, hipMemcpyDeviceToDevice, stream));
-                HIPCHK(hipMemcpyAsync(st + 2 * nv, U[2], sz, hipMemcpyDeviceToDevice, stream));
+                recompute_step(st, st + nv, st + 2 * nv, s + 1 < s1 ? st + 3 * nv : nullptr, fwd_dts[s]);
             }
+            // per step: T = L'(u2)^T A, then L'(u1)^T B, then L'(u^n)^T B with
+            // the RK updates fused into each stage's last axis; stage 2's new
+            // B goes to A's array (free once stage 3 has read it), stage 1
+            // leaves the step's input cotangent in Zb
             for (int64_t s = s1 - 1; s >= s0; --s) {
                 const double dt = fwd_dts[s];
                 const double* st = store + 3 * (s - s0) * nv;
-                adj_apply(st + 2 * nv, A, T, dt);
-                HIPCHK(launch_adj_combine(3, nv, dt, A, T, B, Zb, stream));
-                adj_apply(st + nv, B, T, dt);
-                HIPCHK(launch_adj_combine(2, nv, dt, A, T, B, Zb, stream));
-                adj_apply(st, B, T, dt);
-                HIPCHK(launch_adj_combine(1, nv, dt, A, T, B, Zb, stream));
+                adj_apply(st + 2 * nv, A, T, dt, 3, A, nullptr, Zb, B);
+                adj_apply(st + nv, B, T, dt, 2, nullptr, B, Zb, A);
+                adj_apply(st, A, T, dt, 1, nullptr, A, Zb, nullptr);
                 std::swap(A, Zb);
             }
         }
