@@ -199,6 +199,7 @@ struct Solver {
     int nactive;
     int act[3];
     hipStream_t stream = nullptr;
+    hipStream_t rstream = nullptr;  // the backward sweep's segment recompute (default priority)
     double* U[3] = {nullptr, nullptr, nullptr};  // U^n, stage 1, stage 2
     double* R = nullptr;
     double* pad = nullptr;
@@ -276,6 +277,7 @@ struct Solver {
         if (sc) (void)hipFree(sc);
         if (bcdev) (void)hipFree(bcdev);
         if (stream) (void)hipStreamDestroy(stream);
+        if (rstream) (void)hipStreamDestroy(rstream);
     }
 
     void upload(const double* u) {
@@ -375,14 +377,16 @@ struct Solver {
         return a;
     }
 
-    hipError_t launch(int d, const PassArgs& a) {
-        return precise ? precise::launch_pass(d, mode, cfg[d], a, stream) : fast::launch_pass(d, mode, cfg[d], a, stream);
+    hipError_t launch(int d, const PassArgs& a, hipStream_t st = nullptr) {
+        if (!st) st = stream;
+        return precise ? precise::launch_pass(d, mode, cfg[d], a, st) : fast::launch_pass(d, mode, cfg[d], a, st);
     }
 
     // One stage: residual of Uin over the active axes in order, fused with
     // the SSP-RK3 combination on the last axis (rk_stage 0: residual only).
     void stage(const double* Uin, double* Uout, int rk_stage, bool want_rate, int stage_dt_host, double stage_dt,
-               hipEvent_t* ev, const double* base = nullptr) {
+               hipEvent_t* ev, const double* base = nullptr, hipStream_t st = nullptr) {
+        if (!st) st = stream;
         int last = -1;
         for (int d = 0; d < 3; ++d)
             if (active(d)) last = d;
@@ -399,9 +403,9 @@ struct Solver {
             a.want_rate = (d == last && want_rate) ? 1 : 0;
             a.stage_dt_host = stage_dt_host;
             a.stage_dt = stage_dt;
-            if (ev) HIPCHK(hipEventRecord(ev[2 * d], stream));
-            HIPCHK(launch(d, a));
-            if (ev) HIPCHK(hipEventRecord(ev[2 * d + 1], stream));
+            if (ev) HIPCHK(hipEventRecord(ev[2 * d], st));
+            HIPCHK(launch(d, a, st));
+            if (ev) HIPCHK(hipEventRecord(ev[2 * d + 1], st));
             first = false;
         }
     }
@@ -512,12 +516,12 @@ struct Solver {
     // written straight into the store (u1, u2; unext = U^{n+1}, or null to
     // skip the last stage), with no CFL rate and no clock update (the sweep
     // restores the scalars at its end).
-    void recompute_step(const double* un, double* u1, double* u2, double* unext, double dt) {
-        hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, stream, sc, dt);
+    void recompute_step(const double* un, double* u1, double* u2, double* unext, double dt, hipStream_t st) {
+        hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, st, sc, dt);
         HIPCHK(hipGetLastError());
-        stage(un, u1, 1, false, 0, 0.0, nullptr, un);
-        stage(u1, u2, 2, false, 0, 0.0, nullptr, un);
-        if (unext) stage(u2, unext, 3, false, 0, 0.0, nullptr, un);
+        stage(un, u1, 1, false, 0, 0.0, nullptr, un, st);
+        stage(u1, u2, 2, false, 0, 0.0, nullptr, un, st);
+        if (unext) stage(u2, unext, 3, false, 0, 0.0, nullptr, un, st);
     }
 
     double dt_cfl() {
@@ -655,6 +659,7 @@ struct Solver {
     int ck_every = 0;
     double* adjbuf = nullptr;  // A, B, Z, T: 4 x 5 ncell
     double* store = nullptr;   // u_n, u1, u2 of every step of one segment
+    double* store2 = nullptr;  // a second segment (recomputed while the first is reversed), if it fits
     long long store_steps = 0;
     double* final_state = nullptr;
 
@@ -663,8 +668,9 @@ struct Solver {
         ckpt.clear();
         if (adjbuf) (void)hipFree(adjbuf);
         if (store) (void)hipFree(store);
+        if (store2) (void)hipFree(store2);
         if (final_state) (void)hipFree(final_state);
-        adjbuf = store = final_state = nullptr;
+        adjbuf = store = store2 = final_state = nullptr;
         store_steps = 0;
     }
 
@@ -817,9 +823,18 @@ struct Solver {
         const size_t sz = sizeof(double) * nv;
         if (store_steps < ck_every) {
             if (store) HIPCHK(hipFree(store));
+            if (store2) HIPCHK(hipFree(store2));
+            store = store2 = nullptr;
             HIPCHK(hipMalloc(&store, sz * 3 * ck_every));
+            // the second segment buffer only if it fits (else the recompute
+            // waits for each sweep)
+            if (hipMalloc(&store2, sz * 3 * ck_every) != hipSuccess) {
+                (void)hipGetLastError();
+                store2 = nullptr;
+            }
             store_steps = ck_every;
         }
+        if (!rstream) HIPCHK(hipStreamCreateWithFlags(&rstream, hipStreamNonBlocking));
         std::vector<DevScalars> saved = save_scalars();
         double* A = adjbuf;
         double* B = adjbuf + nv;
@@ -828,25 +843,63 @@ struct Solver {
         host_to_soa(dj_final, A);
         reset_run(0.0);
         const int64_t nsteps = (int64_t)fwd_dts.size();
-        for (int64_t seg = (int64_t)ckpt.size() - 1; seg >= 0; --seg) {
-            const int64_t s0 = seg * ck_every, s1 = std::min<int64_t>(s0 + ck_every, nsteps);
-            // store slots 3 (s - s0) .. +2: U^s and its two stage states
-            HIPCHK(hipMemcpyAsync(store, ckpt[seg], sz, hipMemcpyDeviceToDevice, stream));
-            for (int64_t s = s0; s < s1; ++s) {
-                double* st = store + 3 * (s - s0) * nv;
-                recompute_step(st, st + nv, st + 2 * nv, s + 1 < s1 ? st + 3 * nv : nullptr, fwd_dts[s]);
+        // Segments are recomputed on rstream into store buffer (seg & 1) and
+        // reversed on stream.  With two buffers the recompute of segment
+        // seg - 1 runs while segment seg is reversed (it waits only for the
+        // sweep of seg + 1, the last reader of its buffer); with one it
+        // waits for the sweep of seg.  The sweep never touches sc or R, the
+        // recompute nothing else.
+        double* sbuf[2] = {store, store2 ? store2 : store};
+        struct Events {  // destroyed when backward() returns, after its final synchronisation
+            hipEvent_t e[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+            ~Events() {
+                for (auto x : e)
+                    if (x) (void)hipEventDestroy(x);
             }
+        } evs;
+        for (auto& x : evs.e) HIPCHK(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+        hipEvent_t* ev_rec = evs.e;
+        hipEvent_t* ev_done = evs.e + 2;
+        hipEvent_t ev_start = evs.e[4];
+        HIPCHK(hipEventRecord(ev_start, stream));
+        HIPCHK(hipStreamWaitEvent(rstream, ev_start, 0));
+        auto recompute = [&](int64_t seg) {
+            const int64_t s0 = seg * ck_every, s1 = std::min<int64_t>(s0 + ck_every, nsteps);
+            double* sto = sbuf[seg & 1];
+            // store slots 3 (s - s0) .. +2: U^s and its two stage states
+            HIPCHK(hipMemcpyAsync(sto, ckpt[seg], sz, hipMemcpyDeviceToDevice, rstream));
+            for (int64_t s = s0; s < s1; ++s) {
+                double* st = sto + 3 * (s - s0) * nv;
+                recompute_step(st, st + nv, st + 2 * nv, s + 1 < s1 ? st + 3 * nv : nullptr, fwd_dts[s], rstream);
+            }
+            HIPCHK(hipEventRecord(ev_rec[seg & 1], rstream));
+        };
+        const int64_t last = (int64_t)ckpt.size() - 1;
+        recompute(last);
+        for (int64_t seg = last; seg >= 0; --seg) {
+            if (store2 && seg > 0) {
+                if (seg + 1 <= last) HIPCHK(hipStreamWaitEvent(rstream, ev_done[(seg + 1) & 1], 0));
+                recompute(seg - 1);
+            }
+            HIPCHK(hipStreamWaitEvent(stream, ev_rec[seg & 1], 0));
+            const int64_t s0 = seg * ck_every, s1 = std::min<int64_t>(s0 + ck_every, nsteps);
+            const double* sto = sbuf[seg & 1];
             // per step: T = L'(u2)^T A, then L'(u1)^T B, then L'(u^n)^T B with
             // the RK updates fused into each stage's last axis; stage 2's new
             // B goes to A's array (free once stage 3 has read it), stage 1
             // leaves the step's input cotangent in Zb
             for (int64_t s = s1 - 1; s >= s0; --s) {
                 const double dt = fwd_dts[s];
-                const double* st = store + 3 * (s - s0) * nv;
+                const double* st = sto + 3 * (s - s0) * nv;
                 adj_apply(st + 2 * nv, A, T, dt, 3, A, nullptr, Zb, B);
                 adj_apply(st + nv, B, T, dt, 2, nullptr, B, Zb, A);
                 adj_apply(st, A, T, dt, 1, nullptr, A, Zb, nullptr);
                 std::swap(A, Zb);
+            }
+            HIPCHK(hipEventRecord(ev_done[seg & 1], stream));
+            if (!store2 && seg > 0) {
+                HIPCHK(hipStreamWaitEvent(rstream, ev_done[seg & 1], 0));
+                recompute(seg - 1);
             }
         }
         soa_to_host(A, dj_initial);
