@@ -26,11 +26,15 @@ struct AdjArgs {
     int first;               // MU = contribution (else MU += contribution)
     int ntile;               // tiles per pencil (adj_tiles)
     int ne;                  // output positions of the longest tile (adj_tile_extent)
-    // SSP-RK3 adjoint combination fused into the store of T = MU + this
-    // axis's contribution (the last axis of a stage; k_adj_combine's
-    // updates, T itself not stored): 0 none (MU = T); 3: cBout = 2/3 cA +
-    // 2/3 cdt T, cZ = cA / 3; 2: cZ += 3/4 cB, cBout = 1/4 cB + 1/4 cdt T
-    // (cBout must not be this pass's LAM); 1: cZ = cZ + cB + cdt T
+    // SSP-RK3 adjoint update fused into the store of T = MU + this axis's
+    // contribution on the last axis of a stage (T itself is not stored).
+    // For the forward u1 = u0 + dt L(u0); u2 = 3/4 u0 + 1/4 (u1 + dt L(u1));
+    // u3 = u0 / 3 + 2/3 (u2 + dt L(u2)) and T = L'(u_s)^T (cotangent):
+    //   0: none (MU = T);
+    //   3: cBout = 2/3 cA + 2/3 cdt T, cZ = cA / 3;
+    //   2: cZ += 3/4 cB, cBout = 1/4 cB + 1/4 cdt T (cBout is not this
+    //      pass's LAM: other tiles still read it);
+    //   1: cZ = cZ + cB + cdt T (the cotangent of the step's input).
     int combine;
     double cdt;
     const double* cA;
@@ -48,27 +52,12 @@ struct EnsAdjArgs {
     long long total;
 };
 
-// One instance of the batched RK adjoint combination (dts[dt_index] is the
-// instance's dt of the step being reversed).
-struct AdjCombineTask {
-    double* A;
-    const double* T;
-    double* B;
-    double* Z;
-    long long nv;
-    long long dt_index;
-};
-
 constexpr int kAdjThreads = 256;     // threads per adjoint workgroup (4 waves; three workgroups per CU)
 constexpr int kAdjTileCells = 128;   // interior cells per tile at most (LDS of a tile <= 45 KB)
 int adj_tiles(int n, int kind_lo, int kind_hi);
 int adj_tile_extent(int n, int ntile, int kind_lo, int kind_hi);
 hipError_t launch_ens_adj_pass(int d, const EnsAdjArgs& e, size_t lds, unsigned grid, hipStream_t s);
-hipError_t launch_ens_adj_combine(int stage, const AdjCombineTask* tasks, int ninst, const double* dts,
-                                  long long maxnv, hipStream_t s);
 size_t adj_lds_bytes(int ne);
 hipError_t launch_adj_pass(int d, const AdjArgs& a, hipStream_t s);
-hipError_t launch_adj_combine(int stage, long long nv, double dt, const double* A, const double* T, double* B,
-                              double* Z, hipStream_t s);
 
 }  // namespace em

@@ -1067,8 +1067,10 @@ __global__ __launch_bounds__(kAdjThreads) __attribute__((amdgpu_waves_per_eu(3))
     adj_group<D>(a, w, lds);
 }
 
-// Ensemble VJP: the tiles of every instance in one persistent launch
-// (instance i owns work items prefix[i] .. prefix[i+1]-1).
+// Ensemble VJP: the tiles of every instance in one launch (instance i owns
+// work items prefix[i] .. prefix[i+1]-1), one workgroup per tile: a
+// persistent grid ran 10% slower on 256 instances, its late-starting
+// workgroups setting the end.
 template <int D>
 __global__ __launch_bounds__(kAdjThreads) __attribute__((amdgpu_waves_per_eu(3))) void ens_adj_pass(EnsAdjArgs e) {
     extern __shared__ double lds[];
@@ -1081,7 +1083,8 @@ __global__ __launch_bounds__(kAdjThreads) __attribute__((amdgpu_waves_per_eu(3))
             else
                 hi = mid;
         }
-        adj_group<D>(e.args[lo], w - e.prefix[lo], lds);
+        const AdjArgs a = e.args[lo];  // by value: its fields in registers, not re-read from memory
+        adj_group<D>(a, w - e.prefix[lo], lds);
         __syncthreads();
     }
 }
@@ -1148,62 +1151,6 @@ hipError_t launch_adj_pass(int d, const AdjArgs& a, hipStream_t s) {
         if ((er = allow_lds(adj::adj_pass<2>, lds)) != hipSuccess) return er;
         hipLaunchKernelGGL(adj::adj_pass<2>, dim3(grid), dim3(kAdjThreads), lds, s, a);
     }
-    return hipGetLastError();
-}
-
-// SSP-RK3 adjoint combinations (elementwise over 5 * ncell values), for the
-// forward u1 = u0 + dt L(u0); u2 = 3/4 u0 + 1/4 (u1 + dt L(u1));
-// u3 = u0 / 3 + 2/3 (u2 + dt L(u2)), with T = L'(u_s)^T (cotangent):
-//   stage 3: B = 2/3 A + 2/3 dt T,  Z = A / 3
-//   stage 2: Z += 3/4 B,  B = 1/4 B + 1/4 dt T
-//   stage 1: Z += B + dt T
-__global__ void k_adj_combine(int stage, long long nv, double dt, const double* __restrict__ A,
-                              const double* __restrict__ T, double* __restrict__ B, double* __restrict__ Z) {
-    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < nv; i += (long long)gridDim.x * 256ll) {
-        if (stage == 3) {
-            B[i] = (2.0 / 3.0) * A[i] + (2.0 / 3.0) * dt * T[i];
-            Z[i] = A[i] / 3.0;
-        } else if (stage == 2) {
-            const double b = B[i];
-            Z[i] = Z[i] + 0.75 * b;
-            B[i] = 0.25 * b + 0.25 * dt * T[i];
-        } else {
-            Z[i] = Z[i] + B[i] + dt * T[i];
-        }
-    }
-}
-
-// Ensemble form: instance blockIdx.y with its own arrays and the dt of the
-// step from the device dt history; stage 1 leaves the step's input
-// cotangent in A (A is free once stage 3 has read it).
-__global__ void k_ens_adj_combine(int stage, const AdjCombineTask* __restrict__ tasks, const double* __restrict__ dts) {
-    const AdjCombineTask t = tasks[blockIdx.y];
-    const double dt = dts[t.dt_index];
-    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < t.nv; i += (long long)gridDim.x * 256ll) {
-        if (stage == 3) {
-            t.B[i] = (2.0 / 3.0) * t.A[i] + (2.0 / 3.0) * dt * t.T[i];
-            t.Z[i] = t.A[i] / 3.0;
-        } else if (stage == 2) {
-            const double b = t.B[i];
-            t.Z[i] = t.Z[i] + 0.75 * b;
-            t.B[i] = 0.25 * b + 0.25 * dt * t.T[i];
-        } else {
-            t.A[i] = t.Z[i] + t.B[i] + dt * t.T[i];  // the cotangent of the step's input
-        }
-    }
-}
-
-hipError_t launch_ens_adj_combine(int stage, const AdjCombineTask* tasks, int ninst, const double* dts,
-                                  long long maxnv, hipStream_t s) {
-    const unsigned gx = (unsigned)std::min<long long>((maxnv + 255) / 256, 1024);
-    hipLaunchKernelGGL(k_ens_adj_combine, dim3(gx, (unsigned)ninst), dim3(256), 0, s, stage, tasks, dts);
-    return hipGetLastError();
-}
-
-hipError_t launch_adj_combine(int stage, long long nv, double dt, const double* A, const double* T, double* B,
-                              double* Z, hipStream_t s) {
-    const unsigned g = (unsigned)std::min<long long>((nv + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_adj_combine, dim3(g), dim3(256), 0, s, stage, nv, dt, A, T, B, Z);
     return hipGetLastError();
 }
 

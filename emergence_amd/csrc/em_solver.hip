@@ -989,9 +989,11 @@ struct Ensemble {
     std::vector<double*> ckpt;
     double* dt_hist = nullptr;
     std::vector<double> h_dts;
-    double* work = nullptr;   // A, B, Z, T
-    double* store = nullptr;  // 3 states per step of a segment
-    double* fin = nullptr;    // final states
+    double* work = nullptr;    // A, B, Z, T
+    double* store = nullptr;   // 3 states per step of a segment
+    double* store2 = nullptr;  // a second segment (recomputed while the first is reversed), if it fits
+    double* fin = nullptr;     // final states
+    hipStream_t rstream = nullptr;  // the backward sweep's segment recompute (default priority)
 
     Ensemble(int ninst, const int64_t* nn, const double* lo, const double* hi, int ng, const double e[4],
              const int32_t bk[6], const double bs[30], int mode_, int pos, double cfl_, double tol,
@@ -1050,13 +1052,15 @@ struct Ensemble {
         graph = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
+        if (rstream) (void)hipStreamDestroy(rstream);
+        rstream = nullptr;
     }
     ~Ensemble() { release(); }
 
     void free_adjoint() {
         for (auto* p : ckpt) (void)hipFree(p);
         ckpt.clear();
-        for (double** p : {&dt_hist, &work, &store, &fin})
+        for (double** p : {&dt_hist, &work, &store, &store2, &fin})
             if (*p) {
                 (void)hipFree(*p);
                 *p = nullptr;
@@ -1163,7 +1167,8 @@ struct Ensemble {
         return cfg;
     }
 
-    void stage_launches(int st) {
+    void stage_launches(int st, hipStream_t strm = nullptr) {
+        hipStream_t stream = strm ? strm : this->stream;
         for (int d = 0; d < 3; ++d) {
             if (total[st - 1][d] == 0) continue;
             if (batched) {
@@ -1197,15 +1202,16 @@ struct Ensemble {
     }
 
     // one step: dt (CFL, recorded) or given dt from the history, 3 stages
-    void step_kernels(bool cfl_dt, long long given_step) {
+    void step_kernels(bool cfl_dt, long long given_step, hipStream_t strm = nullptr) {
+        hipStream_t stream = strm ? strm : this->stream;
         if (cfl_dt)
             hipLaunchKernelGGL(k_ens_dt, dim3(small_grid()), dim3(64), 0, stream, d_scs, n, cfl, dt_hist, d_step);
         else
             hipLaunchKernelGGL(k_ens_set_dt, dim3(small_grid()), dim3(64), 0, stream, d_scs, n, dt_hist, given_step);
-        stage_launches(1);
-        stage_launches(2);
+        stage_launches(1, stream);
+        stage_launches(2, stream);
         hipLaunchKernelGGL(k_ens_zero_rate, dim3(n), dim3(kSlots), 0, stream, d_scs);
-        stage_launches(3);
+        stage_launches(3, stream);
         hipLaunchKernelGGL(k_ens_advance, dim3(small_grid()), dim3(64), 0, stream, d_scs, n, d_step);
         HIPCHK(hipGetLastError());
     }
@@ -1257,6 +1263,12 @@ struct Ensemble {
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(stream));  // the task table is rewritten next call
     }
+    // the copies of cnt consecutive tasks of a device table, on stream strm
+    void copy_table(const CopyTask* d_tab, int cnt, hipStream_t strm) {
+        const unsigned gx = (unsigned)std::min<long long>((max_nv + 255) / 256, 1024);
+        hipLaunchKernelGGL(k_ens_copy, dim3(gx, (unsigned)cnt), dim3(256), 0, strm, d_tab);
+        HIPCHK(hipGetLastError());
+    }
     // state buffer b (0: U^n, 1, 2: stage states) of every instance <-> a
     // concatenated buffer
     std::vector<CopyTask> gather_tasks(int b, double* dst) {
@@ -1303,14 +1315,23 @@ struct Ensemble {
     void backward(const double* const* dj_final, double* const* dj_initial) {
         if (ckpt.empty()) throw Error(EM_ERR_STATE, "backward needs a forward run first");
         if (!work) HIPCHK(hipMalloc(&work, sizeof(double) * 4 * tot_nv));
-        if (!store) HIPCHK(hipMalloc(&store, sizeof(double) * 3 * tot_nv * ck_every));
+        if (!store) {
+            const size_t seg_bytes = sizeof(double) * 3 * tot_nv * ck_every;
+            HIPCHK(hipMalloc(&store, seg_bytes));
+            // the second segment buffer only if it fits (else the recompute
+            // waits for each sweep)
+            if (hipMalloc(&store2, seg_bytes) != hipSuccess) {
+                (void)hipGetLastError();
+                store2 = nullptr;
+            }
+        }
+        if (!rstream) HIPCHK(hipStreamCreateWithFlags(&rstream, hipStreamNonBlocking));
         std::vector<std::vector<DevScalars>> saved;
         for (auto* S : in) saved.push_back(S->save_scalars());
-        double* A = work;
-        double* B = work + tot_nv;
-        double* Zb = work + 2 * tot_nv;
-        double* T = work + 3 * tot_nv;
-        for (int i = 0; i < n; ++i) in[i]->host_to_soa(dj_final[i], A + off[i]);
+        // work arrays: A (the cotangent of the state after the step being
+        // reversed) and Z swap roles every step; B and T keep theirs
+        double* W[4] = {work, work + tot_nv, work + 2 * tot_nv, work + 3 * tot_nv};
+        for (int i = 0; i < n; ++i) in[i]->host_to_soa(dj_final[i], W[0] + off[i]);
         // tiles of each instance and axis (its own, as a single-instance
         // backward would use; the sums do not depend on the tiling); LDS per
         // axis: the largest need
@@ -1323,15 +1344,10 @@ struct Ensemble {
                     lds_ax[d] = std::max(lds_ax[d],
                                          adj_lds_bytes(adj_tile_extent(nd, nt, S->bc_kind[d][0], S->bc_kind[d][1])));
                 }
-        AdjArgs* d_aa = nullptr;
-        long long* d_pre = nullptr;
-        AdjCombineTask* d_ct = nullptr;
         const int64_t nsteps = fwd_steps;
-        // argument tables of one segment: [step][stage 0..2][axis]
-        const size_t per_step = (size_t)9 * n;
-        HIPCHK(hipMalloc(&d_aa, sizeof(AdjArgs) * per_step * ck_every));
-        HIPCHK(hipMalloc(&d_pre, sizeof(long long) * 3 * (n + 1)));
-        HIPCHK(hipMalloc(&d_ct, sizeof(AdjCombineTask) * 3 * n));
+        const int64_t nseg = (int64_t)ckpt.size();
+        const int k = ck_every;
+        double* sbuf[2] = {store, store2 ? store2 : store};
         std::vector<long long> pre(3 * (n + 1), 0);
         for (int d = 0; d < 3; ++d)
             for (int i = 0; i < n; ++i) {
@@ -1340,83 +1356,154 @@ struct Ensemble {
                     S.active(d) ? S.ncell / S.n[d] * adj_tiles((int)S.n[d], S.bc_kind[d][0], S.bc_kind[d][1]) : 0;
                 pre[d * (n + 1) + i + 1] = pre[d * (n + 1) + i] + groups;
             }
-        HIPCHK(hipMemcpyAsync(d_pre, pre.data(), sizeof(long long) * pre.size(), hipMemcpyHostToDevice, stream));
-        for (int64_t seg = (int64_t)ckpt.size() - 1; seg >= 0; --seg) {
-            const int64_t s0 = seg * ck_every, s1 = std::min<int64_t>(s0 + ck_every, nsteps);
-            copy_all(scatter_tasks(ckpt[seg], 0));
-            for (int64_t q = s0; q < s1; ++q) {
-                double* st = store + 3 * (q - s0) * tot_nv;
-                copy_all(gather_tasks(0, st));
-                step_kernels(false, q);
-                copy_all(gather_tasks(1, st + tot_nv));
-                copy_all(gather_tasks(2, st + 2 * tot_nv));
-            }
-            // VJP arguments of the segment
-            std::vector<AdjArgs> aa(per_step * (s1 - s0));
-            for (int64_t q = s0; q < s1; ++q)
-                for (int stg = 0; stg < 3; ++stg)
-                    for (int d = 0; d < 3; ++d)
-                        for (int i = 0; i < n; ++i) {
-                            const Solver& S = *in[i];
-                            AdjArgs& a = aa[(size_t)(q - s0) * per_step + (size_t)(stg * 3 + d) * n + i];
-                            a = AdjArgs{};
-                            if (!S.active(d)) continue;
-                            const double dt = h_dts[(size_t)q * n + i];
-                            double* st = store + 3 * (q - s0) * tot_nv;
-                            // stage 3 linearises at u2, stage 2 at u1, stage 1 at u^n
-                            a.U = st + (2 - stg) * tot_nv + off[i];
-                            a.LAM = (stg == 0 ? A : B) + off[i];
-                            a.MU = T + off[i];
-                            for (int e = 0; e < 3; ++e) {
-                                a.n[e] = (int)S.n[e];
-                                a.stride[e] = S.stride[e];
-                            }
-                            a.ncell = S.ncell;
-                            a.eos = S.eos;
-                            for (int sd = 0; sd < 2; ++sd) {
-                                a.bc_kind[sd] = S.bc_kind[d][sd];
-                                for (int v = 0; v < 5; ++v) a.bc_state[sd][v] = S.bc_state[d][sd][v];
-                            }
-                            a.inv_dx = 1.0 / S.dx[d];
-                            a.lam_lim = S.positivity ? (double)S.nactive * dt / S.dx[d] : 0.0;
-                            int first_ax = -1;
-                            for (int e = 0; e < 3; ++e)
-                                if (S.active(e) && first_ax < 0) first_ax = e;
-                            a.first = d == first_ax ? 1 : 0;
-                            a.ntile = adj_tiles((int)S.n[d], S.bc_kind[d][0], S.bc_kind[d][1]);
-                            a.ne = adj_tile_extent((int)S.n[d], a.ntile, S.bc_kind[d][0], S.bc_kind[d][1]);
-                        }
-            HIPCHK(hipMemcpyAsync(d_aa, aa.data(), sizeof(AdjArgs) * aa.size(), hipMemcpyHostToDevice, stream));
-            for (int64_t q = s1 - 1; q >= s0; --q) {
-                std::vector<AdjCombineTask> ct(3 * n);
-                for (int stg = 0; stg < 3; ++stg)
+        // copy tables: per store buffer b and step slot j of a segment the
+        // gathers U^n -> slot 3j, U1 -> 3j + 1, U2 -> 3j + 2 ([b][j][3][n]),
+        // then per segment the scatter of its checkpoint into U^n ([seg][n])
+        std::vector<CopyTask> cp((size_t)(2 * k * 3 + nseg) * n);
+        for (int bb = 0; bb < 2; ++bb)
+            for (int j = 0; j < k; ++j)
+                for (int g = 0; g < 3; ++g)
                     for (int i = 0; i < n; ++i)
-                        ct[stg * n + i] = AdjCombineTask{A + off[i], T + off[i], B + off[i], Zb + off[i],
-                                                         off[i + 1] - off[i], q * n + i};
-                HIPCHK(hipMemcpyAsync(d_ct, ct.data(), sizeof(AdjCombineTask) * ct.size(), hipMemcpyHostToDevice,
-                                      stream));
-                for (int stg = 0; stg < 3; ++stg) {
+                        cp[(((size_t)bb * k + j) * 3 + g) * n + i] =
+                            CopyTask{in[i]->U[g], sbuf[bb] + (3 * (size_t)j + g) * tot_nv + off[i], off[i + 1] - off[i]};
+        const size_t scat0 = (size_t)2 * k * 3 * n;
+        for (int64_t sg = 0; sg < nseg; ++sg)
+            for (int i = 0; i < n; ++i)
+                cp[scat0 + (size_t)sg * n + i] = CopyTask{ckpt[sg] + off[i], in[i]->U[0], off[i + 1] - off[i]};
+        // VJP arguments of every step: [step][stage 0..2][axis][instance],
+        // with the RK updates fused into each stage's last axis as in
+        // Solver::backward (stage 2's new B into A's array; stage 1 leaves
+        // the step's input cotangent in Z, which becomes A)
+        const size_t per_step = (size_t)9 * n;
+        std::vector<AdjArgs> aa((size_t)nsteps * per_step);
+        int ia = 0, iz = 2;
+        for (int64_t q = nsteps - 1; q >= 0; --q) {
+            const int64_t sg = q / k;
+            const double* st = sbuf[sg & 1] + 3 * (q - sg * k) * tot_nv;
+            double* A = W[ia];
+            double* B = W[1];
+            double* Zb = W[iz];
+            double* T = W[3];
+            for (int stg = 0; stg < 3; ++stg)
+                for (int d = 0; d < 3; ++d)
+                    for (int i = 0; i < n; ++i) {
+                        const Solver& S = *in[i];
+                        AdjArgs& a = aa[(size_t)q * per_step + (size_t)(stg * 3 + d) * n + i];
+                        a = AdjArgs{};
+                        if (!S.active(d)) continue;
+                        const double dt = h_dts[(size_t)q * n + i];
+                        // stage 3 linearises at u2, stage 2 at u1, stage 1 at u^n
+                        a.U = st + (2 - stg) * tot_nv + off[i];
+                        a.LAM = (stg == 1 ? B : A) + off[i];
+                        a.MU = T + off[i];
+                        for (int e = 0; e < 3; ++e) {
+                            a.n[e] = (int)S.n[e];
+                            a.stride[e] = S.stride[e];
+                        }
+                        a.ncell = S.ncell;
+                        a.eos = S.eos;
+                        for (int sd = 0; sd < 2; ++sd) {
+                            a.bc_kind[sd] = S.bc_kind[d][sd];
+                            for (int v = 0; v < 5; ++v) a.bc_state[sd][v] = S.bc_state[d][sd][v];
+                        }
+                        a.inv_dx = 1.0 / S.dx[d];
+                        a.lam_lim = S.positivity ? (double)S.nactive * dt / S.dx[d] : 0.0;
+                        int first_ax = -1, last_ax = -1;
+                        for (int e = 0; e < 3; ++e)
+                            if (S.active(e)) {
+                                if (first_ax < 0) first_ax = e;
+                                last_ax = e;
+                            }
+                        a.first = d == first_ax ? 1 : 0;
+                        a.ntile = adj_tiles((int)S.n[d], S.bc_kind[d][0], S.bc_kind[d][1]);
+                        a.ne = adj_tile_extent((int)S.n[d], a.ntile, S.bc_kind[d][0], S.bc_kind[d][1]);
+                        if (d == last_ax) {
+                            a.combine = 3 - stg;
+                            a.cdt = dt;
+                            a.cA = A + off[i];
+                            a.cB = (stg == 1 ? B : A) + off[i];
+                            a.cZ = Zb + off[i];
+                            a.cBout = (stg == 0 ? B : A) + off[i];
+                        }
+                    }
+            std::swap(ia, iz);
+        }
+        AdjArgs* d_aa = nullptr;
+        long long* d_pre = nullptr;
+        CopyTask* d_cp = nullptr;
+        struct Free {  // device tables and events, released when backward() returns
+            std::vector<void*> p;
+            std::vector<hipEvent_t> e;
+            ~Free() {
+                for (auto x : p)
+                    if (x) (void)hipFree(x);
+                for (auto x : e)
+                    if (x) (void)hipEventDestroy(x);
+            }
+        } fr;
+        HIPCHK(hipMalloc(&d_aa, sizeof(AdjArgs) * aa.size()));
+        fr.p.push_back(d_aa);
+        HIPCHK(hipMalloc(&d_pre, sizeof(long long) * pre.size()));
+        fr.p.push_back(d_pre);
+        HIPCHK(hipMalloc(&d_cp, sizeof(CopyTask) * cp.size()));
+        fr.p.push_back(d_cp);
+        HIPCHK(hipMemcpyAsync(d_aa, aa.data(), sizeof(AdjArgs) * aa.size(), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(d_pre, pre.data(), sizeof(long long) * pre.size(), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(d_cp, cp.data(), sizeof(CopyTask) * cp.size(), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        // Segments are recomputed on rstream into store buffer (seg & 1) and
+        // reversed on stream.  With two buffers the recompute of segment
+        // seg - 1 runs while segment seg is reversed (it waits only for the
+        // sweep of seg + 1, the last reader of its buffer); with one it
+        // waits for the sweep of seg.  The sweep never touches the instances'
+        // states, scalars or residuals, the recompute nothing else.
+        hipEvent_t ev_rec[2], ev_done[2];
+        for (auto* x : {&ev_rec[0], &ev_rec[1], &ev_done[0], &ev_done[1]}) {
+            HIPCHK(hipEventCreateWithFlags(x, hipEventDisableTiming));
+            fr.e.push_back(*x);
+        }
+        auto recompute = [&](int64_t sg) {
+            const int64_t s0 = sg * k, s1 = std::min<int64_t>(s0 + k, nsteps);
+            const int bb = (int)(sg & 1);
+            copy_table(d_cp + scat0 + (size_t)sg * n, n, rstream);
+            for (int64_t q = s0; q < s1; ++q) {
+                const size_t j = (size_t)(q - s0);
+                copy_table(d_cp + ((size_t)bb * k + j) * 3 * n, n, rstream);
+                step_kernels(false, q, rstream);
+                copy_table(d_cp + (((size_t)bb * k + j) * 3 + 1) * n, 2 * n, rstream);
+            }
+            HIPCHK(hipEventRecord(ev_rec[bb], rstream));
+        };
+        recompute(nseg - 1);
+        for (int64_t sg = nseg - 1; sg >= 0; --sg) {
+            if (store2 && sg > 0) {
+                if (sg + 1 < nseg) HIPCHK(hipStreamWaitEvent(rstream, ev_done[(sg + 1) & 1], 0));
+                recompute(sg - 1);
+            }
+            HIPCHK(hipStreamWaitEvent(stream, ev_rec[sg & 1], 0));
+            const int64_t s0 = sg * k, s1 = std::min<int64_t>(s0 + k, nsteps);
+            for (int64_t q = s1 - 1; q >= s0; --q)
+                for (int stg = 0; stg < 3; ++stg)
                     for (int d = 0; d < 3; ++d) {
                         const long long tot = pre[d * (n + 1) + n];
                         if (tot == 0) continue;
-                        EnsAdjArgs e{d_aa + (size_t)(q - s0) * per_step + (size_t)(stg * 3 + d) * n,
-                                     d_pre + d * (n + 1), n, tot};
-                        HIPCHK(launch_ens_adj_pass(d, e, lds_ax[d], (unsigned)std::min<long long>(tot, 2048), stream));
+                        EnsAdjArgs e{d_aa + (size_t)q * per_step + (size_t)(stg * 3 + d) * n, d_pre + d * (n + 1), n,
+                                     tot};
+                        HIPCHK(launch_ens_adj_pass(d, e, lds_ax[d], (unsigned)tot, stream));  // one workgroup per tile
                     }
-                    HIPCHK(launch_ens_adj_combine(3 - stg, d_ct + stg * n, n, dt_hist, max_nv, stream));
-                }
-                HIPCHK(hipStreamSynchronize(stream));  // ct is rewritten next step
+            HIPCHK(hipEventRecord(ev_done[sg & 1], stream));
+            if (!store2 && sg > 0) {
+                HIPCHK(hipStreamWaitEvent(rstream, ev_done[sg & 1], 0));
+                recompute(sg - 1);
             }
         }
-        for (int i = 0; i < n; ++i) in[i]->soa_to_host(A + off[i], dj_initial[i]);
+        HIPCHK(hipStreamSynchronize(stream));  // soa_to_host reads on each instance's own stream
+        for (int i = 0; i < n; ++i) in[i]->soa_to_host(W[ia] + off[i], dj_initial[i]);
         copy_all(scatter_tasks(fin, 0));
         for (int i = 0; i < n; ++i) {
             HIPCHK(hipMemcpyAsync(in[i]->sc, saved[i].data(), sizeof(DevScalars), hipMemcpyHostToDevice, stream));
         }
         HIPCHK(hipStreamSynchronize(stream));
-        (void)hipFree(d_aa);
-        (void)hipFree(d_pre);
-        (void)hipFree(d_ct);
     }
 };
 
