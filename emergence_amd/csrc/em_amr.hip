@@ -595,6 +595,7 @@ struct Amr {
             for (auto p : r) f(p);
         f(d_tmp);
         f(d_det);
+        if (pin) (void)hipHostFree(pin);
         if (gexec) (void)hipGraphExecDestroy(gexec);
         for (auto& e : prof_ev) {
             (void)hipEventDestroy(std::get<1>(e));
@@ -659,6 +660,41 @@ struct Amr {
         return p;
     }
     std::unordered_map<const void*, size_t> dev_caps;
+    // Pinned staging of the table uploads of build_tables: one host copy per
+    // table into pinned memory, then an asynchronous copy from there; the
+    // region is reused only after the stream has been synchronised.
+    char* pin = nullptr;
+    size_t pin_cap = 0, pin_off = 0;
+    void pin_reserve(size_t bytes) {
+        if (bytes <= pin_cap) return;
+        if (pin) HCHK(hipHostFree(pin));
+        pin_cap = bytes + bytes / 2 + 4096;
+        HCHK(hipHostMalloc((void**)&pin, pin_cap, hipHostMallocDefault));
+    }
+    template <class T>
+    static size_t staged_bytes(const std::vector<T>& v) {
+        return (sizeof(T) * v.size() + 255) / 256 * 256;
+    }
+    template <class T>
+    T* dev_vec_staged(const std::vector<T>& v, T* old) {
+        const size_t need = sizeof(T) * v.size();
+        if (v.empty()) return old;
+        T* p = old;
+        if (!(old && dev_caps[old] >= need)) {
+            if (old) {
+                HCHK(hipFree(old));
+                dev_caps.erase(old);
+            }
+            const size_t cap = need + need / 2 + 256;
+            HCHK(hipMalloc(&p, cap));
+            dev_caps[p] = cap;
+        }
+        char* h = pin + pin_off;
+        std::memcpy(h, v.data(), need);
+        pin_off += staged_bytes(v);
+        HCHK(hipMemcpyAsync(p, h, need, hipMemcpyHostToDevice, stream));
+        return p;
+    }
 
     // -------------------------------------------------------------- hierarchy
     long long slot(int l, const int64_t idx[3]) const {
@@ -1271,6 +1307,21 @@ struct Amr {
                     }
                     const int nj = get(l, nk);
                     if (!(nj >= 0 && !blocks[nj].leaf)) continue;
+                    // the fine leaves across the face: children (h0, h1) of the
+                    // neighbour next to it, looked up once; fine face (fa, fb)
+                    // of coarse face (a, bb) lies in child ((2a + fa) / B,
+                    // (2bb + fb) / B) at ((2a + fa) % B, (2bb + fb) % B)
+                    const int fa_n = act[e0] ? 2 : 1, fb_n = act[e1] ? 2 : 1;
+                    int kid[2][2] = {{-1, -1}, {-1, -1}};
+                    for (int h0 = 0; h0 < fa_n; ++h0)
+                        for (int h1 = 0; h1 < fb_n; ++h1) {
+                            int64_t ck[3];
+                            ck[d] = 2 * nk[d] + (side == 1 ? 0 : 1);
+                            ck[e0] = act[e0] ? 2 * b.idx[e0] + h0 : 0;
+                            ck[e1] = act[e1] ? 2 * b.idx[e1] + h1 : 0;
+                            const int cid = get(l + 1, ck);
+                            kid[h0][h1] = (cid >= 0 && blocks[cid].leaf) ? cid : -1;
+                        }
                     for (int a = 0; a < bd[e0]; ++a)
                         for (int bb = 0; bb < bd[e1]; ++bb) {
                             RefluxTask t{};
@@ -1282,19 +1333,13 @@ struct Amr {
                             c[e0] = a;
                             c[e1] = bb;
                             t.cell = (c[0] * bd[1] + c[1]) * bd[2] + c[2];
-                            const int fa_n = act[e0] ? 2 : 1, fb_n = act[e1] ? 2 : 1;
                             for (int fa = 0; fa < fa_n; ++fa)
                                 for (int fb = 0; fb < fb_n; ++fb) {
-                                    const int64_t ga = 2 * (b.idx[e0] * bd[e0] + a) + fa;
-                                    const int64_t gb = 2 * (b.idx[e1] * bd[e1] + bb) + fb;
-                                    int64_t ck[3] = {nk[0], nk[1], nk[2]};
-                                    ck[d] = 2 * nk[d] + (side == 1 ? 0 : 1);
-                                    ck[e0] = act[e0] ? hdiv(ga, bd[e0]) : 0;
-                                    ck[e1] = act[e1] ? hdiv(gb, bd[e1]) : 0;
-                                    const int cid = get(l + 1, ck);
-                                    if (cid < 0 || !blocks[cid].leaf) continue;
-                                    const int la = act[e0] ? (int)hrem(ga, bd[e0]) : 0;
-                                    const int lb = act[e1] ? (int)hrem(gb, bd[e1]) : 0;
+                                    const int ga = 2 * a + fa, gb = 2 * bb + fb;
+                                    const int cid = kid[act[e0] ? ga / bd[e0] : 0][act[e1] ? gb / bd[e1] : 0];
+                                    if (cid < 0) continue;
+                                    const int la = act[e0] ? ga % bd[e0] : 0;
+                                    const int lb = act[e1] ? gb % bd[e1] : 0;
                                     t.fslot[t.nfine] = cid;
                                     t.fface[t.nfine] = la * bd[e1] + lb;
                                     t.nfine++;
@@ -1305,11 +1350,6 @@ struct Amr {
                 }
             }
         }
-        d_leaves = dev_vec(leaves, d_leaves);
-        d_leaf_level = dev_vec(leaf_level, d_leaf_level);
-        d_ghost = dev_vec(ghost, d_ghost);
-        d_edge_ids.resize(levels, nullptr);
-        d_edge_idx.resize(levels, nullptr);
         edge_ent.assign(levels, {});
         for (int l = 0; l < levels; ++l) {
             std::vector<int>& ix = edge_ent[l];
@@ -1317,11 +1357,26 @@ struct Amr {
                 for (int d = 0; d < 3; ++d) ix.push_back((int)blocks[id].idx[d]);
                 ix.push_back(box_mask(id));
             }
-            d_edge_ids[l] = dev_vec(edge[l], d_edge_ids[l]);
-            d_edge_idx[l] = dev_vec(ix, d_edge_idx[l]);
+        }
+        // uploads through the pinned staging region (the stream was
+        // synchronised above, so it is free)
+        size_t total = staged_bytes(leaves) + staged_bytes(leaf_level) + staged_bytes(ghost);
+        for (int l = 0; l < levels; ++l) total += staged_bytes(edge[l]) + staged_bytes(edge_ent[l]);
+        for (int d = 0; d < 3; ++d)
+            for (int s = 0; s < 2; ++s) total += staged_bytes(reflux[d][s]);
+        pin_reserve(total);
+        pin_off = 0;
+        d_leaves = dev_vec_staged(leaves, d_leaves);
+        d_leaf_level = dev_vec_staged(leaf_level, d_leaf_level);
+        d_ghost = dev_vec_staged(ghost, d_ghost);
+        d_edge_ids.resize(levels, nullptr);
+        d_edge_idx.resize(levels, nullptr);
+        for (int l = 0; l < levels; ++l) {
+            d_edge_ids[l] = dev_vec_staged(edge[l], d_edge_ids[l]);
+            d_edge_idx[l] = dev_vec_staged(edge_ent[l], d_edge_idx[l]);
         }
         for (int d = 0; d < 3; ++d)
-            for (int s = 0; s < 2; ++s) d_reflux[d][s] = dev_vec(reflux[d][s], d_reflux[d][s]);
+            for (int s = 0; s < 2; ++s) d_reflux[d][s] = dev_vec_staged(reflux[d][s], d_reflux[d][s]);
         HCHK(hipStreamSynchronize(stream));
         tables_dirty = false;
         ++version;
