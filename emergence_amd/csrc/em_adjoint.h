@@ -43,20 +43,10 @@ struct AdjArgs {
     double* cBout;
 };
 
-// Ensemble VJP: per-instance arguments and prefix sums of their work items
-// (pencils x tiles; ninst + 1 entries; total = prefix[ninst]).
-struct EnsAdjArgs {
-    const AdjArgs* args;
-    const long long* prefix;
-    int ninst;
-    long long total;
-};
-
 constexpr int kAdjThreads = 256;     // threads per adjoint workgroup (4 waves; three workgroups per CU)
 constexpr int kAdjTileCells = 128;   // interior cells per tile at most (LDS of a tile <= 45 KB)
 int adj_tiles(int n, int kind_lo, int kind_hi);
 int adj_tile_extent(int n, int ntile, int kind_lo, int kind_hi);
-hipError_t launch_ens_adj_pass(int d, const EnsAdjArgs& e, size_t lds, unsigned grid, hipStream_t s);
 size_t adj_lds_bytes(int ne);
 hipError_t launch_adj_pass(int d, const AdjArgs& a, hipStream_t s);
 

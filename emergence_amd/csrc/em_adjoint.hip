@@ -1067,49 +1067,12 @@ __global__ __launch_bounds__(kAdjThreads) __attribute__((amdgpu_waves_per_eu(3))
     adj_group<D>(a, w, lds);
 }
 
-// Ensemble VJP: the tiles of every instance in one launch (instance i owns
-// work items prefix[i] .. prefix[i+1]-1), one workgroup per tile: a
-// persistent grid ran 10% slower on 256 instances, its late-starting
-// workgroups setting the end.
-template <int D>
-__global__ __launch_bounds__(kAdjThreads) __attribute__((amdgpu_waves_per_eu(3))) void ens_adj_pass(EnsAdjArgs e) {
-    extern __shared__ double lds[];
-    for (long long w = blockIdx.x; w < e.total; w += gridDim.x) {
-        int lo = 0, hi = e.ninst;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (e.prefix[mid] <= w)
-                lo = mid;
-            else
-                hi = mid;
-        }
-        const AdjArgs a = e.args[lo];  // by value: its fields in registers, not re-read from memory
-        adj_group<D>(a, w - e.prefix[lo], lds);
-        __syncthreads();
-    }
-}
-
 }  // namespace adj
 
 template <class K>
 static hipError_t allow_lds(K kernel, size_t lds) {
     return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds);
-}
-
-hipError_t launch_ens_adj_pass(int d, const EnsAdjArgs& e, size_t lds, unsigned grid, hipStream_t s) {
-    hipError_t er;
-    if (d == 0) {
-        if ((er = allow_lds(adj::ens_adj_pass<0>, lds)) != hipSuccess) return er;
-        hipLaunchKernelGGL(adj::ens_adj_pass<0>, dim3(grid), dim3(kAdjThreads), lds, s, e);
-    } else if (d == 1) {
-        if ((er = allow_lds(adj::ens_adj_pass<1>, lds)) != hipSuccess) return er;
-        hipLaunchKernelGGL(adj::ens_adj_pass<1>, dim3(grid), dim3(kAdjThreads), lds, s, e);
-    } else {
-        if ((er = allow_lds(adj::ens_adj_pass<2>, lds)) != hipSuccess) return er;
-        hipLaunchKernelGGL(adj::ens_adj_pass<2>, dim3(grid), dim3(kAdjThreads), lds, s, e);
-    }
-    return hipGetLastError();
 }
 
 // Tiles per pencil: pencils longer than kAdjTileCells cells are split

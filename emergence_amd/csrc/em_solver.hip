@@ -966,7 +966,7 @@ __global__ void k_ens_copy(const CopyTask* __restrict__ t) {
 // An ensemble of independent uniform-grid solvers (own mesh, domain, state,
 // dt and CFL rate each) advanced together: every axis pass of every
 // instance runs in one persistent launch (ens_pass), a step is replayed as
-// one HIP graph; the adjoint runs its VJPs the same way (ens_adj_pass).
+// one HIP graph; the adjoint launches each instance's VJP (adj_pass).
 struct Ensemble {
     std::vector<Solver*> in;
     int n = 0;
@@ -989,11 +989,9 @@ struct Ensemble {
     std::vector<double*> ckpt;
     double* dt_hist = nullptr;
     std::vector<double> h_dts;
-    double* work = nullptr;    // A, B, Z, T
-    double* store = nullptr;   // 3 states per step of a segment
-    double* store2 = nullptr;  // a second segment (recomputed while the first is reversed), if it fits
-    double* fin = nullptr;     // final states
-    hipStream_t rstream = nullptr;  // the backward sweep's segment recompute (default priority)
+    double* work = nullptr;   // A, B, Z, T
+    double* store = nullptr;  // 3 states per step of a segment
+    double* fin = nullptr;    // final states
 
     Ensemble(int ninst, const int64_t* nn, const double* lo, const double* hi, int ng, const double e[4],
              const int32_t bk[6], const double bs[30], int mode_, int pos, double cfl_, double tol,
@@ -1052,15 +1050,13 @@ struct Ensemble {
         graph = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
-        if (rstream) (void)hipStreamDestroy(rstream);
-        rstream = nullptr;
     }
     ~Ensemble() { release(); }
 
     void free_adjoint() {
         for (auto* p : ckpt) (void)hipFree(p);
         ckpt.clear();
-        for (double** p : {&dt_hist, &work, &store, &store2, &fin})
+        for (double** p : {&dt_hist, &work, &store, &fin})
             if (*p) {
                 (void)hipFree(*p);
                 *p = nullptr;
@@ -1315,186 +1311,107 @@ struct Ensemble {
     void backward(const double* const* dj_final, double* const* dj_initial) {
         if (ckpt.empty()) throw Error(EM_ERR_STATE, "backward needs a forward run first");
         if (!work) HIPCHK(hipMalloc(&work, sizeof(double) * 4 * tot_nv));
-        if (!store) {
-            const size_t seg_bytes = sizeof(double) * 3 * tot_nv * ck_every;
-            HIPCHK(hipMalloc(&store, seg_bytes));
-            // the second segment buffer only if it fits (else the recompute
-            // waits for each sweep)
-            if (hipMalloc(&store2, seg_bytes) != hipSuccess) {
-                (void)hipGetLastError();
-                store2 = nullptr;
-            }
-        }
-        if (!rstream) HIPCHK(hipStreamCreateWithFlags(&rstream, hipStreamNonBlocking));
+        if (!store) HIPCHK(hipMalloc(&store, sizeof(double) * 3 * tot_nv * ck_every));
         std::vector<std::vector<DevScalars>> saved;
         for (auto* S : in) saved.push_back(S->save_scalars());
         // work arrays: A (the cotangent of the state after the step being
         // reversed) and Z swap roles every step; B and T keep theirs
         double* W[4] = {work, work + tot_nv, work + 2 * tot_nv, work + 3 * tot_nv};
         for (int i = 0; i < n; ++i) in[i]->host_to_soa(dj_final[i], W[0] + off[i]);
-        // tiles of each instance and axis (its own, as a single-instance
-        // backward would use; the sums do not depend on the tiling); LDS per
-        // axis: the largest need
-        size_t lds_ax[3] = {0, 0, 0};
-        for (auto* S : in)
-            for (int d = 0; d < 3; ++d)
-                if (S->active(d)) {
-                    const int nd = (int)S->n[d];
-                    const int nt = adj_tiles(nd, S->bc_kind[d][0], S->bc_kind[d][1]);
-                    lds_ax[d] = std::max(lds_ax[d],
-                                         adj_lds_bytes(adj_tile_extent(nd, nt, S->bc_kind[d][0], S->bc_kind[d][1])));
-                }
         const int64_t nsteps = fwd_steps;
         const int64_t nseg = (int64_t)ckpt.size();
         const int k = ck_every;
-        double* sbuf[2] = {store, store2 ? store2 : store};
-        std::vector<long long> pre(3 * (n + 1), 0);
-        for (int d = 0; d < 3; ++d)
-            for (int i = 0; i < n; ++i) {
-                const Solver& S = *in[i];
-                const long long groups =
-                    S.active(d) ? S.ncell / S.n[d] * adj_tiles((int)S.n[d], S.bc_kind[d][0], S.bc_kind[d][1]) : 0;
-                pre[d * (n + 1) + i + 1] = pre[d * (n + 1) + i] + groups;
-            }
-        // copy tables: per store buffer b and step slot j of a segment the
-        // gathers U^n -> slot 3j, U1 -> 3j + 1, U2 -> 3j + 2 ([b][j][3][n]),
-        // then per segment the scatter of its checkpoint into U^n ([seg][n])
-        std::vector<CopyTask> cp((size_t)(2 * k * 3 + nseg) * n);
-        for (int bb = 0; bb < 2; ++bb)
-            for (int j = 0; j < k; ++j)
-                for (int g = 0; g < 3; ++g)
-                    for (int i = 0; i < n; ++i)
-                        cp[(((size_t)bb * k + j) * 3 + g) * n + i] =
-                            CopyTask{in[i]->U[g], sbuf[bb] + (3 * (size_t)j + g) * tot_nv + off[i], off[i + 1] - off[i]};
-        const size_t scat0 = (size_t)2 * k * 3 * n;
+        // copy tables: per step slot j of a segment the gathers U^n -> slot
+        // 3j, U1 -> 3j + 1, U2 -> 3j + 2 ([j][3][n]), then per segment the
+        // scatter of its checkpoint into U^n ([seg][n])
+        std::vector<CopyTask> cp((size_t)(k * 3 + nseg) * n);
+        for (int j = 0; j < k; ++j)
+            for (int g = 0; g < 3; ++g)
+                for (int i = 0; i < n; ++i)
+                    cp[((size_t)j * 3 + g) * n + i] =
+                        CopyTask{in[i]->U[g], store + (3 * (size_t)j + g) * tot_nv + off[i], off[i + 1] - off[i]};
+        const size_t scat0 = (size_t)k * 3 * n;
         for (int64_t sg = 0; sg < nseg; ++sg)
             for (int i = 0; i < n; ++i)
                 cp[scat0 + (size_t)sg * n + i] = CopyTask{ckpt[sg] + off[i], in[i]->U[0], off[i + 1] - off[i]};
-        // VJP arguments of every step: [step][stage 0..2][axis][instance],
-        // with the RK updates fused into each stage's last axis as in
-        // Solver::backward (stage 2's new B into A's array; stage 1 leaves
-        // the step's input cotangent in Z, which becomes A)
-        const size_t per_step = (size_t)9 * n;
-        std::vector<AdjArgs> aa((size_t)nsteps * per_step);
-        int ia = 0, iz = 2;
-        for (int64_t q = nsteps - 1; q >= 0; --q) {
-            const int64_t sg = q / k;
-            const double* st = sbuf[sg & 1] + 3 * (q - sg * k) * tot_nv;
-            double* A = W[ia];
-            double* B = W[1];
-            double* Zb = W[iz];
-            double* T = W[3];
-            for (int stg = 0; stg < 3; ++stg)
-                for (int d = 0; d < 3; ++d)
-                    for (int i = 0; i < n; ++i) {
-                        const Solver& S = *in[i];
-                        AdjArgs& a = aa[(size_t)q * per_step + (size_t)(stg * 3 + d) * n + i];
-                        a = AdjArgs{};
-                        if (!S.active(d)) continue;
-                        const double dt = h_dts[(size_t)q * n + i];
-                        // stage 3 linearises at u2, stage 2 at u1, stage 1 at u^n
-                        a.U = st + (2 - stg) * tot_nv + off[i];
-                        a.LAM = (stg == 1 ? B : A) + off[i];
-                        a.MU = T + off[i];
-                        for (int e = 0; e < 3; ++e) {
-                            a.n[e] = (int)S.n[e];
-                            a.stride[e] = S.stride[e];
-                        }
-                        a.ncell = S.ncell;
-                        a.eos = S.eos;
-                        for (int sd = 0; sd < 2; ++sd) {
-                            a.bc_kind[sd] = S.bc_kind[d][sd];
-                            for (int v = 0; v < 5; ++v) a.bc_state[sd][v] = S.bc_state[d][sd][v];
-                        }
-                        a.inv_dx = 1.0 / S.dx[d];
-                        a.lam_lim = S.positivity ? (double)S.nactive * dt / S.dx[d] : 0.0;
-                        int first_ax = -1, last_ax = -1;
-                        for (int e = 0; e < 3; ++e)
-                            if (S.active(e)) {
-                                if (first_ax < 0) first_ax = e;
-                                last_ax = e;
-                            }
-                        a.first = d == first_ax ? 1 : 0;
-                        a.ntile = adj_tiles((int)S.n[d], S.bc_kind[d][0], S.bc_kind[d][1]);
-                        a.ne = adj_tile_extent((int)S.n[d], a.ntile, S.bc_kind[d][0], S.bc_kind[d][1]);
-                        if (d == last_ax) {
-                            a.combine = 3 - stg;
-                            a.cdt = dt;
-                            a.cA = A + off[i];
-                            a.cB = (stg == 1 ? B : A) + off[i];
-                            a.cZ = Zb + off[i];
-                            a.cBout = (stg == 0 ? B : A) + off[i];
-                        }
-                    }
-            std::swap(ia, iz);
-        }
-        AdjArgs* d_aa = nullptr;
-        long long* d_pre = nullptr;
-        CopyTask* d_cp = nullptr;
-        struct Free {  // device tables and events, released when backward() returns
-            std::vector<void*> p;
-            std::vector<hipEvent_t> e;
+        struct Free {  // device table, released when backward() returns
+            void* p = nullptr;
             ~Free() {
-                for (auto x : p)
-                    if (x) (void)hipFree(x);
-                for (auto x : e)
-                    if (x) (void)hipEventDestroy(x);
+                if (p) (void)hipFree(p);
             }
         } fr;
-        HIPCHK(hipMalloc(&d_aa, sizeof(AdjArgs) * aa.size()));
-        fr.p.push_back(d_aa);
-        HIPCHK(hipMalloc(&d_pre, sizeof(long long) * pre.size()));
-        fr.p.push_back(d_pre);
+        CopyTask* d_cp = nullptr;
         HIPCHK(hipMalloc(&d_cp, sizeof(CopyTask) * cp.size()));
-        fr.p.push_back(d_cp);
-        HIPCHK(hipMemcpyAsync(d_aa, aa.data(), sizeof(AdjArgs) * aa.size(), hipMemcpyHostToDevice, stream));
-        HIPCHK(hipMemcpyAsync(d_pre, pre.data(), sizeof(long long) * pre.size(), hipMemcpyHostToDevice, stream));
+        fr.p = d_cp;
         HIPCHK(hipMemcpyAsync(d_cp, cp.data(), sizeof(CopyTask) * cp.size(), hipMemcpyHostToDevice, stream));
         HIPCHK(hipStreamSynchronize(stream));
-        // Segments are recomputed on rstream into store buffer (seg & 1) and
-        // reversed on stream.  With two buffers the recompute of segment
-        // seg - 1 runs while segment seg is reversed (it waits only for the
-        // sweep of seg + 1, the last reader of its buffer); with one it
-        // waits for the sweep of seg.  The sweep never touches the instances'
-        // states, scalars or residuals, the recompute nothing else.
-        hipEvent_t ev_rec[2], ev_done[2];
-        for (auto* x : {&ev_rec[0], &ev_rec[1], &ev_done[0], &ev_done[1]}) {
-            HIPCHK(hipEventCreateWithFlags(x, hipEventDisableTiming));
-            fr.e.push_back(*x);
-        }
-        auto recompute = [&](int64_t sg) {
+        // VJP arguments of one step ([stage 0..2][axis][instance]), with the
+        // RK updates fused into each stage's last axis as in
+        // Solver::backward (stage 2's new B into A's array; stage 1 leaves
+        // the step's input cotangent in Z, which becomes A); each instance's
+        // VJP is its own adj_pass launch, as its single-grid backward runs it
+        std::vector<AdjArgs> aa((size_t)9 * n);
+        int ia = 0, iz = 2;
+        for (int64_t sg = nseg - 1; sg >= 0; --sg) {
             const int64_t s0 = sg * k, s1 = std::min<int64_t>(s0 + k, nsteps);
-            const int bb = (int)(sg & 1);
-            copy_table(d_cp + scat0 + (size_t)sg * n, n, rstream);
+            // recompute: store slots 3 (q - s0) .. +2 hold U^q and its two
+            // stage states
+            copy_table(d_cp + scat0 + (size_t)sg * n, n, stream);
             for (int64_t q = s0; q < s1; ++q) {
                 const size_t j = (size_t)(q - s0);
-                copy_table(d_cp + ((size_t)bb * k + j) * 3 * n, n, rstream);
-                step_kernels(false, q, rstream);
-                copy_table(d_cp + (((size_t)bb * k + j) * 3 + 1) * n, 2 * n, rstream);
+                copy_table(d_cp + j * 3 * n, n, stream);
+                step_kernels(false, q);
+                copy_table(d_cp + (j * 3 + 1) * n, 2 * n, stream);
             }
-            HIPCHK(hipEventRecord(ev_rec[bb], rstream));
-        };
-        recompute(nseg - 1);
-        for (int64_t sg = nseg - 1; sg >= 0; --sg) {
-            if (store2 && sg > 0) {
-                if (sg + 1 < nseg) HIPCHK(hipStreamWaitEvent(rstream, ev_done[(sg + 1) & 1], 0));
-                recompute(sg - 1);
-            }
-            HIPCHK(hipStreamWaitEvent(stream, ev_rec[sg & 1], 0));
-            const int64_t s0 = sg * k, s1 = std::min<int64_t>(s0 + k, nsteps);
-            for (int64_t q = s1 - 1; q >= s0; --q)
+            for (int64_t q = s1 - 1; q >= s0; --q) {
+                const double* st = store + 3 * (q - s0) * tot_nv;
+                double* A = W[ia];
+                double* B = W[1];
+                double* Zb = W[iz];
+                double* T = W[3];
                 for (int stg = 0; stg < 3; ++stg)
-                    for (int d = 0; d < 3; ++d) {
-                        const long long tot = pre[d * (n + 1) + n];
-                        if (tot == 0) continue;
-                        EnsAdjArgs e{d_aa + (size_t)q * per_step + (size_t)(stg * 3 + d) * n, d_pre + d * (n + 1), n,
-                                     tot};
-                        HIPCHK(launch_ens_adj_pass(d, e, lds_ax[d], (unsigned)tot, stream));  // one workgroup per tile
-                    }
-            HIPCHK(hipEventRecord(ev_done[sg & 1], stream));
-            if (!store2 && sg > 0) {
-                HIPCHK(hipStreamWaitEvent(rstream, ev_done[sg & 1], 0));
-                recompute(sg - 1);
+                    for (int d = 0; d < 3; ++d)
+                        for (int i = 0; i < n; ++i) {
+                            const Solver& S = *in[i];
+                            if (!S.active(d)) continue;
+                            AdjArgs a{};
+                            const double dt = h_dts[(size_t)q * n + i];
+                            // stage 3 linearises at u2, stage 2 at u1, stage 1 at u^n
+                            a.U = st + (2 - stg) * tot_nv + off[i];
+                            a.LAM = (stg == 1 ? B : A) + off[i];
+                            a.MU = T + off[i];
+                            for (int e = 0; e < 3; ++e) {
+                                a.n[e] = (int)S.n[e];
+                                a.stride[e] = S.stride[e];
+                            }
+                            a.ncell = S.ncell;
+                            a.eos = S.eos;
+                            for (int sd = 0; sd < 2; ++sd) {
+                                a.bc_kind[sd] = S.bc_kind[d][sd];
+                                for (int v = 0; v < 5; ++v) a.bc_state[sd][v] = S.bc_state[d][sd][v];
+                            }
+                            a.inv_dx = 1.0 / S.dx[d];
+                            a.lam_lim = S.positivity ? (double)S.nactive * dt / S.dx[d] : 0.0;
+                            int first_ax = -1, last_ax = -1;
+                            for (int e = 0; e < 3; ++e)
+                                if (S.active(e)) {
+                                    if (first_ax < 0) first_ax = e;
+                                    last_ax = e;
+                                }
+                            a.first = d == first_ax ? 1 : 0;
+                            a.ntile = adj_tiles((int)S.n[d], S.bc_kind[d][0], S.bc_kind[d][1]);
+                            a.ne = adj_tile_extent((int)S.n[d], a.ntile, S.bc_kind[d][0], S.bc_kind[d][1]);
+                            if (d == last_ax) {
+                                a.combine = 3 - stg;
+                                a.cdt = dt;
+                                a.cA = A + off[i];
+                                a.cB = (stg == 1 ? B : A) + off[i];
+                                a.cZ = Zb + off[i];
+                                a.cBout = (stg == 0 ? B : A) + off[i];
+                            }
+                            HIPCHK(launch_adj_pass(d, a, stream));
+                        }
+                std::swap(ia, iz);
             }
         }
         HIPCHK(hipStreamSynchronize(stream));  // soa_to_host reads on each instance's own stream

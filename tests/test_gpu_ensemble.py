@@ -3,8 +3,9 @@ pass) against the single-grid HIP solver, which is itself bit-identical to
 the oracle:
 * forward steps of every instance bit-identical to running it alone, in
   batched (after the tile autotuner) and per-instance launch modes;
-* the batched adjoint bit-identical to each instance's own backward sweep
-  with the same frozen dt sequence;
+* the ensemble adjoint (batched recompute, each instance's VJP launched on
+  its own) bit-identical to each instance's own backward sweep with the same
+  frozen dt sequence;
 * dJ/dMach of the shock-bubble workload against central differences.
 """
 import numpy as np
