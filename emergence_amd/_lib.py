@@ -29,6 +29,7 @@ EXPORTS = (
     "em_solver_time",
     "em_solver_synchronize",
     "em_solver_adjoint_rhs",
+    "em_solver_set_adjoint_tile",
     "em_solver_forward",
     "em_solver_backward",
     "em_case_init",
@@ -114,6 +115,7 @@ def load():
     lib.em_solver_time.argtypes = [P, D]
     lib.em_solver_synchronize.argtypes = [P]
     lib.em_solver_adjoint_rhs.argtypes = [P, C.c_double, D, D]
+    lib.em_solver_set_adjoint_tile.argtypes = [P, C.c_int32]
     lib.em_solver_forward.argtypes = [P, C.c_int64, C.c_int32, D, D]
     lib.em_solver_backward.argtypes = [P, D, D]
     lib.em_case_init.argtypes = [C.c_int32, I64, D, D, C.c_int32, D, D]

@@ -45,7 +45,7 @@ struct AdjArgs {
 
 constexpr int kAdjThreads = 256;     // threads per adjoint workgroup (4 waves; three workgroups per CU)
 constexpr int kAdjTileCells = 128;   // interior cells per tile at most (LDS of a tile <= 45 KB)
-int adj_tiles(int n, int kind_lo, int kind_hi);
+int adj_tiles(int n, int kind_lo, int kind_hi, int cap = kAdjTileCells);
 int adj_tile_extent(int n, int ntile, int kind_lo, int kind_hi);
 size_t adj_lds_bytes(int ne);
 hipError_t launch_adj_pass(int d, const AdjArgs& a, hipStream_t s);

@@ -1075,12 +1075,14 @@ static hipError_t allow_lds(K kernel, size_t lds) {
                                (int)lds);
 }
 
-// Tiles per pencil: pencils longer than kAdjTileCells cells are split
-// evenly; an axis periodic on one side only keeps whole pencils (its ghosts
-// are copies of the far end).
-int adj_tiles(int n, int kind_lo, int kind_hi) {
+// Tiles per pencil: pencils longer than cap cells (kAdjTileCells unless set
+// otherwise; at least 10, so that an even split leaves every tile the 5
+// cells a reflecting edge routes its ghosts to) are split evenly; an axis
+// periodic on one side only keeps whole pencils (its ghosts are copies of
+// the far end).
+int adj_tiles(int n, int kind_lo, int kind_hi, int cap) {
     if ((kind_lo == 0) != (kind_hi == 0)) return 1;
-    return std::max(1, (n + kAdjTileCells - 1) / kAdjTileCells);
+    return std::max(1, (n + cap - 1) / cap);
 }
 
 // Output positions of a pencil's longest tile: its interior cells plus the

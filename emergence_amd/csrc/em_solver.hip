@@ -658,6 +658,7 @@ struct Solver {
     std::vector<double> fwd_dts;
     int ck_every = 0;
     double* adjbuf = nullptr;  // A, B, Z, T: 4 x 5 ncell
+    int adj_tile_cap = kAdjTileCells;  // interior cells per adjoint tile at most
     double* store = nullptr;   // u_n, u1, u2 of every step of one segment
     double* store2 = nullptr;  // a second segment (recomputed while the first is reversed), if it fits
     long long store_steps = 0;
@@ -688,11 +689,11 @@ struct Solver {
                 if (bc_kind[d][sd] == BC_REFLECT && n[d] < 5)
                     throw Error(EM_ERR_ARG, "the adjoint needs at least 5 cells on a reflecting axis");
             }
-            const int nt = adj_tiles((int)n[d], bc_kind[d][0], bc_kind[d][1]);
+            const int nt = adj_tiles((int)n[d], bc_kind[d][0], bc_kind[d][1], adj_tile_cap);
             const size_t need = adj_lds_bytes(adj_tile_extent((int)n[d], nt, bc_kind[d][0], bc_kind[d][1]));
             if (need > std::max(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin))
-                throw Error(EM_ERR_ARG,
-                            "the adjoint keeps whole pencils in LDS on an axis periodic on one side only: axis too long");
+                throw Error(EM_ERR_ARG, "adjoint tile too long for LDS (whole pencils on an axis periodic on one "
+                                        "side only, or a large tile size)");
         }
     }
 
@@ -725,7 +726,7 @@ struct Solver {
             a.inv_dx = 1.0 / dx[d];
             a.lam_lim = positivity ? (double)nactive * stage_dt / dx[d] : 0.0;  // as the forward pass
             a.first = first ? 1 : 0;
-            a.ntile = adj_tiles((int)n[d], bc_kind[d][0], bc_kind[d][1]);
+            a.ntile = adj_tiles((int)n[d], bc_kind[d][0], bc_kind[d][1], adj_tile_cap);
             a.ne = adj_tile_extent((int)n[d], a.ntile, bc_kind[d][0], bc_kind[d][1]);
             if (d == last && combine != 0) {
                 a.combine = combine;
@@ -1399,7 +1400,7 @@ struct Ensemble {
                                     last_ax = e;
                                 }
                             a.first = d == first_ax ? 1 : 0;
-                            a.ntile = adj_tiles((int)S.n[d], S.bc_kind[d][0], S.bc_kind[d][1]);
+                            a.ntile = adj_tiles((int)S.n[d], S.bc_kind[d][0], S.bc_kind[d][1], S.adj_tile_cap);
                             a.ne = adj_tile_extent((int)S.n[d], a.ntile, S.bc_kind[d][0], S.bc_kind[d][1]);
                             if (d == last_ax) {
                                 a.combine = 3 - stg;
@@ -1660,6 +1661,24 @@ int em_solver_adjoint_rhs(em_solver* h, double stage_dt, const double* lam, doub
         return EM_ERR_ARG;
     }
     return guarded([&] { h->s->adjoint_rhs(stage_dt, lam, out); });
+}
+
+int em_solver_set_adjoint_tile(em_solver* h, int32_t cells) {
+    NEED(h);
+    if (cells < 10) {
+        em::g_last_error = "adjoint tile size must be at least 10 cells";
+        return EM_ERR_ARG;
+    }
+    return guarded([&] {
+        const int keep = h->s->adj_tile_cap;
+        h->s->adj_tile_cap = cells;
+        try {
+            h->s->adj_check();
+        } catch (...) {
+            h->s->adj_tile_cap = keep;
+            throw;
+        }
+    });
 }
 
 int em_solver_forward(em_solver* h, int64_t nsteps, int32_t checkpoint_every, const double* dts, double* dts_out) {

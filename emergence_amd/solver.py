@@ -135,6 +135,10 @@ class HipSolver:
         _lib.check(self.lib.em_solver_synchronize(self.h))
 
     # ---- discrete adjoint (FluxMode::Weno)
+    def set_adjoint_tile(self, cells):
+        """Interior cells per VJP tile at most (>= 10); results do not depend on it."""
+        _lib.check(self.lib.em_solver_set_adjoint_tile(self.h, int(cells)))
+
     def adjoint_rhs(self, stage_dt, lam):
         """(dR/du)^T lam of one residual evaluation at the current state."""
         lam = np.ascontiguousarray(lam, dtype=np.float64).reshape(self.shape)

@@ -129,6 +129,12 @@ int em_solver_synchronize(em_solver* s);
  * weights' derivative on stencils flat to rounding and are held on partly
  * flat ones; the dt sequence is frozen (taken from the forward run). */
 
+/* Interior cells per adjoint tile at most (default 128, at least 10): the
+ * VJP's work unit along each axis.  Results do not depend on it; it exists
+ * for tuning and for the test that shows so.  No counterpart in the
+ * reference. */
+int em_solver_set_adjoint_tile(em_solver* s, int32_t cells);
+
 /* (dR/du)^T lam of one residual evaluation at the current state, with the
  * limiter's lambda from stage_dt; lam and out in the CPU layout (ghosts
  * ignored / written as zero). */

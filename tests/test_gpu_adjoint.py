@@ -179,3 +179,26 @@ def test_adjoint_rhs_long_pencils(em, case):
         ref = o.vjp_rhs(stage_dt, u, lam)
         got = g.adjoint_rhs(stage_dt, lam)
         assert _rel(o.interior(got), o.interior(ref)) < 1e-11, stage_dt
+
+
+@pytest.mark.parametrize("case", ["outflow-reflect-300", "periodic-263"])
+def test_adjoint_results_do_not_depend_on_the_tiling(em, case):
+    # the same VJP with tiles of at most 128 cells (the default), 37 and 10
+    # cells, and with whole pencils: every cotangent sum runs in a fixed order
+    # of global positions, so the bits are the same
+    print(f"adjoint tiling independence {case}", flush=True)
+    if case == "outflow-reflect-300":
+        n, bc = (300, 12, 1), [["outflow", "reflect"], ["reflect", "outflow"], ["periodic", "periodic"]]
+    else:
+        n, bc = (263, 11, 1), [["periodic", "periodic"]] * 3
+    o0 = OracleAdj(n, (0, 0, 0), (1, 1, 1), bc=bc)
+    u = smooth_state(o0, 5)
+    g = em.HipSolver(n, (0, 0, 0), (1, 1, 1), u, bc=bc, mode="weno")
+    rng = np.random.default_rng(4)
+    lam = np.zeros_like(u)
+    o0.interior(lam)[...] = rng.standard_normal(o0.interior(lam).shape)
+    ref = g.adjoint_rhs(0.2, lam)
+    assert np.abs(o0.interior(ref)).max() > 0.0
+    for cells in (37, 10, 300):
+        g.set_adjoint_tile(cells)
+        assert np.array_equal(g.adjoint_rhs(0.2, lam), ref), cells
