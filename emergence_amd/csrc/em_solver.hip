@@ -993,6 +993,8 @@ struct Ensemble {
     double* work = nullptr;   // A, B, Z, T
     double* store = nullptr;  // 3 states per step of a segment
     double* fin = nullptr;    // final states
+    static constexpr int kSweepStreams = 4;
+    hipStream_t sweep[kSweepStreams] = {};  // instance i's VJPs on sweep[i % 4] (default priority)
 
     Ensemble(int ninst, const int64_t* nn, const double* lo, const double* hi, int ng, const double e[4],
              const int32_t bk[6], const double bs[30], int mode_, int pos, double cfl_, double tol,
@@ -1051,6 +1053,10 @@ struct Ensemble {
         graph = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
+        for (auto& x : sweep) {
+            if (x) (void)hipStreamDestroy(x);
+            x = nullptr;
+        }
     }
     ~Ensemble() { release(); }
 
@@ -1335,12 +1341,25 @@ struct Ensemble {
         for (int64_t sg = 0; sg < nseg; ++sg)
             for (int i = 0; i < n; ++i)
                 cp[scat0 + (size_t)sg * n + i] = CopyTask{ckpt[sg] + off[i], in[i]->U[0], off[i + 1] - off[i]};
-        struct Free {  // device table, released when backward() returns
+        struct Free {  // device table and events, released when backward() returns
             void* p = nullptr;
+            std::vector<hipEvent_t> e;
             ~Free() {
                 if (p) (void)hipFree(p);
+                for (auto x : e)
+                    if (x) (void)hipEventDestroy(x);
             }
         } fr;
+        for (auto& x : sweep)
+            if (!x) HIPCHK(hipStreamCreateWithFlags(&x, hipStreamNonBlocking));
+        // ev[0]: the recompute of a segment is done (the sweeps wait for it);
+        // ev[1 + j]: sweep stream j is done with the segment (the next
+        // recompute, which rewrites the store and the states, waits for all)
+        hipEvent_t ev[1 + kSweepStreams];
+        for (auto& x : ev) {
+            HIPCHK(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+            fr.e.push_back(x);
+        }
         CopyTask* d_cp = nullptr;
         HIPCHK(hipMalloc(&d_cp, sizeof(CopyTask) * cp.size()));
         fr.p = d_cp;
@@ -1350,13 +1369,16 @@ struct Ensemble {
         // RK updates fused into each stage's last axis as in
         // Solver::backward (stage 2's new B into A's array; stage 1 leaves
         // the step's input cotangent in Z, which becomes A); each instance's
-        // VJP is its own adj_pass launch, as its single-grid backward runs it
-        std::vector<AdjArgs> aa((size_t)9 * n);
+        // VJP is its own adj_pass launch, as its single-grid backward runs it,
+        // on sweep stream i % 4: instances touch disjoint slices of the work
+        // arrays, so different instances' launches may overlap
         int ia = 0, iz = 2;
         for (int64_t sg = nseg - 1; sg >= 0; --sg) {
             const int64_t s0 = sg * k, s1 = std::min<int64_t>(s0 + k, nsteps);
             // recompute: store slots 3 (q - s0) .. +2 hold U^q and its two
             // stage states
+            if (sg < nseg - 1)
+                for (int j = 0; j < kSweepStreams; ++j) HIPCHK(hipStreamWaitEvent(stream, ev[1 + j], 0));
             copy_table(d_cp + scat0 + (size_t)sg * n, n, stream);
             for (int64_t q = s0; q < s1; ++q) {
                 const size_t j = (size_t)(q - s0);
@@ -1364,6 +1386,8 @@ struct Ensemble {
                 step_kernels(false, q);
                 copy_table(d_cp + (j * 3 + 1) * n, 2 * n, stream);
             }
+            HIPCHK(hipEventRecord(ev[0], stream));
+            for (int j = 0; j < kSweepStreams; ++j) HIPCHK(hipStreamWaitEvent(sweep[j], ev[0], 0));
             for (int64_t q = s1 - 1; q >= s0; --q) {
                 const double* st = store + 3 * (q - s0) * tot_nv;
                 double* A = W[ia];
@@ -1410,11 +1434,13 @@ struct Ensemble {
                                 a.cZ = Zb + off[i];
                                 a.cBout = (stg == 0 ? B : A) + off[i];
                             }
-                            HIPCHK(launch_adj_pass(d, a, stream));
+                            HIPCHK(launch_adj_pass(d, a, sweep[i % kSweepStreams]));
                         }
                 std::swap(ia, iz);
             }
+            for (int j = 0; j < kSweepStreams; ++j) HIPCHK(hipEventRecord(ev[1 + j], sweep[j]));
         }
+        for (int j = 0; j < kSweepStreams; ++j) HIPCHK(hipStreamWaitEvent(stream, ev[1 + j], 0));
         HIPCHK(hipStreamSynchronize(stream));  // soa_to_host reads on each instance's own stream
         for (int i = 0; i < n; ++i) in[i]->soa_to_host(W[ia] + off[i], dj_initial[i]);
         copy_all(scatter_tasks(fin, 0));
