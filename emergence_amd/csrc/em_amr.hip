@@ -259,6 +259,53 @@ __global__ __launch_bounds__(kGhostThreads) void k_amr_ghost(AmrDev a, AmrWeight
     }
 }
 
+// The same-level copies and boundary images of the ghost slabs (task kinds
+// 0 and 1), one thread per slab cell: every task has ng * B^(D-1) cells.
+// Light enough to run at full occupancy, while the coarse-fine predictions
+// (kind 2) keep k_amr_ghost and its register budget.  Tasks write disjoint
+// slabs and read only block data, so the split changes no value.
+constexpr int kGhostCopyThreads = 256;
+__global__ __launch_bounds__(kGhostCopyThreads) void k_amr_ghost_copy(AmrDev a, const GhostTask* __restrict__ tasks,
+                                                                       long long ntot, int per) {
+    for (long long gi = blockIdx.x * (long long)kGhostCopyThreads + threadIdx.x; gi < ntot;
+         gi += (long long)gridDim.x * kGhostCopyThreads) {
+        const GhostTask tk = tasks[gi / per];
+        const int it = (int)(gi % per);
+        const int d = tk.axis, side = tk.side;
+        const int E0 = d == 0 ? 1 : 0, E1 = d == 2 ? 1 : 2;
+        const int B = a.bd[d];
+        const int nt = a.bcells / B;
+        const int ng = a.ng;
+        const int s = 2 * d + side;
+        const int layer = it / nt, t = it % nt;
+        int cc[3];
+        cc[E0] = t / a.bd[E1];
+        cc[E1] = t % a.bd[E1];
+        const int x = side == 0 ? layer - ng : B + layer;
+        double u[5];
+        if (tk.kind == 0) {
+            cc[d] = side == 0 ? B + x : x - B;
+            const long long cell = ((long long)cc[0] * a.bd[1] + cc[1]) * a.bd[2] + cc[2];
+            for (int q = 0; q < 5; ++q) u[q] = a.U[amr_u(a, q, tk.src, cell)];
+        } else {
+            auto ld = [&](int xx, double* o) {
+                int c3[3] = {cc[0], cc[1], cc[2]};
+                c3[d] = xx;
+                const long long cell = ((long long)c3[0] * a.bd[1] + c3[1]) * a.bd[2] + c3[2];
+                for (int q = 0; q < 5; ++q) o[q] = a.U[amr_u(a, q, tk.slot, cell)];
+            };
+            const int kind[2] = {a.bc_kind[d][0], a.bc_kind[d][1]};
+            if (d == 0)
+                bc_ghost<0>(a.eos, B, x, kind, a.bc_state[d], ld, u);
+            else if (d == 1)
+                bc_ghost<1>(a.eos, B, x, kind, a.bc_state[d], ld, u);
+            else
+                bc_ghost<2>(a.eos, B, x, kind, a.bc_state[d], ld, u);
+        }
+        for (int q = 0; q < 5; ++q) a.G[amr_g(a, q, tk.slot, s, layer, t)] = u[q];
+    }
+}
+
 // Restriction of a list of ancestors of one level.  Entry i holds the block
 // coordinates and a write mask: bit (o0+1)*9 + (o1+1)*3 + (o2+1) selects the
 // edge box towards neighbour offset o (cells within w of that side), bit 27
@@ -443,6 +490,9 @@ struct Amr {
     int* d_leaves = nullptr;
     int* d_leaf_level = nullptr;
     GhostTask* d_ghost = nullptr;
+    std::vector<GhostTask> ghost_copy, ghost_pred;  // the batched launch's split of `ghost` by kind
+    GhostTask* d_ghost_copy = nullptr;
+    GhostTask* d_ghost_pred = nullptr;
     std::vector<int*> d_edge_ids, d_edge_idx;
     RefluxTask* d_reflux[3][2] = {};
     int* d_tmp = nullptr;
@@ -589,6 +639,8 @@ struct Amr {
         f(d_leaves);
         f(d_leaf_level);
         f(d_ghost);
+        f(d_ghost_copy);
+        f(d_ghost_pred);
         for (auto p : d_edge_ids) f(p);
         for (auto p : d_edge_idx) f(p);
         for (auto& r : d_reflux)
@@ -1360,7 +1412,11 @@ struct Amr {
         }
         // uploads through the pinned staging region (the stream was
         // synchronised above, so it is free)
-        size_t total = staged_bytes(leaves) + staged_bytes(leaf_level) + staged_bytes(ghost);
+        ghost_copy.clear();
+        ghost_pred.clear();
+        for (const GhostTask& t : ghost) (t.kind == 2 ? ghost_pred : ghost_copy).push_back(t);
+        size_t total = staged_bytes(leaves) + staged_bytes(leaf_level) + staged_bytes(ghost) +
+                       staged_bytes(ghost_copy) + staged_bytes(ghost_pred);
         for (int l = 0; l < levels; ++l) total += staged_bytes(edge[l]) + staged_bytes(edge_ent[l]);
         for (int d = 0; d < 3; ++d)
             for (int s = 0; s < 2; ++s) total += staged_bytes(reflux[d][s]);
@@ -1369,6 +1425,8 @@ struct Amr {
         d_leaves = dev_vec_staged(leaves, d_leaves);
         d_leaf_level = dev_vec_staged(leaf_level, d_leaf_level);
         d_ghost = dev_vec_staged(ghost, d_ghost);
+        d_ghost_copy = dev_vec_staged(ghost_copy, d_ghost_copy);
+        d_ghost_pred = dev_vec_staged(ghost_pred, d_ghost_pred);
         d_edge_ids.resize(levels, nullptr);
         d_edge_idx.resize(levels, nullptr);
         for (int l = 0; l < levels; ++l) {
@@ -1458,9 +1516,21 @@ struct Amr {
         const int n = (int)ghost.size();
         if (!n) return;
         if (batched) {
-            const unsigned g = (unsigned)std::min(n, small_wg);
-            hipLaunchKernelGGL(k_amr_ghost, dim3(g), dim3(kGhostThreads), 0, stream, dev, W, d_ghost, n, small_scr, small_cap);
-            HCHK(hipGetLastError());
+            const int nc = (int)ghost_copy.size(), np = (int)ghost_pred.size();
+            if (nc) {
+                const int per = ng * (bcells / B);
+                const long long ntot = (long long)nc * per;
+                const unsigned g = (unsigned)std::min<long long>((ntot + kGhostCopyThreads - 1) / kGhostCopyThreads, 65536);
+                hipLaunchKernelGGL(k_amr_ghost_copy, dim3(g), dim3(kGhostCopyThreads), 0, stream, dev, d_ghost_copy, ntot,
+                                   per);
+                HCHK(hipGetLastError());
+            }
+            if (np) {
+                const unsigned g = (unsigned)std::min(np, small_wg);
+                hipLaunchKernelGGL(k_amr_ghost, dim3(g), dim3(kGhostThreads), 0, stream, dev, W, d_ghost_pred, np, small_scr,
+                                   small_cap);
+                HCHK(hipGetLastError());
+            }
         } else {
             // one launch per leaf (its 2*D slabs)
             const int per = 2 * ndim;
