@@ -304,87 +304,302 @@ __device__ double* predict_region(const AmrDev& a, const AmrWeights& W, int lf, 
     return in;
 }
 
+// A box of level-l cells g0 .. g0+sz-1 read in place: at box cell c, the
+// value and validity that gathering the box into scratch (blocks of level l,
+// periodic wrap) and then filling it with box_bc would leave there, for boxes
+// that reach past no characteristic boundary.  A cell with no coordinate
+// outside a non-periodic boundary is its block's data (invalid, zero, where
+// the block is absent).  A cell outside is box_bc's image along the last axis
+// it lies outside of, of a source cell that box_bc has filled along the
+// earlier axes first: so the source chain is walked from the last outside
+// axis down.  Outflow moves to the boundary cell, reflect to the mirror cell
+// and negates that axis's momentum, a Dirichlet state ends the chain; each
+// axis negates a different component, so the sign changes commute.  Along an
+// axis the step depends on that coordinate alone, so it is tabulated per axis
+// (map, built by patch_map).
+constexpr int kPatchLen = 72;  // 2b + 8 for the largest block side, 32
+constexpr int kMapInvalid = 1 << 12, kMapFlip = 1 << 13, kMapDirichlet = 1 << 14;  // low bits: position or side
+struct FinePatch {
+    int f0[3], sz[3], n3[3];
+    int l;
+    bool cached;  // bids/bb hold the box's block ids (box_blocks)
+    const int* bids;
+    const int* map;  // [3][kPatchLen]
+    BoxBlocks bb;
+};
+
+// map[d][p] for every box position p along every axis; every thread of the
+// workgroup calls it, a barrier must follow before the map is read
+__device__ void patch_map(const AmrDev& a, const FinePatch& P, int* map) {
+    for (int i = threadIdx.x; i < 3 * kPatchLen; i += blockDim.x) {
+        const int d = i / kPatchLen, p = i - d * kPatchLen;
+        const int g = P.f0[d] + p;
+        int e = p;
+        if (a.act[d] && p < P.sz[d] && (g < 0 || g >= P.n3[d])) {
+            const int side = g >= P.n3[d] ? 1 : 0;
+            const int kind = a.bc_kind[d][side];
+            // box_bc's source position: the boundary cell (outflow) or the
+            // mirror cell (reflect), which lies inside the domain because a
+            // patch spans -4 .. n3 + 3 and n3 >= 16
+            int src = 0;
+            bool moved = false;
+            if (kind == BC_DIRICHLET) {
+                e = kMapDirichlet | side;
+            } else if (kind == BC_OUTFLOW) {
+                src = side == 0 ? 0 : P.n3[d] - 1;
+                moved = true;
+            } else if (kind == BC_REFLECT) {
+                src = side == 0 ? -1 - g : 2 * P.n3[d] - 1 - g;
+                moved = true;
+            } else if (kind != BC_PERIODIC) {
+                e = kMapInvalid;  // characteristic boxes take the general path
+            }
+            if (moved) {
+                const int ps = src - P.f0[d];
+                e = ps >= 0 && ps < P.sz[d] ? ps | (kind == BC_REFLECT ? kMapFlip : 0) : kMapInvalid;
+            }
+        }
+        map[i] = e;
+    }
+}
+
+// block id (-1: absent) and cell index of a box cell with no outside axis
+__device__ __forceinline__ int patch_block(const AmrDev& a, const FinePatch& P, const int c[3], int* cell) {
+    int g[3] = {P.f0[0] + c[0], P.f0[1] + c[1], P.f0[2] + c[2]};
+    if (P.cached) return P.bids[box_block_of(a, P.bb, g, cell)];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        if (a.act[d]) g[d] = rem_e(g[d], P.n3[d]);  // out of range only across a periodic boundary
+    return amr_find(a, P.l, g, cell);
+}
+
+// source cell s of box cell c; returns -1 (s is read), -2 (invalid) or the
+// Dirichlet boundary 2 d + side; neg collects the negated axes
+__device__ __forceinline__ int patch_source(const FinePatch& P, const int c[3], int s[3], unsigned& neg) {
+    int r = -1;
+    neg = 0;
+#pragma unroll
+    for (int d = 2; d >= 0; --d) {
+        const int e = P.map[d * kPatchLen + c[d]];
+        s[d] = e & 0xfff;
+        if (r == -1) {
+            if (e & kMapDirichlet)
+                r = 2 * d + (e & 1);
+            else if (e & kMapInvalid)
+                r = -2;
+            else if (e & kMapFlip)
+                neg |= 1u << d;
+        }
+    }
+    return r;
+}
+
+// validity of box cell c (needs P.cached)
+__device__ __forceinline__ bool patch_ok(const AmrDev& a, const FinePatch& P, const int c[3]) {
+    int s[3];
+    unsigned neg;
+    const int r = patch_source(P, c, s, neg);
+    if (r != -1) return r >= 0;
+    int cell;
+    const int g[3] = {P.f0[0] + s[0], P.f0[1] + s[1], P.f0[2] + s[2]};
+    return P.bids[box_block_of(a, P.bb, g, &cell)] >= 0;
+}
+
+// box cell c as a load (needs P.cached): returns -1 (block cell, U index base + q * vs),
+// -2 (invalid) or the Dirichlet boundary 2 d + side; neg as patch_source
+__device__ __forceinline__ int patch_resolve(const AmrDev& a, const FinePatch& P, const int c[3], long long& base,
+                                             unsigned& neg) {
+    int s[3];
+    int r = patch_source(P, c, s, neg);
+    base = 0;
+    if (r == -1) {
+        int cell;
+        const int g[3] = {P.f0[0] + s[0], P.f0[1] + s[1], P.f0[2] + s[2]};
+        const int id = P.bids[box_block_of(a, P.bb, g, &cell)];
+        if (id >= 0)
+            base = (long long)id * a.bcells + cell;
+        else
+            r = -2;
+    }
+    return r;
+}
+
+// value of a resolved box cell (zero where invalid)
+__device__ __forceinline__ void patch_load(const AmrDev& a, int r, long long base, unsigned neg, double v[5]) {
+    if (r == -1) {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) v[q] = a.U[q * a.vs + base];
+    } else if (r >= 0) {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) v[q] = a.bc_state[r >> 1][r & 1][q];
+    } else {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) v[q] = 0.0;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        if ((neg >> d) & 1) v[1 + d] *= -1.0;
+}
+
+// First restriction pass, along axis D, reading the fine patch in place:
+// output cell c of the (patch with b cells along D) box takes the first of
+// the shifted 6-point stencils whose cells are all valid, else the average
+// of its two children, else zero and invalid.
+template <int D>
+__device__ __forceinline__ void restrict_first_pass(const AmrDev& a, const AmrWeights& W, const FinePatch& P,
+                                                    const int nsz[3], double* out, double* ook, long long cap) {
+    const int shifts[5] = {0, -1, 1, -2, 2};
+    const int ntot = nsz[0] * nsz[1] * nsz[2];
+    for (int i = threadIdx.x; i < ntot; i += blockDim.x) {
+        int c[3];
+        unflat(i, nsz, c);
+        int s3[3] = {c[0], c[1], c[2]};
+        const int base = 2 * c[D] + 2;
+        unsigned okw = 0;  // bit j: patch cell base - 2 + j is valid
+#pragma unroll 1
+        for (int j = 0; j < 10; ++j) {
+            s3[D] = base - 2 + j;
+            if (s3[D] >= 0 && s3[D] < P.sz[D] && patch_ok(a, P, s3)) okw |= 1u << j;
+        }
+        bool done = false;
+        double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+        for (int m = 0; m < 5 && !done; ++m) {
+            const int st = base + shifts[m];
+            if (st < 0 || st + 5 >= P.sz[D]) continue;
+            if (((okw >> (st - base + 2)) & 0x3fu) != 0x3fu) continue;
+            // resolve the six cells, then issue their loads together
+            long long bs[6];
+            int rr[6];
+            unsigned ng[6];
+#pragma unroll
+            for (int kk = 0; kk < 6; ++kk) {
+                s3[D] = st + kk;
+                rr[kk] = patch_resolve(a, P, s3, bs[kk], ng[kk]);
+            }
+            double v[6][5];
+#pragma unroll
+            for (int kk = 0; kk < 6; ++kk) patch_load(a, rr[kk], bs[kk], ng[kk], v[kk]);
+#pragma unroll
+            for (int kk = 0; kk < 6; ++kk)
+#pragma unroll
+                for (int q = 0; q < 5; ++q) acc[q] += W.wts[m][kk] * v[kk][q];
+            done = true;
+        }
+        double okv = done ? 1.0 : 0.0;
+        if (!done && ((okw >> 4) & 3u) == 3u) {
+            double va[5], vb[5];
+            long long ba, bb;
+            unsigned na, nb;
+            s3[D] = base + 2;
+            const int ra = patch_resolve(a, P, s3, ba, na);
+            s3[D] = base + 3;
+            const int rb = patch_resolve(a, P, s3, bb, nb);
+            patch_load(a, ra, ba, na, va);
+            patch_load(a, rb, bb, nb, vb);
+#pragma unroll
+            for (int q = 0; q < 5; ++q) acc[q] = 0.5 * (va[q] + vb[q]);
+            okv = 1.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 5; ++q) out[q * cap + i] = acc[q];
+        ook[i] = okv;
+    }
+}
+
 // restrict_box of amr.rs:622-730 for a whole ancestor block (level l) by
-// one workgroup: the (2b+8)^D fine patch with its validity mask (level l+1
-// blocks, periodic wrap, bc_box), then one pass per axis with the centred
+// one workgroup: over the (2b+8)^D fine patch with its validity mask (level
+// l+1 blocks, periodic wrap, bc_box), one pass per axis with the centred
 // 6-point stencil, shifted where the fine data ends, the child average as a
 // last resort.  Cells left invalid are written as zero, as the reference does.
-// Only cells selected by `mask` are stored (see k_amr_restrict); a coarse
-// cell's stencils stay inside its own box's fine patch, so its value is the
-// same whichever box the reference restricts it in.
-__device__ void restrict_block(const AmrDev& a, const AmrWeights& W, int l, int slot, const int idx[3], int mask,
-                               int w, double* scr, long long cap) {
+// The first pass reads the patch in place (FinePatch); a patch reaching past
+// a characteristic boundary is gathered into scratch and filled by box_bc
+// first.  Later passes read the previous pass from scratch.  Only cells
+// selected by `mask` are stored (see k_amr_restrict); a coarse cell's
+// stencils stay inside its own box's fine patch, so its value is the same
+// whichever box the reference restricts it in.
+__device__ __forceinline__ void restrict_block(const AmrDev& a, const AmrWeights& W, int l, int slot,
+                                               const int idx[3], int mask, int w, double* scr, long long cap) {
     double* A = scr;
     double* Bf = scr + 5 * cap;
     double* okA = scr + 10 * cap;
     double* okB = scr + 11 * cap;
-    __shared__ int any_out;
-    int f0[3], sz[3];
-    const int lf = l + 1;
-    const int nf[3] = {amr_n_level(a, lf, 0), amr_n_level(a, lf, 1), amr_n_level(a, lf, 2)};
+    int sz[3];
+    FinePatch P;
+    P.l = l + 1;
+    bool general = false;
+    int d0 = 3;  // first active axis
     for (int d = 0; d < 3; ++d) {
-        f0[d] = a.act[d] ? 2 * idx[d] * a.bd[d] - 4 : 0;
-        sz[d] = a.act[d] ? 2 * a.bd[d] + 8 : 1;
+        P.f0[d] = a.act[d] ? 2 * idx[d] * a.bd[d] - 4 : 0;
+        P.sz[d] = sz[d] = a.act[d] ? 2 * a.bd[d] + 8 : 1;
+        P.n3[d] = amr_n_level(a, P.l, d);
+        if (a.act[d]) {
+            d0 = d < d0 ? d : d0;
+            general = general || P.sz[d] > kPatchLen ||
+                      (P.f0[d] < 0 && a.bc_kind[d][0] == BC_CHARACTERISTIC) ||
+                      (P.f0[d] + P.sz[d] > P.n3[d] && a.bc_kind[d][1] == BC_CHARACTERISTIC);
+        }
     }
-    if (threadIdx.x == 0) any_out = 0;
     __shared__ int bids[kBoxBlocks];
-    BoxBlocks bb;
-    const bool cached = box_blocks(a, lf, f0, sz, bids, bb);  // synchronises
-    const int tot = sz[0] * sz[1] * sz[2];
-    for (int i = threadIdx.x; i < tot && cached; i += blockDim.x) {
-        int c[3];
-        unflat(i, sz, c);
-        const int g[3] = {f0[0] + c[0], f0[1] + c[1], f0[2] + c[2]};
-        int cell;
-        const int id = bids[box_block_of(a, bb, g, &cell)];
-        double okv = 0.0;
-        if (id == -2) {
-            any_out = 1;
-            for (int q = 0; q < 5; ++q) A[q * cap + i] = 0.0;
-        } else if (id >= 0) {
-            for (int q = 0; q < 5; ++q) A[q * cap + i] = a.U[amr_u(a, q, id, cell)];
-            okv = 1.0;
-        } else {
-            for (int q = 0; q < 5; ++q) A[q * cap + i] = 0.0;
-        }
-        okA[i] = okv;
-    }
-    for (int i = threadIdx.x; i < tot && !cached; i += blockDim.x) {
-        int c[3];
-        unflat(i, sz, c);
-        int g[3] = {f0[0] + c[0], f0[1] + c[1], f0[2] + c[2]};
-        bool outside = false;
-        for (int d = 0; d < 3; ++d)
-            if (a.act[d] && (g[d] < 0 || g[d] >= nf[d])) {
-                if (a.bc_kind[d][g[d] >= nf[d] ? 1 : 0] == BC_PERIODIC)
-                    g[d] = rem_e(g[d], nf[d]);
-                else
-                    outside = true;
-            }
-        double okv = 0.0;
-        if (outside) {
-            any_out = 1;
-            for (int q = 0; q < 5; ++q) A[q * cap + i] = 0.0;
-        } else {
-            int cell;
-            const int id = amr_find(a, lf, g, &cell);
-            if (id >= 0) {
-                for (int q = 0; q < 5; ++q) A[q * cap + i] = a.U[amr_u(a, q, id, cell)];
-                okv = 1.0;
-            } else {
-                for (int q = 0; q < 5; ++q) A[q * cap + i] = 0.0;
-            }
-        }
-        okA[i] = okv;
-    }
-    __syncthreads();
-    if (any_out) box_bc(a, lf, f0, sz, A, okA, cap);
+    __shared__ int pmap[3 * kPatchLen];
+    P.map = pmap;
+    if (!general) patch_map(a, P, pmap);
+    P.cached = box_blocks(a, P.l, P.f0, P.sz, bids, P.bb);  // synchronises
+    P.bids = bids;
+    general = general || !P.cached;  // the in-place reads use the block-id cache
     const int shifts[5] = {0, -1, 1, -2, 2};
     double* in = A;
     double* out = Bf;
     double* iok = okA;
     double* ook = okB;
-    for (int d = 0; d < 3; ++d) {
+    int dstart = d0;
+    if (general) {
+        const int tot = sz[0] * sz[1] * sz[2];
+        for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+            int c[3];
+            unflat(i, sz, c);
+            double v[5];
+            bool ok = false;
+            bool inside = true;
+            for (int d = 0; d < 3; ++d) {
+                const int g = P.f0[d] + c[d];
+                inside = inside && !(a.act[d] && (g < 0 || g >= P.n3[d]) &&
+                                     a.bc_kind[d][g >= P.n3[d] ? 1 : 0] != BC_PERIODIC);
+            }
+            if (inside) {
+                int cell;
+                const int id = patch_block(a, P, c, &cell);
+                ok = id >= 0;
+#pragma unroll
+                for (int q = 0; q < 5; ++q) v[q] = ok ? a.U[amr_u(a, q, id, cell)] : 0.0;
+            } else {
+#pragma unroll
+                for (int q = 0; q < 5; ++q) v[q] = 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < 5; ++q) A[q * cap + i] = v[q];
+            okA[i] = ok ? 1.0 : 0.0;
+        }
+        __syncthreads();
+        box_bc(a, P.l, P.f0, sz, A, okA, cap);
+    } else if (d0 < 3) {
+        int nsz[3] = {sz[0], sz[1], sz[2]};
+        nsz[d0] = a.bd[d0];
+        if (d0 == 0)
+            restrict_first_pass<0>(a, W, P, nsz, Bf, okB, cap);
+        else if (d0 == 1)
+            restrict_first_pass<1>(a, W, P, nsz, Bf, okB, cap);
+        else
+            restrict_first_pass<2>(a, W, P, nsz, Bf, okB, cap);
+        __syncthreads();
+        in = Bf;
+        out = A;
+        iok = okB;
+        ook = okA;
+        for (int e = 0; e < 3; ++e) sz[e] = nsz[e];
+        dstart = d0 + 1;
+    }
+    for (int d = dstart; d < 3; ++d) {
         if (!a.act[d]) continue;
         int nsz[3] = {sz[0], sz[1], sz[2]};
         nsz[d] = a.bd[d];
