@@ -232,6 +232,38 @@ def test_shock_bubble_3d(em, mode):
         assert _rel(g.download(), o.get_u()) < 1e-12
 
 
+def test_restriction_images_over_three_axes(em):
+    # The shock starts next to the outflow x-low face and refines a layer of
+    # blocks across the whole y-z section, so the fine patches of edge
+    # ancestors reach past the x, y and z boundaries at once: their corner
+    # cells are images of images (outflow, reflect, a Dirichlet state; the
+    # momentum sign changes of several axes combined).  Residuals, the
+    # restricted ancestors and states after steps stay bit-identical.
+    print("amr restriction images over three axes, 3-D shock-bubble", flush=True)
+    wall = (1.0, 0.0, 0.0, 0.0, 2.5)
+    bc = [["outflow", ("dirichlet", wall)], ["reflect", ("dirichlet", wall)], ["reflect", "outflow"]]
+    prm = [2.0, 0.06, 0.6, 0.25, 0.25, 0.15, 0.1]
+    o = OracleAmr("shock_bubble", prm, (4, 2, 2), 8, 3, (0, 0, 0), (1, 0.5, 0.5), eps=0.2, bc=bc, mode="weno",
+                  regrid_every=2)
+    lv, idx, leaf = o.blocks()
+    anc = (lv == 1) & ~leaf
+    corner = anc & (idx[:, 0] == 0) & np.isin(idx[:, 1], (0, 3)) & np.isin(idx[:, 2], (0, 3))
+    assert corner.sum() >= 1 and ((lv == 1) & leaf).sum() >= 1
+    g = _from_oracle(em, o, bc=bc, mode="weno", eps=0.2, regrid_every=2)
+    rg, sg = g.rhs(1e-3)
+    ro, so = o.compute_rhs(1e-3)
+    assert np.array_equal(rg, ro)
+    assert sg == so
+    assert np.array_equal(g.download(), o.get_u())
+    for _ in range(3):
+        dt = o.dt_cfl()
+        assert g.dt_cfl() == dt
+        o.step(dt)
+        g.step(dt)
+    _same_tree(g, o)
+    assert np.array_equal(g.download(), o.get_u())
+
+
 def test_characteristic_and_dirichlet_boundaries(em):
     print("amr characteristic / dirichlet boundaries", flush=True)
     far = (1.0, 0.1, 0.0, 0.0, 2.5)
