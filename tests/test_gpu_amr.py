@@ -118,6 +118,70 @@ def test_steps_with_regrids_2d(em):
     assert g.counters() == o.stats()[1:]
 
 
+@pytest.mark.parametrize("b,base,nsteps", [(16, (4, 2, 0), 56), (32, (2, 1, 0), 12)])
+def test_larger_blocks_2d(em, b, base, nsteps):
+    # 16^2 and 32^2 blocks: their own flux-pass shapes, index shifts, ghost
+    # slabs and restriction patches (2b + 8 cells).  With 16^2 blocks the
+    # hierarchy changes after about 41 steps.
+    print(f"amr 2-D shock-bubble weno, {b}^2 blocks, {nsteps} steps", flush=True)
+    o = OracleAmr("shock_bubble", [3.0, 0.3, 0.6, 0.25, 0.0, 0.12, 0.1], base, b, 3, (0, 0, 0), (1, 0.5, 1),
+                  eps=1e-2, bc=SB2D_BC, mode="weno", regrid_every=2)
+    lv, _, leaf = o.blocks()
+    assert len(set(lv[leaf])) >= 2
+    g = _from_oracle(em, o, bc=SB2D_BC, mode="weno", eps=1e-2, regrid_every=2)
+    rg, sg = g.rhs(1e-3)
+    ro, so = o.compute_rhs(1e-3)
+    assert np.array_equal(rg, ro)
+    assert sg == so
+    assert np.array_equal(g.download(), o.get_u())
+    # rhs() adds its faces to the device counters, compute_rhs not to the
+    # oracle's: compare what the steps add
+    c0, s0 = g.counters(), o.stats()[1:]
+    sizes = set()
+    for k in range(nsteps):
+        dt = o.dt_cfl()
+        assert g.dt_cfl() == dt, k
+        o.step(dt)
+        g.step(dt)
+        _same_tree(g, o)
+        sizes.add(o.nblocks)
+    assert np.array_equal(g.download(), o.get_u())
+    assert [x - y for x, y in zip(g.counters(), c0)] == [x - y for x, y in zip(o.stats()[1:], s0)]
+    if b == 16:
+        assert len(sizes) >= 2, "the hierarchy should have changed"
+
+
+def test_16cubed_blocks_3d(em):
+    # 3-D shock-bubble on 16^3 blocks with leaves on two levels: residuals and
+    # restricted ancestors for both flux-pass shapes of this block size, then
+    # steps, bit-identical; per-leaf launches give the same bits
+    print("amr 3-D shock-bubble weno, 16^3 blocks", flush=True)
+    bc = [["outflow", "outflow"], ["reflect", "reflect"], ["reflect", "reflect"]]
+    prm = [2.0, 0.3, 0.6, 0.25, 0.25, 0.15, 0.1]
+    o = OracleAmr("shock_bubble", prm, (4, 2, 2), 16, 2, (0, 0, 0), (1, 0.5, 0.5), eps=0.5, bc=bc, mode="weno",
+                  regrid_every=2)
+    lv, _, leaf = o.blocks()
+    assert len(set(lv[leaf])) == 2
+    ro, so = o.compute_rhs(1e-3)
+    for shape in range(2):
+        g = _from_oracle(em, o, bc=bc, mode="weno", eps=0.5, regrid_every=2)
+        g.set_shapes([shape] * 3)
+        rg, sg = g.rhs(1e-3)
+        assert np.array_equal(rg, ro), shape
+        assert sg == so, shape
+        assert np.array_equal(g.download(), o.get_u()), shape
+    per = _from_oracle(em, o, bc=bc, mode="weno", eps=0.5, regrid_every=2, batched=False)
+    for _ in range(3):
+        dt = o.dt_cfl()
+        assert g.dt_cfl() == dt
+        o.step(dt)
+        g.step(dt)
+        per.step(dt)
+    _same_tree(g, o)
+    assert np.array_equal(g.download(), o.get_u())
+    assert np.array_equal(per.download(), o.get_u())
+
+
 def test_details_and_regrid(em):
     print("amr details + regrid 2-D riemann", flush=True)
     o = _riemann("weno")
