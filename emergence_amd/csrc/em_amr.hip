@@ -491,6 +491,9 @@ struct Amr {
     int* d_leaf_level = nullptr;
     GhostTask* d_ghost = nullptr;
     std::vector<GhostTask> ghost_copy, ghost_pred;  // the batched launch's split of `ghost` by kind
+    int n_bnd_copy = 0;  // ghost_copy holds its boundary images (kind 1) first, then the same-level copies
+    std::vector<int> gsrc;  // same-level source per (block, face) for the batched passes (AmrPassArgs::gsrc)
+    int* d_gsrc = nullptr;
     GhostTask* d_ghost_copy = nullptr;
     GhostTask* d_ghost_pred = nullptr;
     std::vector<int*> d_edge_ids, d_edge_idx;
@@ -641,6 +644,7 @@ struct Amr {
         f(d_ghost);
         f(d_ghost_copy);
         f(d_ghost_pred);
+        f(d_gsrc);
         for (auto p : d_edge_ids) f(p);
         for (auto p : d_edge_idx) f(p);
         for (auto& r : d_reflux)
@@ -1414,9 +1418,17 @@ struct Amr {
         // synchronised above, so it is free)
         ghost_copy.clear();
         ghost_pred.clear();
-        for (const GhostTask& t : ghost) (t.kind == 2 ? ghost_pred : ghost_copy).push_back(t);
+        gsrc.assign(blocks.size() * 6, -1);
+        for (const GhostTask& t : ghost) {
+            if (t.kind == 1) ghost_copy.push_back(t);
+            if (t.kind == 2) ghost_pred.push_back(t);
+            if (t.kind == 0) gsrc[(size_t)t.slot * 6 + 2 * t.axis + t.side] = t.src;
+        }
+        n_bnd_copy = (int)ghost_copy.size();
+        for (const GhostTask& t : ghost)
+            if (t.kind == 0) ghost_copy.push_back(t);
         size_t total = staged_bytes(leaves) + staged_bytes(leaf_level) + staged_bytes(ghost) +
-                       staged_bytes(ghost_copy) + staged_bytes(ghost_pred);
+                       staged_bytes(ghost_copy) + staged_bytes(ghost_pred) + staged_bytes(gsrc);
         for (int l = 0; l < levels; ++l) total += staged_bytes(edge[l]) + staged_bytes(edge_ent[l]);
         for (int d = 0; d < 3; ++d)
             for (int s = 0; s < 2; ++s) total += staged_bytes(reflux[d][s]);
@@ -1427,6 +1439,7 @@ struct Amr {
         d_ghost = dev_vec_staged(ghost, d_ghost);
         d_ghost_copy = dev_vec_staged(ghost_copy, d_ghost_copy);
         d_ghost_pred = dev_vec_staged(ghost_pred, d_ghost_pred);
+        d_gsrc = dev_vec_staged(gsrc, d_gsrc);
         d_edge_ids.resize(levels, nullptr);
         d_edge_idx.resize(levels, nullptr);
         for (int l = 0; l < levels; ++l) {
@@ -1512,11 +1525,14 @@ struct Amr {
         }
     }
 
-    void fill_ghosts() {
+    // same_level false: the batched passes read same-level neighbours through
+    // gsrc, so their slab copies are left out (the detail pass, which reads
+    // every slab, fills them all)
+    void fill_ghosts(bool same_level = true) {
         const int n = (int)ghost.size();
         if (!n) return;
         if (batched) {
-            const int nc = (int)ghost_copy.size(), np = (int)ghost_pred.size();
+            const int nc = same_level ? (int)ghost_copy.size() : n_bnd_copy, np = (int)ghost_pred.size();
             if (nc) {
                 const int per = ng * (bcells / B);
                 const long long ntot = (long long)nc * per;
@@ -1569,6 +1585,7 @@ struct Amr {
             pa.tol = tol;
             pa.stage_dt_host = stage_dt_host;
             pa.stage_dt = stage_dt;
+            pa.gsrc = batched ? d_gsrc : nullptr;
             const int nl = (int)leaves.size();
             if (batched) {
                 pa.leaf0 = 0;
@@ -1603,7 +1620,7 @@ struct Amr {
     // compute_rhs (amr.rs:1302-1323): restriction, ghosts, fluxes, reflux
     void compute_rhs(int stage_dt_host, double stage_dt) {
         restrict_all();
-        fill_ghosts();
+        fill_ghosts(false);
         passes(stage_dt_host, stage_dt);
         refluxes();
     }
@@ -1672,7 +1689,7 @@ struct Amr {
         phase(3, [&] { rk(0); });
         for (int s = 1; s <= 3; ++s) {
             phase(0, [&] { restrict_all(); });
-            phase(1, [&] { fill_ghosts(); });
+            phase(1, [&] { fill_ghosts(false); });
             phase(2, [&] { passes(0, 0.0); });
             phase(3, [&] {
                 refluxes();

@@ -2,7 +2,8 @@
 //
 // A work item is (leaf, group of P pencils along the pass axis D); the whole
 // pencil of B cells plus the 5 ghost layers on each side is staged in LDS
-// (interior cells from the pool, ghosts from the leaf's face slabs), the face
+// (interior cells from the pool; ghosts from a same-level neighbour's cells
+// where there is one, else from the leaf's face slabs), the face
 // fluxes come from tile_faces (the uniform pass's arithmetic), and the
 // residual is accumulated as in block_rhs (amr.rs:1224-1299):
 //   rhs -= (F_{i+1/2} - F_{i-1/2}) / dx_level
@@ -52,7 +53,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_WAVES_PER
         const double dx = a.dx[lvl][D];
         const int t0 = gp * P;
         const int Pv = min(P, nt - t0);
-        // ---- load the pencils (interior from the pool, ghosts from the slabs)
+        // ---- load the pencils (interior and same-level ghosts from the pool, others from the slabs)
         for (int it = tid; it < C * P; it += NT) {
             int c, p;
             if (pa.dfast) {
@@ -76,9 +77,21 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_WAVES_PER
                 for (int q = 0; q < 5; ++q) u[q] = a.U[amr_u(a, q, slot, cell)];
             } else {
                 const int s = 2 * D + (x < 0 ? 0 : 1);
-                const int layer = x < 0 ? x + ngh : x - B;
+                const int src = pa.gsrc ? pa.gsrc[slot * 6 + s] : -1;
+                if (src >= 0) {
+                    // a same-level neighbour's cells, as its slab copy holds them
+                    int cc[3];
+                    cc[D] = x < 0 ? x + B : x - B;
+                    cc[E0] = t / be1;
+                    cc[E1] = t % be1;
+                    const long long cell = ((long long)cc[0] * a.bd[1] + cc[1]) * a.bd[2] + cc[2];
 #pragma unroll
-                for (int q = 0; q < 5; ++q) u[q] = a.G[amr_g(a, q, slot, s, layer, t)];
+                    for (int q = 0; q < 5; ++q) u[q] = a.U[amr_u(a, q, src, cell)];
+                } else {
+                    const int layer = x < 0 ? x + ngh : x - B;
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) u[q] = a.G[amr_g(a, q, slot, s, layer, t)];
+                }
             }
             to_prim(e, u, wv);
 #pragma unroll

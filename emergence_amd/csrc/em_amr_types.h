@@ -74,6 +74,10 @@ struct AmrPassArgs {
     double tol;
     int stage_dt_host;
     double stage_dt;
+    // same-level source block of each (block, face) ghost slab, indexed
+    // block * 6 + 2 * axis + side, -1 where the slab itself is filled; null:
+    // every ghost is read from the slabs
+    const int* gsrc;
 };
 
 // Shapes of the batched pass with a compiled kernel (launch_amr_b): P pencils
