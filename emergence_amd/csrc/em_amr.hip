@@ -1784,15 +1784,27 @@ struct Amr {
 
     void graph_step() {
         if (!gexec || graph_version != version) {
-            if (gexec) {
-                HCHK(hipGraphExecDestroy(gexec));
-                gexec = nullptr;
-            }
             hipGraph_t g;
             HCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
             device_step();
             HCHK(hipStreamEndCapture(stream, &g));
-            const hipError_t e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+            // a new hierarchy usually records the same launches with other
+            // arguments: update the executable graph in place (instantiating
+            // costs milliseconds), else instantiate anew
+            hipError_t e = hipSuccess;
+            bool updated = false;
+            if (gexec) {
+                HCHK(hipStreamSynchronize(stream));  // no launch of it in flight
+                hipGraphNode_t bad = nullptr;
+                hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+                updated = hipGraphExecUpdate(gexec, g, &bad, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+                if (!updated) {
+                    (void)hipGetLastError();
+                    (void)hipGraphExecDestroy(gexec);
+                    gexec = nullptr;
+                }
+            }
+            if (!updated) e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
             HCHK(e);
             graph_version = version;
@@ -1891,8 +1903,13 @@ struct Amr {
     void regrid() {  // amr.rs:1039-1046
         std::vector<int8_t> f = flags_from_details(details());
         std::set<Key> prot = regrid_refine(f);
-        tables();
-        f = flags_from_details(details());
+        // the coarsening reads the details of the refined hierarchy; with no
+        // refinement (tables still clean) tree and data are unchanged, and so
+        // are the details and flags just computed
+        if (tables_dirty) {
+            tables();
+            f = flags_from_details(details());
+        }
         regrid_coarsen(f, prot);
         tables();
     }

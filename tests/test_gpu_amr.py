@@ -118,6 +118,25 @@ def test_steps_with_regrids_2d(em):
     assert g.counters() == o.stats()[1:]
 
 
+def test_graph_replay_across_regrids(em):
+    # run_steps replays one captured step per hierarchy; when a regrid changes
+    # the hierarchy the executable graph is updated in place (or instantiated
+    # anew), and the replayed steps keep the oracle's bits
+    print("amr graph replay across regrids, 2-D shock-bubble", flush=True)
+    o = OracleAmr("shock_bubble", [3.0, 0.3, 0.6, 0.25, 0.0, 0.12, 0.1], (8, 4, 0), 8, 3, (0, 0, 0), (1, 0.5, 1),
+                  eps=1e-2, bc=SB2D_BC, mode="weno", regrid_every=2)
+    g = _from_oracle(em, o, bc=SB2D_BC, mode="weno", eps=1e-2, regrid_every=2)
+    sizes = set()
+    for _ in range(56):
+        o.step(o.dt_cfl())
+        sizes.add(o.nblocks)
+    g.run_steps(56)
+    assert len(sizes) >= 2, "the hierarchy should have changed"
+    assert g.t == o.t
+    _same_tree(g, o)
+    assert np.array_equal(g.download(), o.get_u())
+
+
 @pytest.mark.parametrize("b,base,nsteps", [(16, (4, 2, 0), 56), (32, (2, 1, 0), 12)])
 def test_larger_blocks_2d(em, b, base, nsteps):
     # 16^2 and 32^2 blocks: their own flux-pass shapes, index shifts, ghost
