@@ -311,14 +311,17 @@ __global__ __launch_bounds__(kGhostCopyThreads) void k_amr_ghost_copy(AmrDev a, 
 // edge box towards neighbour offset o (cells within w of that side), bit 27
 // the whole block (restrict_all's boxes, amr.rs:760-800).  Values do not
 // depend on the box they are computed in, so the whole block is computed and
-// only the selected cells are stored.
+// only the selected cells are stored.  xoff/xs (optional, coarsening
+// replay): ancestor i treats the blocks xs[xoff[i] .. xoff[i+1]) as absent.
 __global__ __launch_bounds__(kPatchThreads) void k_amr_restrict(AmrDev a, AmrWeights W, const int* __restrict__ ids,
                                                      const int* __restrict__ ent, int n, int level, int w,
-                                                     double* scratch, long long cap) {
+                                                     double* scratch, long long cap, const int* __restrict__ xoff,
+                                                     const int* __restrict__ xs) {
     double* scr = scratch + (long long)blockIdx.x * 12 * cap;
     for (int i = blockIdx.x; i < n; i += gridDim.x) {
         const int bi[3] = {ent[4 * i], ent[4 * i + 1], ent[4 * i + 2]};
-        restrict_block(a, W, level, ids[i], bi, ent[4 * i + 3], w, scr, cap);
+        const int nx = xoff ? xoff[i + 1] - xoff[i] : 0;
+        restrict_block(a, W, level, ids[i], bi, ent[4 * i + 3], w, scr, cap, nx ? xs + xoff[i] : nullptr, nx);
     }
 }
 
@@ -1027,6 +1030,7 @@ struct Amr {
         int pid0;                       // the parent's pool slot when the coarsening phase began
         int batch;
         std::vector<long long> removed;  // index entries of its children
+        std::vector<int> kids0;          // its children's pool slots when the phase began
     };
     std::vector<CoOp> co_ops;
     std::vector<int> orig_slot;  // phase-start pool slot of the data that belongs at slot s
@@ -1044,6 +1048,7 @@ struct Amr {
         for (auto& c : kids) ids.push_back(get(l + 1, c.data()));
         std::sort(ids.begin(), ids.end(), [](int a, int b) { return a > b; });
         for (int i : ids) {
+            op.kids0.push_back(orig_slot[i]);
             op.removed.push_back(map_off[blocks[i].level] + slot(blocks[i].level, blocks[i].idx));
             map_host_set(blocks[i].level, blocks[i].idx, -1);
             const int last = (int)blocks.size() - 1;
@@ -1088,11 +1093,15 @@ struct Amr {
     // l+1 through the index and writes its parent; it sees the same data as
     // in the reference's sequential loop when every earlier coarsening it
     // depends on (co_depends) has been applied and no later one it would see
-    // has.  Batch b holds the coarsenings whose dependencies lie in batches
-    // < b; its restrictions run in one launch per level against the pool as
-    // it was when the phase began, with the children of batches < b removed
-    // from the index.  The swap-remove moves are then applied at once as one
-    // block permutation, and the index entries of moved blocks rewritten.
+    // has.  A coarsening one level finer writes a parent the restriction
+    // reads: levels run finest first, one launch each.  A coarsening on the
+    // same level only removes children from the index: each restriction
+    // masks the children of the earlier ones it depends on (their pool slots
+    // when the phase began) out of its patch, so a level's restrictions run
+    // together against the index and pool as they were when the phase began.
+    // Then the removals are applied to the index, the swap-remove moves at
+    // once as one block permutation, and the index entries of moved blocks
+    // rewritten.
     double* d_co_tmp = nullptr;
     long long d_co_tmp_n = 0;
     int* d_co_int = nullptr;
@@ -1100,36 +1109,39 @@ struct Amr {
     void replay_coarsening() {
         const int n = (int)co_ops.size();
         if (n == 0) return;
-        int nb = 0;
-        for (int k = 0; k < n; ++k) {
-            int b = 0;
-            for (int j = 0; j < k; ++j)
-                if (co_ops[j].batch + 1 > b && co_depends(co_ops[k], co_ops[j])) b = co_ops[j].batch + 1;
-            co_ops[k].batch = b;
-            nb = std::max(nb, b + 1);
-        }
-        // int tables: per (batch, level) restriction lists (ids, then 4 ints each)
+        // int tables per level: parent slots, 4 ints each (index, whole-block
+        // mask), exclusion offsets (n + 1) and the excluded slots
         struct Launch {
-            int level, ids, ent, n;
+            int level, ids, ent, xoff, xs, n;
         };
         std::vector<int> ib;
-        std::vector<std::vector<Launch>> rl(nb);
-        for (int b = 0; b < nb; ++b)
-            for (int l = levels - 1; l >= 0; --l) {
-                std::vector<int> ids, ent;
-                for (const CoOp& op : co_ops)
-                    if (op.batch == b && op.l == l) {
-                        ids.push_back(op.pid0);
-                        for (int d = 0; d < 3; ++d) ent.push_back((int)op.idx[d]);
-                        ent.push_back(1 << 27);  // the whole block
-                    }
-                if (ids.empty()) continue;
-                Launch L{l, (int)ib.size(), 0, (int)ids.size()};
-                ib.insert(ib.end(), ids.begin(), ids.end());
-                L.ent = (int)ib.size();
-                ib.insert(ib.end(), ent.begin(), ent.end());
-                rl[b].push_back(L);
+        std::vector<Launch> rl;
+        for (int l = levels - 1; l >= 0; --l) {
+            std::vector<int> ids, ent, xoff, xs;
+            for (int k = 0; k < n; ++k) {
+                const CoOp& op = co_ops[k];
+                if (op.l != l) continue;
+                ids.push_back(op.pid0);
+                for (int d = 0; d < 3; ++d) ent.push_back((int)op.idx[d]);
+                ent.push_back(1 << 27);  // the whole block
+                xoff.push_back((int)xs.size());
+                for (int j = 0; j < k; ++j)
+                    if (co_ops[j].l == l && co_depends(op, co_ops[j]))
+                        xs.insert(xs.end(), co_ops[j].kids0.begin(), co_ops[j].kids0.end());
             }
+            if (ids.empty()) continue;
+            xoff.push_back((int)xs.size());
+            Launch L{l, 0, 0, 0, 0, (int)ids.size()};
+            L.ids = (int)ib.size();
+            ib.insert(ib.end(), ids.begin(), ids.end());
+            L.ent = (int)ib.size();
+            ib.insert(ib.end(), ent.begin(), ent.end());
+            L.xoff = (int)ib.size();
+            ib.insert(ib.end(), xoff.begin(), xoff.end());
+            L.xs = (int)ib.size();
+            ib.insert(ib.end(), xs.begin(), xs.end());
+            rl.push_back(L);
+        }
         // block permutation of the moves
         std::vector<int> src, dst;
         for (size_t sl = 0; sl < orig_slot.size(); ++sl)
@@ -1141,26 +1153,21 @@ struct Amr {
         ib.insert(ib.end(), src.begin(), src.end());
         const int pdst = (int)ib.size();
         ib.insert(ib.end(), dst.begin(), dst.end());
-        // index edits: the removals of each batch, then the moved blocks'
-        // entries (all entries distinct), uploaded once
+        // index edits: every removal, then the moved blocks' entries (all
+        // entries distinct), uploaded once
         std::vector<long long> eo;
         std::vector<int> ev;
-        std::vector<int> eb(nb + 2, 0);  // batch b's removals: eo[eb[b] .. eb[b+1]); final: eb[nb] .. eb[nb+1]
-        for (int b = 0; b < nb; ++b) {
-            eb[b] = (int)eo.size();
-            for (const CoOp& op : co_ops)
-                if (op.batch == b)
-                    for (long long e : op.removed) {
-                        eo.push_back(e);
-                        ev.push_back(-1);
-                    }
-        }
-        eb[nb] = (int)eo.size();
+        for (const CoOp& op : co_ops)
+            for (long long e : op.removed) {
+                eo.push_back(e);
+                ev.push_back(-1);
+            }
+        const int nrem = (int)eo.size();
         for (int sl : dst) {
             eo.push_back(map_off[blocks[sl].level] + slot(blocks[sl].level, blocks[sl].idx));
             ev.push_back(sl);
         }
-        eb[nb + 1] = (int)eo.size();
+        const int nall = (int)eo.size();
         apply_edits();  // earlier edits (refinement) first; synchronises
         d_co_int = dev_vec(ib, d_co_int);
         d_eoff = dev_vec(eo, d_eoff);
@@ -1177,16 +1184,14 @@ struct Amr {
             hipLaunchKernelGGL(k_amr_map_set, dim3(g), dim3(256), 0, stream, d_map, d_eoff + lo, d_eval + lo, hi - lo);
             HCHK(hipGetLastError());
         };
-        for (int b = 0; b < nb; ++b) {
-            if (b > 0) edits(eb[b - 1], eb[b]);
-            for (const Launch& L : rl[b]) {
-                const unsigned g = (unsigned)std::min(L.n, big_wg);
-                hipLaunchKernelGGL(k_amr_restrict, dim3(g), dim3(kPatchThreads), 0, stream, dev, W, d_co_int + L.ids,
-                                   d_co_int + L.ent, L.n, L.level, box_w(), big_scr, big_cap);
-                HCHK(hipGetLastError());
-            }
+        for (const Launch& L : rl) {
+            const unsigned g = (unsigned)std::min(L.n, big_wg);
+            hipLaunchKernelGGL(k_amr_restrict, dim3(g), dim3(kPatchThreads), 0, stream, dev, W, d_co_int + L.ids,
+                               d_co_int + L.ent, L.n, L.level, box_w(), big_scr, big_cap, d_co_int + L.xoff,
+                               d_co_int + L.xs);
+            HCHK(hipGetLastError());
         }
-        edits(eb[nb - 1], eb[nb]);  // the last batch's removals
+        edits(0, nrem);
         if (!src.empty()) {
             const int np = (int)src.size();
             const unsigned g = (unsigned)std::min<long long>(((long long)np * bcells + 255) / 256, 4096);
@@ -1195,7 +1200,7 @@ struct Amr {
             hipLaunchKernelGGL(k_amr_blocks_in, dim3(g), dim3(256), 0, stream, dev, d_co_int + pdst, np, d_co_tmp);
             HCHK(hipGetLastError());
         }
-        edits(eb[nb], eb[nb + 1]);
+        edits(nrem, nall);
         HCHK(hipStreamSynchronize(stream));  // ib, eo, ev are copied from pageable memory
         co_ops.clear();
     }
@@ -1515,11 +1520,11 @@ struct Amr {
             if (batched) {
                 const unsigned g = (unsigned)std::min(n, big_wg);
                 hipLaunchKernelGGL(k_amr_restrict, dim3(g), dim3(kPatchThreads), 0, stream, dev, W, d_edge_ids[l], d_edge_idx[l],
-                                   n, l, box_w(), big_scr, big_cap);
+                                   n, l, box_w(), big_scr, big_cap, nullptr, nullptr);
             } else {
                 for (int i = 0; i < n; ++i)
                     hipLaunchKernelGGL(k_amr_restrict, dim3(1), dim3(kPatchThreads), 0, stream, dev, W, d_edge_ids[l] + i,
-                                       d_edge_idx[l] + 4 * i, 1, l, box_w(), big_scr, big_cap);
+                                       d_edge_idx[l] + 4 * i, 1, l, box_w(), big_scr, big_cap, nullptr, nullptr);
             }
             HCHK(hipGetLastError());
         }

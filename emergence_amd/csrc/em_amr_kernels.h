@@ -325,6 +325,8 @@ struct FinePatch {
     bool cached;  // bids/bb hold the box's block ids (box_blocks)
     const int* bids;
     const int* map;  // [3][kPatchLen]
+    const int* xs;   // blocks treated as absent (nx of them)
+    int nx;
     BoxBlocks bb;
 };
 
@@ -370,7 +372,10 @@ __device__ __forceinline__ int patch_block(const AmrDev& a, const FinePatch& P, 
 #pragma unroll
     for (int d = 0; d < 3; ++d)
         if (a.act[d]) g[d] = rem_e(g[d], P.n3[d]);  // out of range only across a periodic boundary
-    return amr_find(a, P.l, g, cell);
+    int id = amr_find(a, P.l, g, cell);
+    for (int k = 0; k < P.nx; ++k)
+        if (id == P.xs[k]) id = -1;
+    return id;
 }
 
 // source cell s of box cell c; returns -1 (s is read), -2 (invalid) or the
@@ -517,9 +522,11 @@ __device__ __forceinline__ void restrict_first_pass(const AmrDev& a, const AmrWe
 // first.  Later passes read the previous pass from scratch.  Only cells
 // selected by `mask` are stored (see k_amr_restrict); a coarse cell's
 // stencils stay inside its own box's fine patch, so its value is the same
-// whichever box the reference restricts it in.
+// whichever box the reference restricts it in.  The nx blocks xs are treated
+// as absent from the index (a coarsening that removed them came first).
 __device__ __forceinline__ void restrict_block(const AmrDev& a, const AmrWeights& W, int l, int slot,
-                                               const int idx[3], int mask, int w, double* scr, long long cap) {
+                                               const int idx[3], int mask, int w, double* scr, long long cap,
+                                               const int* xs, int nx) {
     double* A = scr;
     double* Bf = scr + 5 * cap;
     double* okA = scr + 10 * cap;
@@ -543,9 +550,18 @@ __device__ __forceinline__ void restrict_block(const AmrDev& a, const AmrWeights
     __shared__ int bids[kBoxBlocks];
     __shared__ int pmap[3 * kPatchLen];
     P.map = pmap;
+    P.xs = xs;
+    P.nx = nx;
     if (!general) patch_map(a, P, pmap);
     P.cached = box_blocks(a, P.l, P.f0, P.sz, bids, P.bb);  // synchronises
     P.bids = bids;
+    if (P.cached && nx > 0) {
+        const int nbid = P.bb.nbk[0] * P.bb.nbk[1] * P.bb.nbk[2];
+        for (int i = threadIdx.x; i < nbid; i += blockDim.x)
+            for (int k = 0; k < nx; ++k)
+                if (bids[i] == xs[k]) bids[i] = -1;
+        __syncthreads();
+    }
     general = general || !P.cached;  // the in-place reads use the block-id cache
     const int shifts[5] = {0, -1, 1, -2, 2};
     double* in = A;

@@ -118,6 +118,34 @@ def test_steps_with_regrids_2d(em):
     assert g.counters() == o.stats()[1:]
 
 
+def test_mass_coarsening_regrids(em):
+    # Flattened data (deviations from the mean scaled by 1e-7) coarsen the 28
+    # level-1 parents in one regrid (70 neighbouring pairs) and the 12 level-0
+    # ones in the next: the restriction of each coarsening sees the earlier
+    # coarsenings of its neighbours (their children gone from the index), as
+    # in the reference's sequential loop.  Trees and data stay bit-identical.
+    print("amr mass coarsening, 2-D riemann flattened", flush=True)
+    o = _riemann("weno")
+    _, _, leaf = o.blocks()
+    u = o.get_u()
+    ref = u[leaf].reshape(-1, 5).mean(0)
+    o.set_u(ref + 1e-7 * (u - ref))
+    g = _from_oracle(em, o, bc=RIEMANN_BC, mode="weno")
+    for r in range(2):
+        n0 = o.nblocks
+        o.regrid()
+        g.regrid()
+        assert o.nblocks < n0, r
+        _same_tree(g, o)
+        assert np.array_equal(g.download(), o.get_u()), r
+    for _ in range(2):
+        dt = o.dt_cfl()
+        assert g.dt_cfl() == dt
+        o.step(dt)
+        g.step(dt)
+    assert np.array_equal(g.download(), o.get_u())
+
+
 def test_graph_replay_across_regrids(em):
     # run_steps replays one captured step per hierarchy; when a regrid changes
     # the hierarchy the executable graph is updated in place (or instantiated
