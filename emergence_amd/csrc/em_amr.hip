@@ -365,6 +365,30 @@ __global__ void k_amr_reflux(AmrDev a, const RefluxTask* __restrict__ t, int n, 
     }
 }
 
+// Every reflux correction of the batched step in one launch: the tasks of
+// all (axis, side) lists grouped by the coarse cell they correct, each group
+// in the (axis, side) order of k_amr_reflux's launches, so a cell's residual
+// takes its corrections in the same order and every cell is one thread's.
+__global__ void k_amr_reflux_cells(AmrDev a, const RefluxTask* __restrict__ t, const int* __restrict__ grp, int ng) {
+    for (int gi = blockIdx.x * 128 + threadIdx.x; gi < ng; gi += gridDim.x * 128) {
+        for (int i = grp[gi]; i < grp[gi + 1]; ++i) {
+            const RefluxTask k = t[i];
+            const int d = k.axis, side = k.side;
+            const double dx = a.dx[k.level][d];
+            const double cnt = (double)k.nfine;
+            for (int q = 0; q < 5; ++q) {
+                double sum = 0.0;
+                for (int f = 0; f < k.nfine; ++f) sum += a.F[amr_f(a, q, k.fslot[f], 2 * d + (1 - side), k.fface[f])];
+                const double fc = a.F[amr_f(a, q, k.slot, 2 * d + side, k.face)];
+                const double df = sum / cnt - fc;
+                const double dr = side == 1 ? -df / dx : df / dx;
+                const long long ix = amr_u(a, q, k.slot, k.cell);
+                a.R[ix] += dr;
+            }
+        }
+    }
+}
+
 // Largest relative wavelet detail of each leaf (block_detail, amr.rs:823-867):
 // |s - sum_m wr[m] s(+-1, +-3, +-5)| of the sensor values ln rho,
 // ln(p + p_inf) along every active axis; infinite if not finite.
@@ -501,6 +525,10 @@ struct Amr {
     GhostTask* d_ghost_pred = nullptr;
     std::vector<int*> d_edge_ids, d_edge_idx;
     RefluxTask* d_reflux[3][2] = {};
+    std::vector<RefluxTask> reflux_cells;  // all reflux tasks grouped by coarse cell (k_amr_reflux_cells)
+    std::vector<int> reflux_grp;           // group starts, and the end
+    RefluxTask* d_reflux_cells = nullptr;
+    int* d_reflux_grp = nullptr;
     int* d_tmp = nullptr;
     long long d_tmp_n = 0;
     double* d_det = nullptr;
@@ -650,6 +678,8 @@ struct Amr {
         f(d_gsrc);
         for (auto p : d_edge_ids) f(p);
         for (auto p : d_edge_idx) f(p);
+        f(d_reflux_cells);
+        f(d_reflux_grp);
         for (auto& r : d_reflux)
             for (auto p : r) f(p);
         f(d_tmp);
@@ -1406,6 +1436,8 @@ struct Amr {
                                     t.nfine++;
                                 }
                             if (t.nfine == 0) continue;
+                            t.axis = d;
+                            t.side = side;
                             reflux[d][side].push_back(t);
                         }
                 }
@@ -1432,8 +1464,21 @@ struct Amr {
         n_bnd_copy = (int)ghost_copy.size();
         for (const GhostTask& t : ghost)
             if (t.kind == 0) ghost_copy.push_back(t);
+        // reflux tasks by coarse cell, each cell's in (axis, side) order
+        reflux_cells.clear();
+        for (int d = 0; d < 3; ++d)
+            for (int sd = 0; sd < 2; ++sd) reflux_cells.insert(reflux_cells.end(), reflux[d][sd].begin(), reflux[d][sd].end());
+        std::stable_sort(reflux_cells.begin(), reflux_cells.end(), [](const RefluxTask& x, const RefluxTask& y) {
+            return x.slot != y.slot ? x.slot < y.slot : x.cell < y.cell;
+        });
+        reflux_grp.clear();
+        for (size_t i = 0; i < reflux_cells.size(); ++i)
+            if (i == 0 || reflux_cells[i].slot != reflux_cells[i - 1].slot || reflux_cells[i].cell != reflux_cells[i - 1].cell)
+                reflux_grp.push_back((int)i);
+        reflux_grp.push_back((int)reflux_cells.size());
         size_t total = staged_bytes(leaves) + staged_bytes(leaf_level) + staged_bytes(ghost) +
-                       staged_bytes(ghost_copy) + staged_bytes(ghost_pred) + staged_bytes(gsrc);
+                       staged_bytes(ghost_copy) + staged_bytes(ghost_pred) + staged_bytes(gsrc) +
+                       staged_bytes(reflux_cells) + staged_bytes(reflux_grp);
         for (int l = 0; l < levels; ++l) total += staged_bytes(edge[l]) + staged_bytes(edge_ent[l]);
         for (int d = 0; d < 3; ++d)
             for (int s = 0; s < 2; ++s) total += staged_bytes(reflux[d][s]);
@@ -1453,6 +1498,8 @@ struct Amr {
         }
         for (int d = 0; d < 3; ++d)
             for (int s = 0; s < 2; ++s) d_reflux[d][s] = dev_vec_staged(reflux[d][s], d_reflux[d][s]);
+        d_reflux_cells = dev_vec_staged(reflux_cells, d_reflux_cells);
+        d_reflux_grp = dev_vec_staged(reflux_grp, d_reflux_grp);
         HCHK(hipStreamSynchronize(stream));
         tables_dirty = false;
         ++version;
@@ -1610,6 +1657,15 @@ struct Amr {
     }
 
     void refluxes() {
+        if (batched) {
+            const int ng = (int)reflux_grp.size() - 1;
+            if (ng > 0) {
+                hipLaunchKernelGGL(k_amr_reflux_cells, dim3((unsigned)std::min((ng + 127) / 128, 4096)), dim3(128), 0,
+                                   stream, dev, d_reflux_cells, d_reflux_grp, ng);
+                HCHK(hipGetLastError());
+            }
+            return;
+        }
         for (int d = 0; d < 3; ++d) {
             if (!act[d]) continue;
             for (int s = 0; s < 2; ++s) {

@@ -60,6 +60,7 @@ struct GhostTask {
 struct RefluxTask {
     int slot, cell, face, nfine, level;
     int fslot[4], fface[4];
+    int axis, side;
 };
 
 struct AmrPassArgs {
