@@ -261,7 +261,9 @@ int em_amr_set_variant(em_amr* h, int32_t precise);
 int em_amr_set_batched(em_amr* h, int32_t batched);
 
 /* 1 (default): batched em_amr_run_steps replays each step as one captured
- * HIP graph (captured again after every hierarchy edit); 0: eager launches. */
+ * HIP graph (captured again after every hierarchy edit, and the executable
+ * graph updated in place when the launches keep their topology); 0: eager
+ * launches. */
 int em_amr_set_graph(em_amr* h, int32_t on);
 
 /* Pass shape per axis (pencils per work item and threads of the batched
