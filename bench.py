@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--amr-only", action="store_true", help="only the block-AMR workload (profiling)")
     ap.add_argument("--amr-steps", type=int, default=20)
     ap.add_argument("--amr-per-block-steps", type=int, default=3)
+    ap.add_argument("--amr-no-graph", action="store_true",
+                    help="batched AMR steps launched eagerly, not replayed as graphs (for counter collection)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload")
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint workload (profiling)")
     ap.add_argument("--adj-n", type=int, default=512)
@@ -337,19 +339,23 @@ def amr_bench(em, args):
     nblocks, nleaves = root.counts()
     del root
     bat = em.HipAmr(c["base"], c["b"], c["levels"], c["lo"], c["hi"], blocks=hier, **kw)
+    if args.amr_no_graph:
+        bat.set_graph(False)
     shapes = bat.autotune(3)
-    per = em.HipAmr(c["base"], c["b"], c["levels"], c["lo"], c["hi"], blocks=hier, batched=False, **kw)
-    per.set_shapes(shapes)
     k = args.amr_per_block_steps
-    bat.run_steps(k)
-    per.synchronize()
-    t0 = time.perf_counter()
-    per.run_steps(k)
-    per.synchronize()
-    el_per = time.perf_counter() - t0
-    same = bool(np.array_equal(bat.download(), per.download()))
-    cells_per = per.leaf_count() * c["b"] ** 3
-    del per
+    same, el_per, cells_per = None, None, None
+    if k > 0:  # 0: no per-block comparison (counter runs)
+        per = em.HipAmr(c["base"], c["b"], c["levels"], c["lo"], c["hi"], blocks=hier, batched=False, **kw)
+        per.set_shapes(shapes)
+        bat.run_steps(k)
+        per.synchronize()
+        t0 = time.perf_counter()
+        per.run_steps(k)
+        per.synchronize()
+        el_per = time.perf_counter() - t0
+        same = bool(np.array_equal(bat.download(), per.download()))
+        cells_per = per.leaf_count() * c["b"] ** 3
+        del per
     bat.synchronize()
     leaves0 = bat.leaf_count()
     t0 = time.perf_counter()
@@ -362,7 +368,7 @@ def amr_bench(em, args):
     bat.run_steps(16)
     phase = [v / 16 for v in bat.profile()]
     value = cells * args.amr_steps * 3 / el
-    value_per = cells_per * k * 3 / el_per
+    value_per = cells_per * k * 3 / el_per if k > 0 else None
     cpu = None if args.no_cpu_baseline else amr_cpu_baseline(nleaves * c["b"] ** 3)
     return {
         "workload": "3-D shock-bubble block AMR, 8^3 blocks, 3 levels, Hybrid, CFL 0.4, regrid every 4",
@@ -372,9 +378,9 @@ def amr_bench(em, args):
         "unit": "cell-updates/s per RK stage",
         "ms_per_step": el / args.amr_steps * 1e3,
         "steps": args.amr_steps,
-        "per_block": {"value": value_per, "ms_per_step": el_per / k * 1e3, "steps": k},
+        "per_block": {"value": value_per, "ms_per_step": el_per / k * 1e3, "steps": k} if k > 0 else None,
         "pass_shapes": shapes,
-        "speedup_vs_per_block": value / value_per,
+        "speedup_vs_per_block": value / value_per if k > 0 else None,
         "identical_to_per_block": same,
         "phase_ms_per_step": dict(zip(["restrict", "ghosts", "fluxes", "reflux_update", "regrid"], phase)),
         "cpu_baseline": cpu,
