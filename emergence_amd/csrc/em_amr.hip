@@ -443,6 +443,90 @@ __global__ __launch_bounds__(256) void k_amr_details(AmrDev a, AmrWeights W, con
     }
 }
 
+// k_amr_details for blocks of at most 512 cells with face slabs of at most
+// 64 cells: every sensor value a leaf's stencils read (its cells, and the
+// five slab layers nearest each face) is computed once into LDS, instead of
+// once per stencil that reads it; the values and the order of the sums are
+// those of k_amr_details.
+constexpr int kDetCells = 512, kDetFace = 64;
+__global__ __launch_bounds__(256) void k_amr_details_lds(AmrDev a, AmrWeights W, const int* __restrict__ leaves, int n,
+                                                        double* __restrict__ det) {
+    __shared__ double red[4];
+    __shared__ double si[2][kDetCells];
+    __shared__ double sg[2][6][5][kDetFace];
+    auto sensor = [&](const double u[5], double s[2]) {
+        double w[5];
+        to_prim(a.eos, u, w);
+        s[0] = log(fmax(w[0], 1e-300));
+        s[1] = log(fmax(w[4] + a.eos.p_inf, 1e-300));
+    };
+    for (int li = blockIdx.x; li < n; li += gridDim.x) {
+        const long long slot = leaves[li];
+        for (int c = threadIdx.x; c < a.bcells; c += blockDim.x) {
+            double u[5], sv[2];
+            for (int q = 0; q < 5; ++q) u[q] = a.U[amr_u(a, q, slot, c)];
+            sensor(u, sv);
+            si[0][c] = sv[0];
+            si[1][c] = sv[1];
+        }
+        // slab entry (face f = 2 d + side, layer k = 0..4 from the face outwards, t)
+        for (int e = threadIdx.x; e < 6 * 5 * kDetFace; e += blockDim.x) {
+            const int f = e / (5 * kDetFace), k = (e / kDetFace) % 5, t = e % kDetFace;
+            const int d = f >> 1, side = f & 1;
+            if (!a.act[d] || t >= a.bcells / a.bd[d]) continue;
+            const int layer = side == 0 ? a.ng - 1 - k : k;
+            double u[5], sv[2];
+            for (int q = 0; q < 5; ++q) u[q] = a.G[amr_g(a, q, slot, f, layer, t)];
+            sensor(u, sv);
+            sg[0][f][k][t] = sv[0];
+            sg[1][f][k][t] = sv[1];
+        }
+        __syncthreads();
+        double mx = 0.0;
+        for (int c = threadIdx.x; c < a.bcells; c += blockDim.x) {
+            int ci[3];
+            unflat(c, a.bd, ci);
+            auto sens = [&](int d, int x, double s[2]) {
+                if (x >= 0 && x < a.bd[d]) {
+                    int c3[3] = {ci[0], ci[1], ci[2]};
+                    c3[d] = x;
+                    const int cell = (c3[0] * a.bd[1] + c3[1]) * a.bd[2] + c3[2];
+                    s[0] = si[0][cell];
+                    s[1] = si[1][cell];
+                } else {
+                    const int E0 = d == 0 ? 1 : 0, E1 = d == 2 ? 1 : 2;
+                    const int t = ci[E0] * a.bd[E1] + ci[E1];
+                    const int f = 2 * d + (x < 0 ? 0 : 1);
+                    const int k = x < 0 ? -1 - x : x - a.bd[d];
+                    s[0] = sg[0][f][k][t];
+                    s[1] = sg[1][f][k][t];
+                }
+            };
+            for (int d = 0; d < 3; ++d) {
+                if (!a.act[d]) continue;
+                double s0[2];
+                sens(d, ci[d], s0);
+                double pred[2] = {0.0, 0.0};
+                const int offs[6] = {-5, -3, -1, 1, 3, 5};
+                for (int m = 0; m < 6; ++m) {
+                    double sv[2];
+                    sens(d, ci[d] + offs[m], sv);
+                    for (int q = 0; q < 2; ++q) pred[q] += W.wr[m] * sv[q];
+                }
+                for (int q = 0; q < 2; ++q) {
+                    const double dd = fabs(s0[q] - pred[q]);
+                    mx = isfinite(dd) ? fmax(mx, dd) : INFINITY;
+                }
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+        __syncthreads();
+        if (threadIdx.x == 0) det[li] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------------ host
 
 static void lagrange6_h(double x0, double x, double out[6]) {  // amr.rs:41-47
@@ -1949,8 +2033,12 @@ struct Amr {
         }
         std::vector<double> det(blocks.size(), 0.0);
         if (n) {
-            hipLaunchKernelGGL(k_amr_details, dim3((unsigned)std::min(n, 2048)), dim3(256), 0, stream, dev, W, d_leaves,
-                               n, d_det);
+            if (bcells <= kDetCells && tmax <= kDetFace)
+                hipLaunchKernelGGL(k_amr_details_lds, dim3((unsigned)std::min(n, 2048)), dim3(256), 0, stream, dev, W,
+                                   d_leaves, n, d_det);
+            else
+                hipLaunchKernelGGL(k_amr_details, dim3((unsigned)std::min(n, 2048)), dim3(256), 0, stream, dev, W, d_leaves,
+                                   n, d_det);
             HCHK(hipGetLastError());
             std::vector<double> h(n);
             HCHK(hipMemcpyAsync(h.data(), d_det, sizeof(double) * n, hipMemcpyDeviceToHost, stream));
