@@ -5,7 +5,12 @@
  * which a host binds for the Euler system the way the reference's CPU
  * `Solver` (fluid/crates/compressible/src/solver.rs:139-795) is driven:
  * plain pointers and sizes, caller-owned host buffers, library-owned device
- * buffers, one handle = one HIP stream, not thread-safe per handle.
+ * buffers, not thread-safe per handle.  A handle owns its HIP streams, all at
+ * default priority: one main stream for stepping (and everything a call
+ * without a stream note does), plus a second stream on which a solver's
+ * backward sweep recomputes checkpoint segments, and four sweep streams in an
+ * ensemble's backward.  Every call returns with its streams' work ordered
+ * before the handle's next call.
  *
  * Host state arrays use the reference's CPU layout: ghost-padded AoS float64,
  * cell index (i*m1 + j)*m2 + k with m_d = n_d + 2*ng on active axes (n_d > 1)

@@ -3,10 +3,15 @@
 // index (i*b1 + j)*b2 + k (amr.rs:278-281).
 #include <cmath>
 #include <cstring>
+#include <map>
 
 #include "emref_amr.h"
 
 using namespace emref;
+
+// Blocks protected by the last regrid_refine of a handle, kept until its
+// next regrid_coarsen (or destroy).
+static std::map<void*, std::set<AmrKey>> g_protected;
 
 namespace {
 
@@ -107,7 +112,10 @@ void* emref_amr_create(int case_id, const double* params, const int64_t base[3],
     return a;
 }
 
-void emref_amr_destroy(void* h) { delete (Amr*)h; }
+void emref_amr_destroy(void* h) {
+    g_protected.erase(h);
+    delete (Amr*)h;
+}
 int64_t emref_amr_nblocks(void* h) { return (int64_t)((Amr*)h)->blocks.size(); }
 int64_t emref_amr_bcells(void* h) { return (int64_t)((Amr*)h)->bcells(); }
 int64_t emref_amr_ng(void* h) { return (int64_t)((Amr*)h)->ng; }
@@ -186,4 +194,37 @@ void emref_amr_stats(void* h, uint64_t out[4]) {
     out[3] = st.limited_faces;
 }
 
+// Regrid rules with caller-given flags (amr.rs:877-892, 1051-1106,
+// 1545-1548), for tests that pin them directly.
+
+void emref_amr_flags_from_details(void* h, const double* det, int8_t* flags) {
+    Amr* a = (Amr*)h;
+    std::vector<double> d(det, det + a->blocks.size());
+    std::vector<int8_t> f = a->flags_from_details(d);
+    std::memcpy(flags, f.data(), f.size());
+}
+
+void emref_amr_regrid_refine(void* h, const int8_t* flags) {
+    Amr* a = (Amr*)h;
+    std::vector<int8_t> f(flags, flags + a->blocks.size());
+    g_protected[h] = a->regrid_refine(f);
+}
+
+void emref_amr_regrid_coarsen(void* h, const int8_t* flags) {
+    Amr* a = (Amr*)h;
+    std::vector<int8_t> f(flags, flags + a->blocks.size());
+    a->regrid_coarsen(f, g_protected[h]);
+    g_protected.erase(h);
+}
+
+// schedule_regrid(changed); returns steps, regrid interval, next regrid step
+void emref_amr_schedule_regrid(void* h, int changed, int64_t out[3]) {
+    Amr* a = (Amr*)h;
+    if (changed >= 0) a->schedule_regrid(changed != 0);
+    out[0] = (int64_t)a->steps;
+    out[1] = (int64_t)a->regrid_interval;
+    out[2] = (int64_t)a->next_regrid;
+}
+
 }  // extern "C"
+

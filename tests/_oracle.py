@@ -179,6 +179,11 @@ def _load_amr(lib):
     lib.emref_amr_details.argtypes = [P, D]
     lib.emref_amr_totals.argtypes = [P, D]
     lib.emref_amr_stats.argtypes = [P, U]
+    I8 = C.POINTER(C.c_int8)
+    lib.emref_amr_flags_from_details.argtypes = [P, D, I8]
+    lib.emref_amr_regrid_refine.argtypes = [P, I8]
+    lib.emref_amr_regrid_coarsen.argtypes = [P, I8]
+    lib.emref_amr_schedule_regrid.argtypes = [P, C.c_int, I64]
     lib._amr_ready = True
     return lib
 
@@ -290,6 +295,28 @@ class OracleAmr:
         st = (C.c_uint64 * 4)()
         self.lib.emref_amr_stats(self.h, st)
         return list(st)
+
+    def flags_from_details(self, det):
+        det = np.ascontiguousarray(det, dtype=np.float64)
+        out = np.empty(self.nblocks, np.int8)
+        self.lib.emref_amr_flags_from_details(self.h, _dp(det), out.ctypes.data_as(C.POINTER(C.c_int8)))
+        return out
+
+    def regrid_refine(self, flags):
+        f = np.ascontiguousarray(flags, dtype=np.int8)
+        assert f.shape == (self.nblocks,)
+        self.lib.emref_amr_regrid_refine(self.h, f.ctypes.data_as(C.POINTER(C.c_int8)))
+
+    def regrid_coarsen(self, flags):
+        f = np.ascontiguousarray(flags, dtype=np.int8)
+        assert f.shape == (self.nblocks,)
+        self.lib.emref_amr_regrid_coarsen(self.h, f.ctypes.data_as(C.POINTER(C.c_int8)))
+
+    def schedule_regrid(self, changed=None):
+        """schedule_regrid(changed) (None: only read); -> (steps, interval, next)."""
+        out = (C.c_int64 * 3)()
+        self.lib.emref_amr_schedule_regrid(self.h, -1 if changed is None else int(bool(changed)), out)
+        return tuple(out)
 
     def dx(self, level):
         return tuple((h - l) / (nb * self.b * 2 ** level) if nb > 0 else (h - l)

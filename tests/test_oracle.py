@@ -288,3 +288,142 @@ def test_hybrid_sensor_limits():
     r4, _ = s4.rhs(1e-3)
     assert st3[1] == st3[0]
     assert np.array_equal(r3, r4)
+
+
+# ---- Hybrid blend weight, brute force on tiny lines (scheme.rs:94-99, 160-183, 268-276)
+
+def _line_fluxes(w, mode, tol, eos=(1.4, 0.0, 0.0, 0.0), ng=6, d=0):
+    import ctypes as C
+
+    lib = _oracle.load()
+    lib.emref_line_fluxes.argtypes = [C.POINTER(C.c_double)] * 2 + [C.c_int64, C.c_int, C.c_int64, C.c_int, C.c_int,
+                                                                      C.c_double, C.c_double, C.POINTER(C.c_double),
+                                                                      C.POINTER(C.c_uint64)]
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    n = len(w) - 2 * ng
+    out = np.empty((n + 1, 5))
+    st = (C.c_uint64 * 4)()
+    e = np.array(eos, dtype=np.float64)
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    lib.emref_line_fluxes(dp(e), dp(w), len(w), d, ng, _oracle.MODE[mode], 0, tol, 0.0, dp(out), st)
+    return out, list(st)
+
+
+def _theta_by_hand(w, tol, p_inf=0.0, ng=6):
+    """theta of every face, restated independently: sensor (ln rho, ln(p+p_inf))
+    floored at 1e-300 with fmax semantics (NaN ignored), the two 5-point fourth
+    differences over cells i-2..i+2 and i-1..i+3 of face i+1/2, max over both
+    sensors skipping NaN, smoothstep of (r/tol - 1/2)/(1/2), 1 when r is not finite."""
+    def fmax(a, b):
+        return b if math.isnan(a) else a if math.isnan(b) else max(a, b)
+
+    sens = [(math.log(fmax(row[0], 1e-300)), math.log(fmax(row[4] + p_inf, 1e-300))) for row in w]
+    n = len(w) - 2 * ng
+    th, rs = [], []
+    for f in range(n + 1):
+        i = ng - 1 + f
+        r = 0.0
+        for k in range(2):
+            s = [sens[j][k] for j in range(i - 2, i + 4)]
+            a = abs(s[0] - 4.0 * s[1] + 6.0 * s[2] - 4.0 * s[3] + s[4])
+            b = abs(s[1] - 4.0 * s[2] + 6.0 * s[3] - 4.0 * s[4] + s[5])
+            r = fmax(fmax(r, a), b)
+        rs.append(r)
+        if not math.isfinite(r):
+            th.append(1.0)
+            continue
+        x = min(max((r / tol - 0.5) / 0.5, 0.0), 1.0)
+        th.append(x * x * (3.0 - 2.0 * x))
+    return th, rs
+
+
+def _blend_expected(w, tol, eos=(1.4, 0.0, 0.0, 0.0)):
+    th, rs = _theta_by_hand(w, tol, eos[1])
+    fw, _ = _line_fluxes(w, "weno", tol, eos)
+    fc, _ = _line_fluxes(w, "central", tol, eos)
+    exp = np.empty_like(fw)
+    for f, t in enumerate(th):
+        if t == 0.0:
+            exp[f] = fc[f]
+        elif t == 1.0:
+            exp[f] = fw[f]
+        else:
+            exp[f] = [t * fw[f][k] + (1.0 - t) * fc[f][k] for k in range(5)]
+    return exp, th, rs
+
+
+def _tiny_line(seed, n=7, ng=6, jump=0.0):
+    rng = np.random.default_rng(seed)
+    m = n + 2 * ng
+    x = np.arange(m, dtype=np.float64)
+    rho = 1.0 + 0.05 * np.sin(0.7 * x + rng.uniform(0, 6)) + jump * (x >= m // 2 + rng.integers(-2, 3))
+    p = 1.0 + 0.04 * np.cos(0.9 * x + rng.uniform(0, 6)) + 0.01 * rng.standard_normal(m)
+    w = np.stack([rho, 0.3 + 0.01 * rng.standard_normal(m), 0.1 * np.ones(m), -0.05 * np.ones(m), p], axis=1)
+    return w
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+@pytest.mark.parametrize("eos", [(1.4, 0.0, 0.0, 0.0), (1.4, 0.5, 0.0, 0.0)], ids=["ideal", "stiffened"])
+def test_hybrid_theta_matches_brute_force_at_intermediate_ratios(seed, eos):
+    w = _tiny_line(seed, jump=0.3 * seed)
+    _, rs = _theta_by_hand(w, 1.0, eos[1])
+    f_star = int(np.argmax(rs))
+    r_star = rs[f_star]
+    seen = set()
+    # r/tol = 1/2 (theta 0), 3/4 (theta 1/2), 1 (theta 1) at the face with the
+    # largest r, plus tolerances that put several faces strictly inside (0, 1)
+    for tol in (2.0 * r_star, r_star / 0.75, r_star, float(np.median(rs)) / 0.7, r_star / 0.6):
+        exp, th, _ = _blend_expected(w, tol, eos)
+        got, st = _line_fluxes(w, "hybrid", tol, eos)
+        assert np.array_equal(got, exp), (tol, th)
+        assert st[0] == len(th) and st[1] == sum(1 for t in th if t > 0.0)
+        seen.update("zero" if t == 0.0 else "one" if t == 1.0 else "mid" for t in th)
+    assert {"zero", "one", "mid"} <= seen
+    # the three exact points of the smoothstep on the face with the largest r
+    assert _theta_by_hand(w, 2.0 * r_star, eos[1])[0][f_star] == 0.0
+    assert _theta_by_hand(w, r_star / 0.75, eos[1])[0][f_star] == 0.5
+    assert _theta_by_hand(w, r_star, eos[1])[0][f_star] == 1.0
+
+
+def test_hybrid_theta_window_is_asymmetric_as_the_reference():
+    # a sensor bump seen by cells i-2..i+3 of face i+1/2: faces whose window
+    # reaches the bump switch, the next face over does not.  A window shifted
+    # by one cell, or only one of the two fourth differences, fails here.
+    ng, n = 6, 9
+    w = np.tile([1.0, 0.2, 0.0, 0.0, 1.0], (n + 2 * ng, 1))
+    bump = ng + 4
+    w[bump, 0] = 1.5
+    th, _ = _theta_by_hand(w, 1e-3)
+    i_of = lambda f: ng - 1 + f
+    for f, t in enumerate(th):
+        inside = i_of(f) - 2 <= bump <= i_of(f) + 3
+        assert (t == 1.0) == inside, (f, t)
+    exp, _, _ = _blend_expected(w, 1e-3)
+    got, st = _line_fluxes(w, "hybrid", 1e-3)
+    assert np.array_equal(got, exp)
+    assert st[1] == 6
+
+
+def test_hybrid_theta_non_finite_sensor_gives_weno():
+    # an infinite density makes r infinite on the faces whose windows hold it:
+    # theta = 1 there (the shock path), whatever tol is.  A NaN density is
+    # floored to 1e-300 by the fmax (NaN-ignoring) sensor, so r stays finite.
+    ng = 6
+    w = _tiny_line(7)
+    w[ng + 3, 0] = np.inf
+    th, rs = _theta_by_hand(w, 1e6)
+    assert any(not math.isfinite(r) for r in rs)
+    assert all(t == 1.0 for t, r in zip(th, rs) if not math.isfinite(r))
+    got, st = _line_fluxes(w, "hybrid", 1e6)
+    fw, _ = _line_fluxes(w, "weno", 1e6)
+    for f, r in enumerate(rs):
+        if not math.isfinite(r):
+            assert np.array_equal(got[f], fw[f], equal_nan=True)
+    assert st[1] == sum(1 for t in th if t > 0.0)
+    w2 = _tiny_line(7)
+    w2[ng + 3, 0] = np.nan
+    _, rs2 = _theta_by_hand(w2, 1.0)
+    assert all(math.isfinite(r) for r in rs2)
+    exp2, th2, _ = _blend_expected(w2, max(rs2) / 0.8)
+    got2, _ = _line_fluxes(w2, "hybrid", max(rs2) / 0.8)
+    assert np.array_equal(got2, exp2, equal_nan=True)

@@ -7,6 +7,8 @@ and the reference's own validation facts determine:
 * 6th-order prediction and restriction are exact on quintic polynomials;
 * refluxing conserves mass to round-off across refinement levels.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -169,3 +171,235 @@ def test_amr_tracks_uniform_finest_grid():
                 fine[gi:gi + r, gj:gj + r] = ua[i, ii, jj, 0, 0]
     ref = s.interior(s.get_u())[:, :, 0, 0]
     assert np.abs(fine - ref).mean() < 1e-3
+
+
+# ---- regrid rules (amr.rs:818-892, 952-1106, 1545-1548), pinned by closed
+# forms on the wavelet detail and by invariants of the refined/coarsened tree
+
+_WR = {-5: 3.0 / 256.0, -3: -25.0 / 256.0, -1: 150.0 / 256.0, 1: 150.0 / 256.0, 3: -25.0 / 256.0, 5: 3.0 / 256.0}
+
+
+def _uniform_single_level(n_blocks=4, b=8, bc=None):
+    return OracleAmr("smooth", None, (n_blocks, n_blocks, 0), b, 1, (0, 0, 0), (1, 1, 1), eps=1e-3, bc=bc)
+
+
+def _set_cells(a, rho_fn, p_fn):
+    """Every block's cells from global cell indices (i, j) (single level)."""
+    lv, idx, _ = a.blocks()
+    u = a.get_u()
+    for bi in range(len(lv)):
+        for ii in range(a.b):
+            for jj in range(a.b):
+                gi, gj = idx[bi][0] * a.b + ii, idx[bi][1] * a.b + jj
+                u[bi, ii, jj, 0] = [rho_fn(gi, gj), 0.0, 0.0, 0.0, p_fn(gi, gj) / 0.4]
+    a.set_u(u)
+    return lv, idx
+
+
+def test_block_detail_closed_forms_on_polynomial_sensors():
+    # The detail is |s_i - sum_m wr_m s_{i+o_m}|, o = -5,-3,-1,1,3,5, on both
+    # sensors (ln rho, ln p) along every active axis: 6-point interpolation
+    # over spacing 2h, exact on quintics, and on (x - x0)^6 its error is
+    # prod_m (o_m h) = -225 h^6 at every cell, whatever x0.
+    a = _uniform_single_level()
+    h = 1.0 / 16.0
+    X = lambda i: (i - 16) * h
+    q = lambda x: 0.3 * x ** 5 - 0.2 * x ** 3 + 0.5 * x ** 2 + 0.1 * x
+    interior = lambda idx: all(1 <= idx[d] <= 2 for d in range(2))
+    cases = [
+        (lambda i, j: math.exp(q(X(i)) + q(X(j))), lambda i, j: math.exp(q(X(j)) - q(X(i))), 0.0),
+        (lambda i, j: math.exp(q(X(i)) + (X(i) - 0.3) ** 6), lambda i, j: 1.0, 225.0 * h ** 6),
+        (lambda i, j: 1.0, lambda i, j: math.exp(0.5 * q(X(i)) + (X(j) + 0.1) ** 6), 225.0 * h ** 6),
+    ]
+    for rho_fn, p_fn, want in cases:
+        lv, idx = _set_cells(a, rho_fn, p_fn)
+        det = a.details()
+        for bi in range(len(lv)):
+            if interior(idx[bi]):
+                assert abs(det[bi] - want) <= 1e-12 + 1e-9 * want, (bi, det[bi], want)
+
+
+def test_block_detail_of_a_one_cell_bump_has_the_wr_pattern():
+    # s = ln rho is zero except delta at one cell, p = 1: the detail of a cell
+    # in the bump's row or column at odd distance o <= 5 is |wr(o)| delta
+    # (150, 25, 3 over 256), the bump cell itself has delta, every other cell
+    # zero; a block's detail is the largest over its cells (periodic wrap).
+    a = _uniform_single_level()
+    ncell = 32
+    for (ib, jb) in [(14, 9), (8, 21), (3, 30)]:
+        rho_b = math.exp(0.37)
+        lv, idx = _set_cells(a, lambda i, j: rho_b if (i, j) == (ib, jb) else 1.0, lambda i, j: 1.0)
+        s_b = math.log(rho_b)
+        det = a.details()
+        for bi in range(len(lv)):
+            want = 0.0
+            for ii in range(a.b):
+                for jj in range(a.b):
+                    gi, gj = idx[bi][0] * a.b + ii, idx[bi][1] * a.b + jj
+                    if (gi, gj) == (ib, jb):
+                        want = max(want, s_b)
+                        continue
+                    for (same, d_off) in ((gj == jb, ib - gi), (gi == ib, jb - gj)):
+                        if not same:
+                            continue
+                        o = (d_off + ncell // 2) % ncell - ncell // 2
+                        if o in _WR:
+                            want = max(want, abs(_WR[o] * s_b))
+            assert det[bi] == want, (ib, jb, bi, det[bi], want)
+
+
+def _riemann_hierarchy(levels=3, base=(4, 4, 0), bc=None, eps=1e-3):
+    bc = bc or [["outflow"] * 2, ["outflow"] * 2, ["periodic"] * 2]
+    return OracleAmr("riemann2d", [0.55, 0.55], base, 8, levels, (0, 0, 0), (1, 1, 1), eps=eps, bc=bc)
+
+
+def test_flags_from_details_thresholds():
+    # +1 iff det > eps and the level can refine; -1 iff det < eps/128; ancestors 0
+    a = _riemann_hierarchy()
+    eps, levels = 1e-3, 3
+    lv, _, leaf = a.blocks()
+    assert leaf.sum() < len(lv) and (lv[leaf] == levels - 1).any() and (lv[leaf] < levels - 1).any()
+    probes = [eps, np.nextafter(eps, 1.0), 10 * eps, eps / 128.0, np.nextafter(eps / 128.0, 0.0), 0.0, 0.5 * eps]
+    for shift in range(len(probes)):
+        det = np.array([probes[(i + shift) % len(probes)] for i in range(len(lv))])
+        got = a.flags_from_details(det)
+        for i in range(len(lv)):
+            if not leaf[i]:
+                want = 0
+            elif det[i] > eps and lv[i] + 1 < levels:
+                want = 1
+            elif det[i] < eps / 128.0:
+                want = -1
+            else:
+                want = 0
+            assert got[i] == want, (i, lv[i], det[i], got[i])
+    # the edges themselves, on a refinable leaf
+    i = int(np.flatnonzero(leaf & (lv < levels - 1))[0])
+    det = np.zeros(len(lv))
+    for v, want in [(eps, 0), (np.nextafter(eps, 1.0), 1), (eps / 128.0, 0), (np.nextafter(eps / 128.0, 0.0), -1)]:
+        det[i] = v
+        assert a.flags_from_details(det)[i] == want, v
+
+
+def _tree(a):
+    lv, idx, leaf = a.blocks()
+    return {(int(lv[i]), tuple(int(v) for v in idx[i][:2])): bool(leaf[i]) for i in range(len(lv))}, (lv, idx, leaf)
+
+
+def _neigh(key, base, periodic):
+    l, (x, y) = key
+    nb = (base[0] << l, base[1] << l)
+    out = []
+    for dx in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            if dx == 0 and dy == 0:
+                continue
+            k = [x + dx, y + dy]
+            ok = True
+            for d in range(2):
+                if k[d] < 0 or k[d] >= nb[d]:
+                    if periodic[d]:
+                        k[d] %= nb[d]
+                    else:
+                        ok = False
+            if ok:
+                out.append((l, tuple(k)))
+    return out
+
+
+def _check_tiling_and_balance(tree, base, levels, periodic):
+    """Leaves tile the domain exactly once and adjacent leaves (diagonals
+    included, wrapped on periodic axes) differ by at most one level."""
+    L = levels - 1
+    nf = (base[0] << L, base[1] << L)
+    grid = -np.ones(nf, dtype=int)
+    for (l, (x, y)), is_leaf in tree.items():
+        if not is_leaf:
+            continue
+        r = 1 << (L - l)
+        sub = grid[x * r:(x + 1) * r, y * r:(y + 1) * r]
+        assert (sub == -1).all(), "leaves overlap"
+        sub[...] = l
+    assert (grid >= 0).all(), "leaves leave a hole"
+    for dx, dy in [(1, 0), (0, 1), (1, 1), (1, -1)]:
+        sh = np.roll(np.roll(grid, -dx, 0), -dy, 1)
+        diff = np.abs(grid - sh)
+        valid = np.ones(nf, dtype=bool)
+        if not periodic[0] and dx:
+            valid[nf[0] - 1, :] = False
+        if not periodic[1] and dy:
+            valid[:, nf[1] - 1 if dy > 0 else 0] = False
+        assert (diff[valid] <= 1).all(), "2:1 balance broken"
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+@pytest.mark.parametrize("periodic", [False, True], ids=["outflow", "periodic"])
+def test_regrid_refine_and_coarsen_rules_on_random_flags(seed, periodic):
+    levels, base = 3, (4, 4, 0)
+    bc = [["periodic"] * 2] * 3 if periodic else [["outflow"] * 2, ["outflow"] * 2, ["periodic"] * 2]
+    per = (periodic, periodic)
+    a = _riemann_hierarchy(levels, base, bc)
+    before, (lv, idx, leaf) = _tree(a)
+    _check_tiling_and_balance(before, base, levels, per)
+    rng = np.random.default_rng(seed)
+    # about one leaf in eight flagged for refinement
+    flags = np.where(leaf, rng.choice([-1, 0, 0, 1], size=len(lv), p=[0.3, 0.3, 0.275, 0.125]), 0).astype(np.int8)
+    flags[rng.choice(np.flatnonzero(leaf & (lv + 1 < levels)))] = 1
+    # refine: flagged leaves and their same-level leaf neighbours (one-block
+    # buffer), each if its level can refine; the buffered set is protected
+    want = set()
+    for i in np.flatnonzero(flags > 0):
+        k = (int(lv[i]), tuple(int(v) for v in idx[i][:2]))
+        want.add(k)
+        want.update(nk for nk in _neigh(k, base, per) if before.get(nk) is True)
+    a.regrid_refine(flags)
+    after, _ = _tree(a)
+    _check_tiling_and_balance(after, base, levels, per)
+    for k in want:
+        if k[0] + 1 < levels:
+            assert after.get(k) is False, ("flagged or buffered leaf not refined", k)
+    newly = {k for k, v in before.items() if v and after.get(k) is False}
+    assert len(newly) > 0
+    for k in newly - {k for k in want if k[0] + 1 < levels}:
+        # any other refinement is forced by 2:1 balance: k covers a missing
+        # same-level neighbour position of a block refined at a finer level
+        forced = any(
+            r[0] > k[0] and
+            any((nk[1][0] >> (r[0] - k[0]), nk[1][1] >> (r[0] - k[0])) == k[1] for nk in _neigh(r, base, per))
+            for r in newly)
+        assert forced, ("refinement neither flagged, buffered nor forced by balance", k)
+    # coarsen, every leaf flagged -1: candidates are ancestors whose children
+    # are all unprotected leaves; a candidate is vetoed when a same-level
+    # neighbour holds a non-leaf child (a block two levels finer would touch it)
+    tree1 = after
+    kids = lambda k: [(k[0] + 1, (2 * k[1][0] + ox, 2 * k[1][1] + oy)) for ox in (0, 1) for oy in (0, 1)]
+    cand = [k for k, v in tree1.items() if not v and all(tree1.get(c) is True and c not in want for c in kids(k))]
+    a.regrid_coarsen(np.where(_tree(a)[1][2], -1, 0).astype(np.int8))
+    final, _ = _tree(a)
+    _check_tiling_and_balance(final, base, levels, per)
+    for k in want:
+        if tree1.get(k) is True:
+            assert final.get(k) is True, ("protected leaf coarsened away", k)
+    coarsened = {k for k, v in tree1.items() if not v and final.get(k) is True}
+    assert coarsened <= set(cand)
+    for k in cand:
+        veto = any(final.get(c) is False for nk in _neigh(k, base, per) for c in kids(nk))
+        assert (k in coarsened) == (not veto), (k, veto)
+    assert coarsened
+
+
+def test_regrid_interval_doubles_up_to_twice_when_nothing_changes():
+    # schedule_regrid: changed -> regrid_every; unchanged -> min(2 interval, 2 regrid_every)
+    a = OracleAmr("smooth", None, (2, 2, 0), 8, 1, (0, 0, 0), (1, 1, 1), eps=1e9, regrid_every=3)
+    assert a.schedule_regrid() == (0, 3, 3)
+    assert a.schedule_regrid(False) == (0, 6, 6)
+    assert a.schedule_regrid(False) == (0, 6, 6)
+    assert a.schedule_regrid(True) == (0, 3, 3)
+    # a single-level hierarchy never changes: regrids at steps 3, 9, 15, ...
+    a.schedule_regrid(True)
+    seen = []
+    for _ in range(16):
+        a.step(1e-4)
+        s, itv, nxt = a.schedule_regrid()
+        seen.append(nxt)
+    assert seen == [3, 3, 9, 9, 9, 9, 9, 9, 15, 15, 15, 15, 15, 15, 21, 21]
