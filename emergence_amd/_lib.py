@@ -33,6 +33,7 @@ EXPORTS = (
     "em_solver_forward",
     "em_solver_backward",
     "em_case_init",
+    "em_rhs_noalloc",
     "em_ensemble_create",
     "em_ensemble_destroy",
     "em_ensemble_set_batched",
@@ -119,6 +120,9 @@ def load():
     lib.em_solver_forward.argtypes = [P, C.c_int64, C.c_int32, D, D]
     lib.em_solver_backward.argtypes = [P, D, D]
     lib.em_case_init.argtypes = [C.c_int32, I64, D, D, C.c_int32, D, D]
+    # device pointers (u, rhs, stats) and the stream travel as plain addresses
+    lib.em_rhs_noalloc.argtypes = [I64, D, D, D, I32, D, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_double,
+                                   P, P, P, P]
     U8 = C.POINTER(C.c_uint8)
     DP = C.POINTER(D)
     lib.em_ensemble_create.argtypes = [C.POINTER(P), C.c_int32, I64, D, D, C.c_int32, D, I32, D, C.c_int32,

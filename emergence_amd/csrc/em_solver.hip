@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -183,7 +184,11 @@ __global__ void k_ghost_sweep(double* __restrict__ Pd, long long len, int m0, in
 
 // ------------------------------------------------------------------ solver
 
-struct Solver {
+// Mesh, boundary conditions, scheme options and tile shapes of one uniform
+// grid: everything an axis pass needs besides its buffers.  Validates its
+// arguments and allocates nothing (the allocation-free residual entry point
+// builds one on the stack).
+struct Grid {
     int64_t n[3];
     double lo[3], dx[3];
     int ng;
@@ -192,33 +197,22 @@ struct Solver {
     double bc_state[3][2][5];
     int mode, positivity;
     double cfl, tol;
-    int precise = 1;  // reference arithmetic by default
     int cfg[3] = {6, 6, 6};
     long long ncell;
     long long stride[3];
     int nactive;
     int act[3];
-    hipStream_t stream = nullptr;
-    hipStream_t rstream = nullptr;  // the backward sweep's segment recompute (default priority)
-    double* U[3] = {nullptr, nullptr, nullptr};  // U^n, stage 1, stage 2
-    double* R = nullptr;
-    double* pad = nullptr;
-    double* bcdev = nullptr;  // [3][2][5] boundary states for the ghost sweep
-    DevScalars* sc = nullptr;
-    hipGraphExec_t graph[2] = {nullptr, nullptr};  // per run mode
-    int graph_variant[2] = {-1, -1};
-    int graph_cfg[2][3];
-    bool rate_valid = false;
 
     bool active(int d) const { return n[d] > 1; }
     int m(int d) const { return active(d) ? (int)(n[d] + 2 * ng) : 1; }
     int g(int d) const { return active(d) ? ng : 0; }
     long long len() const { return (long long)m(0) * m(1) * m(2); }
 
-    Solver(const int64_t nn[3], const double lo_[3], const double hi[3], int ng_, const double e[4],
-           const int32_t bk[6], const double bs[30], int mode_, int pos, double cfl_, double tol_, double t0) {
+    Grid(const int64_t nn[3], const double lo_[3], const double hi[3], int ng_, const double e[4],
+         const int32_t bk[6], const double bs[30], int mode_, int pos, double cfl_, double tol_) {
         for (int d = 0; d < 3; ++d) {
             if (nn[d] < 1) throw Error(EM_ERR_ARG, "mesh extents must be >= 1");
+            if (nn[d] > (1 << 30)) throw Error(EM_ERR_ARG, "mesh extents must be <= 2^30");
             n[d] = nn[d];
             lo[d] = lo_[d];
             dx[d] = (hi[d] - lo_[d]) / (double)nn[d];  // Mesh::new, solver.rs:32
@@ -246,9 +240,75 @@ struct Solver {
         }
         if (nactive == 0) throw Error(EM_ERR_ARG, "mesh has no active axis");
         ncell = n[0] * n[1] * n[2];
+        if (ncell > (1ll << 40)) throw Error(EM_ERR_ARG, "mesh has more than 2^40 cells");
         stride[0] = n[1] * n[2];
         stride[1] = n[2];
         stride[2] = 1;
+    }
+
+    PassArgs pass_args(int d, DevScalars* sc) const {
+        PassArgs a{};
+        a.sc = sc;
+        for (int e = 0; e < 3; ++e) {
+            a.n[e] = (int)n[e];
+            a.stride[e] = stride[e];
+            a.dx_all[e] = dx[e];
+            a.active[e] = act[e];
+        }
+        a.ncell = ncell;
+        a.eos = eos;
+        for (int sd = 0; sd < 2; ++sd) {
+            a.bc_kind[sd] = bc_kind[d][sd];
+            for (int q = 0; q < 5; ++q) a.bc_state[sd][q] = bc_state[d][sd][q];
+        }
+        a.inv_dx = 1.0 / dx[d];
+        a.dx_d = dx[d];
+        a.ndim_pos = positivity ? (double)nactive : 0.0;
+        a.tol = tol;
+        // transverse axes: the pencils of a workgroup are neighbours along the
+        // innermost active transverse axis
+        int t[2], nt = 0;
+        for (int e = 0; e < 3; ++e)
+            if (e != d) t[nt++] = e;
+        if (active(t[1]) || !active(t[0])) {
+            a.fast_axis = t[1];
+            a.slow_axis = t[0];
+        } else {
+            a.fast_axis = t[0];
+            a.slow_axis = t[1];
+        }
+        a.dfast = stride[d] == 1 ? 1 : 0;
+        const TileCfg tc = kTileCfgs[cfg[d]];
+        a.ntile = (int)((n[d] + tc.T - 1) / tc.T);
+        a.ngroup = (int)((n[a.fast_axis] + tc.P - 1) / tc.P);
+        return a;
+    }
+
+    uint64_t faces_per_stage() const {
+        uint64_t f = 0;
+        for (int d = 0; d < 3; ++d)
+            if (active(d)) f += (uint64_t)(n[d] + 1) * (ncell / n[d]);
+        return f;
+    }
+};
+
+struct Solver : Grid {
+    int precise = 1;  // reference arithmetic by default
+    hipStream_t stream = nullptr;
+    hipStream_t rstream = nullptr;  // the backward sweep's segment recompute (default priority)
+    double* U[3] = {nullptr, nullptr, nullptr};  // U^n, stage 1, stage 2
+    double* R = nullptr;
+    double* pad = nullptr;
+    double* bcdev = nullptr;  // [3][2][5] boundary states for the ghost sweep
+    DevScalars* sc = nullptr;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};  // per run mode
+    int graph_variant[2] = {-1, -1};
+    int graph_cfg[2][3];
+    bool rate_valid = false;
+
+    Solver(const int64_t nn[3], const double lo_[3], const double hi[3], int ng_, const double e[4],
+           const int32_t bk[6], const double bs[30], int mode_, int pos, double cfl_, double tol_, double t0)
+        : Grid(nn, lo_, hi, ng_, e, bk, bs, mode_, pos, cfl_, tol_) {
         HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         for (int b = 0; b < 3; ++b) HIPCHK(hipMalloc(&U[b], sizeof(double) * 5 * ncell));
         HIPCHK(hipMalloc(&R, sizeof(double) * 5 * ncell));
@@ -339,43 +399,7 @@ struct Solver {
             for (int v = 0; v < 5; ++v) u[q * 5 + v] = h[v * L + q];
     }
 
-    PassArgs pass_args(int d) const {
-        PassArgs a{};
-        a.sc = sc;
-        for (int e = 0; e < 3; ++e) {
-            a.n[e] = (int)n[e];
-            a.stride[e] = stride[e];
-            a.dx_all[e] = dx[e];
-            a.active[e] = act[e];
-        }
-        a.ncell = ncell;
-        a.eos = eos;
-        for (int sd = 0; sd < 2; ++sd) {
-            a.bc_kind[sd] = bc_kind[d][sd];
-            for (int q = 0; q < 5; ++q) a.bc_state[sd][q] = bc_state[d][sd][q];
-        }
-        a.inv_dx = 1.0 / dx[d];
-        a.dx_d = dx[d];
-        a.ndim_pos = positivity ? (double)nactive : 0.0;
-        a.tol = tol;
-        // transverse axes: the pencils of a workgroup are neighbours along the
-        // innermost active transverse axis
-        int t[2], nt = 0;
-        for (int e = 0; e < 3; ++e)
-            if (e != d) t[nt++] = e;
-        if (active(t[1]) || !active(t[0])) {
-            a.fast_axis = t[1];
-            a.slow_axis = t[0];
-        } else {
-            a.fast_axis = t[0];
-            a.slow_axis = t[1];
-        }
-        a.dfast = stride[d] == 1 ? 1 : 0;
-        const TileCfg tc = kTileCfgs[cfg[d]];
-        a.ntile = (int)((n[d] + tc.T - 1) / tc.T);
-        a.ngroup = (int)((n[a.fast_axis] + tc.P - 1) / tc.P);
-        return a;
-    }
+    PassArgs pass_args(int d) const { return Grid::pass_args(d, sc); }
 
     hipError_t launch(int d, const PassArgs& a, hipStream_t st = nullptr) {
         if (!st) st = stream;
@@ -493,13 +517,6 @@ struct Solver {
             stats[0] = faces_per_stage();
             for (int q = 0; q < 3; ++q) stats[1 + q] = after.counters[q] - before.counters[q];
         }
-    }
-
-    uint64_t faces_per_stage() const {
-        uint64_t f = 0;
-        for (int d = 0; d < 3; ++d)
-            if (active(d)) f += (uint64_t)(n[d] + 1) * (ncell / n[d]);
-        return f;
     }
 
     // one step with the given dt, queued on the stream (no host wait)
@@ -1539,6 +1556,101 @@ static void case_init(int id, const int64_t n[3], const double lo[3], const doub
             }
 }
 
+// ------------------------------------------- allocation-free residual
+
+// Scratch of em_rhs_noalloc: the device scalars its passes accumulate into
+// (64 counter slots on separate 128-byte lines, the unused CFL-rate slots and
+// the run header).  One copy per GPU, kept until the process ends; every call
+// on a GPU reuses that GPU's copy, so calls that overlap on one GPU race.
+constexpr size_t kNoAllocScratchBytes = EM_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static_assert(sizeof(DevScalars) <= kNoAllocScratchBytes, "scratch smaller than the device scalars");
+
+constexpr int kMaxDevices = 64;
+static DevScalars* g_scratch_ptr[kMaxDevices];  // device address of `scratch`, per GPU
+static std::mutex g_scratch_mu;
+
+// Device address of the current GPU's scratch, looked up once per GPU with
+// that GPU current.  Host code never passes `scratch` itself to a kernel: on
+// the host that name is only a stand-in for the symbol.
+static DevScalars* noalloc_scratch() {
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw Error(EM_ERR_ARG, "device ordinal out of range (at most 64 GPUs)");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw Error(EM_ERR_HIP, "hipGetSymbolAddress returned a null scratch address");
+        g_scratch_ptr[dev] = (DevScalars*)p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+// Puts the scratch in the state a residual pass expects: active run, counter
+// slots zero (a kernel, not hipMemsetAsync, so callers may capture the call).
+__global__ void k_noalloc_reset(DevScalars* sc) {
+    const int t = threadIdx.x;
+    if (t == 0) {
+        sc->dt = 0.0;
+        sc->t = 0.0;
+        sc->t_end = 0.0;
+        sc->active = 1;
+        sc->err = 0;
+        sc->steps = 0;
+    }
+    if (t < kSlots) {
+        sc->rate[t][0] = 0ull;
+        for (int q = 0; q < 3; ++q) sc->counters[q][t][0] = 0ull;
+    }
+}
+
+// stats = [faces, shock-path faces, fallbacks, limited faces], slots reduced.
+__global__ void k_noalloc_stats(const DevScalars* sc, unsigned long long faces, unsigned long long* out) {
+    const int t = threadIdx.x;
+    if (t == 0) out[0] = faces;
+    if (t < 3) {
+        unsigned long long c = 0;
+        for (int i = 0; i < kSlots; ++i) c += sc->counters[t][i][0];
+        out[1 + t] = c;
+    }
+}
+
+static void rhs_noalloc(const Grid& G, int precise, double stage_dt, const double* u, double* rhs, uint64_t* stats,
+                        hipStream_t st) {
+    if (sizeof(DevScalars) > kNoAllocScratchBytes) throw Error(EM_ERR_ARG, "scratch too small for the device scalars");
+    for (int d = 0; d < 3; ++d) {
+        if (!G.active(d)) continue;
+        const PassArgs a = G.pass_args(d, nullptr);
+        const long long nblk = (long long)a.ntile * a.ngroup * a.n[a.slow_axis];
+        if (nblk > 0x7fffffffll) throw Error(EM_ERR_ARG, "mesh too large for one axis-pass launch");
+    }
+    DevScalars* sc = noalloc_scratch();
+    hipLaunchKernelGGL(k_noalloc_reset, dim3(1), dim3(kSlots), 0, st, sc);
+    HIPCHK(hipGetLastError());
+    bool first = true;
+    for (int d = 0; d < 3; ++d) {
+        if (!G.active(d)) continue;
+        PassArgs a = G.pass_args(d, sc);
+        a.U = u;
+        a.R = rhs;
+        a.U0 = u;
+        a.Uout = nullptr;
+        a.first = first ? 1 : 0;
+        a.rk_stage = 0;
+        a.want_rate = 0;
+        a.stage_dt_host = 1;
+        a.stage_dt = stage_dt;
+        HIPCHK(precise ? precise::launch_pass(d, G.mode, G.cfg[d], a, st) : fast::launch_pass(d, G.mode, G.cfg[d], a, st));
+        first = false;
+    }
+    if (stats) {
+        hipLaunchKernelGGL(k_noalloc_stats, dim3(1), dim3(64), 0, st, sc, (unsigned long long)G.faces_per_stage(),
+                           (unsigned long long*)stats);
+        HIPCHK(hipGetLastError());
+    }
+}
+
 }  // namespace em
 
 // ------------------------------------------------------------------ C ABI
@@ -1831,6 +1943,23 @@ int em_case_init(int32_t case_id, const int64_t n[3], const double lo[3], const 
         return EM_ERR_ARG;
     }
     return guarded([&] { em::case_init(case_id, n, lo, hi, ng, params, u); });
+}
+
+int em_rhs_noalloc(const int64_t n[3], const double lo[3], const double hi[3], const double eos[4],
+                   const int32_t bc_kind[6], const double bc_state[30], int32_t mode, int32_t positivity, double tol,
+                   int32_t precise, double stage_dt, const double* u, double* rhs, uint64_t* stats, void* stream) {
+    if (!n || !lo || !hi || !eos || !bc_kind || !u || !rhs) {
+        em::g_last_error = "null argument";
+        return EM_ERR_ARG;
+    }
+    if ((const void*)u == (const void*)rhs) {
+        em::g_last_error = "u and rhs must not alias";
+        return EM_ERR_ARG;
+    }
+    return guarded([&] {
+        const em::Grid G(n, lo, hi, 6, eos, bc_kind, bc_state, mode, positivity, 0.5, tol);
+        em::rhs_noalloc(G, precise, stage_dt, u, rhs, stats, (hipStream_t)stream);
+    });
 }
 
 }  // extern "C"

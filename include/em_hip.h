@@ -210,6 +210,31 @@ int em_ensemble_backward(em_ensemble* h, const double* const* dj_final, double* 
 int em_case_init(int32_t case_id, const int64_t n[3], const double lo[3], const double hi[3], int32_t ng,
                  const double* params, double* u);
 
+/* Allocation-free per-stage residual.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * residual and stats as em_solver_rhs for the same mesh, boundaries and
+ * options (ng >= 5 assumed), but on caller-owned device buffers and without a
+ * handle:
+ *   u       device, interior-only SoA conserved state: 5 planes of
+ *           n0*n1*n2 doubles, cell index (i*n1 + j)*n2 + k (read only)
+ *   rhs     device, same layout, receives the residual (must not alias u)
+ *   stats   device uint64[4] or NULL: [faces, shock-path faces, first-order
+ *           fallbacks, limited faces] of this evaluation
+ *   stream  hipStream_t the work is queued on (NULL: the null stream); the
+ *           call returns without waiting for it
+ *   precise 1: reference arithmetic (no FMA contraction), 0: fast variant
+ * It calls no hipMalloc/hipFree.  Its only scratch is a fixed
+ * EM_NOALLOC_SCRATCH_BYTES static device array per GPU (the counter slots),
+ * independent of the mesh; the counters are cleared by a kernel at the start
+ * of every call.  Calls that overlap on one GPU share that array and race.
+ * Meshes whose axis-pass launch would exceed 2^31 - 1 workgroups, or with
+ * more than 2^40 cells, are rejected with EM_ERR_ARG. */
+#define EM_NOALLOC_SCRATCH_BYTES 65536
+int em_rhs_noalloc(const int64_t n[3], const double lo[3], const double hi[3], const double eos[4],
+                   const int32_t bc_kind[6], const double bc_state[30], int32_t mode, int32_t positivity, double tol,
+                   int32_t precise, double stage_dt, const double* u, double* rhs, uint64_t* stats, void* stream);
+
 /* ------------------------------------------------------------------------
  * Block-structured AMR (global SSP-RK3 step), drop-in for
  *   GpuAmr<f64, 5, Euler> (fluid/crates/compressible-cuda/src/amr.rs:95-858)
