@@ -1,9 +1,10 @@
 // Restatement of the reference's CPU known-answer probe
-// (fluid/crates/compressible/examples/cpu_validate.rs:23-99): `smooth`,
-// `weno`, `fdweno` and `open` cases; prints l1_rho / mass_error (or max_dp)
-// and the FNV-1a state hash in the reference's output format.
-// Test infrastructure: run by tests/test_oracle_golden.py.
-//   cpu_validate [smooth|weno|fdweno|open] [N=32] [steps=40]
+// (fluid/crates/compressible/examples/cpu_validate.rs:23-149): `smooth`,
+// `weno`, `fdweno`, `interface` (two-phase FiveEq, Hybrid) and `open` cases;
+// prints l1_rho / mass_error (or max_dp, or max_relative_dp / max_du) and the
+// FNV-1a state hash in the reference's output format.
+// Test infrastructure: run by tests/test_oracle.py.
+//   cpu_validate [smooth|weno|fdweno|interface|open] [N=32] [steps=40]
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -12,6 +13,7 @@
 #include <string>
 
 #include "emref.h"
+#include "emref_multi.h"
 
 using namespace emref;
 
@@ -101,8 +103,32 @@ int main(int argc, char** argv) {
                 }
         std::printf("open n=%zu steps=%zu wall_s=%.9f max_dp=%s hash=%016llx\n", n, steps, el,
                     rust_exp9(pressure).c_str(), (unsigned long long)state_hash(s));
+    } else if (cs == "interface") {
+        // examples/cpu_validate.rs:100-146
+        gen::FiveEq fe;
+        fe.eos[0] = Nasg{2.0, 1e6, 0.0, 0.0};
+        fe.eos[1] = Nasg{1.4, 0.0, 0.0, 0.0};
+        fe.kapila = true;
+        gen::SolverG<gen::FiveEq> s(fe, mesh);
+        gen::interface_init(s);
+        s.mode = FluxMode::Hybrid;
+        const double dt = 1e-5;
+        auto t0 = std::chrono::steady_clock::now();
+        for (size_t q = 0; q < steps; ++q) s.step(dt);
+        double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        double dp = 0.0, du = 0.0;
+        for (size_t i = 0; i < n; ++i)
+            for (size_t j = 0; j < n; ++j)
+                for (size_t k = 0; k < n; ++k) {
+                    gen::Vn<7> w = fe.to_prim(s.u[mesh.idx(i + mesh.g(0), j + mesh.g(1), k + mesh.g(2))]);
+                    if (!fe.admissible(w)) return 2;
+                    dp = std::fmax(dp, std::fabs(w[5] / 1e5 - 1.0));
+                    du = std::fmax(du, std::fabs(w[2] - 0.1));
+                }
+        std::printf("interface n=%zu steps=%zu wall_s=%.9f max_relative_dp=%s max_du=%s hash=%016llx\n", n, steps, el,
+                    rust_exp9(dp).c_str(), rust_exp9(du).c_str(), (unsigned long long)s.hash());
     } else {
-        std::fprintf(stderr, "smooth, weno, fdweno or open expected\n");
+        std::fprintf(stderr, "smooth, weno, fdweno, interface or open expected\n");
         return 1;
     }
     return 0;

@@ -4,6 +4,7 @@
 // (i*m1 + j)*m2 + k, five conserved values per cell (solver.rs:72-74).
 #include <omp.h>
 
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <exception>
@@ -49,6 +50,23 @@ void emref_set_u(void* h, const double* u) {
 void emref_get_u(void* h, double* u) {
     Solver* s = (Solver*)h;
     std::memcpy(u, s->u.data(), sizeof(V) * s->u.size());
+}
+
+// Initial state of the reference's `open` probe (examples/cpu_validate.rs:
+// 68-80): a Gaussian pressure pulse of 1e-4 at the centre of [0,1]^3 at rest,
+// rho = 1 + dp/1.4, p = 1 + dp, dp = 1e-4 exp(-r^2/0.04).
+void emref_init_open(void* h) {
+    Solver* s = (Solver*)h;
+    const Mesh& m = s->mesh;
+    for (size_t i = 0; i < m.n[0]; ++i)
+        for (size_t j = 0; j < m.n[1]; ++j)
+            for (size_t k = 0; k < m.n[2]; ++k) {
+                const double xs[3] = {m.x(i, 0), m.x(j, 1), m.x(k, 2)};
+                double r2 = -0.0;  // Rust's f64 Sum starts from -0.0
+                for (int d = 0; d < 3; ++d) r2 += (xs[d] - 0.5) * (xs[d] - 0.5);
+                const double dp = 1e-4 * std::exp(-r2 / 0.04);
+                s->u[m.idx(i + m.g(0), j + m.g(1), k + m.g(2))] = s->sys.to_cons(V{1.0 + dp / 1.4, 0.0, 0.0, 0.0, 1.0 + dp});
+            }
 }
 
 void emref_fill_ghosts(void* h) {

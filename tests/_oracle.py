@@ -57,6 +57,7 @@ def load():
     lib.emref_stats.argtypes = [P, U]
     lib.emref_totals.argtypes = [P, D]
     lib.emref_hash.restype = C.c_uint64
+    lib.emref_init_open.argtypes = [P]
     lib.emref_hash.argtypes = [P]
     lib.emref_set_threads.argtypes = [C.c_int]
     lib.emref_max_threads.restype = C.c_int
@@ -141,6 +142,155 @@ class OracleSolver:
 
     def hash(self):
         return self.lib.emref_hash(self.h)
+
+    def init_open(self):
+        """State of the reference's `open` probe (Gaussian pressure pulse)."""
+        self.lib.emref_init_open(self.h)
+
+    def interior(self, a):
+        g = [self.ng if v > 1 else 0 for v in self.n]
+        return a[g[0]:g[0] + self.n[0], g[1]:g[1] + self.n[1], g[2]:g[2] + self.n[2]]
+
+
+def _load_gen(lib):
+    if getattr(lib, "_gen_ready", False):
+        return lib
+    P = C.c_void_p
+    D = C.POINTER(C.c_double)
+    U = C.POINTER(C.c_uint64)
+    I64 = C.POINTER(C.c_int64)
+    lib.emref_gen_create.restype = P
+    lib.emref_gen_create.argtypes = [C.c_int, D, I64, D, D, C.c_int64]
+    lib.emref_gen_destroy.argtypes = [P]
+    lib.emref_gen_nv.restype = C.c_int
+    lib.emref_gen_nv.argtypes = [P]
+    lib.emref_gen_set_bc.argtypes = [P, C.c_int, C.c_int, C.c_int, D]
+    lib.emref_gen_set_opts.argtypes = [P, C.c_int, C.c_double, C.c_int, C.c_double]
+    lib.emref_gen_set_u.argtypes = [P, D]
+    lib.emref_gen_get_u.argtypes = [P, D]
+    lib.emref_gen_rhs.argtypes = [P, C.c_double, D, U]
+    lib.emref_gen_step.argtypes = [P, C.c_double]
+    lib.emref_gen_dt_cfl.restype = C.c_double
+    lib.emref_gen_dt_cfl.argtypes = [P]
+    lib.emref_gen_run_to.restype = C.c_int64
+    lib.emref_gen_run_to.argtypes = [P, C.c_double]
+    lib.emref_gen_time.restype = C.c_double
+    lib.emref_gen_time.argtypes = [P]
+    lib.emref_gen_stats.argtypes = [P, U]
+    lib.emref_gen_hash.restype = C.c_uint64
+    lib.emref_gen_hash.argtypes = [P]
+    lib.emref_gen_to_prim.argtypes = [P, D, D]
+    lib.emref_gen_init_interface.argtypes = [P]
+    lib.emref_gen_eigen.argtypes = [P, D, C.c_int, D, D]
+    lib.emref_gen_riemann.argtypes = [P, D, D, C.c_int, D, D]
+    lib.emref_gen_phys_flux.argtypes = [P, D, C.c_int, D]
+    lib._gen_ready = True
+    return lib
+
+
+# the reference's `interface` probe materials (examples/cpu_validate.rs:101-112):
+# water as a stiffened gas (gamma 2, p_inf 1e6) and air (ideal, gamma 1.4)
+FIVEEQ_WATER_AIR = (2.0, 1e6, 0.0, 0.0, 1.4, 0.0, 0.0, 0.0, 1.0)
+
+
+class OracleGen:
+    """The oracle's generic solver (emref_gen.h) of the Euler system
+    (sys="euler", params = gamma, p_inf, b, q) or of the Kapila five-equation
+    two-phase system (sys="fiveeq", params = both phases' gamma, p_inf, b, q
+    and the Kapila flag).  Ghost-padded AoS arrays of shape (m0, m1, m2, nv)."""
+
+    def __init__(self, sys, n, lo, hi, params=None, ng=6):
+        self.lib = _load_gen(load())
+        self.sysid = {"euler": 0, "fiveeq": 1}[sys]
+        if params is None:
+            params = (1.4, 0.0, 0.0, 0.0) if self.sysid == 0 else FIVEEQ_WATER_AIR
+        self.n = tuple(int(v) for v in n)
+        self.ng = ng
+        self.h = self.lib.emref_gen_create(self.sysid, _dp(np.asarray(params, dtype=np.float64)),
+                                           (C.c_int64 * 3)(*self.n), (C.c_double * 3)(*lo), (C.c_double * 3)(*hi), ng)
+        self.nv = self.lib.emref_gen_nv(self.h)
+        self.shape = tuple(v + 2 * ng if v > 1 else 1 for v in self.n) + (self.nv,)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.emref_gen_destroy(self.h)
+            self.h = None
+
+    def set_bc(self, d, side, kind, state=None):
+        st = None if state is None else _dp(np.asarray(state, dtype=np.float64))
+        self.lib.emref_gen_set_bc(self.h, d, side, BC[kind], st)
+
+    def set_opts(self, mode="hybrid", tol=1e-3, positivity=True, cfl=0.5):
+        self.lib.emref_gen_set_opts(self.h, MODE[mode], tol, int(positivity), cfl)
+
+    def set_u(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(self.shape)
+        self.lib.emref_gen_set_u(self.h, _dp(u))
+
+    def get_u(self):
+        u = np.empty(self.shape)
+        self.lib.emref_gen_get_u(self.h, _dp(u))
+        return u
+
+    def rhs(self, stage_dt):
+        out = np.empty(self.shape)
+        st = (C.c_uint64 * 4)()
+        self.lib.emref_gen_rhs(self.h, stage_dt, _dp(out), st)
+        return out, list(st)
+
+    def step(self, dt):
+        self.lib.emref_gen_step(self.h, dt)
+
+    def dt_cfl(self):
+        return self.lib.emref_gen_dt_cfl(self.h)
+
+    def run_to(self, t_end):
+        return self.lib.emref_gen_run_to(self.h, t_end)
+
+    @property
+    def t(self):
+        return self.lib.emref_gen_time(self.h)
+
+    def stats(self):
+        st = (C.c_uint64 * 4)()
+        self.lib.emref_gen_stats(self.h, st)
+        return list(st)
+
+    def hash(self):
+        return self.lib.emref_gen_hash(self.h)
+
+    def to_prim(self, u):
+        """Primitive states of an array of conserved states (..., nv)."""
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1, self.nv)
+        w = np.empty_like(u)
+        for r in range(u.shape[0]):
+            self.lib.emref_gen_to_prim(self.h, _dp(u[r]), _dp(w[r]))
+        return w
+
+    def eigen(self, w, d):
+        """(L, R) of the FiveEq primitive system at w along axis d."""
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        l, r = np.empty((7, 7)), np.empty((7, 7))
+        self.lib.emref_gen_eigen(self.h, _dp(w), d, _dp(l), _dp(r))
+        return l, r
+
+    def riemann(self, wl, wr, d):
+        """HLLC flux and contact speed of two FiveEq primitive states."""
+        wl = np.ascontiguousarray(wl, dtype=np.float64)
+        wr = np.ascontiguousarray(wr, dtype=np.float64)
+        f, ss = np.empty(7), C.c_double()
+        self.lib.emref_gen_riemann(self.h, _dp(wl), _dp(wr), d, _dp(f), C.byref(ss))
+        return f, ss.value
+
+    def phys_flux(self, w, d):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        f = np.empty(7)
+        self.lib.emref_gen_phys_flux(self.h, _dp(w), d, _dp(f))
+        return f
+
+    def init_interface(self):
+        """State of the reference's `interface` probe (FiveEq handles)."""
+        self.lib.emref_gen_init_interface(self.h)
 
     def interior(self, a):
         g = [self.ng if v > 1 else 0 for v in self.n]

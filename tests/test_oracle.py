@@ -36,9 +36,25 @@ def test_cpu_validate_golden_hash(case):
     line = subprocess.run([_oracle.CPU_VALIDATE] + args, check=True, capture_output=True, text=True).stdout
     got = _parse(line.strip())
     assert got["hash"] == case["hash"], line
-    for key in ("l1_rho", "mass_error", "max_dp"):
+    for key in ("l1_rho", "mass_error", "max_dp", "max_relative_dp", "max_du"):
         if key in case:
             assert got[key] == case[key], line
+
+
+def test_open_probe_initialiser_reproduces_the_golden_hash():
+    # emref_init_open (the state the GPU open-probe test starts from) stepped
+    # by the oracle gives the reference's recorded `open 24 60` line
+    case = next(c for c in GOLD["cases"] if c["args"] == ["open", 24, 60])
+    o = OracleSolver((24, 24, 24), (0, 0, 0), (1, 1, 1))
+    o.init_open()
+    far = [1.0, 0.0, 0.0, 0.0, 1.0 / (1.4 - 1.0)]
+    for d in range(3):
+        for sd in range(2):
+            o.set_bc(d, sd, "characteristic", far)
+    o.set_opts(mode="central")
+    for _ in range(60):
+        o.step(0.02 / 60)
+    assert f"{o.hash():016x}" == case["hash"]
 
 
 # ----------------------------------------------------------- closed forms
