@@ -56,7 +56,7 @@ def parse():
     ap.add_argument("--fast", action="store_true",
                     help="FMA-contracted variant (differs from the reference arithmetic by rounding)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-repeats", type=int, default=4, help="CPU-baseline repeats (median)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shapes (short profiled runs)")
     ap.add_argument("--no-amr", action="store_true", help="skip the block-AMR workload")
     ap.add_argument("--amr-only", action="store_true", help="only the block-AMR workload (profiling)")
@@ -131,14 +131,22 @@ def ensemble_bench(em, args):
         del ens
     ens = em.HipEnsemble(sizes, states, bc=bc)
     ens.set_tiles(tiles)
+
+    def objective():
+        djs = []
+        for i, (n, _, r, *_) in enumerate(inst):
+            djs.append(bubble(ens.download(i), n, r)[1])
+        return djs
+
+    # warm-up pair of the same length: the timed pair allocates nothing
+    ens.forward(args.ens_steps, args.ens_checkpoint)
+    ens.backward(objective())
+    for i, u in enumerate(states):
+        ens.upload(i, u)
     t0 = time.perf_counter()
     ens.forward(args.ens_steps, args.ens_checkpoint)
     t_fwd = time.perf_counter() - t0
-    djs, js = [], []
-    for i, (n, _, r, *_) in enumerate(inst):
-        j, g = bubble(ens.download(i), n, r)
-        js.append(j)
-        djs.append(g)
+    djs = objective()
     t0 = time.perf_counter()
     du0 = ens.backward(djs)
     t_bwd = time.perf_counter() - t0
@@ -248,11 +256,15 @@ def adjoint_bench(em, args):
     bc = [["outflow", "outflow"], ["outflow", "outflow"], ["periodic", "periodic"]]
     mk = lambda u: em.HipSolver((n, n, 1), (0, 0, 0), (1, 1, 1), u, bc=bc, mode="weno", cfl=0.5)
     g = mk(u0)
-    g.forward(2, 2)  # warm-up (kernels, buffers)
+    # warm-up: one full forward/backward pair of the same length, so that the
+    # timed pair allocates nothing (checkpoints, dt history, segment stores,
+    # adjoint work arrays) and replays already-instantiated graphs
+    g.forward(args.adj_steps, args.adj_checkpoint)
+    g.backward(pressure_window(g.download(), n)[1])
     g.upload(u0)
     g.synchronize()
     t0 = time.perf_counter()
-    dts = g.forward(args.adj_steps, args.adj_checkpoint)
+    dts = g.forward(args.adj_steps, args.adj_checkpoint)  # CFL dt computed and recorded on the device
     g.synchronize()
     t_fwd = time.perf_counter() - t0
     J, dj = pressure_window(g.download(), n)
@@ -369,7 +381,7 @@ def amr_bench(em, args):
     phase = [v / 16 for v in bat.profile()]
     value = cells * args.amr_steps * 3 / el
     value_per = cells_per * k * 3 / el_per if k > 0 else None
-    cpu = None if args.no_cpu_baseline else amr_cpu_baseline(nleaves * c["b"] ** 3)
+    cpu = None if args.no_cpu_baseline else amr_cpu_baseline(nleaves * c["b"] ** 3, args.cpu_repeats)
     return {
         "workload": "3-D shock-bubble block AMR, 8^3 blocks, 3 levels, Hybrid, CFL 0.4, regrid every 4",
         "blocks": nblocks,
@@ -387,7 +399,7 @@ def amr_bench(em, args):
     }
 
 
-def amr_cpu_baseline(cells):
+def amr_cpu_baseline(cells, repeats):
     """Oracle block AMR (restated reference CPU Amr) on the host cores: one
     global SSP-RK3 step of the same hierarchy."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -395,13 +407,19 @@ def amr_cpu_baseline(cells):
 
     c = AMR_CASE
     threads = _oracle.load().emref_max_threads()
-    o = _oracle.OracleAmr("shock_bubble", c["params"], c["base"], c["b"], c["levels"], c["lo"], c["hi"], eps=c["eps"],
-                          regrid_every=c["regrid_every"], bc=AMR_BC, mode=c["mode"], cfl=c["cfl"])
-    t0 = time.perf_counter()
-    o.step(o.dt_cfl())
-    el = time.perf_counter() - t0
-    return {"value": cells * 3 / el, "unit": "cell-updates/s per RK stage", "cores": int(threads), "kind": "port",
-            "sample": f"1 global SSP-RK3 step of the same hierarchy, OpenMP over leaves, {el:.1f} s wall"}
+    runs = []
+    for _ in range(repeats):
+        o = _oracle.OracleAmr("shock_bubble", c["params"], c["base"], c["b"], c["levels"], c["lo"], c["hi"],
+                              eps=c["eps"], regrid_every=c["regrid_every"], bc=AMR_BC, mode=c["mode"], cfl=c["cfl"])
+        t0 = time.perf_counter()
+        o.step(o.dt_cfl())
+        runs.append(time.perf_counter() - t0)
+    el = float(np.median(runs))
+    rec = {"value": cells * 3 / el, "unit": "cell-updates/s per RK stage", "cores": int(threads), "kind": "port",
+           "sample": f"1 global SSP-RK3 step of the same hierarchy, OpenMP over leaves, median of {repeats} runs "
+                     f"({el:.2f} s each)", "repeats_s": runs}
+    rec.update(host_info())
+    return rec
 
 
 def profiled_traffic(variant, d):
@@ -421,29 +439,79 @@ def profiled_traffic(variant, d):
     return None
 
 
-def cpu_baseline(n, u, steps):
-    """Oracle (restated reference CPU solver) on the host cores: `steps`
-    SSP-RK3 steps of the same workload, CFL dt."""
+def host_info():
+    """CPU model and core counts of the host the CPU baseline runs on."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+
+
+def cpu_baseline(em, n, u, repeats):
+    """Oracle (restated reference CPU solver, bit-identical to it) on the
+    host cores, by the reference's cpu-bench protocol (median of alternating
+    repeats, 1 thread and all threads, hash check):
+    * headline: one SSP-RK3 step (dt_cfl + 3 stages) of the same n^3 Weno
+      workload on all threads, median of `repeats` runs;
+    * 1 thread vs all threads, alternating, on a 64^3 sample of the same
+      workload (same initial-condition construction; the per-cell work of a
+      periodic 3-D Weno step does not depend on the size);
+    * the 64^3 sample's state after its step must hash equal to the GPU's
+      (precise variant) after the same step."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle  # noqa: E402  (CPU-baseline leg only)
 
     lib = _oracle.load()
-    threads = lib.emref_max_threads()
-    o = _oracle.OracleSolver((n, n, n), (0, 0, 0), (2 * np.pi,) * 3)
-    o.set_u(u)
-    o.set_opts(mode="weno", cfl=CFL)
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        o.step(o.dt_cfl())
-    el = time.perf_counter() - t0
-    return {
-        "value": n ** 3 * steps * 3 / el,
+    threads = int(lib.emref_max_threads())
+    hi = (2 * np.pi,) * 3
+
+    def one_step(nn, uu, nthreads):
+        lib.emref_set_threads(nthreads)
+        o = _oracle.OracleSolver((nn, nn, nn), (0, 0, 0), hi)
+        o.set_u(uu)
+        o.set_opts(mode="weno", cfl=CFL)
+        t0 = time.perf_counter()
+        dt = o.dt_cfl()
+        o.step(dt)
+        el = time.perf_counter() - t0
+        return el, o, dt
+
+    runs = [one_step(n, u, threads)[0] for _ in range(repeats)]
+    m = 64
+    us = em.case_init(1, (m, m, m), (0, 0, 0), hi, 6, [0.3, float(0x5EED)])
+    t1, tall = [], []
+    for _ in range(repeats):  # alternating
+        t1.append(one_step(m, us, 1)[0])
+        el, o, dt = one_step(m, us, threads)
+        tall.append(el)
+    lib.emref_set_threads(threads)
+    g = em.HipSolver((m, m, m), (0, 0, 0), hi, us, mode="weno", cfl=CFL, precise=True)
+    g.step(dt)
+    same = bool(np.array_equal(g.interior(g.download()), o.interior(o.get_u())))
+    med = float(np.median(runs))
+    rec = {
+        "value": n ** 3 * 3 / med,
         "unit": "cell-updates/s per RK stage",
-        "cores": int(threads),
+        "cores": threads,
         "kind": "port",
-        "sample": f"{steps} SSP-RK3 step(s) (dt_cfl + 3 stages) of the same {n}^3 Weno workload, "
-                  f"OpenMP over pencils, {el:.1f} s wall",
+        "sample": f"1 SSP-RK3 step (dt_cfl + 3 stages) of the same {n}^3 Weno workload, OpenMP over pencils, "
+                  f"median of {repeats} runs ({med:.2f} s each)",
+        "repeats_s": runs,
+        "threads_1_vs_all": {
+            "sample": f"1 SSP-RK3 step of the {m}^3 version of the workload, {repeats} alternating repeats",
+            "value_1_thread": m ** 3 * 3 / float(np.median(t1)),
+            "value_all_threads": m ** 3 * 3 / float(np.median(tall)),
+            "threads": threads,
+        },
+        "state_identical_to_gpu": same,
     }
+    rec.update(host_info())
+    return rec
 
 
 def main():
@@ -507,7 +575,7 @@ def main():
             # compute roof: MI355X's FP64 peak, 78.6 TFLOP/s for vector and
             # matrix FP64 alike; the pass runs FP64 VALU code, no MFMA
             "bound": "mfma",
-            "roof": "fp64 compute (VALU; no MFMA instructions)",
+            "roof": "fp64-valu (the compute roof: 78.6 TFLOP/s FP64; the pass issues VALU FP64, no MFMA)",
             "kernel": f"axis_pass d={dom}",
             "achieved": achieved / 1e12,
             "peak": PEAK_FP64 / 1e12,
@@ -519,7 +587,7 @@ def main():
         },
     }
     if not args.no_cpu_baseline:
-        rec["cpu_baseline"] = cpu_baseline(n, u, args.cpu_steps)
+        rec["cpu_baseline"] = cpu_baseline(em, n, u, args.cpu_repeats)
     del g
     if not args.no_amr:
         rec["amr"] = amr_bench(em, args)

@@ -40,6 +40,7 @@ EXPORTS = (
     "em_ensemble_set_tiles",
     "em_ensemble_autotune",
     "em_ensemble_run_steps",
+    "em_ensemble_upload",
     "em_ensemble_download",
     "em_ensemble_time",
     "em_ensemble_forward",
@@ -70,6 +71,18 @@ EXPORTS = (
     "em_amr_counters",
     "em_amr_time",
     "em_amr_synchronize",
+    "em_fiveeq_last_error",
+    "em_fiveeq_create",
+    "em_fiveeq_destroy",
+    "em_fiveeq_upload",
+    "em_fiveeq_download",
+    "em_fiveeq_rhs",
+    "em_fiveeq_step",
+    "em_fiveeq_dt_cfl",
+    "em_fiveeq_run_to",
+    "em_fiveeq_counters",
+    "em_fiveeq_time",
+    "em_fiveeq_synchronize",
 )
 
 _lib = None
@@ -132,6 +145,7 @@ def load():
     lib.em_ensemble_set_tiles.argtypes = [P, C.c_int32]
     lib.em_ensemble_autotune.argtypes = [P, C.c_int32, I32]
     lib.em_ensemble_run_steps.argtypes = [P, C.c_int64]
+    lib.em_ensemble_upload.argtypes = [P, C.c_int32, D]
     lib.em_ensemble_download.argtypes = [P, C.c_int32, D]
     lib.em_ensemble_time.argtypes = [P, C.c_int32, D]
     lib.em_ensemble_forward.argtypes = [P, C.c_int64, C.c_int32]
@@ -164,6 +178,19 @@ def load():
     lib.em_amr_counters.argtypes = [P, U64]
     lib.em_amr_time.argtypes = [P, D]
     lib.em_amr_synchronize.argtypes = [P]
+    lib.em_fiveeq_last_error.argtypes = []
+    lib.em_fiveeq_create.argtypes = [C.POINTER(P), I64, D, D, C.c_int32, D, C.c_int32, I32, D, C.c_int32, C.c_int32,
+                                     C.c_double, C.c_double, C.c_double, D]
+    lib.em_fiveeq_destroy.argtypes = [P]
+    lib.em_fiveeq_upload.argtypes = [P, D]
+    lib.em_fiveeq_download.argtypes = [P, D]
+    lib.em_fiveeq_rhs.argtypes = [P, C.c_double, D, U64]
+    lib.em_fiveeq_step.argtypes = [P, C.c_double]
+    lib.em_fiveeq_dt_cfl.argtypes = [P, D]
+    lib.em_fiveeq_run_to.argtypes = [P, C.c_double, I64]
+    lib.em_fiveeq_counters.argtypes = [P, U64]
+    lib.em_fiveeq_time.argtypes = [P, D]
+    lib.em_fiveeq_synchronize.argtypes = [P]
     for name in EXPORTS:
         getattr(lib, name).restype = C.c_char_p if name.endswith("last_error") else C.c_int
     _lib = lib
@@ -173,6 +200,12 @@ def load():
 def check(rc):
     if rc != 0:
         raise EmError(rc, load().em_last_error().decode())
+    return rc
+
+
+def check_fiveeq(rc):
+    if rc != 0:
+        raise EmError(rc, load().em_fiveeq_last_error().decode())
     return rc
 
 

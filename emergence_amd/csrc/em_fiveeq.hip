@@ -1,0 +1,940 @@
+// Two-phase Kapila five-equation system (FiveEq, seven variables) on a
+// uniform grid: the reference's GpuSolver with GpuSystem::FiveEq
+// (fluid/crates/compressible-cuda/src/lib.rs:50-82, 155-477), with the
+// arithmetic of the CPU Solver<7, FiveEq> (fluid/crates/compressible/src/
+// solver.rs, scheme.rs, multiphase.rs), which is its parity target.
+//
+// Kernels:
+//   fe_pass<D, MODE, T, P, NT>  the whole per-axis residual of a tile of T
+//       cells x P pencils in one launch, LDS-staged like the Euler pass:
+//       load + primitives (+ sensor), hybrid blend weight, WCNS midpoints
+//       (flux and contact speed, each once), faces (S6 combination, KEEP-PE
+//       central flux, blend, positivity limiter, face velocity), cells
+//       (divergence, the non-conservative (alpha + K) div u term of the
+//       volume fraction, fused SSP-RK3 stage on the last axis);
+//   k_fe_rate   CFL rate (sum of (|u_d| + c)/dx_d) and the dilatation-rate
+//       limit |Dp/Dt|/(p + p_inf,min) of every interior cell
+//       (solver.rs:478-523, multiphase.rs:373-391), wave-reduced into slots.
+// Built without FMA contraction: results are bit-identical to the oracle.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/em_hip.h"
+#include "em_fiveeq.h"
+#include "em_types.h"
+
+namespace em {
+namespace fe {
+
+static thread_local std::string g_err;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+static void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Error(EM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define FECHK(x) ::em::fe::hip_check((x), #x)
+
+// Device scalars of one FiveEq solver; slot accumulators on separate
+// 128-byte lines, reduced on read.
+struct alignas(256) FeScalars {
+    double dt;
+    int active;
+    int pad_;
+    unsigned long long rate[kSlots][16];
+    unsigned long long source[kSlots][16];
+    unsigned long long counters[3][kSlots][16];
+};
+
+struct FePassArgs {
+    const double* U;  // stage input (7 planes of ncell)
+    double* R;        // residual accumulator
+    const double* U0;
+    double* Uout;
+    FeScalars* sc;
+    int n[3];
+    long long stride[3];
+    long long ncell;
+    Sys sys;
+    int bc_kind[2];
+    double bc_state[2][7];
+    double inv_dx, ndim_pos, dx_d, tol;
+    int first, rk_stage;
+    int fast_axis, slow_axis, dfast, ntile, ngroup;
+    int stage_dt_host;
+    double stage_dt;
+};
+
+template <int MODE, int T, int P>
+struct FeLds {
+    static constexpr bool HYB = MODE == MODE_HYBRID;
+    double sw[7][T + 10][P];
+    double sm[8][T + 5][P];  // midpoints: flux + contact speed; then faces: flux + face velocity
+    double ss[3][HYB ? T + 10 : 1][P];
+    double sth[HYB ? T + 1 : 1][P];
+    unsigned int scnt[3];
+};
+
+template <int D, int MODE, int T, int P, int NT>
+__global__ __launch_bounds__(NT) void fe_pass(FePassArgs a) {
+    constexpr int C = T + 10;
+    constexpr int M = T + 5;
+    constexpr int NF = T + 1;
+    constexpr bool HYB = MODE == MODE_HYBRID;
+    constexpr bool CEN = MODE == MODE_CENTRAL;
+    constexpr int un = IU + D;
+    __shared__ FeLds<MODE, T, P> L;
+    auto& sw = L.sw;
+    auto& sm = L.sm;
+    auto& ss = L.ss;
+    auto& sth = L.sth;
+    if (a.sc->active == 0) return;
+    const int tid = threadIdx.x;
+    const int bid = blockIdx.x;
+    const int tile = bid % a.ntile;
+    const int rest = bid / a.ntile;
+    const int gF = rest % a.ngroup;
+    const int sI = rest / a.ngroup;
+    const int nD = a.n[D];
+    const int nF = a.n[a.fast_axis];
+    const int c0 = tile * T;
+    const int Tv = min(T, nD - c0);
+    const int p0 = gF * P;
+    const int Pv = min(P, nF - p0);
+    const long long sD = a.stride[D];
+    const long long sF = a.stride[a.fast_axis];
+    const long long base = (long long)sI * a.stride[a.slow_axis] + (long long)p0 * sF;
+    const long long nc = a.ncell;
+    const Sys& sys = a.sys;
+    if (tid < 3) L.scnt[tid] = 0;
+
+    // ---- 1. load + primitives (+ sensor ln rho, ln(p + p_inf,mix), alpha)
+    for (int it = tid; it < C * P; it += NT) {
+        int c, p;
+        if (a.dfast) {
+            c = it % C;
+            p = it / C;
+        } else {
+            p = it % P;
+            c = it / P;
+        }
+        if (p >= Pv || c >= Tv + 10) continue;
+        const int g = c0 - 5 + c;
+        const long long pen = base + (long long)p * sF;
+        double u[7], w[7];
+        if (g >= 0 && g < nD) {
+            const long long idx = pen + (long long)g * sD;
+#pragma unroll
+            for (int k = 0; k < 7; ++k) u[k] = a.U[k * nc + idx];
+        } else {
+            auto ld = [&](int x, double* o) {
+#pragma unroll
+                for (int k = 0; k < 7; ++k) o[k] = a.U[k * nc + pen + (long long)x * sD];
+            };
+            bc_ghost<D>(sys, nD, g, a.bc_kind, a.bc_state, ld, u);
+        }
+        to_prim(sys, u, w);
+#pragma unroll
+        for (int k = 0; k < 7; ++k) sw[k][c][p] = w[k];
+        if constexpr (HYB) {
+            // FiveEq::sensor, multiphase.rs:385-390
+            ss[0][c][p] = log(fmax(mix_rho(w), 1e-300));
+            ss[1][c][p] = log(fmax(w[IP] + p_inf_mix(sys, w), 1e-300));
+            ss[2][c][p] = w[IA];
+        }
+    }
+    __syncthreads();
+
+    double lam = 0.0;
+    if (a.ndim_pos > 0.0) {
+        const double dt = a.stage_dt_host ? a.stage_dt : a.sc->dt;
+        lam = a.ndim_pos * dt / a.dx_d;  // solver.rs:393
+    }
+    unsigned n_shock = 0, n_fb = 0, n_lim = 0;
+
+    // ---- 2. hybrid blend weights (scheme.rs:171-183)
+    if constexpr (HYB) {
+        for (int it = tid; it < NF * P; it += NT) {
+            const int p = it % P, f = it / P;
+            if (p >= Pv || f > Tv) continue;
+            double r = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                double q[6];
+#pragma unroll
+                for (int j = 0; j < 6; ++j) q[j] = ss[k][f + 2 + j][p];
+                r = fmax(fmax(r, d4f(q[0], q[1], q[2], q[3], q[4])), d4f(q[1], q[2], q[3], q[4], q[5]));
+            }
+            double x = (r / a.tol - 0.5) / 0.5;
+            if (x < 0.0) x = 0.0;
+            if (x > 1.0) x = 1.0;
+            sth[f][p] = isfinite(r) ? x * x * (3.0 - 2.0 * x) : 1.0;
+        }
+        __syncthreads();
+    }
+
+    // ---- 3. WCNS midpoints (midpoint m between tile cells m+2 and m+3)
+    if constexpr (!CEN) {
+        for (int it = tid; it < M * P; it += NT) {
+            const int p = it % P, m = it / P;
+            if (p >= Pv || m >= Tv + 5) continue;
+            if constexpr (HYB) {
+                bool need = false;
+                const int flo = max(0, m - 4), fhi = min(Tv, m);
+                for (int f = flo; f <= fhi; ++f) need = need || sth[f][p] > 0.0;
+                if (!need) continue;
+            }
+            double st[6][7];
+#pragma unroll
+            for (int j = 0; j < 6; ++j)
+#pragma unroll
+                for (int k = 0; k < 7; ++k) st[j][k] = sw[k][m + j][p];
+            double f[7], us;
+            const bool fb = midpoint_flux<D>(sys, st, f, us);
+#pragma unroll
+            for (int k = 0; k < 7; ++k) sm[k][m][p] = f[k];
+            sm[7][m][p] = us;
+            if (fb) {
+                const int j = c0 - 3 + m;  // global midpoint index, counted once per pencil
+                const bool own = (j >= c0 && j < c0 + Tv) || (j < 0 && c0 == 0) || (j >= nD && c0 + Tv == nD);
+                n_fb += own ? 1u : 0u;
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- 4. faces (face f between tile cells f+4 and f+5)
+    constexpr int FITER = (NF * P + NT - 1) / NT;
+    double FR[FITER][8];
+#pragma unroll
+    for (int fi = 0; fi < FITER; ++fi) {
+        const int it = tid + fi * NT;
+        const int p = it % P, f = it / P;
+        if (it >= NF * P || p >= Pv || f > Tv) continue;
+        const unsigned own_face = (f < Tv || c0 + Tv == nD) ? 1u : 0u;
+        double theta;
+        if constexpr (MODE == MODE_WENO)
+            theta = 1.0;
+        else if constexpr (CEN)
+            theta = 0.0;
+        else
+            theta = sth[f][p];
+        double fo[7], cen[7];
+        double uo = 0.0, uc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            fo[k] = 0.0;
+            cen[k] = 0.0;
+        }
+        if (theta < 1.0) {
+            // KEEP-PE 6th-order central flux and face velocity, scheme.rs:188-214
+            const double a6[3] = {0.75, -0.15, 1.0 / 60.0};
+#pragma unroll
+            for (int li = 0; li < 3; ++li) {
+                const int lag = li + 1;
+#pragma unroll
+                for (int mm = 0; mm < lag; ++mm) {
+                    double wa[7], wb[7], ft[7];
+#pragma unroll
+                    for (int k = 0; k < 7; ++k) {
+                        wa[k] = sw[k][f + 4 - mm][p];
+                        wb[k] = sw[k][f + 4 - mm + lag][p];
+                    }
+                    split_flux<D>(sys, wa, wb, ft);
+#pragma unroll
+                    for (int k = 0; k < 7; ++k) cen[k] += 2.0 * a6[li] * ft[k];
+                    uc += a6[li] * (wa[un] + wb[un]);
+                }
+            }
+        }
+        if (theta == 0.0) {
+#pragma unroll
+            for (int k = 0; k < 7; ++k) fo[k] = cen[k];
+            uo = uc;
+        } else {
+            n_shock += own_face;
+            // F = sum_s S6[s] (F_{i+s} + F_{i-s}), order s=0, i+1, i-1, i+2, i-2
+            const double s0 = 75.0 / 64.0 - 25.0 / 384.0 + 3.0 / 640.0;
+            const double s1 = -25.0 / 384.0 + 3.0 / 640.0;
+            const double s2 = 3.0 / 640.0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                double v = 0.0;
+                v += s0 * sm[k][f + 2][p];
+                v += s1 * sm[k][f + 3][p];
+                v += s1 * sm[k][f + 1][p];
+                v += s2 * sm[k][f + 4][p];
+                v += s2 * sm[k][f][p];
+                if (k < 7)
+                    fo[k] = v;
+                else
+                    uo = v;
+            }
+            if (theta < 1.0) {
+#pragma unroll
+                for (int k = 0; k < 7; ++k) fo[k] = theta * fo[k] + (1.0 - theta) * cen[k];
+                uo = theta * uo + (1.0 - theta) * uc;
+            }
+        }
+        if (lam > 0.0) {
+            // positivity limiter, scheme.rs:401-450
+            double wl[7], wr[7], ul[7], ur[7], fl[7], fr[7];
+#pragma unroll
+            for (int k = 0; k < 7; ++k) {
+                wl[k] = sw[k][f + 4][p];
+                wr[k] = sw[k][f + 5][p];
+            }
+            to_cons(sys, wl, ul);
+            to_cons(sys, wr, ur);
+            phys_flux<D>(wl, ul, fl);
+            phys_flux<D>(wr, ur, fr);
+            const double tl = 2.0 * lam;
+            auto ok = [&](const double* fx) {
+                double qa[7], qb[7], pa[7], pb[7];
+#pragma unroll
+                for (int k = 0; k < 7; ++k) {
+                    qa[k] = ul[k] - tl * (fx[k] - fl[k]);
+                    qb[k] = ur[k] + tl * (fx[k] - fr[k]);
+                }
+                to_prim(sys, qa, pa);
+                to_prim(sys, qb, pb);
+                return admissible(sys, pa) && admissible(sys, pb);
+            };
+            bool fin = true;
+#pragma unroll
+            for (int k = 0; k < 7; ++k) fin = fin && isfinite(fo[k]);
+            if (!(fin && ok(fo))) {
+                const double al = fmax(fabs(wl[un]) + sound_speed(sys, wl), fabs(wr[un]) + sound_speed(sys, wr));
+                double flf[7], fh[7];
+#pragma unroll
+                for (int k = 0; k < 7; ++k) {
+                    flf[k] = 0.5 * (fl[k] + fr[k]) - 0.5 * al * (ur[k] - ul[k]);
+                    fh[k] = fo[k];
+                }
+                n_lim += own_face;
+                double lo = 0.0, hi = 1.0;
+                for (int itb = 0; itb < 30; ++itb) {
+                    const double mid = 0.5 * (lo + hi);
+                    double bl[7];
+#pragma unroll
+                    for (int k = 0; k < 7; ++k) bl[k] = isfinite(fh[k]) ? mid * fh[k] + (1.0 - mid) * flf[k] : flf[k];
+                    if (ok(bl))
+                        lo = mid;
+                    else
+                        hi = mid;
+                }
+#pragma unroll
+                for (int k = 0; k < 7; ++k) fo[k] = isfinite(fh[k]) ? lo * fh[k] + (1.0 - lo) * flf[k] : flf[k];
+                const double um = 0.5 * (wl[un] + wr[un]);
+                uo = isfinite(uo) ? lo * uo + (1.0 - lo) * um : um;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 7; ++k) FR[fi][k] = fo[k];
+        FR[fi][7] = uo;
+    }
+    __syncthreads();  // every midpoint read is done: reuse sm for the faces
+#pragma unroll
+    for (int fi = 0; fi < FITER; ++fi) {
+        const int it = tid + fi * NT;
+        const int p = it % P, f = it / P;
+        if (it >= NF * P || p >= Pv || f > Tv) continue;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) sm[k][f][p] = FR[fi][k];
+    }
+    __syncthreads();
+
+    // ---- 5. cells: divergence, non-conservative term, fused RK stage
+    const double dt = a.sc->dt;
+    for (int it = tid; it < T * P; it += NT) {
+        int c, p;
+        if (a.dfast) {
+            c = it % T;
+            p = it / T;
+        } else {
+            p = it % P;
+            c = it / P;
+        }
+        if (p >= Pv || c >= Tv) continue;
+        const long long idx = base + (long long)p * sF + (long long)(c0 + c) * sD;
+        double r[7], w[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            const double prev = a.first ? 0.0 : a.R[k * nc + idx];
+            r[k] = prev - (sm[k][c + 1][p] - sm[k][c][p]) * a.inv_dx;
+            w[k] = sw[k][c + 5][p];
+        }
+        // solver.rs:440-442: + c(w_i) (U_{i+1/2} - U_{i-1/2}) / dx
+        r[IA] += nc_coef(sys, w) * (sm[7][c + 1][p] - sm[7][c][p]) * a.inv_dx;
+        if (a.rk_stage == 0) {
+#pragma unroll
+            for (int k = 0; k < 7; ++k) a.R[k * nc + idx] = r[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 7; ++k) {
+                const double u0 = a.U0[k * nc + idx];
+                double o;
+                if (a.rk_stage == 1) {
+                    o = u0 + dt * r[k];
+                } else {
+                    const double us = a.U[k * nc + idx];
+                    // solver.rs:113-116 and 716-726: dest = U^n, old = the stage
+                    if (a.rk_stage == 2)
+                        o = 0.75 * u0 + 0.25 * (us + dt * r[k]);
+                    else
+                        o = u0 / 3.0 + 2.0 / 3.0 * (us + dt * r[k]);
+                }
+                a.Uout[k * nc + idx] = o;
+            }
+        }
+    }
+
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        n_shock += __shfl_xor(n_shock, off);
+        n_fb += __shfl_xor(n_fb, off);
+        n_lim += __shfl_xor(n_lim, off);
+    }
+    if ((tid & 63) == 0) {
+        if (n_shock) atomicAdd(&L.scnt[0], n_shock);
+        if (n_fb) atomicAdd(&L.scnt[1], n_fb);
+        if (n_lim) atomicAdd(&L.scnt[2], n_lim);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const int slot = bid % kSlots;
+        for (int q = 0; q < 3; ++q)
+            if (L.scnt[q]) atomicAdd(&a.sc->counters[q][slot][0], (unsigned long long)L.scnt[q]);
+    }
+}
+
+// Conserved neighbour of interior cell x (pencil coordinate) along d through
+// the boundary condition when x is a boundary cell (solver.rs:592-640).
+template <int D, class LD>
+__device__ __forceinline__ void neighbor_conserved(const Sys& s, int n, int x, bool high, const int kind[2],
+                                                   const double (*state)[7], LD ld, double* out) {
+    const bool boundary = high ? x == n - 1 : x == 0;
+    if (!boundary) {
+        ld(high ? x + 1 : x - 1, out);
+        return;
+    }
+    const int side = high ? 1 : 0;
+    switch (kind[side]) {
+        case 0: ld(high ? 0 : n - 1, out); return;
+        case 1: ld(x, out); return;
+        case 3:
+#pragma unroll
+            for (int q = 0; q < 7; ++q) out[q] = state[side][q];
+            return;
+        case 2:
+            ld(x, out);
+            out[IU + D] *= -1.0;
+            return;
+        default: {
+            double ub[7], ub1[7];
+            ld(x, ub);
+            ld(high ? x - 1 : x + 1, ub1);
+            characteristic_ghost<D>(s, ub, ub1, state[side], side, 1.0, out);
+            return;
+        }
+    }
+}
+
+struct FeRateArgs {
+    const double* U;
+    FeScalars* sc;
+    long long ncell;
+    int n[3];
+    long long stride[3];
+    double dx[3];
+    int act[3];
+    Sys sys;
+    int bc_kind[3][2];
+    double bc_state[3][2][7];
+};
+
+template <int D>
+__device__ __forceinline__ double divu_axis(const FeRateArgs& a, long long cell, int x) {
+    auto ld = [&](int y, double* o) {
+        const long long q = cell + (long long)(y - x) * a.stride[D];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) o[k] = a.U[k * a.ncell + q];
+    };
+    double lo[7], hi[7];
+    neighbor_conserved<D>(a.sys, a.n[D], x, false, a.bc_kind[D], a.bc_state[D], ld, lo);
+    neighbor_conserved<D>(a.sys, a.n[D], x, true, a.bc_kind[D], a.bc_state[D], ld, hi);
+    const double va = lo[IU + D] / (lo[0] + lo[1]);  // velocity_conserved, multiphase.rs:158-161
+    const double vb = hi[IU + D] / (hi[0] + hi[1]);
+    return fabs(vb - va) / (2.0 * a.dx[D]);
+}
+
+// CFL rate and dilatation-rate limit of every interior cell (solver.rs:478-
+// 523): rate = max sum_d (|u_d| + c)/dx_d, source = max dilatation_rate(w,
+// sum_d |v_{+d} - v_{-d}| / (2 dx_d)); one atomicMax per workgroup per slot.
+__global__ __launch_bounds__(256) void k_fe_rate(FeRateArgs a) {
+    double rate = 0.0, source = 0.0;
+    for (long long cell = blockIdx.x * 256ll + threadIdx.x; cell < a.ncell; cell += (long long)gridDim.x * 256ll) {
+        double u[7], w[7];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) u[k] = a.U[k * a.ncell + cell];
+        to_prim(a.sys, u, w);
+        const double c = sound_speed(a.sys, w);
+        double ac = -0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            if (a.act[d]) ac += (fabs(w[IU + d]) + c) / a.dx[d];
+        rate = fmax(rate, ac);
+        const int x0 = (int)(cell / a.stride[0]);
+        const int x1 = (int)((cell / a.stride[1]) % a.n[1]);
+        const int x2 = (int)(cell % a.n[2]);
+        double divu = 0.0;
+        if (a.act[0]) divu += divu_axis<0>(a, cell, x0);
+        if (a.act[1]) divu += divu_axis<1>(a, cell, x1);
+        if (a.act[2]) divu += divu_axis<2>(a, cell, x2);
+        source = fmax(source, dilatation_rate(a.sys, w, divu));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        rate = fmax(rate, __shfl_xor(rate, off));
+        source = fmax(source, __shfl_xor(source, off));
+    }
+    __shared__ double sr[4], sq[4];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        sr[wid] = rate;
+        sq[wid] = source;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double rm = 0.0, qm = 0.0;
+        for (int i = 0; i < 4; ++i) {
+            rm = fmax(rm, sr[i]);
+            qm = fmax(qm, sq[i]);
+        }
+        rm = rm > 0.0 ? rm : 0.0;  // IEEE bits of +x order like x
+        qm = qm > 0.0 ? qm : 0.0;
+        const int slot = blockIdx.x % kSlots;
+        atomicMax(&a.sc->rate[slot][0], (unsigned long long)__double_as_longlong(rm));
+        atomicMax(&a.sc->source[slot][0], (unsigned long long)__double_as_longlong(qm));
+    }
+}
+
+__global__ void k_fe_zero_rate(FeScalars* sc) {
+    if (threadIdx.x < kSlots) {
+        sc->rate[threadIdx.x][0] = 0ull;
+        sc->source[threadIdx.x][0] = 0ull;
+    }
+}
+__global__ void k_fe_set_dt(FeScalars* sc, double dt) { sc->dt = dt; }
+
+// Tile shapes (cells along the axis x pencils, threads)
+struct FeTile {
+    int T, P, NT;
+};
+constexpr FeTile kFeTiles[] = {{27, 8, 256}, {59, 4, 256}};
+constexpr int kNumFeTiles = 2;
+
+template <int D, int MODE, int T, int P, int NT>
+static hipError_t launch_fe(const FePassArgs& a, hipStream_t st) {
+    const long long nblk = (long long)a.ntile * a.ngroup * a.n[a.slow_axis];
+    hipLaunchKernelGGL((fe_pass<D, MODE, T, P, NT>), dim3((unsigned)nblk), dim3(NT), 0, st, a);
+    return hipGetLastError();
+}
+template <int D, int MODE>
+static hipError_t launch_fe_cfg(int cfg, const FePassArgs& a, hipStream_t st) {
+    switch (cfg) {
+        case 0: return launch_fe<D, MODE, 27, 8, 256>(a, st);
+        case 1: return launch_fe<D, MODE, 59, 4, 256>(a, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+template <int D>
+static hipError_t launch_fe_mode(int mode, int cfg, const FePassArgs& a, hipStream_t st) {
+    switch (mode) {
+        case MODE_HYBRID: return launch_fe_cfg<D, MODE_HYBRID>(cfg, a, st);
+        case MODE_WENO: return launch_fe_cfg<D, MODE_WENO>(cfg, a, st);
+        case MODE_CENTRAL: return launch_fe_cfg<D, MODE_CENTRAL>(cfg, a, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+static hipError_t launch_fe_pass(int d, int mode, int cfg, const FePassArgs& a, hipStream_t st) {
+    switch (d) {
+        case 0: return launch_fe_mode<0>(mode, cfg, a, st);
+        case 1: return launch_fe_mode<1>(mode, cfg, a, st);
+        case 2: return launch_fe_mode<2>(mode, cfg, a, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// Host object: mesh, boundaries, options, device state
+struct FeSolver {
+    int64_t n[3];
+    double lo[3], dx[3];
+    int ng;
+    Sys sys;
+    int bc_kind[3][2];
+    double bc_state[3][2][7];
+    int mode, positivity;
+    double cfl, tol, t = 0.0;
+    int cfg[3] = {0, 0, 0};
+    long long ncell;
+    long long stride[3];
+    int nactive = 0, act[3];
+    hipStream_t stream = nullptr;
+    double* U[3] = {nullptr, nullptr, nullptr};
+    double* R = nullptr;
+    FeScalars* sc = nullptr;
+    uint64_t counters_host[3] = {0, 0, 0};
+
+    bool active(int d) const { return n[d] > 1; }
+    int m(int d) const { return active(d) ? (int)(n[d] + 2 * ng) : 1; }
+    int g(int d) const { return active(d) ? ng : 0; }
+
+    FeSolver(const int64_t nn[3], const double lo_[3], const double hi[3], int ng_, const double ph[8], int kapila,
+             const int32_t bk[6], const double bs[42], int mode_, int pos, double cfl_, double tol_, double t0) {
+        for (int d = 0; d < 3; ++d) {
+            if (nn[d] < 1 || nn[d] > (1 << 30)) throw Error(EM_ERR_ARG, "mesh extents must be in [1, 2^30]");
+            n[d] = nn[d];
+            lo[d] = lo_[d];
+            dx[d] = (hi[d] - lo_[d]) / (double)nn[d];  // Mesh::new, solver.rs:32
+        }
+        ng = ng_;
+        for (int k = 0; k < 2; ++k) sys.e[k] = Phase{ph[4 * k], ph[4 * k + 1], ph[4 * k + 2], ph[4 * k + 3]};
+        sys.kapila = kapila ? 1 : 0;
+        mode = mode_;
+        positivity = pos;
+        cfl = cfl_;
+        tol = tol_;
+        t = t0;
+        if (mode < 0 || mode > 2) throw Error(EM_ERR_ARG, "mode must be 0 (Hybrid), 1 (Weno) or 2 (Central)");
+        if (ng < (mode == MODE_CENTRAL ? 3 : 5))
+            throw Error(EM_ERR_ARG, "ghost layers: need ng >= 5 for Hybrid/Weno, >= 3 for Central");
+        for (int d = 0; d < 3; ++d) {
+            act[d] = active(d) ? 1 : 0;
+            nactive += act[d];
+            for (int sd = 0; sd < 2; ++sd) {
+                bc_kind[d][sd] = bk[2 * d + sd];
+                if (bc_kind[d][sd] < 0 || bc_kind[d][sd] > 4) throw Error(EM_ERR_ARG, "boundary kind out of range");
+                for (int q = 0; q < 7; ++q) bc_state[d][sd][q] = bs ? bs[(2 * d + sd) * 7 + q] : 0.0;
+            }
+        }
+        if (nactive == 0) throw Error(EM_ERR_ARG, "mesh has no active axis");
+        ncell = n[0] * n[1] * n[2];
+        stride[0] = n[1] * n[2];
+        stride[1] = n[2];
+        stride[2] = 1;
+        for (int d = 0; d < 3; ++d) cfg[d] = stride[d] == 1 ? 1 : 0;
+        FECHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        for (auto& b : U) FECHK(hipMalloc(&b, sizeof(double) * 7 * ncell));
+        FECHK(hipMalloc(&R, sizeof(double) * 7 * ncell));
+        FECHK(hipMalloc(&sc, sizeof(FeScalars)));
+        std::vector<FeScalars> h(1);
+        std::memset(h.data(), 0, sizeof(FeScalars));
+        h[0].active = 1;
+        FECHK(hipMemcpyAsync(sc, h.data(), sizeof(FeScalars), hipMemcpyHostToDevice, stream));
+        FECHK(hipStreamSynchronize(stream));
+    }
+    ~FeSolver() {
+        for (auto* b : U)
+            if (b) (void)hipFree(b);
+        if (R) (void)hipFree(R);
+        if (sc) (void)hipFree(sc);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void upload(const double* u) {
+        std::vector<double> h(7 * ncell);
+        const int m1 = m(1), m2 = m(2);
+        for (long long i = 0; i < n[0]; ++i)
+            for (long long j = 0; j < n[1]; ++j)
+                for (long long k = 0; k < n[2]; ++k) {
+                    const long long q = ((i + g(0)) * m1 + (j + g(1))) * m2 + (k + g(2));
+                    const long long c = (i * n[1] + j) * n[2] + k;
+                    for (int v = 0; v < 7; ++v) h[v * ncell + c] = u[q * 7 + v];
+                }
+        FECHK(hipMemcpyAsync(U[0], h.data(), sizeof(double) * 7 * ncell, hipMemcpyHostToDevice, stream));
+        FECHK(hipStreamSynchronize(stream));
+    }
+    // ghost-padded AoS with the interior filled and zero ghosts
+    void to_host(const double* dev, double* u) {
+        std::vector<double> h(7 * ncell);
+        FECHK(hipMemcpyAsync(h.data(), dev, sizeof(double) * 7 * ncell, hipMemcpyDeviceToHost, stream));
+        FECHK(hipStreamSynchronize(stream));
+        const long long L = (long long)m(0) * m(1) * m(2);
+        std::fill(u, u + 7 * L, 0.0);
+        const int m1 = m(1), m2 = m(2);
+        for (long long i = 0; i < n[0]; ++i)
+            for (long long j = 0; j < n[1]; ++j)
+                for (long long k = 0; k < n[2]; ++k) {
+                    const long long q = ((i + g(0)) * m1 + (j + g(1))) * m2 + (k + g(2));
+                    const long long c = (i * n[1] + j) * n[2] + k;
+                    for (int v = 0; v < 7; ++v) u[q * 7 + v] = h[v * ncell + c];
+                }
+    }
+
+    FePassArgs pass_args(int d) const {
+        FePassArgs a{};
+        a.sc = sc;
+        for (int e = 0; e < 3; ++e) {
+            a.n[e] = (int)n[e];
+            a.stride[e] = stride[e];
+        }
+        a.ncell = ncell;
+        a.sys = sys;
+        for (int sd = 0; sd < 2; ++sd) {
+            a.bc_kind[sd] = bc_kind[d][sd];
+            for (int q = 0; q < 7; ++q) a.bc_state[sd][q] = bc_state[d][sd][q];
+        }
+        a.inv_dx = 1.0 / dx[d];
+        a.dx_d = dx[d];
+        a.ndim_pos = positivity ? (double)nactive : 0.0;
+        a.tol = tol;
+        int tt[2], nt = 0;
+        for (int e = 0; e < 3; ++e)
+            if (e != d) tt[nt++] = e;
+        if (active(tt[1]) || !active(tt[0])) {
+            a.fast_axis = tt[1];
+            a.slow_axis = tt[0];
+        } else {
+            a.fast_axis = tt[0];
+            a.slow_axis = tt[1];
+        }
+        a.dfast = stride[d] == 1 ? 1 : 0;
+        const FeTile tc = kFeTiles[cfg[d]];
+        a.ntile = (int)((n[d] + tc.T - 1) / tc.T);
+        a.ngroup = (int)((n[a.fast_axis] + tc.P - 1) / tc.P);
+        const long long nblk = (long long)a.ntile * a.ngroup * a.n[a.slow_axis];
+        if (nblk > 0x7fffffffll) throw Error(EM_ERR_ARG, "mesh too large for one axis-pass launch");
+        return a;
+    }
+
+    // residual of Uin over the active axes in order; fused RK stage on the last
+    void stage(const double* Uin, double* Uout, int rk_stage, int stage_dt_host, double stage_dt) {
+        int last = -1;
+        for (int d = 0; d < 3; ++d)
+            if (active(d)) last = d;
+        bool first = true;
+        for (int d = 0; d < 3; ++d) {
+            if (!active(d)) continue;
+            FePassArgs a = pass_args(d);
+            a.U = Uin;
+            a.R = R;
+            a.U0 = U[0];
+            a.Uout = Uout;
+            a.first = first ? 1 : 0;
+            a.rk_stage = d == last ? rk_stage : 0;
+            a.stage_dt_host = stage_dt_host;
+            a.stage_dt = stage_dt;
+            FECHK(launch_fe_pass(d, mode, cfg[d], a, stream));
+            first = false;
+        }
+    }
+
+    void read_counters(uint64_t out[3]) {
+        std::vector<FeScalars> h(1);
+        FECHK(hipMemcpyAsync(h.data(), sc, sizeof(FeScalars), hipMemcpyDeviceToHost, stream));
+        FECHK(hipStreamSynchronize(stream));
+        for (int q = 0; q < 3; ++q) {
+            out[q] = 0;
+            for (int i = 0; i < kSlots; ++i) out[q] += h[0].counters[q][i][0];
+        }
+    }
+
+    uint64_t faces_per_stage() const {
+        uint64_t f = 0;
+        for (int d = 0; d < 3; ++d)
+            if (active(d)) f += (uint64_t)(n[d] + 1) * (ncell / n[d]);
+        return f;
+    }
+
+    void rhs(double stage_dt, double* out, uint64_t stats[4]) {
+        uint64_t before[3], after[3];
+        read_counters(before);
+        stage(U[0], nullptr, 0, 1, stage_dt);
+        read_counters(after);
+        to_host(R, out);
+        if (stats) {
+            stats[0] = faces_per_stage();
+            for (int q = 0; q < 3; ++q) stats[1 + q] = after[q] - before[q];
+        }
+    }
+
+    // SSP-RK3 step (solver.rs:659-738), dt from the host
+    void step(double dt) {
+        hipLaunchKernelGGL(k_fe_set_dt, dim3(1), dim3(1), 0, stream, sc, dt);
+        FECHK(hipGetLastError());
+        stage(U[0], U[1], 1, 0, 0.0);
+        stage(U[1], U[2], 2, 0, 0.0);
+        stage(U[2], U[0], 3, 0, 0.0);
+        FECHK(hipStreamSynchronize(stream));
+        t += dt;
+    }
+
+    double dt_cfl() {
+        hipLaunchKernelGGL(k_fe_zero_rate, dim3(1), dim3(kSlots), 0, stream, sc);
+        FECHK(hipGetLastError());
+        FeRateArgs a{};
+        a.U = U[0];
+        a.sc = sc;
+        a.ncell = ncell;
+        a.sys = sys;
+        for (int d = 0; d < 3; ++d) {
+            a.n[d] = (int)n[d];
+            a.stride[d] = stride[d];
+            a.dx[d] = dx[d];
+            a.act[d] = act[d];
+            for (int sd = 0; sd < 2; ++sd) {
+                a.bc_kind[d][sd] = bc_kind[d][sd];
+                for (int q = 0; q < 7; ++q) a.bc_state[d][sd][q] = bc_state[d][sd][q];
+            }
+        }
+        const long long nb = std::min<long long>((ncell + 255) / 256, 2048);
+        hipLaunchKernelGGL(k_fe_rate, dim3((unsigned)nb), dim3(256), 0, stream, a);
+        FECHK(hipGetLastError());
+        std::vector<FeScalars> h(1);
+        FECHK(hipMemcpyAsync(h.data(), sc, sizeof(FeScalars), hipMemcpyDeviceToHost, stream));
+        FECHK(hipStreamSynchronize(stream));
+        uint64_t rb = 0, qb = 0;
+        for (int i = 0; i < kSlots; ++i) {
+            rb = std::max<uint64_t>(rb, h[0].rate[i][0]);
+            qb = std::max<uint64_t>(qb, h[0].source[i][0]);
+        }
+        const double rate = __builtin_bit_cast(double, rb), source = __builtin_bit_cast(double, qb);
+        double dt = cfl / rate;  // solver.rs:514-519
+        if (source > 0.0) dt = std::fmin(dt, 0.5 / source);
+        return dt;
+    }
+
+    // solver.rs:758-768
+    int64_t run_to(double t_end) {
+        int64_t steps = 0;
+        while (t < t_end - 1e-10 * std::fabs(t_end)) {
+            const double dt = std::fmin(dt_cfl(), t_end - t);
+            if (!(std::isfinite(dt) && dt > 1e-12 * std::fmax(std::fabs(t_end), 1e-300)))
+                throw Error(EM_ERR_COLLAPSE, "time step collapsed");
+            step(dt);
+            ++steps;
+        }
+        return steps;
+    }
+};
+
+}  // namespace fe
+}  // namespace em
+
+// ------------------------------------------------------------------ C ABI
+
+struct em_fiveeq {
+    em::fe::FeSolver* s;
+};
+
+template <class F>
+static int fe_guarded(F&& f) {
+    try {
+        f();
+        em::fe::g_err.clear();
+        return EM_OK;
+    } catch (const em::fe::Error& e) {
+        em::fe::g_err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        em::fe::g_err = e.what();
+        return EM_ERR_STATE;
+    }
+}
+
+#define FENEED(h)                                \
+    if (!(h) || !(h)->s) {                       \
+        em::fe::g_err = "null FiveEq handle";    \
+        return EM_ERR_ARG;                       \
+    }
+
+extern "C" {
+
+const char* em_fiveeq_last_error(void) { return em::fe::g_err.c_str(); }
+
+int em_fiveeq_create(em_fiveeq** out, const int64_t n[3], const double lo[3], const double hi[3], int32_t ng,
+                     const double phases[8], int32_t kapila, const int32_t bc_kind[6], const double bc_state[42],
+                     int32_t mode, int32_t positivity, double cfl, double tol, double t0, const double* u) {
+    if (!out || !n || !lo || !hi || !phases || !bc_kind || !u) {
+        em::fe::g_err = "null argument";
+        return EM_ERR_ARG;
+    }
+    return fe_guarded([&] {
+        auto* s = new em::fe::FeSolver(n, lo, hi, ng, phases, kapila, bc_kind, bc_state, mode, positivity, cfl,
+                                       tol, t0);
+        try {
+            s->upload(u);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = new em_fiveeq{s};
+    });
+}
+
+int em_fiveeq_destroy(em_fiveeq* h) {
+    if (!h) return EM_OK;
+    delete h->s;
+    delete h;
+    return EM_OK;
+}
+
+int em_fiveeq_upload(em_fiveeq* h, const double* u) {
+    FENEED(h);
+    return fe_guarded([&] { h->s->upload(u); });
+}
+
+int em_fiveeq_download(em_fiveeq* h, double* u) {
+    FENEED(h);
+    return fe_guarded([&] { h->s->to_host(h->s->U[0], u); });
+}
+
+int em_fiveeq_rhs(em_fiveeq* h, double stage_dt, double* rhs, uint64_t stats[4]) {
+    FENEED(h);
+    return fe_guarded([&] { h->s->rhs(stage_dt, rhs, stats); });
+}
+
+int em_fiveeq_step(em_fiveeq* h, double dt) {
+    FENEED(h);
+    return fe_guarded([&] { h->s->step(dt); });
+}
+
+int em_fiveeq_dt_cfl(em_fiveeq* h, double* dt) {
+    FENEED(h);
+    return fe_guarded([&] { *dt = h->s->dt_cfl(); });
+}
+
+int em_fiveeq_run_to(em_fiveeq* h, double t_end, int64_t* steps) {
+    FENEED(h);
+    return fe_guarded([&] {
+        const int64_t k = h->s->run_to(t_end);
+        if (steps) *steps = k;
+    });
+}
+
+int em_fiveeq_counters(em_fiveeq* h, uint64_t out[3]) {
+    FENEED(h);
+    return fe_guarded([&] { h->s->read_counters(out); });
+}
+
+int em_fiveeq_time(em_fiveeq* h, double* t) {
+    FENEED(h);
+    *t = h->s->t;
+    return EM_OK;
+}
+
+int em_fiveeq_synchronize(em_fiveeq* h) {
+    FENEED(h);
+    return fe_guarded([&] { FECHK(hipStreamSynchronize(h->s->stream)); });
+}
+
+}  // extern "C"

@@ -73,6 +73,13 @@ __global__ void k_advance(DevScalars* sc) {
     sc->steps += 1;
 }
 
+// Records the step's dt at hist[steps] (the forward run's dt history, kept
+// on the device so that steps can be replayed as a graph).
+__global__ void k_record_dt(const DevScalars* sc, double* hist) {
+    if (sc->active == 0) return;
+    hist[sc->steps] = sc->dt;
+}
+
 // dt of the next step from the CFL rate of the current state.
 //   run_to == 0: dt = cfl / rate (fixed number of steps)
 //   run_to == 1: Solver::run_to rule (solver.rs:758-768): stop once
@@ -301,9 +308,10 @@ struct Solver : Grid {
     double* pad = nullptr;
     double* bcdev = nullptr;  // [3][2][5] boundary states for the ghost sweep
     DevScalars* sc = nullptr;
-    hipGraphExec_t graph[2] = {nullptr, nullptr};  // per run mode
-    int graph_variant[2] = {-1, -1};
-    int graph_cfg[2][3];
+    // per run mode: 0 run_steps, 1 run_to, 2 forward run recording dt
+    hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
+    int graph_variant[3] = {-1, -1, -1};
+    int graph_cfg[3][3];
     bool rate_valid = false;
 
     Solver(const int64_t nn[3], const double lo_[3], const double hi[3], int ng_, const double e[4],
@@ -549,28 +557,30 @@ struct Solver : Grid {
     }
 
     // One step driven by device-resident dt: dt kernel + stages + advance.
-    void device_step(int run_to) {
-        hipLaunchKernelGGL(k_dt_from_rate, dim3(1), dim3(1), 0, stream, sc, cfl, run_to);
+    // Mode 2 (forward run) also records dt in dt_hist[steps].
+    void device_step(int run_mode) {
+        hipLaunchKernelGGL(k_dt_from_rate, dim3(1), dim3(1), 0, stream, sc, cfl, run_mode == 1 ? 1 : 0);
+        if (run_mode == 2) hipLaunchKernelGGL(k_record_dt, dim3(1), dim3(1), 0, stream, sc, dt_hist);
         step_kernels();
     }
 
-    hipGraphExec_t get_graph(int run_to) {
-        bool same = graph[run_to] && graph_variant[run_to] == precise;
-        for (int d = 0; d < 3 && same; ++d) same = graph_cfg[run_to][d] == cfg[d];
-        if (same) return graph[run_to];
-        if (graph[run_to]) {
-            HIPCHK(hipGraphExecDestroy(graph[run_to]));
-            graph[run_to] = nullptr;
+    hipGraphExec_t get_graph(int run_mode) {
+        bool same = graph[run_mode] && graph_variant[run_mode] == precise;
+        for (int d = 0; d < 3 && same; ++d) same = graph_cfg[run_mode][d] == cfg[d];
+        if (same) return graph[run_mode];
+        if (graph[run_mode]) {
+            HIPCHK(hipGraphExecDestroy(graph[run_mode]));
+            graph[run_mode] = nullptr;
         }
         hipGraph_t gr;
         HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        device_step(run_to);
+        device_step(run_mode);
         HIPCHK(hipStreamEndCapture(stream, &gr));
-        HIPCHK(hipGraphInstantiate(&graph[run_to], gr, nullptr, nullptr, 0));
+        HIPCHK(hipGraphInstantiate(&graph[run_mode], gr, nullptr, nullptr, 0));
         HIPCHK(hipGraphDestroy(gr));
-        graph_variant[run_to] = precise;
-        for (int d = 0; d < 3; ++d) graph_cfg[run_to][d] = cfg[d];
-        return graph[run_to];
+        graph_variant[run_mode] = precise;
+        for (int d = 0; d < 3; ++d) graph_cfg[run_mode][d] = cfg[d];
+        return graph[run_mode];
     }
 
     int64_t run_to(double t_end) {
@@ -671,8 +681,11 @@ struct Solver : Grid {
     // ---------------------------------------------------------- adjoint
     // Checkpoints of the last forward run (every ck_every steps), its dt
     // sequence, and the device buffers of the backward sweep.
-    std::vector<double*> ckpt;
+    std::vector<double*> ckpt;  // kept across forward runs; the last run uses nck_used
+    int64_t nck_used = 0;
     std::vector<double> fwd_dts;
+    double* dt_hist = nullptr;  // device dt history of a CFL-driven forward run
+    int64_t dt_hist_cap = 0;
     int ck_every = 0;
     double* adjbuf = nullptr;  // A, B, Z, T: 4 x 5 ncell
     int adj_tile_cap = kAdjTileCells;  // interior cells per adjoint tile at most
@@ -684,6 +697,10 @@ struct Solver : Grid {
     void free_adjoint() {
         for (auto* p : ckpt) (void)hipFree(p);
         ckpt.clear();
+        nck_used = 0;
+        if (dt_hist) (void)hipFree(dt_hist);
+        dt_hist = nullptr;
+        dt_hist_cap = 0;
         if (adjbuf) (void)hipFree(adjbuf);
         if (store) (void)hipFree(store);
         if (store2) (void)hipFree(store2);
@@ -803,29 +820,55 @@ struct Solver : Grid {
     }
 
     // Forward run of nsteps SSP-RK3 steps with checkpoints every k steps;
-    // dt from dts or, if null, the CFL dt of each step (recorded).
+    // dt from dts or, if null, the CFL dt of each step, computed and recorded
+    // on the device (graph-replayed steps, no host wait per step).  Device
+    // buffers of earlier runs (checkpoints, dt history, final state) are
+    // reused, so a repeated run of the same length allocates nothing.
     void forward(int64_t nsteps, int k, const double* dts, double* dts_out) {
         if (nsteps < 1 || k < 1) throw Error(EM_ERR_ARG, "nsteps and checkpoint_every must be >= 1");
         adj_check();
-        free_adjoint();
+        if (dts)
+            for (int64_t s = 0; s < nsteps; ++s)
+                if (!(std::isfinite(dts[s]) && dts[s] > 0.0)) throw Error(EM_ERR_COLLAPSE, "time step collapsed");
         ck_every = k;
         fwd_dts.assign((size_t)nsteps, 0.0);
         const int64_t nck = (nsteps + k - 1) / k;
-        for (int64_t c = 0; c < nck; ++c) {
+        while ((int64_t)ckpt.size() < nck) {
             double* p;
             HIPCHK(hipMalloc(&p, sizeof(double) * 5 * ncell));
             ckpt.push_back(p);
         }
-        for (int64_t s = 0; s < nsteps; ++s) {
-            if (s % k == 0)
-                HIPCHK(hipMemcpyAsync(ckpt[s / k], U[0], sizeof(double) * 5 * ncell, hipMemcpyDeviceToDevice, stream));
-            const double dt = dts ? dts[s] : dt_cfl();
-            if (!(std::isfinite(dt) && dt > 0.0)) throw Error(EM_ERR_COLLAPSE, "time step collapsed");
-            fwd_dts[s] = dt;
-            step_async(dt);  // dt_cfl waits for the stream before the next step
+        nck_used = nck;
+        if (!final_state) HIPCHK(hipMalloc(&final_state, sizeof(double) * 5 * ncell));
+        const size_t sz = sizeof(double) * 5 * ncell;
+        if (dts) {
+            for (int64_t s = 0; s < nsteps; ++s) {
+                if (s % k == 0) HIPCHK(hipMemcpyAsync(ckpt[s / k], U[0], sz, hipMemcpyDeviceToDevice, stream));
+                fwd_dts[s] = dts[s];
+                step_async(dts[s]);
+            }
+        } else {
+            if (dt_hist_cap < nsteps) {
+                if (dt_hist) HIPCHK(hipFree(dt_hist));
+                HIPCHK(hipMalloc(&dt_hist, sizeof(double) * nsteps));
+                dt_hist_cap = nsteps;
+                if (graph[2]) {  // the captured step records into the old buffer
+                    HIPCHK(hipGraphExecDestroy(graph[2]));
+                    graph[2] = nullptr;
+                }
+            }
+            reset_run(0.0);
+            compute_rate();
+            hipGraphExec_t gx = get_graph(2);
+            for (int64_t s = 0; s < nsteps; ++s) {
+                if (s % k == 0) HIPCHK(hipMemcpyAsync(ckpt[s / k], U[0], sz, hipMemcpyDeviceToDevice, stream));
+                HIPCHK(hipGraphLaunch(gx, stream));
+            }
+            HIPCHK(hipMemcpyAsync(fwd_dts.data(), dt_hist, sizeof(double) * nsteps, hipMemcpyDeviceToHost, stream));
+            Scalars h = read_scalars();  // waits for the stream
+            if (h.err & 1) throw Error(EM_ERR_COLLAPSE, "time step collapsed");
         }
-        HIPCHK(hipMalloc(&final_state, sizeof(double) * 5 * ncell));
-        HIPCHK(hipMemcpyAsync(final_state, U[0], sizeof(double) * 5 * ncell, hipMemcpyDeviceToDevice, stream));
+        HIPCHK(hipMemcpyAsync(final_state, U[0], sz, hipMemcpyDeviceToDevice, stream));
         HIPCHK(hipStreamSynchronize(stream));
         rate_valid = false;
         if (dts_out) std::copy(fwd_dts.begin(), fwd_dts.end(), dts_out);
@@ -835,7 +878,7 @@ struct Solver : Grid {
     // are recomputed from its checkpoint (u_n and both RK stage states of
     // every step kept), then the steps are reversed.
     void backward(const double* dj_final, double* dj_initial) {
-        if (ckpt.empty()) throw Error(EM_ERR_STATE, "backward needs a forward run with checkpoints first");
+        if (nck_used == 0) throw Error(EM_ERR_STATE, "backward needs a forward run with checkpoints first");
         ensure_adjbuf();
         const long long nv = 5 * ncell;
         const size_t sz = sizeof(double) * nv;
@@ -892,7 +935,7 @@ struct Solver : Grid {
             }
             HIPCHK(hipEventRecord(ev_rec[seg & 1], rstream));
         };
-        const int64_t last = (int64_t)ckpt.size() - 1;
+        const int64_t last = nck_used - 1;
         recompute(last);
         for (int64_t seg = last; seg >= 0; --seg) {
             if (store2 && seg > 0) {
@@ -1004,11 +1047,14 @@ struct Ensemble {
     // adjoint
     int ck_every = 0;
     long long fwd_steps = 0;
-    std::vector<double*> ckpt;
+    std::vector<double*> ckpt;  // kept across forward runs; the last run uses nck_used
+    int64_t nck_used = 0;
     double* dt_hist = nullptr;
+    long long dt_hist_cap = 0;
     std::vector<double> h_dts;
     double* work = nullptr;   // A, B, Z, T
     double* store = nullptr;  // 3 states per step of a segment
+    int store_steps = 0;
     double* fin = nullptr;    // final states
     static constexpr int kSweepStreams = 4;
     hipStream_t sweep[kSweepStreams] = {};  // instance i's VJPs on sweep[i % 4] (default priority)
@@ -1080,11 +1126,14 @@ struct Ensemble {
     void free_adjoint() {
         for (auto* p : ckpt) (void)hipFree(p);
         ckpt.clear();
+        nck_used = 0;
         for (double** p : {&dt_hist, &work, &store, &fin})
             if (*p) {
                 (void)hipFree(*p);
                 *p = nullptr;
             }
+        dt_hist_cap = 0;
+        store_steps = 0;
     }
 
     // pass arguments of every instance for stage s (1..3) and axis d, as
@@ -1221,11 +1270,13 @@ struct Ensemble {
         }
     }
 
-    // one step: dt (CFL, recorded) or given dt from the history, 3 stages
-    void step_kernels(bool cfl_dt, long long given_step, hipStream_t strm = nullptr) {
+    // one step: dt (CFL; recorded in dt_hist when `record`) or the given
+    // step's dt from the history, 3 stages
+    void step_kernels(bool cfl_dt, long long given_step, hipStream_t strm = nullptr, bool record = false) {
         hipStream_t stream = strm ? strm : this->stream;
         if (cfl_dt)
-            hipLaunchKernelGGL(k_ens_dt, dim3(small_grid()), dim3(64), 0, stream, d_scs, n, cfl, dt_hist, d_step);
+            hipLaunchKernelGGL(k_ens_dt, dim3(small_grid()), dim3(64), 0, stream, d_scs, n, cfl,
+                               record ? dt_hist : nullptr, d_step);
         else
             hipLaunchKernelGGL(k_ens_set_dt, dim3(small_grid()), dim3(64), 0, stream, d_scs, n, dt_hist, given_step);
         stage_launches(1, stream);
@@ -1302,28 +1353,35 @@ struct Ensemble {
         return t;
     }
 
-    // forward with checkpoints every k steps; CFL dt per instance recorded
+    // forward with checkpoints every k steps; CFL dt per instance recorded.
+    // Device buffers of earlier runs are reused (a repeated run of the same
+    // length allocates nothing).
     void forward(int64_t nsteps, int k) {
         if (nsteps < 1 || k < 1) throw Error(EM_ERR_ARG, "nsteps and checkpoint_every must be >= 1");
         for (auto* S : in) S->adj_check();
-        free_adjoint();
         ck_every = k;
         fwd_steps = nsteps;
-        HIPCHK(hipMalloc(&dt_hist, sizeof(double) * nsteps * n));
+        if (dt_hist_cap < nsteps * n) {
+            if (dt_hist) HIPCHK(hipFree(dt_hist));
+            dt_hist = nullptr;
+                HIPCHK(hipMalloc(&dt_hist, sizeof(double) * nsteps * n));
+            dt_hist_cap = nsteps * n;
+        }
         const int64_t nck = (nsteps + k - 1) / k;
-        for (int64_t c = 0; c < nck; ++c) {
+        while ((int64_t)ckpt.size() < nck) {
             double* p;
             HIPCHK(hipMalloc(&p, sizeof(double) * tot_nv));
             ckpt.push_back(p);
         }
+        nck_used = nck;
         start_run();
         for (int64_t s0 = 0; s0 < nsteps; s0 += k) {
             copy_all(gather_tasks(0, ckpt[s0 / k]));
             const int64_t s1 = std::min<int64_t>(s0 + k, nsteps);
-            for (int64_t q = s0; q < s1; ++q) step_kernels(true, 0);
+            for (int64_t q = s0; q < s1; ++q) step_kernels(true, 0, nullptr, true);
         }
         check_err();
-        HIPCHK(hipMalloc(&fin, sizeof(double) * tot_nv));
+        if (!fin) HIPCHK(hipMalloc(&fin, sizeof(double) * tot_nv));
         copy_all(gather_tasks(0, fin));
         h_dts.resize((size_t)nsteps * n);
         HIPCHK(hipMemcpyAsync(h_dts.data(), dt_hist, sizeof(double) * nsteps * n, hipMemcpyDeviceToHost, stream));
@@ -1333,9 +1391,14 @@ struct Ensemble {
     // dJ/du_0 of every instance from dJ/du_N (host arrays in each
     // instance's CPU layout)
     void backward(const double* const* dj_final, double* const* dj_initial) {
-        if (ckpt.empty()) throw Error(EM_ERR_STATE, "backward needs a forward run first");
+        if (nck_used == 0) throw Error(EM_ERR_STATE, "backward needs a forward run first");
         if (!work) HIPCHK(hipMalloc(&work, sizeof(double) * 4 * tot_nv));
-        if (!store) HIPCHK(hipMalloc(&store, sizeof(double) * 3 * tot_nv * ck_every));
+        if (store_steps < ck_every) {
+            if (store) HIPCHK(hipFree(store));
+            store = nullptr;
+            HIPCHK(hipMalloc(&store, sizeof(double) * 3 * tot_nv * ck_every));
+            store_steps = ck_every;
+        }
         std::vector<std::vector<DevScalars>> saved;
         for (auto* S : in) saved.push_back(S->save_scalars());
         // work arrays: A (the cotangent of the state after the step being
@@ -1343,7 +1406,7 @@ struct Ensemble {
         double* W[4] = {work, work + tot_nv, work + 2 * tot_nv, work + 3 * tot_nv};
         for (int i = 0; i < n; ++i) in[i]->host_to_soa(dj_final[i], W[0] + off[i]);
         const int64_t nsteps = fwd_steps;
-        const int64_t nseg = (int64_t)ckpt.size();
+        const int64_t nseg = nck_used;
         const int k = ck_every;
         // copy tables: per step slot j of a segment the gathers U^n -> slot
         // 3j, U1 -> 3j + 1, U2 -> 3j + 2 ([j][3][n]), then per segment the
@@ -1890,6 +1953,15 @@ int em_ensemble_set_tiles(em_ensemble* h, int32_t cfg) {
 int em_ensemble_run_steps(em_ensemble* h, int64_t nsteps) {
     ENEED(h);
     return guarded([&] { h->e->run_steps(nsteps); });
+}
+
+int em_ensemble_upload(em_ensemble* h, int32_t i, const double* u) {
+    ENEED(h);
+    if (i < 0 || i >= h->e->n || !u) {
+        em::g_last_error = "instance index out of range or null buffer";
+        return EM_ERR_ARG;
+    }
+    return guarded([&] { h->e->in[i]->upload(u); });
 }
 
 int em_ensemble_download(em_ensemble* h, int32_t i, double* u) {

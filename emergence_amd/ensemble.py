@@ -117,6 +117,10 @@ class HipEnsemble:
     def run_steps(self, nsteps):
         _lib.check(self.lib.em_ensemble_run_steps(self.h, int(nsteps)))
 
+    def upload(self, i, u):
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(padded_shape(self.sizes[i], self.ng))
+        _lib.check(self.lib.em_ensemble_upload(self.h, int(i), _dp(u)))
+
     def download(self, i):
         u = np.empty(padded_shape(self.sizes[i], self.ng))
         _lib.check(self.lib.em_ensemble_download(self.h, int(i), _dp(u)))

@@ -188,6 +188,8 @@ int em_ensemble_autotune(em_ensemble* h, int32_t repeats, int32_t* cfg_out);
 /* nsteps CFL steps of every instance (each its own dt). */
 int em_ensemble_run_steps(em_ensemble* h, int64_t nsteps);
 
+/* Replace / read instance i's conserved state (its CPU layout). */
+int em_ensemble_upload(em_ensemble* h, int32_t i, const double* u);
 int em_ensemble_download(em_ensemble* h, int32_t i, double* u);
 int em_ensemble_time(em_ensemble* h, int32_t i, double* t);
 
@@ -360,6 +362,45 @@ int em_amr_counters(em_amr* h, uint64_t out[3]);
 int em_amr_time(em_amr* h, double* t);
 
 int em_amr_synchronize(em_amr* h);
+
+/* ------------------------------------------------------------------------
+ * Two-phase Kapila five-equation system on a uniform grid: drop-in for
+ *   GpuSolver<f64> built with GpuSystem::FiveEq { eos, kapila }
+ *   (fluid/crates/compressible-cuda/src/lib.rs:50-82, 155-477),
+ * with the arithmetic of the CPU Solver<7, FiveEq>
+ * (fluid/crates/compressible/src/multiphase.rs), its parity target.
+ * Conserved variables (a1, a2, rho u, rho v, rho w, E, alpha1) with
+ * a_k = alpha_k rho_k; host arrays are ghost-padded AoS with 7 values per
+ * cell, same cell order as the Euler solver.  Same error convention; the
+ * messages come from em_fiveeq_last_error(). */
+typedef struct em_fiveeq em_fiveeq;
+
+const char* em_fiveeq_last_error(void);
+
+/* GpuSolver::from_cpu for FiveEq (lib.rs:186-282).
+ *   phases[8]     gamma, p_inf, b, q of phase 1 then phase 2 (Nasg, eos.rs)
+ *   kapila        1: Kapila's K term in the volume-fraction equation
+ *                 (multiphase.rs:92-104), 0: Allaire's model
+ *   bc_state[42]  [axis][side][7] conserved Dirichlet / far-field states
+ * other arguments as em_solver_create; u has 7 values per cell. */
+int em_fiveeq_create(em_fiveeq** out, const int64_t n[3], const double lo[3], const double hi[3], int32_t ng,
+                     const double phases[8], int32_t kapila, const int32_t bc_kind[6], const double bc_state[42],
+                     int32_t mode, int32_t positivity, double cfl, double tol, double t0, const double* u);
+int em_fiveeq_destroy(em_fiveeq* h);
+/* upload / download the conserved state (download leaves the ghosts zero) */
+int em_fiveeq_upload(em_fiveeq* h, const double* u);
+int em_fiveeq_download(em_fiveeq* h, double* u);
+/* Per-stage residual of the current state, including the non-conservative
+ * (alpha1 + K) div u term (solver.rs:350-475); stats as em_solver_rhs. */
+int em_fiveeq_rhs(em_fiveeq* h, double stage_dt, double* rhs, uint64_t stats[4]);
+/* GpuSolver::step / dt_cfl / run_to: dt_cfl includes the dilatation-rate
+ * limit of the stiff non-conservative term (solver.rs:478-523). */
+int em_fiveeq_step(em_fiveeq* h, double dt);
+int em_fiveeq_dt_cfl(em_fiveeq* h, double* dt);
+int em_fiveeq_run_to(em_fiveeq* h, double t_end, int64_t* steps);
+int em_fiveeq_counters(em_fiveeq* h, uint64_t out[3]);
+int em_fiveeq_time(em_fiveeq* h, double* t);
+int em_fiveeq_synchronize(em_fiveeq* h);
 
 #ifdef __cplusplus
 }
