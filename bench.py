@@ -72,6 +72,10 @@ def parse():
     ap.add_argument("--adj-fd-cells", type=int, default=64, help="central-difference checks (seeded cells)")
     ap.add_argument("--adj-fd-modes", type=int, default=16, help="central-difference checks along density modes")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble workload")
+    ap.add_argument("--no-fiveeq", action="store_true", help="skip the two-phase FiveEq workload")
+    ap.add_argument("--fiveeq-only", action="store_true", help="only the FiveEq workload (profiling)")
+    ap.add_argument("--fe-n", type=int, default=256)
+    ap.add_argument("--fe-steps", type=int, default=10)
     ap.add_argument("--ensemble-only", action="store_true", help="only the ensemble workload (profiling)")
     ap.add_argument("--ens-n", type=int, default=256, help="instances")
     ap.add_argument("--ens-steps", type=int, default=300)
@@ -514,6 +518,44 @@ def cpu_baseline(em, n, u, repeats):
     return rec
 
 
+def fiveeq_bench(em, args):
+    """The reference's `interface` probe at n^3 (examples/cpu_validate.rs:100-
+    146): a water drop (stiffened gas) in air moving at 0.1 through p = 1e5,
+    Kapila five-equation model, Hybrid mode, 3-D periodic; fixed dt = half the
+    CFL dt of the initial state, steps timed after a warm-up step."""
+    n = args.fe_n
+    x = (np.arange(n) + 0.5) / n
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    r = np.sqrt((X - 0.5) ** 2 + (Y - 0.5) ** 2 + (Z - 0.5) ** 2)
+    a = np.clip(0.5 * (1.0 + np.tanh((0.25 - r) * n)), 1e-6, 1.0 - 1e-6)
+    (g1, pi1, _, _), (g2, pi2, _, _) = em.fiveeq.WATER_AIR
+    a1, a2, vx = a * 1000.0, (1.0 - a) * 1.2, 0.1
+    rho = a1 + a2
+    e = 1e5 * (a / (g1 - 1.0) + (1.0 - a) / (g2 - 1.0)) + g1 * pi1 * a / (g1 - 1.0) + 0.5 * rho * vx * vx
+    u = np.zeros((n + 12, n + 12, n + 12, 7))
+    u[6:-6, 6:-6, 6:-6] = np.stack([a1, a2, rho * vx, 0 * a, 0 * a, e, a], axis=-1)
+    g = em.HipFiveEq((n, n, n), (0, 0, 0), (1, 1, 1), u, mode="hybrid")
+    dt = 0.5 * g.dt_cfl()
+    g.step(dt)
+    g.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.fe_steps):
+        g.step(dt)
+    g.synchronize()
+    el = time.perf_counter() - t0
+    shock = g.counters()[0] / (3 * (args.fe_steps + 1) * 3 * (n + 1) * n * n)
+    return {
+        "workload": f"3-D periodic {n}^3 water drop in air, Kapila five-equation model (7 variables), Hybrid, "
+                    "the reference's interface probe at this size",
+        "value": n ** 3 * args.fe_steps * 3 / el,
+        "unit": "cell-updates/s per RK stage",
+        "ms_per_step": el / args.fe_steps * 1e3,
+        "steps": args.fe_steps,
+        "dt": dt,
+        "shock_path_face_fraction": shock,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -530,6 +572,9 @@ def main():
         return
     if args.ensemble_only:
         print(json.dumps({"ensemble": ensemble_bench(em, args)}), flush=True)
+        return
+    if args.fiveeq_only:
+        print(json.dumps({"fiveeq": fiveeq_bench(em, args)}), flush=True)
         return
     n = args.n
     lo, hi = (0.0, 0.0, 0.0), (2 * np.pi,) * 3
@@ -596,6 +641,8 @@ def main():
         rec["adjoint_forward_cost_ratio"] = rec["adjoint"]["cost_ratio"]
     if not args.no_ensemble:
         rec["ensemble"] = ensemble_bench(em, args)
+    if not args.no_fiveeq:
+        rec["fiveeq"] = fiveeq_bench(em, args)
     print(json.dumps(rec), flush=True)
 
 

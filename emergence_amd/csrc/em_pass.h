@@ -8,10 +8,12 @@
 namespace em {
 
 // Tile shapes (cells along the axis x pencils, threads); index = cfg id.
-// Shapes 4..8 give exactly one midpoint per thread ((T+5)*P == NT).
+// Shapes 4..9 give exactly one midpoint per thread ((T+5)*P == NT); shape 9
+// puts 16 pencils in a workgroup, so that a strided pass reads whole 128-byte
+// lines (16 neighbouring cells of the innermost axis).
 constexpr TileCfg kTileCfgs[] = {{32, 16, 640}, {64, 8, 576}, {128, 4, 576}, {256, 2, 576}, {27, 8, 256},
-                                 {59, 4, 256},  {59, 8, 512}, {27, 16, 512}, {123, 4, 512}};
-constexpr int kNumTileCfgs = 9;
+                                 {59, 4, 256},  {59, 8, 512}, {27, 16, 512}, {123, 4, 512}, {59, 16, 1024}};
+constexpr int kNumTileCfgs = 10;
 
 #define EM_DECLARE_LAUNCHERS                                                               \
     hipError_t launch_pass_hybrid(int d, int cfg, const PassArgs& a, hipStream_t s);      \

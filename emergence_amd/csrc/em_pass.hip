@@ -181,14 +181,16 @@ __device__ __forceinline__ void axis_tile(const PassArgs& a, int bid, TileLds<MO
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
                 const double u0 = a.U0[k * nc + idx];
-                const double us = a.U[k * nc + idx];
                 double o;
-                if (a.rk_stage == 1)
-                    o = u0 + dt * r[k];
-                else if (a.rk_stage == 2)
-                    o = 0.75 * u0 + 0.25 * (us + dt * r[k]);
-                else
-                    o = u0 / 3.0 + 2.0 / 3.0 * (us + dt * r[k]);
+                if (a.rk_stage == 1) {
+                    o = u0 + dt * r[k];  // the stage input is U^n itself: no second load
+                } else {
+                    const double us = a.U[k * nc + idx];
+                    if (a.rk_stage == 2)
+                        o = 0.75 * u0 + 0.25 * (us + dt * r[k]);
+                    else
+                        o = u0 / 3.0 + 2.0 / 3.0 * (us + dt * r[k]);
+                }
                 un_[k] = o;
                 a.Uout[k * nc + idx] = o;
             }
@@ -289,6 +291,7 @@ static hipError_t launch_cfg(int cfg, const PassArgs& a, hipStream_t s) {
         case 6: return launch_one<D, MODE, 59, 8, 512>(a, s);
         case 7: return launch_one<D, MODE, 27, 16, 512>(a, s);
         case 8: return launch_one<D, MODE, 123, 4, 512>(a, s);
+        case 9: return launch_one<D, MODE, 59, 16, 1024>(a, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -305,6 +308,7 @@ static hipError_t launch_ens_cfg(int cfg, const EnsPassArgs& e, unsigned grid, h
         case 6: return launch_ens_one<D, MODE, 59, 8, 512>(e, grid, s);
         case 7: return launch_ens_one<D, MODE, 27, 16, 512>(e, grid, s);
         case 8: return launch_ens_one<D, MODE, 123, 4, 512>(e, grid, s);
+        case 9: return launch_ens_one<D, MODE, 59, 16, 1024>(e, grid, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -125,7 +125,7 @@ int em_solver_synchronize(em_solver* s);
  * only, periodic / outflow / reflect (n >= 5) / Dirichlet boundaries, axes of
  * any length (pencils are cut into tiles of at most 128 cells with their
  * halo; an axis periodic on one side only keeps whole pencils in LDS, up to
- * 562 cells).  Conventions: branches
+ * 544 cells: longer ones are refused with EM_ERR_ARG).  Conventions: branches
  * follow the forward run, min/max differentiate the returned operand (the
  * first on a tie), d|x|/dx = sign(x) with sign(0) = 0; the positivity
  * limiter's blend weight is differentiated implicitly at the admissibility

@@ -388,6 +388,36 @@ def test_characteristic_and_dirichlet_boundaries(em):
     assert sg == so
 
 
+def test_nested_characteristic_corners(em):
+    """Characteristic boundaries on two axes meeting at a refined corner
+    (compressible-cuda/src/kernels.cu:722-786, amr.rs:450-504): the corner
+    ancestor's restriction patch and the corner leaves' ghosts reach past both
+    boundaries, whose images nest (the y image of an x image); residual,
+    steps with regrids, trees and data bit-identical to the oracle."""
+    print("amr nested characteristic corners", flush=True)
+    far = (1.0, 0.0, 0.0, 0.0, 2.5)
+    bc = [[("characteristic", far), "outflow"], [("characteristic", far), ("characteristic", far)],
+          ["periodic", "periodic"]]
+    # a Mach 1.5 shock at x = 0.6 and a light bubble in the (0, 0) corner
+    prm = [1.5, 0.6, 0.1, 0.1, 0.0, 0.18, 0.2]
+    o = OracleAmr("shock_bubble", prm, (4, 4, 0), 8, 2, (0, 0, 0), (1, 1, 1), eps=1e-2, bc=bc, mode="weno",
+                  regrid_every=2)
+    lv, idx, leaf = o.blocks()
+    assert ((lv == 0) & ~leaf & (idx[:, 0] == 0) & (idx[:, 1] == 0)).sum() == 1  # the corner is refined
+    g = _from_oracle(em, o, bc=bc, mode="weno", eps=1e-2, regrid_every=2)
+    rg, sg = g.rhs(1e-3)
+    ro, so = o.compute_rhs(1e-3)
+    assert np.array_equal(rg, ro)
+    assert sg == so
+    for _ in range(3):
+        dt = o.dt_cfl()
+        assert g.dt_cfl() == dt
+        o.step(dt)
+        g.step(dt)
+    _same_tree(g, o)
+    assert np.array_equal(g.download(), o.get_u())
+
+
 def test_pass_shapes_and_autotune_give_identical_results(em):
     # Every compiled shape of the batched flux pass (pencils per work item,
     # threads) gives the same bits: a cell's flux differences and their

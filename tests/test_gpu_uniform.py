@@ -142,7 +142,7 @@ def test_rhs_3d_precise_bit_identical_all_tiles(em):
     n = (33, 20, 45)
     g, o, _ = _pair(em, 1, n, (0, 0, 0), (2 * np.pi,) * 3, "weno")
     ro, so = o.rhs(1e-3)
-    for cfg in range(9):
+    for cfg in range(10):
         g.set_tiles([cfg, cfg, cfg])
         rg, sg = g.rhs(1e-3)
         assert np.array_equal(rg, ro), cfg
