@@ -41,7 +41,12 @@ CFL = 0.5
 # Algorithmic FP64 work of the WCNS (Weno-mode) residual, counted from the
 # reference formulas with sparse eigenvectors, div/sqrt counted as 1 FLOP:
 # 3,500 FLOP per cell per RK stage in 3-D, i.e. 3,500/3 per cell per axis pass.
+# tile shapes of the axis pass (T cells, P pencils, threads) by cfg id, as
+# the library's table
+TILE_SHAPES = [(32, 16, 640), (64, 8, 576), (128, 4, 576), (256, 2, 576), (27, 8, 256), (59, 4, 256), (59, 8, 512),
+               (27, 16, 512), (123, 4, 512), (59, 16, 1024)]
 FLOP_PER_CELL_STAGE = 3500.0
+FLOP_PER_CELL_STAGE_CENTRAL = 1600.0  # KEEP-PE central flux on every face, 3-D
 BYTES_PER_CELL_STAGE = 120.0  # read U^(s), read U^n, write U^(s+1) (5 x f64 each)
 PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
@@ -60,7 +65,7 @@ def parse():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shapes (short profiled runs)")
     ap.add_argument("--no-amr", action="store_true", help="skip the block-AMR workload")
     ap.add_argument("--amr-only", action="store_true", help="only the block-AMR workload (profiling)")
-    ap.add_argument("--amr-steps", type=int, default=20)
+    ap.add_argument("--amr-steps", type=int, default=100)
     ap.add_argument("--amr-per-block-steps", type=int, default=3)
     ap.add_argument("--amr-no-graph", action="store_true",
                     help="batched AMR steps launched eagerly, not replayed as graphs (for counter collection)")
@@ -71,8 +76,10 @@ def parse():
     ap.add_argument("--adj-checkpoint", type=int, default=20)
     ap.add_argument("--adj-fd-cells", type=int, default=64, help="central-difference checks (seeded cells)")
     ap.add_argument("--adj-fd-modes", type=int, default=16, help="central-difference checks along density modes")
+    ap.add_argument("--adj-fd-sweep", type=int, default=16, help="cells also checked at h = 1e-5 and 1e-6 rho")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble workload")
     ap.add_argument("--no-fiveeq", action="store_true", help="skip the two-phase FiveEq workload")
+    ap.add_argument("--no-hybrid", action="store_true", help="skip the Hybrid-mode 256^3 line")
     ap.add_argument("--fiveeq-only", action="store_true", help="only the FiveEq workload (profiling)")
     ap.add_argument("--fe-n", type=int, default=256)
     ap.add_argument("--fe-steps", type=int, default=10)
@@ -296,13 +303,20 @@ def adjoint_bench(em, args):
     rng = np.random.default_rng(0x5EED)
     support = np.argwhere(np.abs(grad_rho) >= 1e-2 * scale)
     cells = support[rng.choice(len(support), size=min(args.adj_fd_cells, len(support)), replace=False)]
-    errs = []
-    for i, j in cells:
+    errs, best = [], []
+    for q, (i, j) in enumerate(cells):
         du = np.zeros_like(u0)
         du[6 + i, 6 + j, 0] = dirn[i, j]
-        fd = jfd(du, 1e-7 * q0[i, j, 0])
         ad = float(grad_rho[i, j])
+        fd = jfd(du, 1e-7 * q0[i, j, 0])
         errs.append(abs(fd - ad) / abs(ad))
+        if q < args.adj_fd_sweep:
+            # the same cell at h = 1e-5 and 1e-6 rho: truncation falls like h^2
+            # and the rounding of J over h grows like 1/h, so the best h
+            # shows the adjoint's agreement (tests/test_oracle_adjoint.py
+            # holds the full step-size study on a reduced case)
+            e = [errs[-1]] + [abs(jfd(du, h * q0[i, j, 0]) - ad) / abs(ad) for h in (1e-5, 1e-6)]
+            best.append(min(e))
     # (2) along each seeded density mode (at fixed velocity and pressure):
     # dJ/da_m for rho = rho_q (1 + 0.01 sum_m a_m cos(k_m . x + phi_m)),
     # a sum over the whole support, with h = 1e-5 in a_m
@@ -334,7 +348,10 @@ def adjoint_bench(em, args):
         "grad_max_abs": scale,
         "fd_cells": {"cells": len(errs), "h": "1e-7 rho", "drawn_from": "|dJ/drho_0| >= 1e-2 max",
                      "max_rel_err": max(errs) if errs else None,
-                     "median_rel_err": float(np.median(errs)) if errs else None},
+                     "median_rel_err": float(np.median(errs)) if errs else None,
+                     "best_of_h": {"cells": len(best), "h": "1e-5, 1e-6, 1e-7 rho",
+                                   "max_rel_err": max(best) if best else None,
+                                   "median_rel_err": float(np.median(best)) if best else None}},
         "fd_modes": {"modes": len(merr), "h": 1e-5, "dJ_da": mgrads, "normwise_rel_err": mnorm,
                      "median_rel_err": float(np.median(merr)) if merr else None,
                      "max_rel_err": max(merr) if merr else None, "rel_err_floor": "1e-2 x max |dJ/da|"},
@@ -374,17 +391,25 @@ def amr_bench(em, args):
         del per
     bat.synchronize()
     leaves0 = bat.leaf_count()
+    c0 = bat.counters()[0]
     t0 = time.perf_counter()
     bat.run_steps(args.amr_steps)
     bat.synchronize()
     el = time.perf_counter() - t0
     leaves1 = bat.leaf_count()
+    b = c["b"]
+    faces = 0.5 * (leaves0 + leaves1) * 3 * (b + 1) * b * b * 3 * args.amr_steps
+    share = (bat.counters()[0] - c0) / faces
     cells = 0.5 * (leaves0 + leaves1) * c["b"] ** 3
     bat.set_profile(True)
     bat.run_steps(16)
     phase = [v / 16 for v in bat.profile()]
     value = cells * args.amr_steps * 3 / el
     value_per = cells_per * k * 3 / el_per if k > 0 else None
+    # algorithmic FLOPs of the flux phase (shock-path share at the Weno
+    # figure, the rest central) over its measured time per step
+    flop = share * FLOP_PER_CELL_STAGE + (1.0 - share) * FLOP_PER_CELL_STAGE_CENTRAL
+    achieved = cells * 3 * flop / (phase[2] * 1e-3)
     cpu = None if args.no_cpu_baseline else amr_cpu_baseline(nleaves * c["b"] ** 3, args.cpu_repeats)
     return {
         "workload": "3-D shock-bubble block AMR, 8^3 blocks, 3 levels, Hybrid, CFL 0.4, regrid every 4",
@@ -399,6 +424,10 @@ def amr_bench(em, args):
         "speedup_vs_per_block": value / value_per if k > 0 else None,
         "identical_to_per_block": same,
         "phase_ms_per_step": dict(zip(["restrict", "ghosts", "fluxes", "reflux_update", "regrid"], phase)),
+        "shock_path_face_share": share,
+        "roofline": {"bound": "mfma", "roof": "fp64-valu", "kernel": "amr_pass (flux phase, 3 axes)",
+                     "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64, "flop_per_cell_stage": flop},
         "cpu_baseline": cpu,
     }
 
@@ -426,17 +455,19 @@ def amr_cpu_baseline(cells, repeats):
     return rec
 
 
-def profiled_traffic(variant, d):
-    """HBM bytes per launch of the axis pass along d, from the committed
-    rocprofv3 summary (FETCH_SIZE and WRITE_SIZE runs scaled by a plain-copy
-    calibration); None when no summary for this variant exists."""
+def profiled_traffic(variant, d, shape):
+    """HBM bytes per launch of the axis pass along d with tile shape
+    (T, P, threads), from the committed rocprofv3 summary (FETCH_SIZE and
+    WRITE_SIZE runs scaled by a plain-copy calibration); None when the
+    summary has no row for this kernel."""
     import re
 
     path = os.path.join(ROOT, "profiles", f"axis_pass_256cubed_weno_{variant}.md")
     if not os.path.exists(path):
         return None
+    sig = f"axis_pass<{d}, 1, {shape[0]}, {shape[1]}, {shape[2]}>"
     for line in open(path):
-        if f"axis_pass<{d}," in line and "GB |" in line:
+        if sig in line and "GB |" in line:
             m = re.search(r"\| ([0-9.]+) GB \|\s*$", line.strip())
             if m:
                 return float(m.group(1)) * 1e9
@@ -556,6 +587,41 @@ def fiveeq_bench(em, args):
     }
 
 
+def hybrid_bench(em, args, n, u):
+    """The same 256^3 workload in the reference's default mode (Hybrid: the
+    KEEP-PE central flux on smooth faces, WCNS where the sensor flags, a blend
+    between).  Algorithmic FLOPs per cell and stage: the shock-path share s
+    of the faces at the Weno figure, the rest at the central one."""
+    lo, hi = (0.0, 0.0, 0.0), (2 * np.pi,) * 3
+    g = em.HipSolver((n, n, n), lo, hi, u, mode="hybrid", cfl=CFL, precise=not args.fast)
+    tiles = None if args.no_autotune else g.autotune(3)
+    g.upload(u)
+    g.run_steps(args.warmup)
+    c0 = g.counters()[0]
+    g.synchronize()
+    t0 = time.perf_counter()
+    pass_ms = g.run_steps(args.steps, timed=True)
+    g.synchronize()
+    el = time.perf_counter() - t0
+    share = (g.counters()[0] - c0) / (args.steps * 3 * 3 * (n + 1) * n * n)
+    value = n ** 3 * args.steps * 3 / el
+    flop = share * FLOP_PER_CELL_STAGE + (1.0 - share) * FLOP_PER_CELL_STAGE_CENTRAL
+    dom = int(np.argmax(pass_ms))
+    achieved = flop / 3.0 * n ** 3 / (pass_ms[dom] * 1e-3)
+    return {
+        "workload": f"3-D periodic {n}^3 (same state), FluxMode::Hybrid (tol 1e-3), CFL {CFL}",
+        "value": value,
+        "unit": "cell-updates/s per RK stage",
+        "ms_per_step": el / args.steps * 1e3,
+        "tiles": tiles,
+        "pass_ms": pass_ms,
+        "shock_path_face_share": share,
+        "roofline": {"bound": "mfma", "roof": "fp64-valu", "kernel": f"axis_pass d={dom}", "achieved": achieved / 1e12,
+                     "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
+                     "flop_per_cell_stage": flop, "hbm_fraction": value * BYTES_PER_CELL_STAGE / PEAK_HBM},
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -626,7 +692,7 @@ def main():
             "peak": PEAK_FP64 / 1e12,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64,
-            "traffic": profiled_traffic("fast" if args.fast else "precise", dom),
+            "traffic": profiled_traffic("fast" if args.fast else "precise", dom, TILE_SHAPES[(tiles or [6, 6, 6])[dom]]),
             "hbm_fraction": value * BYTES_PER_CELL_STAGE / PEAK_HBM,
             "fp64_fraction": value * FLOP_PER_CELL_STAGE / PEAK_FP64,
         },
@@ -634,6 +700,8 @@ def main():
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline(em, n, u, args.cpu_repeats)
     del g
+    if not args.no_hybrid:
+        rec["hybrid"] = hybrid_bench(em, args, n, u)
     if not args.no_amr:
         rec["amr"] = amr_bench(em, args)
     if not args.no_adjoint:
