@@ -28,6 +28,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_WAVES_PER
     constexpr int E0 = D == 0 ? 1 : 0;
     constexpr int E1 = D == 2 ? 1 : 2;
     __shared__ double sw[5][C][P];
+    __shared__ double sre[C][P];  // rho_e of each cell
     __shared__ double sm[5][M][P];
     __shared__ double ss[2][CS][P];
     __shared__ double sth[FS][P];
@@ -96,6 +97,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_WAVES_PER
             to_prim(e, u, wv);
 #pragma unroll
             for (int q = 0; q < 5; ++q) sw[q][c][p] = wv[q];
+            sre[c][p] = eos_rho_e(e, wv[0], wv[4]);
             if constexpr (HYB) {
                 ss[0][c][p] = log(fmax(wv[0], 1e-300));
                 ss[1][c][p] = log(fmax(wv[4] + e.p_inf, 1e-300));
@@ -103,7 +105,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_WAVES_PER
         }
         __syncthreads();
         const double lam = pa.ndim_pos > 0.0 ? pa.ndim_pos * dtv / dx : 0.0;  // amr.rs:1254
-        tile_faces<D, MODE, B, P, NT>(sw, sm, ss, sth, B, Pv, 0, B, e, lam, pa.tol, n_shock, n_fb, n_lim);
+        tile_faces<D, MODE, B, P, NT>(sw, sre, sm, ss, sth, B, Pv, 0, B, e, lam, pa.tol, n_shock, n_fb, n_lim);
         // ---- residual and boundary face fluxes
         for (int it = tid; it < B * P; it += NT) {
             int c, p;

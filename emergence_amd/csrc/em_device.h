@@ -64,6 +64,18 @@ __device__ __forceinline__ void to_cons(const Eos& e, const double w[5], double 
     u[3] = rho * w[3];
     u[4] = eos_rho_e(e, rho, w[4]) + ke;
 }
+// to_cons with the cell's rho_e = eos_rho_e(e, w[0], w[4]) already at hand
+// (the same expressions, system.rs:210-217): a cell's rho_e is computed once
+// per tile and reused by every face that needs its conserved state.
+__device__ __forceinline__ void to_cons_re(const double w[5], double re, double u[5]) {
+    double rho = w[0];
+    double ke = 0.5 * rho * (w[1] * w[1] + w[2] * w[2] + w[3] * w[3]);
+    u[0] = rho;
+    u[1] = rho * w[1];
+    u[2] = rho * w[2];
+    u[3] = rho * w[3];
+    u[4] = re + ke;
+}
 // Physical flux from (w, q = to_cons(w)): system.rs:336-348 and the closure
 // of riemann_flux (system.rs:268-279) compute the same expressions.
 template <int D>
@@ -138,6 +150,25 @@ __device__ __forceinline__ void split_flux(const Eos& e, const double a[5], cons
     f[un] += 0.5 * (a[4] + b[4]);
     double kin = c * 0.5 * (a[1] * b[1] + a[2] * b[2] + a[3] * b[3]);
     double rhoe = 0.5 * (eos_rho_e(e, a[0], a[4]) + eos_rho_e(e, b[0], b[4]));
+    double pw = 0.5 * (a[un] * b[4] + b[un] * a[4]);
+    f[4] = kin + u * rhoe + pw;
+}
+
+// split_flux with the two cells' rho_e given (each computed once per tile)
+template <int D>
+__device__ __forceinline__ void split_flux_re(const double a[5], const double b[5], double rea, double reb,
+                                              double f[5]) {
+    constexpr int un = 1 + D;
+    double rho = 0.5 * (a[0] + b[0]);
+    double u = 0.5 * (a[un] + b[un]);
+    double c = rho * u;
+    f[0] = c;
+    f[1] = c * 0.5 * (a[1] + b[1]);
+    f[2] = c * 0.5 * (a[2] + b[2]);
+    f[3] = c * 0.5 * (a[3] + b[3]);
+    f[un] += 0.5 * (a[4] + b[4]);
+    double kin = c * 0.5 * (a[1] * b[1] + a[2] * b[2] + a[3] * b[3]);
+    double rhoe = 0.5 * (rea + reb);
     double pw = 0.5 * (a[un] * b[4] + b[un] * a[4]);
     f[4] = kin + u * rhoe + pw;
 }

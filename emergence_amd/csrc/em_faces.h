@@ -6,9 +6,11 @@
 // AMR pass so both evaluate identical arithmetic.
 //
 // On entry (after a barrier): sw[k][c][p] holds the primitive state of the
-// tile cells c = 0 .. Tv+9 (cell c0-5+c of the pencil), ss the hybrid
-// sensor.  On exit (after a barrier): sm[k][f][p] holds the flux of face f =
-// 0 .. Tv (between cells c0+f-1 and c0+f).  Counters of midpoints and faces
+// tile cells c = 0 .. Tv+9 (cell c0-5+c of the pencil), sre[c][p] its
+// rho_e = eos_rho_e(rho, p) (used by the central pair fluxes and the
+// limiter's conserved states), ss the hybrid sensor.  On exit (after a
+// barrier): sm[k][f][p] holds the flux of face f = 0 .. Tv (between cells
+// c0+f-1 and c0+f).  Counters of midpoints and faces
 // owned by the tile (tiles of one pencil share their halo) are accumulated.
 #pragma once
 #include "em_device.h"
@@ -18,7 +20,8 @@ namespace em {
 namespace EM_VARIANT {
 
 template <int D, int MODE, int T, int P, int NT>
-__device__ __forceinline__ void tile_faces(double (&sw)[5][T + 10][P], double (&sm)[5][T + 5][P],
+__device__ __forceinline__ void tile_faces(double (&sw)[5][T + 10][P], double (&sre)[T + 10][P],
+                                           double (&sm)[5][T + 5][P],
                                            double (&ss)[2][MODE == MODE_HYBRID ? T + 10 : 1][P],
                                            double (&sth)[MODE == MODE_HYBRID ? T + 1 : 1][P], int Tv, int Pv,
                                            int c0, int nD, const Eos& e, double lam, double tol,
@@ -112,7 +115,7 @@ __device__ __forceinline__ void tile_faces(double (&sw)[5][T + 10][P], double (&
                         wa[k] = sw[k][f + 4 - mm][p];
                         wb[k] = sw[k][f + 4 - mm + lag][p];
                     }
-                    split_flux<D>(e, wa, wb, ft);
+                    split_flux_re<D>(wa, wb, sre[f + 4 - mm][p], sre[f + 4 - mm + lag][p], ft);
 #pragma unroll
                     for (int k = 0; k < 5; ++k) cen[k] += 2.0 * a6[li] * ft[k];
                 }
@@ -147,8 +150,8 @@ __device__ __forceinline__ void tile_faces(double (&sw)[5][T + 10][P], double (&
                 wl[k] = sw[k][f + 4][p];
                 wr[k] = sw[k][f + 5][p];
             }
-            to_cons(e, wl, ul);
-            to_cons(e, wr, ur);
+            to_cons_re(wl, sre[f + 4][p], ul);
+            to_cons_re(wr, sre[f + 5][p], ur);
             phys_flux<D>(wl, ul, fl);
             phys_flux<D>(wr, ur, fr);
             const double tl = 2.0 * lam;

@@ -74,6 +74,7 @@ template <int MODE, int T, int P, int NT>
 struct TileLds {
     static constexpr bool HYB = MODE == MODE_HYBRID;
     double sw[5][T + 10][P];
+    double sre[T + 10][P];  // rho_e of each cell
     double sm[5][T + 5][P];
     double ss[2][HYB ? T + 10 : 1][P];
     double sth[HYB ? T + 1 : 1][P];
@@ -88,6 +89,7 @@ __device__ __forceinline__ void axis_tile(const PassArgs& a, int bid, TileLds<MO
     constexpr int C = T + 10;  // cells held in LDS
     constexpr bool HYB = MODE == MODE_HYBRID;
     auto& sw = L.sw;
+    auto& sre = L.sre;
     auto& sm = L.sm;
     auto& ss = L.ss;
     auto& sth = L.sth;
@@ -137,6 +139,7 @@ __device__ __forceinline__ void axis_tile(const PassArgs& a, int bid, TileLds<MO
         to_prim(e, u, w);
 #pragma unroll
         for (int k = 0; k < 5; ++k) sw[k][c][p] = w[k];
+        sre[c][p] = eos_rho_e(e, w[0], w[4]);
         if constexpr (HYB) {
             // Euler::sensor, system.rs:409-413
             ss[0][c][p] = log(fmax(w[0], 1e-300));
@@ -151,7 +154,7 @@ __device__ __forceinline__ void axis_tile(const PassArgs& a, int bid, TileLds<MO
         const double dt = a.stage_dt_host ? a.stage_dt : a.sc->dt;
         lam = a.ndim_pos * dt / a.dx_d;  // solver.rs:393
     }
-    tile_faces<D, MODE, T, P, NT>(sw, sm, ss, sth, Tv, Pv, c0, nD, e, lam, a.tol, n_shock, n_fb, n_lim);
+    tile_faces<D, MODE, T, P, NT>(sw, sre, sm, ss, sth, Tv, Pv, c0, nD, e, lam, a.tol, n_shock, n_fb, n_lim);
 
     // ---- 5. divergence, fused RK stage and CFL rate
     double rate = 0.0;
