@@ -63,6 +63,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-repeats", type=int, default=4, help="CPU-baseline repeats (median)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shapes (short profiled runs)")
+    ap.add_argument("--tiles", type=str, default=None,
+                    help="fixed Weno tile shapes per axis, e.g. 6,6,5 (indices into TILE_SHAPES; no autotuning)")
+    ap.add_argument("--hybrid-tiles", type=str, default=None, help="fixed Hybrid tile shapes per axis")
     ap.add_argument("--no-amr", action="store_true", help="skip the block-AMR workload")
     ap.add_argument("--amr-only", action="store_true", help="only the block-AMR workload (profiling)")
     ap.add_argument("--amr-steps", type=int, default=100)
@@ -566,6 +569,7 @@ def fiveeq_bench(em, args):
     u = np.zeros((n + 12, n + 12, n + 12, 7))
     u[6:-6, 6:-6, 6:-6] = np.stack([a1, a2, rho * vx, 0 * a, 0 * a, e, a], axis=-1)
     g = em.HipFiveEq((n, n, n), (0, 0, 0), (1, 1, 1), u, mode="hybrid")
+    tiles = None if args.no_autotune else g.autotune(3)
     dt = 0.5 * g.dt_cfl()
     g.step(dt)
     g.synchronize()
@@ -582,9 +586,20 @@ def fiveeq_bench(em, args):
         "unit": "cell-updates/s per RK stage",
         "ms_per_step": el / args.fe_steps * 1e3,
         "steps": args.fe_steps,
+        "tiles": tiles,
         "dt": dt,
         "shock_path_face_fraction": shock,
     }
+
+
+def pick_tiles(g, args, fixed):
+    """Tile shape per axis: the given indices, the defaults, or the
+    autotuner's pick (a few timed passes of every shape)."""
+    if fixed:
+        tiles = [int(v) for v in fixed.split(",")]
+        g.set_tiles(tiles)
+        return tiles
+    return None if args.no_autotune else g.autotune(3)
 
 
 def hybrid_bench(em, args, n, u):
@@ -594,7 +609,7 @@ def hybrid_bench(em, args, n, u):
     of the faces at the Weno figure, the rest at the central one."""
     lo, hi = (0.0, 0.0, 0.0), (2 * np.pi,) * 3
     g = em.HipSolver((n, n, n), lo, hi, u, mode="hybrid", cfl=CFL, precise=not args.fast)
-    tiles = None if args.no_autotune else g.autotune(3)
+    tiles = pick_tiles(g, args, args.hybrid_tiles)
     g.upload(u)
     g.run_steps(args.warmup)
     c0 = g.counters()[0]
@@ -646,7 +661,7 @@ def main():
     lo, hi = (0.0, 0.0, 0.0), (2 * np.pi,) * 3
     u = em.case_init(1, (n, n, n), lo, hi, 6, [0.3, float(0x5EED)])
     g = em.HipSolver((n, n, n), lo, hi, u, mode="weno", cfl=CFL, precise=not args.fast)
-    tiles = None if args.no_autotune else g.autotune(3)
+    tiles = pick_tiles(g, args, args.tiles)
     g.upload(u)
     if args.warmup:
         g.run_steps(args.warmup)

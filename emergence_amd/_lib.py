@@ -80,6 +80,8 @@ EXPORTS = (
     "em_fiveeq_step",
     "em_fiveeq_dt_cfl",
     "em_fiveeq_run_to",
+    "em_fiveeq_set_tiles",
+    "em_fiveeq_autotune",
     "em_fiveeq_counters",
     "em_fiveeq_time",
     "em_fiveeq_synchronize",
@@ -188,6 +190,8 @@ def load():
     lib.em_fiveeq_step.argtypes = [P, C.c_double]
     lib.em_fiveeq_dt_cfl.argtypes = [P, D]
     lib.em_fiveeq_run_to.argtypes = [P, C.c_double, I64]
+    lib.em_fiveeq_set_tiles.argtypes = [P, I32]
+    lib.em_fiveeq_autotune.argtypes = [P, C.c_int32, I32]
     lib.em_fiveeq_counters.argtypes = [P, U64]
     lib.em_fiveeq_time.argtypes = [P, D]
     lib.em_fiveeq_synchronize.argtypes = [P]

@@ -132,6 +132,17 @@ __device__ __forceinline__ void to_cons(const Sys& s, const double* w, double* u
     u[5] = rho_e(s, w) + 0.5 * r * (w[2] * w[2] + w[3] * w[3] + w[4] * w[4]);
     u[6] = w[IA];
 }
+// to_cons with rho e(w) already evaluated (the same value rho_e returns)
+__device__ __forceinline__ void to_cons_re(const double* w, double re, double* u) {
+    const double r = w[0] + w[1];
+    u[0] = w[0];
+    u[1] = w[1];
+    u[2] = r * w[2];
+    u[3] = r * w[3];
+    u[4] = r * w[4];
+    u[5] = re + 0.5 * r * (w[2] * w[2] + w[3] * w[3] + w[4] * w[4]);
+    u[6] = w[IA];
+}
 // 319-331, from w and q = to_cons(w)
 template <int D>
 __device__ __forceinline__ void phys_flux(const double* w, const double* q, double* f) {
@@ -215,9 +226,9 @@ __device__ __forceinline__ void hllc(const Sys& s, const double* wl, const doubl
     us = ss;
 }
 
-// KEEP-PE pair flux: 312-331
+// KEEP-PE pair flux: 312-331, with rho e of the two states given
 template <int D>
-__device__ __forceinline__ void split_flux(const Sys& s, const double* a, const double* b, double* f) {
+__device__ __forceinline__ void split_flux_re(const double* a, const double* b, double rea, double reb, double* f) {
     constexpr int un = IU + D;
     const double u = 0.5 * (a[un] + b[un]);
     const double rho = 0.5 * (mix_rho(a) + mix_rho(b));
@@ -228,7 +239,7 @@ __device__ __forceinline__ void split_flux(const Sys& s, const double* a, const 
     for (int k = IU; k < IU + 3; ++k) f[k] = c * 0.5 * (a[k] + b[k]);
     f[un] += 0.5 * (a[IP] + b[IP]);
     const double kin = c * 0.5 * (a[2] * b[2] + a[3] * b[3] + a[4] * b[4]);
-    const double rhoe = 0.5 * (rho_e(s, a) + rho_e(s, b));
+    const double rhoe = 0.5 * (rea + reb);
     const double pw = 0.5 * (a[un] * b[IP] + b[un] * a[IP]);
     f[5] = kin + u * rhoe + pw;
     f[6] = 0.5 * (a[IA] + b[IA]) * u;
@@ -260,18 +271,22 @@ __device__ __forceinline__ double weno5z(double v0, double v1, double v2, double
     return (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
 }
 
-// WCNS midpoint j+1/2 from the six primitive states st[0..5] = w_{j-2..j+3}:
-// scheme.rs:286-333 with FiveEq::eigen (multiphase.rs:219-256) in sparse
-// form.  Returns true on the first-order fallback; f = HLLC flux, us its
-// contact speed.
-template <int D>
-__device__ __forceinline__ bool midpoint_flux(const Sys& s, const double (&st)[6][7], double* f, double& us) {
+// WCNS midpoint j+1/2 from the six primitive states w_{j-2..j+3}, read
+// through ld(j, k) (component k of stencil state j, j = 0..5): scheme.rs:
+// 286-333 with FiveEq::eigen (multiphase.rs:219-256) in sparse form.
+// Returns true on the first-order fallback; f = HLLC flux, us its contact
+// speed.  The characteristic fields are taken one at a time, each stencil
+// value read when it is needed and each interpolated value folded into the
+// primitive states at once (in the reference's column order), so that only
+// one field's stencil is live: this keeps the pass within 256 registers.
+template <int D, class LD>
+__device__ __forceinline__ bool midpoint_flux(const Sys& s, LD&& ld, double* f, double& us) {
     constexpr int un = IU + D;
     constexpr int t1 = IU + (D + 1) % 3;
     constexpr int t2 = IU + (D + 2) % 3;
     double avg[7];
 #pragma unroll
-    for (int k = 0; k < 7; ++k) avg[k] = 0.5 * (st[2][k] + st[3][k]);
+    for (int k = 0; k < 7; ++k) avg[k] = 0.5 * (ld(2, k) + ld(3, k));
     const double rho = mix_rho(avg);
     const double c = sound_speed(s, avg);
     const double kk = kapila_k(s, avg);
@@ -284,52 +299,77 @@ __device__ __forceinline__ bool midpoint_flux(const Sys& s, const double (&st)[6
     const double l5u = rho / (2.0 * c);
     const double l5p = 1.0 / (2.0 * c2);
     const double l6p = kk / rho / c2;
-    double ch[6][7];
-#pragma unroll
-    for (int j = 0; j < 6; ++j) {
-        // dense order: sum over columns k = 0..6 from 0.0, zero terms skipped
-        ch[j][0] = 0.0 + 1.0 * st[j][0] + l0p * st[j][IP];
-        ch[j][1] = 0.0 + 1.0 * st[j][1] + l1p * st[j][IP];
-        ch[j][2] = 0.0 + l2u * st[j][un] + l2p * st[j][IP];
-        ch[j][3] = 0.0 + 1.0 * st[j][t1];
-        ch[j][4] = 0.0 + 1.0 * st[j][t2];
-        ch[j][5] = 0.0 + l5u * st[j][un] + l5p * st[j][IP];
-        ch[j][6] = 0.0 + l6p * st[j][IP] + 1.0 * st[j][IA];
-    }
-    double cl[7], cr[7];
-#pragma unroll
-    for (int a = 0; a < 7; ++a) {
-        cl[a] = weno5z(ch[0][a], ch[1][a], ch[2][a], ch[3][a], ch[4][a]);
-        cr[a] = weno5z(ch[5][a], ch[4][a], ch[3][a], ch[2][a], ch[1][a]);
-    }
     // R: r[0][0]=r[1][1]=r[t1][3]=r[t2][4]=r[IA][6]=1; columns 2 (sign -1)
     // and 5 (sign +1): r[0]=a1/rho, r[1]=a2/rho, r[un]=sgn c/rho, r[IP]=c^2,
     // r[IA]=-K/rho
     const double r0 = avg[0] / rho, r1 = avg[1] / rho;
     const double rum = -1.0 * c / rho, rup = 1.0 * c / rho;
     const double ra = -kk / rho;
+    // w = R c, each row summed over columns a = 0..6 from 0.0 (zero terms
+    // skipped)
     double wl[7], wr[7];
-    wl[0] = 0.0 + 1.0 * cl[0] + r0 * cl[2] + r0 * cl[5];
-    wr[0] = 0.0 + 1.0 * cr[0] + r0 * cr[2] + r0 * cr[5];
-    wl[1] = 0.0 + 1.0 * cl[1] + r1 * cl[2] + r1 * cl[5];
-    wr[1] = 0.0 + 1.0 * cr[1] + r1 * cr[2] + r1 * cr[5];
-    wl[un] = 0.0 + rum * cl[2] + rup * cl[5];
-    wr[un] = 0.0 + rum * cr[2] + rup * cr[5];
-    wl[t1] = 0.0 + 1.0 * cl[3];
-    wr[t1] = 0.0 + 1.0 * cr[3];
-    wl[t2] = 0.0 + 1.0 * cl[4];
-    wr[t2] = 0.0 + 1.0 * cr[4];
-    wl[IP] = 0.0 + c2 * cl[2] + c2 * cl[5];
-    wr[IP] = 0.0 + c2 * cr[2] + c2 * cr[5];
-    wl[IA] = 0.0 + ra * cl[2] + ra * cl[5] + 1.0 * cl[6];
-    wr[IA] = 0.0 + ra * cr[2] + ra * cr[5] + 1.0 * cr[6];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        wl[k] = 0.0;
+        wr[k] = 0.0;
+    }
+#pragma unroll
+    for (int a = 0; a < 7; ++a) {
+        // c_a = L[a] . w_j, summed over columns k = 0..6 from 0.0
+        double ch[6];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) {
+            if (a == 0) ch[j] = 0.0 + 1.0 * ld(j, 0) + l0p * ld(j, IP);
+            if (a == 1) ch[j] = 0.0 + 1.0 * ld(j, 1) + l1p * ld(j, IP);
+            if (a == 2) ch[j] = 0.0 + l2u * ld(j, un) + l2p * ld(j, IP);
+            if (a == 3) ch[j] = 0.0 + 1.0 * ld(j, t1);
+            if (a == 4) ch[j] = 0.0 + 1.0 * ld(j, t2);
+            if (a == 5) ch[j] = 0.0 + l5u * ld(j, un) + l5p * ld(j, IP);
+            if (a == 6) ch[j] = 0.0 + l6p * ld(j, IP) + 1.0 * ld(j, IA);
+        }
+        const double cl = weno5z(ch[0], ch[1], ch[2], ch[3], ch[4]);
+        const double cr = weno5z(ch[5], ch[4], ch[3], ch[2], ch[1]);
+        if (a == 0) {
+            wl[0] = wl[0] + 1.0 * cl;
+            wr[0] = wr[0] + 1.0 * cr;
+        }
+        if (a == 1) {
+            wl[1] = wl[1] + 1.0 * cl;
+            wr[1] = wr[1] + 1.0 * cr;
+        }
+        if (a == 2 || a == 5) {
+            const double ru = a == 2 ? rum : rup;
+            wl[0] = wl[0] + r0 * cl;
+            wr[0] = wr[0] + r0 * cr;
+            wl[1] = wl[1] + r1 * cl;
+            wr[1] = wr[1] + r1 * cr;
+            wl[un] = wl[un] + ru * cl;
+            wr[un] = wr[un] + ru * cr;
+            wl[IP] = wl[IP] + c2 * cl;
+            wr[IP] = wr[IP] + c2 * cr;
+            wl[IA] = wl[IA] + ra * cl;
+            wr[IA] = wr[IA] + ra * cr;
+        }
+        if (a == 3) {
+            wl[t1] = wl[t1] + 1.0 * cl;
+            wr[t1] = wr[t1] + 1.0 * cr;
+        }
+        if (a == 4) {
+            wl[t2] = wl[t2] + 1.0 * cl;
+            wr[t2] = wr[t2] + 1.0 * cr;
+        }
+        if (a == 6) {
+            wl[IA] = wl[IA] + 1.0 * cl;
+            wr[IA] = wr[IA] + 1.0 * cr;
+        }
+    }
     bool fb = false;
     if (!admissible(s, wl) || !admissible(s, wr)) {
         fb = true;
 #pragma unroll
         for (int k = 0; k < 7; ++k) {
-            wl[k] = st[2][k];
-            wr[k] = st[3][k];
+            wl[k] = ld(2, k);
+            wr[k] = ld(3, k);
         }
     }
     hllc<D>(s, wl, wr, f, us);

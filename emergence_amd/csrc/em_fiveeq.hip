@@ -77,14 +77,19 @@ template <int MODE, int T, int P>
 struct FeLds {
     static constexpr bool HYB = MODE == MODE_HYBRID;
     double sw[7][T + 10][P];
+    double sre[T + 10][P];  // rho e(w) of each tile cell (central pair fluxes, limiter)
     double sm[8][T + 5][P];  // midpoints: flux + contact speed; then faces: flux + face velocity
     double ss[3][HYB ? T + 10 : 1][P];
     double sth[HYB ? T + 1 : 1][P];
     unsigned int scnt[3];
 };
 
+#ifndef EM_FE_WAVES_PER_EU
+#define EM_FE_WAVES_PER_EU 2
+#endif
+
 template <int D, int MODE, int T, int P, int NT>
-__global__ __launch_bounds__(NT) void fe_pass(FePassArgs a) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_FE_WAVES_PER_EU))) void fe_pass(FePassArgs a) {
     constexpr int C = T + 10;
     constexpr int M = T + 5;
     constexpr int NF = T + 1;
@@ -93,6 +98,7 @@ __global__ __launch_bounds__(NT) void fe_pass(FePassArgs a) {
     constexpr int un = IU + D;
     __shared__ FeLds<MODE, T, P> L;
     auto& sw = L.sw;
+    auto& sre = L.sre;
     auto& sm = L.sm;
     auto& ss = L.ss;
     auto& sth = L.sth;
@@ -144,6 +150,7 @@ __global__ __launch_bounds__(NT) void fe_pass(FePassArgs a) {
         to_prim(sys, u, w);
 #pragma unroll
         for (int k = 0; k < 7; ++k) sw[k][c][p] = w[k];
+        sre[c][p] = rho_e(sys, w);
         if constexpr (HYB) {
             // FiveEq::sensor, multiphase.rs:385-390
             ss[0][c][p] = log(fmax(mix_rho(w), 1e-300));
@@ -192,13 +199,8 @@ __global__ __launch_bounds__(NT) void fe_pass(FePassArgs a) {
                 for (int f = flo; f <= fhi; ++f) need = need || sth[f][p] > 0.0;
                 if (!need) continue;
             }
-            double st[6][7];
-#pragma unroll
-            for (int j = 0; j < 6; ++j)
-#pragma unroll
-                for (int k = 0; k < 7; ++k) st[j][k] = sw[k][m + j][p];
             double f[7], us;
-            const bool fb = midpoint_flux<D>(sys, st, f, us);
+            const bool fb = midpoint_flux<D>(sys, [&](int j, int k) { return sw[k][m + j][p]; }, f, us);
 #pragma unroll
             for (int k = 0; k < 7; ++k) sm[k][m][p] = f[k];
             sm[7][m][p] = us;
@@ -248,7 +250,7 @@ __global__ __launch_bounds__(NT) void fe_pass(FePassArgs a) {
                         wa[k] = sw[k][f + 4 - mm][p];
                         wb[k] = sw[k][f + 4 - mm + lag][p];
                     }
-                    split_flux<D>(sys, wa, wb, ft);
+                    split_flux_re<D>(wa, wb, sre[f + 4 - mm][p], sre[f + 4 - mm + lag][p], ft);
 #pragma unroll
                     for (int k = 0; k < 7; ++k) cen[k] += 2.0 * a6[li] * ft[k];
                     uc += a6[li] * (wa[un] + wb[un]);
@@ -292,8 +294,8 @@ __global__ __launch_bounds__(NT) void fe_pass(FePassArgs a) {
                 wl[k] = sw[k][f + 4][p];
                 wr[k] = sw[k][f + 5][p];
             }
-            to_cons(sys, wl, ul);
-            to_cons(sys, wr, ur);
+            to_cons_re(wl, sre[f + 4][p], ul);
+            to_cons_re(wr, sre[f + 5][p], ur);
             phys_flux<D>(wl, ul, fl);
             phys_flux<D>(wr, ur, fr);
             const double tl = 2.0 * lam;
@@ -539,8 +541,8 @@ __global__ void k_fe_set_dt(FeScalars* sc, double dt) { sc->dt = dt; }
 struct FeTile {
     int T, P, NT;
 };
-constexpr FeTile kFeTiles[] = {{27, 8, 256}, {59, 4, 256}};
-constexpr int kNumFeTiles = 2;
+constexpr FeTile kFeTiles[] = {{27, 8, 256}, {59, 4, 256}, {59, 8, 512}, {27, 16, 512}, {123, 4, 512}};
+constexpr int kNumFeTiles = 5;
 
 template <int D, int MODE, int T, int P, int NT>
 static hipError_t launch_fe(const FePassArgs& a, hipStream_t st) {
@@ -553,6 +555,9 @@ static hipError_t launch_fe_cfg(int cfg, const FePassArgs& a, hipStream_t st) {
     switch (cfg) {
         case 0: return launch_fe<D, MODE, 27, 8, 256>(a, st);
         case 1: return launch_fe<D, MODE, 59, 4, 256>(a, st);
+        case 2: return launch_fe<D, MODE, 59, 8, 512>(a, st);
+        case 3: return launch_fe<D, MODE, 27, 16, 512>(a, st);
+        case 4: return launch_fe<D, MODE, 123, 4, 512>(a, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -813,6 +818,55 @@ struct FeSolver {
         return dt;
     }
 
+    // Times every tile shape of each active axis on the current state (one
+    // residual pass at the CFL stage dt, after a warm-up pass) and keeps the
+    // fastest; the device scalars (counters) are restored afterwards.
+    void autotune(int repeats, int32_t out[3]) {
+        std::vector<FeScalars> saved(1);
+        FECHK(hipMemcpyAsync(saved.data(), sc, sizeof(FeScalars), hipMemcpyDeviceToHost, stream));
+        FECHK(hipStreamSynchronize(stream));
+        const double dt = dt_cfl();
+        hipEvent_t e0, e1;
+        FECHK(hipEventCreate(&e0));
+        FECHK(hipEventCreate(&e1));
+        for (int d = 0; d < 3; ++d) {
+            if (!active(d)) continue;
+            double best = 1e300;
+            int bestc = cfg[d];
+            for (int c = 0; c < kNumFeTiles; ++c) {
+                if (kFeTiles[c].T > 2 * n[d] && c != 0) continue;
+                cfg[d] = c;
+                FePassArgs a = pass_args(d);
+                a.U = U[0];
+                a.R = R;
+                a.U0 = U[0];
+                a.Uout = nullptr;
+                a.first = 1;
+                a.rk_stage = 0;
+                a.stage_dt_host = 1;
+                a.stage_dt = std::isfinite(dt) ? dt : 0.0;
+                FECHK(launch_fe_pass(d, mode, c, a, stream));  // warm-up
+                FECHK(hipEventRecord(e0, stream));
+                for (int r = 0; r < repeats; ++r) FECHK(launch_fe_pass(d, mode, c, a, stream));
+                FECHK(hipEventRecord(e1, stream));
+                FECHK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                FECHK(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) {
+                    best = ms;
+                    bestc = c;
+                }
+            }
+            cfg[d] = bestc;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        FECHK(hipMemcpyAsync(sc, saved.data(), sizeof(FeScalars), hipMemcpyHostToDevice, stream));
+        FECHK(hipStreamSynchronize(stream));
+        if (out)
+            for (int d = 0; d < 3; ++d) out[d] = cfg[d];
+    }
+
     // solver.rs:758-768
     int64_t run_to(double t_end) {
         int64_t steps = 0;
@@ -919,6 +973,27 @@ int em_fiveeq_run_to(em_fiveeq* h, double t_end, int64_t* steps) {
         const int64_t k = h->s->run_to(t_end);
         if (steps) *steps = k;
     });
+}
+
+int em_fiveeq_set_tiles(em_fiveeq* h, const int32_t cfg[3]) {
+    FENEED(h);
+    for (int d = 0; d < 3; ++d)
+        if (cfg[d] < 0 || cfg[d] >= em::fe::kNumFeTiles) {
+            em::fe::g_err = "tile configuration out of range";
+            return EM_ERR_ARG;
+        }
+    for (int d = 0; d < 3; ++d) h->s->cfg[d] = cfg[d];
+    em::fe::g_err.clear();
+    return EM_OK;
+}
+
+int em_fiveeq_autotune(em_fiveeq* h, int32_t repeats, int32_t cfg_out[3]) {
+    FENEED(h);
+    if (repeats < 1) {
+        em::fe::g_err = "repeats must be >= 1";
+        return EM_ERR_ARG;
+    }
+    return fe_guarded([&] { h->s->autotune(repeats, cfg_out); });
 }
 
 int em_fiveeq_counters(em_fiveeq* h, uint64_t out[3]) {

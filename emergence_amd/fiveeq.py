@@ -95,6 +95,14 @@ class HipFiveEq:
         _lib.check_fiveeq(self.lib.em_fiveeq_run_to(self.h, t_end, C.byref(k)))
         return k.value
 
+    def set_tiles(self, cfg):
+        _lib.check_fiveeq(self.lib.em_fiveeq_set_tiles(self.h, (C.c_int32 * 3)(*cfg)))
+
+    def autotune(self, repeats=3):
+        out = (C.c_int32 * 3)()
+        _lib.check_fiveeq(self.lib.em_fiveeq_autotune(self.h, repeats, out))
+        return list(out)
+
     def counters(self):
         out = (C.c_uint64 * 3)()
         _lib.check_fiveeq(self.lib.em_fiveeq_counters(self.h, out))

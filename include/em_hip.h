@@ -398,6 +398,11 @@ int em_fiveeq_rhs(em_fiveeq* h, double stage_dt, double* rhs, uint64_t stats[4])
 int em_fiveeq_step(em_fiveeq* h, double dt);
 int em_fiveeq_dt_cfl(em_fiveeq* h, double* dt);
 int em_fiveeq_run_to(em_fiveeq* h, double t_end, int64_t* steps);
+/* Tile shape per axis (index into the FiveEq tile table) and the project
+ * autotuner over it, as em_solver_set_tiles / em_solver_autotune (no
+ * reference counterpart).  cfg_out may be NULL. */
+int em_fiveeq_set_tiles(em_fiveeq* h, const int32_t cfg[3]);
+int em_fiveeq_autotune(em_fiveeq* h, int32_t repeats, int32_t cfg_out[3]);
 int em_fiveeq_counters(em_fiveeq* h, uint64_t out[3]);
 int em_fiveeq_time(em_fiveeq* h, double* t);
 int em_fiveeq_synchronize(em_fiveeq* h);
