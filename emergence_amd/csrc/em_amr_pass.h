@@ -46,6 +46,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_WAVES_PER
     const Eos e = a.eos;
     unsigned int n_shock = 0, n_fb = 0, n_lim = 0;
     const double dtv = pa.stage_dt_host ? pa.stage_dt : a.sc->dt;
+    // plain stride: the chunked XCD order (xcd_stride) measured no faster here
     for (long long w = blockIdx.x; w < nwork; w += gridDim.x) {
         const int li = pa.leaf0 + (int)(w / pa.ngroup);
         const int gp = (int)(w % pa.ngroup);

@@ -104,7 +104,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_FE_WAVES_
     auto& sth = L.sth;
     if (a.sc->active == 0) return;
     const int tid = threadIdx.x;
-    const int bid = blockIdx.x;
+    const int bid = xcd_major((int)blockIdx.x, (int)gridDim.x);
     const int tile = bid % a.ntile;
     const int rest = bid / a.ntile;
     const int gF = rest % a.ngroup;

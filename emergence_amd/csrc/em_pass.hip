@@ -243,7 +243,7 @@ __device__ __forceinline__ void axis_tile(const PassArgs& a, int bid, TileLds<MO
 template <int D, int MODE, int T, int P, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_WAVES_PER_EU))) void axis_pass(PassArgs a) {
     __shared__ TileLds<MODE, T, P, NT> L;
-    axis_tile<D, MODE, T, P, NT>(a, blockIdx.x, L);
+    axis_tile<D, MODE, T, P, NT>(a, xcd_major((int)blockIdx.x, (int)gridDim.x), L);
 }
 
 // Ensemble pass: the tiles of every instance in one persistent launch.  Work
@@ -253,7 +253,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_WAVES_PER
 template <int D, int MODE, int T, int P, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_WAVES_PER_EU))) void ens_pass(EnsPassArgs e) {
     __shared__ TileLds<MODE, T, P, NT> L;
-    for (long long w = blockIdx.x; w < e.total; w += gridDim.x) {
+    const XcdStride xs = xcd_stride();
+    for (long long t = xs.first;; t += xs.step) {
+        const long long w = xs.item(t);
+        if (w >= e.total) break;
         int lo = 0, hi = e.ninst;
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;

@@ -70,4 +70,49 @@ struct TileCfg {
     int T, P, NT;
 };
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup w runs on
+// XCD w % 8, and each XCD has its own L2).  Tiles that share cache lines are
+// neighbours in tile order: adjacent tiles of a pencil share 10 halo cells,
+// and the pencil groups 2k and 2k+1 of a strided pass read the two halves of
+// the same 128-byte lines.  xcd_major hands out the tile order in chunks of
+// kXcdChunk consecutive tiles, chunk g to XCD g % 8, so that neighbours run
+// on one XCD at about the same time and a line fetched for one tile is still
+// in that L2 for the next; the chunks of one XCD are spread over the whole
+// tile range, so work that varies in space stays balanced over the XCDs.
+// A bijection of [0, nblk) for any nblk (the last nblk mod 256 tiles keep
+// their own index).
+#ifndef EM_XCD_ORDER
+#define EM_XCD_ORDER 1
+#endif
+constexpr int kXcds = 8;
+constexpr int kXcdChunk = 32;
+__device__ __forceinline__ int xcd_major(int w, int nblk) {
+    constexpr int span = kXcds * kXcdChunk;
+    if (!EM_XCD_ORDER || w >= nblk / span * span) return w;
+    const int x = w % kXcds, m = w / kXcds;
+    return ((m / kXcdChunk) * kXcds + x) * kXcdChunk + m % kXcdChunk;
+}
+
+// The same order for a persistent grid of workgroups striding over work
+// items: the grid/8 workgroups of XCD x take turns over the items of the
+// chunks g = x, x + 8, x + 16, ... in order.  Loop:
+//   for (long long t = s.first;; t += s.step) {
+//       const long long w = s.item(t);
+//       if (w >= nwork) break;  ...
+// item() increases with t, so every item is visited once.  A grid that is
+// not a multiple of 8 keeps the plain stride.
+struct XcdStride {
+    long long first, step;
+    int x, chunked;
+    __device__ __forceinline__ long long item(long long t) const {
+        if (!chunked) return t;
+        return ((t / kXcdChunk) * kXcds + x) * kXcdChunk + t % kXcdChunk;
+    }
+};
+__device__ __forceinline__ XcdStride xcd_stride() {
+    const int grid = (int)gridDim.x, b = (int)blockIdx.x;
+    if (!EM_XCD_ORDER || grid % kXcds != 0) return XcdStride{b, grid, 0, 0};
+    return XcdStride{b / kXcds, grid / kXcds, b % kXcds, 1};
+}
+
 }  // namespace em
