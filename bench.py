@@ -86,6 +86,7 @@ def parse():
     ap.add_argument("--fiveeq-only", action="store_true", help="only the FiveEq workload (profiling)")
     ap.add_argument("--fe-n", type=int, default=256)
     ap.add_argument("--fe-steps", type=int, default=10)
+    ap.add_argument("--fe-tiles", type=str, default=None, help="fixed FiveEq tile shapes per axis (no autotuning)")
     ap.add_argument("--ensemble-only", action="store_true", help="only the ensemble workload (profiling)")
     ap.add_argument("--ens-n", type=int, default=256, help="instances")
     ap.add_argument("--ens-steps", type=int, default=300)
@@ -458,22 +459,20 @@ def amr_cpu_baseline(cells, repeats):
     return rec
 
 
-def profiled_traffic(variant, d, shape):
-    """HBM bytes per launch of the axis pass along d with tile shape
-    (T, P, threads), from the committed rocprofv3 summary (FETCH_SIZE and
-    WRITE_SIZE runs scaled by a plain-copy calibration); None when the
-    summary has no row for this kernel."""
-    import re
-
-    path = os.path.join(ROOT, "profiles", f"axis_pass_256cubed_weno_{variant}.md")
+def profiled_traffic(variant, mode, d, shape):
+    """HBM bytes (read + write) per launch of the axis pass along d in flux
+    mode `mode` (0 Hybrid, 1 Weno) with tile shape (T, P, threads), from the
+    committed rocprofv3 summary (128/64/32-byte read requests and WRITE_SIZE);
+    None when the summary has no row for this kernel."""
+    path = os.path.join(ROOT, "profiles", "axis_pass_256cubed.md")
     if not os.path.exists(path):
         return None
-    sig = f"axis_pass<{d}, 1, {shape[0]}, {shape[1]}, {shape[2]}>"
+    sig = f"`em::{variant}::axis_pass<{d}, {mode}, {shape[0]}, {shape[1]}, {shape[2]}>`"
     for line in open(path):
-        if sig in line and "GB |" in line:
-            m = re.search(r"\| ([0-9.]+) GB \|\s*$", line.strip())
-            if m:
-                return float(m.group(1)) * 1e9
+        cells = [c.strip() for c in line.split("|")]
+        # | kernel | read requests | read | write | read + write | algorithmic | ratio |
+        if len(cells) >= 8 and cells[1] == sig and cells[5].endswith(" GB"):
+            return float(cells[5].split()[0]) * 1e9
     return None
 
 
@@ -569,7 +568,7 @@ def fiveeq_bench(em, args):
     u = np.zeros((n + 12, n + 12, n + 12, 7))
     u[6:-6, 6:-6, 6:-6] = np.stack([a1, a2, rho * vx, 0 * a, 0 * a, e, a], axis=-1)
     g = em.HipFiveEq((n, n, n), (0, 0, 0), (1, 1, 1), u, mode="hybrid")
-    tiles = None if args.no_autotune else g.autotune(3)
+    tiles = pick_tiles(g, args, args.fe_tiles)
     dt = 0.5 * g.dt_cfl()
     g.step(dt)
     g.synchronize()
@@ -633,6 +632,8 @@ def hybrid_bench(em, args, n, u):
         "shock_path_face_share": share,
         "roofline": {"bound": "mfma", "roof": "fp64-valu", "kernel": f"axis_pass d={dom}", "achieved": achieved / 1e12,
                      "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP64,
+                     "traffic": profiled_traffic("fast" if args.fast else "precise", 0, dom,
+                                                 TILE_SHAPES[(tiles or [6, 6, 6])[dom]]),
                      "flop_per_cell_stage": flop, "hbm_fraction": value * BYTES_PER_CELL_STAGE / PEAK_HBM},
     }
 
@@ -707,7 +708,7 @@ def main():
             "peak": PEAK_FP64 / 1e12,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64,
-            "traffic": profiled_traffic("fast" if args.fast else "precise", dom, TILE_SHAPES[(tiles or [6, 6, 6])[dom]]),
+            "traffic": profiled_traffic("fast" if args.fast else "precise", 1, dom, TILE_SHAPES[(tiles or [6, 6, 6])[dom]]),
             "hbm_fraction": value * BYTES_PER_CELL_STAGE / PEAK_HBM,
             "fp64_fraction": value * FLOP_PER_CELL_STAGE / PEAK_FP64,
         },
