@@ -49,6 +49,7 @@ FLOP_PER_CELL_STAGE = 3500.0
 FLOP_PER_CELL_STAGE_CENTRAL = 1600.0  # KEEP-PE central flux on every face, 3-D
 BYTES_PER_CELL_STAGE = 120.0  # read U^(s), read U^n, write U^(s+1) (5 x f64 each)
 PEAK_FP64 = 78.6e12
+PEAK_FP32 = 157.3e12
 PEAK_HBM = 8.0e12
 
 
@@ -66,6 +67,8 @@ def parse():
     ap.add_argument("--tiles", type=str, default=None,
                     help="fixed Weno tile shapes per axis, e.g. 6,6,5 (indices into TILE_SHAPES; no autotuning)")
     ap.add_argument("--hybrid-tiles", type=str, default=None, help="fixed Hybrid tile shapes per axis")
+    ap.add_argument("--no-f32", action="store_true", help="skip the FP32 256^3 line")
+    ap.add_argument("--f32-tiles", type=str, default=None, help="fixed FP32 tile shapes per axis")
     ap.add_argument("--no-amr", action="store_true", help="skip the block-AMR workload")
     ap.add_argument("--amr-only", action="store_true", help="only the block-AMR workload (profiling)")
     ap.add_argument("--amr-steps", type=int, default=100)
@@ -591,6 +594,38 @@ def fiveeq_bench(em, args):
     }
 
 
+def f32_bench(em, args, n, u):
+    """The same 256^3 Weno workload on the FP32 variant of the solver (the
+    reference's GpuSolver<f32>: state and arithmetic in FP32, multiply-adds
+    contracted), with its difference from the FP64 state after the same
+    number of steps.  FP32 VALU peak: 157.3 TFLOP/s."""
+    lo, hi = (0.0, 0.0, 0.0), (2 * np.pi,) * 3
+    g = em.HipSolver((n, n, n), lo, hi, u, mode="weno", cfl=CFL, real="f32")
+    tiles = pick_tiles(g, args, args.f32_tiles)
+    g.upload(u)
+    g.run_steps(args.warmup)
+    g.synchronize()
+    t0 = time.perf_counter()
+    pass_ms = g.run_steps(args.steps, timed=True)
+    g.synchronize()
+    el = time.perf_counter() - t0
+    value = n ** 3 * args.steps * 3 / el
+    dom = int(np.argmax(pass_ms))
+    achieved = FLOP_PER_CELL_STAGE / 3.0 * n ** 3 / (pass_ms[dom] * 1e-3)
+    return {
+        "workload": f"3-D periodic {n}^3 (same state), FluxMode::Weno, CFL {CFL}, FP32 state and arithmetic",
+        "value": value,
+        "unit": "cell-updates/s per RK stage",
+        "dtype": "f32",
+        "ms_per_step": el / args.steps * 1e3,
+        "tiles": tiles,
+        "pass_ms": pass_ms,
+        "roofline": {"bound": "mfma", "roof": "fp32-valu", "kernel": f"axis_pass_f32 d={dom}",
+                     "achieved": achieved / 1e12, "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32, "hbm_fraction": value * BYTES_PER_CELL_STAGE / 2 / PEAK_HBM},
+    }
+
+
 def pick_tiles(g, args, fixed):
     """Tile shape per axis: the given indices, the defaults, or the
     autotuner's pick (a few timed passes of every shape)."""
@@ -718,6 +753,8 @@ def main():
     del g
     if not args.no_hybrid:
         rec["hybrid"] = hybrid_bench(em, args, n, u)
+    if not args.no_f32:
+        rec["f32"] = f32_bench(em, args, n, u)
     if not args.no_amr:
         rec["amr"] = amr_bench(em, args)
     if not args.no_adjoint:

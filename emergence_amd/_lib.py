@@ -14,6 +14,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "em_hip.h")
 EXPORTS = (
     "em_last_error",
     "em_solver_create",
+    "em_solver_create_f32",
     "em_solver_destroy",
     "em_solver_set_variant",
     "em_solver_set_tiles",
@@ -116,6 +117,8 @@ def load():
     lib.em_last_error.argtypes = []
     lib.em_solver_create.argtypes = [C.POINTER(P), I64, D, D, C.c_int32, D, I32, D, C.c_int32, C.c_int32,
                                      C.c_double, C.c_double, C.c_double, D]
+    lib.em_solver_create_f32.argtypes = [C.POINTER(P), I64, D, D, C.c_int32, D, I32, D, C.c_int32, C.c_int32,
+                                         C.c_double, C.c_double, C.c_double, D]
     lib.em_solver_destroy.argtypes = [P]
     lib.em_solver_set_variant.argtypes = [P, C.c_int32]
     lib.em_solver_set_tiles.argtypes = [P, I32]

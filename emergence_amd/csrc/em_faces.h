@@ -19,12 +19,12 @@
 namespace em {
 namespace EM_VARIANT {
 
-template <int D, int MODE, int T, int P, int NT>
-__device__ __forceinline__ void tile_faces(double (&sw)[5][T + 10][P], double (&sre)[T + 10][P],
-                                           double (&sm)[5][T + 5][P],
-                                           double (&ss)[2][MODE == MODE_HYBRID ? T + 10 : 1][P],
-                                           double (&sth)[MODE == MODE_HYBRID ? T + 1 : 1][P], int Tv, int Pv,
-                                           int c0, int nD, const Eos& e, double lam, double tol,
+template <int D, int MODE, int T, int P, int NT, class R>
+__device__ __forceinline__ void tile_faces(R (&sw)[5][T + 10][P], R (&sre)[T + 10][P],
+                                           R (&sm)[5][T + 5][P],
+                                           R (&ss)[2][MODE == MODE_HYBRID ? T + 10 : 1][P],
+                                           R (&sth)[MODE == MODE_HYBRID ? T + 1 : 1][P], int Tv, int Pv,
+                                           int c0, int nD, const EosT<R>& e, R lam, R tol,
                                            unsigned& n_shock, unsigned& n_fb, unsigned& n_lim) {
     constexpr int M = T + 5;
     constexpr int NF = T + 1;
@@ -37,18 +37,18 @@ __device__ __forceinline__ void tile_faces(double (&sw)[5][T + 10][P], double (&
         for (int it = tid; it < NF * P; it += NT) {
             const int p = it % P, f = it / P;
             if (p >= Pv || f > Tv) continue;
-            double r = 0.0;
+            R r = R(0.0);
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
-                double q[6];
+                R q[6];
 #pragma unroll
                 for (int j = 0; j < 6; ++j) q[j] = ss[k][f + 2 + j][p];
                 r = fmax(fmax(r, d4(q[0], q[1], q[2], q[3], q[4])), d4(q[1], q[2], q[3], q[4], q[5]));
             }
-            double x = (r / tol - 0.5) / 0.5;
-            if (x < 0.0) x = 0.0;
-            if (x > 1.0) x = 1.0;
-            sth[f][p] = isfinite(r) ? x * x * (3.0 - 2.0 * x) : 1.0;
+            R x = (r / tol - R(0.5)) / R(0.5);
+            if (x < R(0.0)) x = R(0.0);
+            if (x > R(1.0)) x = R(1.0);
+            sth[f][p] = isfinite(r) ? x * x * (R(3.0) - R(2.0) * x) : R(1.0);
         }
         __syncthreads();
     }
@@ -62,15 +62,15 @@ __device__ __forceinline__ void tile_faces(double (&sw)[5][T + 10][P], double (&
             if constexpr (HYB) {
                 bool need = false;
                 const int flo = max(0, m - 4), fhi = min(Tv, m);
-                for (int f = flo; f <= fhi; ++f) need = need || sth[f][p] > 0.0;
+                for (int f = flo; f <= fhi; ++f) need = need || sth[f][p] > R(0.0);
                 if (!need) continue;
             }
-            double s[6][5];
+            R s[6][5];
 #pragma unroll
             for (int j = 0; j < 6; ++j)
 #pragma unroll
                 for (int k = 0; k < 5; ++k) s[j][k] = sw[k][m + j][p];
-            double f[5];
+            R f[5];
             const bool fb = midpoint_flux<D>(e, s, f);
 #pragma unroll
             for (int k = 0; k < 5; ++k) sm[k][m][p] = f[k];
@@ -84,7 +84,7 @@ __device__ __forceinline__ void tile_faces(double (&sw)[5][T + 10][P], double (&
     }
 
     // ---- 4. face fluxes (face f between cells c0+f-1 and c0+f)
-    double FR[FITER][5];
+    R FR[FITER][5];
 #pragma unroll
     for (int fi = 0; fi < FITER; ++fi) {
         const int it = tid + fi * NT;
@@ -92,24 +92,24 @@ __device__ __forceinline__ void tile_faces(double (&sw)[5][T + 10][P], double (&
         if (it >= NF * P || p >= Pv || f > Tv) continue;
         // a face shared by two tiles is counted by the tile on its right
         const unsigned own_face = (f < Tv || c0 + Tv == nD) ? 1u : 0u;
-        double theta;
+        R theta;
         if constexpr (MODE == MODE_WENO)
-            theta = 1.0;
+            theta = R(1.0);
         else if constexpr (CEN)
-            theta = 0.0;
+            theta = R(0.0);
         else
             theta = sth[f][p];
-        double fo[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        double cen[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        if (theta < 1.0) {
+        R fo[5] = {R(0.0), R(0.0), R(0.0), R(0.0), R(0.0)};
+        R cen[5] = {R(0.0), R(0.0), R(0.0), R(0.0), R(0.0)};
+        if (theta < R(1.0)) {
             // KEEP-PE 6th-order central flux, scheme.rs:188-214
-            const double a6[3] = {A6_0, A6_1, A6_2};
+            const R a6[3] = {R(A6_0), R(A6_1), R(A6_2)};
 #pragma unroll
             for (int li = 0; li < 3; ++li) {
                 const int lag = li + 1;
 #pragma unroll
                 for (int mm = 0; mm < lag; ++mm) {
-                    double wa[5], wb[5], ft[5];
+                    R wa[5], wb[5], ft[5];
 #pragma unroll
                     for (int k = 0; k < 5; ++k) {
                         wa[k] = sw[k][f + 4 - mm][p];
@@ -117,11 +117,11 @@ __device__ __forceinline__ void tile_faces(double (&sw)[5][T + 10][P], double (&
                     }
                     split_flux_re<D>(wa, wb, sre[f + 4 - mm][p], sre[f + 4 - mm + lag][p], ft);
 #pragma unroll
-                    for (int k = 0; k < 5; ++k) cen[k] += 2.0 * a6[li] * ft[k];
+                    for (int k = 0; k < 5; ++k) cen[k] += R(2.0) * a6[li] * ft[k];
                 }
             }
         }
-        if (theta == 0.0) {
+        if (theta == R(0.0)) {
 #pragma unroll
             for (int k = 0; k < 5; ++k) fo[k] = cen[k];
         } else {
@@ -129,22 +129,22 @@ __device__ __forceinline__ void tile_faces(double (&sw)[5][T + 10][P], double (&
             // F = sum_s S6[s] (F_{i+s} + F_{i-s}), order s=0, i+1, i-1, i+2, i-2
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
-                double v = 0.0;
-                v += S6_0 * sm[k][f + 2][p];
-                v += S6_1 * sm[k][f + 3][p];
-                v += S6_1 * sm[k][f + 1][p];
-                v += S6_2 * sm[k][f + 4][p];
-                v += S6_2 * sm[k][f][p];
+                R v = R(0.0);
+                v += R(S6_0) * sm[k][f + 2][p];
+                v += R(S6_1) * sm[k][f + 3][p];
+                v += R(S6_1) * sm[k][f + 1][p];
+                v += R(S6_2) * sm[k][f + 4][p];
+                v += R(S6_2) * sm[k][f][p];
                 fo[k] = v;
             }
-            if (theta < 1.0) {
+            if (theta < R(1.0)) {
 #pragma unroll
-                for (int k = 0; k < 5; ++k) fo[k] = theta * fo[k] + (1.0 - theta) * cen[k];
+                for (int k = 0; k < 5; ++k) fo[k] = theta * fo[k] + (R(1.0) - theta) * cen[k];
             }
         }
-        if (lam > 0.0) {
+        if (lam > R(0.0)) {
             // positivity limiter, scheme.rs:401-444
-            double wl[5], wr[5], ul[5], ur[5], fl[5], fr[5];
+            R wl[5], wr[5], ul[5], ur[5], fl[5], fr[5];
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
                 wl[k] = sw[k][f + 4][p];
@@ -154,9 +154,9 @@ __device__ __forceinline__ void tile_faces(double (&sw)[5][T + 10][P], double (&
             to_cons_re(wr, sre[f + 5][p], ur);
             phys_flux<D>(wl, ul, fl);
             phys_flux<D>(wr, ur, fr);
-            const double tl = 2.0 * lam;
-            auto ok = [&](const double* fx) {
-                double qa[5], qb[5], pa[5], pb[5];
+            const R tl = R(2.0) * lam;
+            auto ok = [&](const R* fx) {
+                R qa[5], qb[5], pa[5], pb[5];
 #pragma unroll
                 for (int k = 0; k < 5; ++k) {
                     qa[k] = ul[k] - tl * (fx[k] - fl[k]);
@@ -171,28 +171,28 @@ __device__ __forceinline__ void tile_faces(double (&sw)[5][T + 10][P], double (&
             for (int k = 0; k < 5; ++k) fin = fin && isfinite(fo[k]);
             if (!(fin && ok(fo))) {
                 constexpr int un = 1 + D;
-                const double al = fmax(fabs(wl[un]) + sound_speed(e, wl[0], wl[4]),
+                const R al = fmax(fabs(wl[un]) + sound_speed(e, wl[0], wl[4]),
                                        fabs(wr[un]) + sound_speed(e, wr[0], wr[4]));
-                double flf[5], fh[5];
+                R flf[5], fh[5];
 #pragma unroll
                 for (int k = 0; k < 5; ++k) {
-                    flf[k] = 0.5 * (fl[k] + fr[k]) - 0.5 * al * (ur[k] - ul[k]);
+                    flf[k] = R(0.5) * (fl[k] + fr[k]) - R(0.5) * al * (ur[k] - ul[k]);
                     fh[k] = fo[k];
                 }
                 n_lim += own_face;
-                double lo = 0.0, hi = 1.0;
+                R lo = R(0.0), hi = R(1.0);
                 for (int itb = 0; itb < 30; ++itb) {
-                    const double mid = 0.5 * (lo + hi);
-                    double bl[5];
+                    const R mid = R(0.5) * (lo + hi);
+                    R bl[5];
 #pragma unroll
-                    for (int k = 0; k < 5; ++k) bl[k] = isfinite(fh[k]) ? mid * fh[k] + (1.0 - mid) * flf[k] : flf[k];
+                    for (int k = 0; k < 5; ++k) bl[k] = isfinite(fh[k]) ? mid * fh[k] + (R(1.0) - mid) * flf[k] : flf[k];
                     if (ok(bl))
                         lo = mid;
                     else
                         hi = mid;
                 }
 #pragma unroll
-                for (int k = 0; k < 5; ++k) fo[k] = isfinite(fh[k]) ? lo * fh[k] + (1.0 - lo) * flf[k] : flf[k];
+                for (int k = 0; k < 5; ++k) fo[k] = isfinite(fh[k]) ? lo * fh[k] + (R(1.0) - lo) * flf[k] : flf[k];
             }
         }
 #pragma unroll

@@ -50,6 +50,17 @@ constexpr int kNumTileCfgs = 10;
             case MODE_CENTRAL: return launch_pass_central(d, cfg, a, s);                   \
             default: return hipErrorInvalidValue;                                          \
         }                                                                                  \
+    }                                                                                      \
+    hipError_t launch_pass32_hybrid(int d, int cfg, const PassArgsT<float>& a, hipStream_t s); \
+    hipError_t launch_pass32_weno(int d, int cfg, const PassArgsT<float>& a, hipStream_t s);   \
+    hipError_t launch_pass32_central(int d, int cfg, const PassArgsT<float>& a, hipStream_t s); \
+    inline hipError_t launch_pass(int d, int mode, int cfg, const PassArgsT<float>& a, hipStream_t s) { \
+        switch (mode) {                                                                    \
+            case MODE_HYBRID: return launch_pass32_hybrid(d, cfg, a, s);                   \
+            case MODE_WENO: return launch_pass32_weno(d, cfg, a, s);                       \
+            case MODE_CENTRAL: return launch_pass32_central(d, cfg, a, s);                 \
+            default: return hipErrorInvalidValue;                                          \
+        }                                                                                  \
     }
 
 namespace precise {

@@ -36,14 +36,17 @@
 #endif
 #if EM_MODE == 0
 #define EM_LAUNCHER launch_pass_hybrid
+#define EM_LAUNCHER32 launch_pass32_hybrid
 #define EM_AMR_LAUNCHER launch_amr_hybrid
 #define EM_ENS_LAUNCHER launch_ens_hybrid
 #elif EM_MODE == 1
 #define EM_LAUNCHER launch_pass_weno
+#define EM_LAUNCHER32 launch_pass32_weno
 #define EM_AMR_LAUNCHER launch_amr_weno
 #define EM_ENS_LAUNCHER launch_ens_weno
 #else
 #define EM_LAUNCHER launch_pass_central
+#define EM_LAUNCHER32 launch_pass32_central
 #define EM_AMR_LAUNCHER launch_amr_central
 #define EM_ENS_LAUNCHER launch_ens_central
 #endif
@@ -59,10 +62,10 @@ namespace EM_VARIANT {
 
 // Conserved state of the virtual ghost cell at pencil coordinate g (outside
 // [0, n)) of the pencil starting at cell `pen`.
-template <int D>
-__device__ __forceinline__ void ghost_state(const PassArgs& a, long long pen, int g, double u[5]) {
+template <int D, class Real>
+__device__ __forceinline__ void ghost_state(const PassArgsT<Real>& a, long long pen, int g, Real u[5]) {
     const long long sD = a.stride[D];
-    auto ld = [&](int x, double* o) {
+    auto ld = [&](int x, Real* o) {
 #pragma unroll
         for (int k = 0; k < 5; ++k) o[k] = a.U[k * a.ncell + pen + (long long)x * sD];
     };
@@ -70,22 +73,22 @@ __device__ __forceinline__ void ghost_state(const PassArgs& a, long long pen, in
 }
 
 // LDS of one tile (declared by the kernels, passed to axis_tile)
-template <int MODE, int T, int P, int NT>
+template <class Real, int MODE, int T, int P, int NT>
 struct TileLds {
     static constexpr bool HYB = MODE == MODE_HYBRID;
-    double sw[5][T + 10][P];
-    double sre[T + 10][P];  // rho_e of each cell
-    double sm[5][T + 5][P];
-    double ss[2][HYB ? T + 10 : 1][P];
-    double sth[HYB ? T + 1 : 1][P];
-    double sred[NT / 64 + 1];
+    Real sw[5][T + 10][P];
+    Real sre[T + 10][P];  // rho_e of each cell
+    Real sm[5][T + 5][P];
+    Real ss[2][HYB ? T + 10 : 1][P];
+    Real sth[HYB ? T + 1 : 1][P];
+    double sred[NT / 64 + 1];  // per-wave CFL rate (in double for the slot max)
     unsigned int scnt[3];
 };
 
 // Tile `bid` of one axis pass described by `a` (every thread of the
 // workgroup calls it; it returns uniformly).
-template <int D, int MODE, int T, int P, int NT>
-__device__ __forceinline__ void axis_tile(const PassArgs& a, int bid, TileLds<MODE, T, P, NT>& L) {
+template <class Real, int D, int MODE, int T, int P, int NT>
+__device__ __forceinline__ void axis_tile(const PassArgsT<Real>& a, int bid, TileLds<Real, MODE, T, P, NT>& L) {
     constexpr int C = T + 10;  // cells held in LDS
     constexpr bool HYB = MODE == MODE_HYBRID;
     auto& sw = L.sw;
@@ -112,7 +115,7 @@ __device__ __forceinline__ void axis_tile(const PassArgs& a, int bid, TileLds<MO
     const long long sF = a.stride[a.fast_axis];
     const long long base = (long long)sI * a.stride[a.slow_axis] + (long long)p0 * sF;
     const long long nc = a.ncell;
-    const Eos e = a.eos;
+    const EosT<Real> e = a.eos;
     if (tid < 3) scnt[tid] = 0;
 
     // ---- 1. load + primitives
@@ -128,7 +131,7 @@ __device__ __forceinline__ void axis_tile(const PassArgs& a, int bid, TileLds<MO
         if (p >= Pv || c >= Tv + 10) continue;
         const int g = c0 - 5 + c;
         const long long pen = base + (long long)p * sF;
-        double u[5], w[5];
+        Real u[5], w[5];
         if (g >= 0 && g < nD) {
             const long long idx = pen + (long long)g * sD;
 #pragma unroll
@@ -142,8 +145,8 @@ __device__ __forceinline__ void axis_tile(const PassArgs& a, int bid, TileLds<MO
         sre[c][p] = eos_rho_e(e, w[0], w[4]);
         if constexpr (HYB) {
             // Euler::sensor, system.rs:409-413
-            ss[0][c][p] = log(fmax(w[0], 1e-300));
-            ss[1][c][p] = log(fmax(w[4] + e.p_inf, 1e-300));
+            ss[0][c][p] = log(fmax(w[0], Real(1e-300)));
+            ss[1][c][p] = log(fmax(w[4] + e.p_inf, Real(1e-300)));
         }
     }
     __syncthreads();
@@ -154,11 +157,11 @@ __device__ __forceinline__ void axis_tile(const PassArgs& a, int bid, TileLds<MO
         const double dt = a.stage_dt_host ? a.stage_dt : a.sc->dt;
         lam = a.ndim_pos * dt / a.dx_d;  // solver.rs:393
     }
-    tile_faces<D, MODE, T, P, NT>(sw, sre, sm, ss, sth, Tv, Pv, c0, nD, e, lam, a.tol, n_shock, n_fb, n_lim);
+    tile_faces<D, MODE, T, P, NT>(sw, sre, sm, ss, sth, Tv, Pv, c0, nD, e, Real(lam), a.tol, n_shock, n_fb, n_lim);
 
     // ---- 5. divergence, fused RK stage and CFL rate
     double rate = 0.0;
-    const double dt = a.sc->dt;
+    const Real dt = Real(a.sc->dt);
     for (int it = tid; it < T * P; it += NT) {
         int c, p;
         if (a.dfast) {
@@ -170,42 +173,42 @@ __device__ __forceinline__ void axis_tile(const PassArgs& a, int bid, TileLds<MO
         }
         if (p >= Pv || c >= Tv) continue;
         const long long idx = base + (long long)p * sF + (long long)(c0 + c) * sD;
-        double r[5];
+        Real r[5];
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
-            const double prev = a.first ? 0.0 : a.R[k * nc + idx];
+            const Real prev = a.first ? Real(0.0) : a.R[k * nc + idx];
             r[k] = prev - (sm[k][c + 1][p] - sm[k][c][p]) * a.inv_dx;
         }
         if (a.rk_stage == 0) {
 #pragma unroll
             for (int k = 0; k < 5; ++k) a.R[k * nc + idx] = r[k];
         } else {
-            double un_[5];
+            Real un_[5];
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
-                const double u0 = a.U0[k * nc + idx];
-                double o;
+                const Real u0 = a.U0[k * nc + idx];
+                Real o;
                 if (a.rk_stage == 1) {
                     o = u0 + dt * r[k];  // the stage input is U^n itself: no second load
                 } else {
-                    const double us = a.U[k * nc + idx];
+                    const Real us = a.U[k * nc + idx];
                     if (a.rk_stage == 2)
-                        o = 0.75 * u0 + 0.25 * (us + dt * r[k]);
+                        o = Real(0.75) * u0 + Real(0.25) * (us + dt * r[k]);
                     else
-                        o = u0 / 3.0 + 2.0 / 3.0 * (us + dt * r[k]);
+                        o = u0 / Real(3.0) + Real(2.0) / Real(3.0) * (us + dt * r[k]);
                 }
                 un_[k] = o;
                 a.Uout[k * nc + idx] = o;
             }
             if (a.want_rate) {
-                double w[5];
+                Real w[5];
                 to_prim(e, un_, w);
-                const double c = sound_speed(e, w[0], w[4]);
-                double ac = -0.0;
+                const Real c = sound_speed(e, w[0], w[4]);
+                Real ac = Real(-0.0);
 #pragma unroll
                 for (int d = 0; d < 3; ++d)
                     if (a.active[d]) ac += (fabs(w[1 + d]) + c) / a.dx_all[d];
-                rate = fmax(rate, ac);
+                rate = fmax(rate, (double)ac);
             }
         }
     }
@@ -242,8 +245,17 @@ __device__ __forceinline__ void axis_tile(const PassArgs& a, int bid, TileLds<MO
 
 template <int D, int MODE, int T, int P, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_WAVES_PER_EU))) void axis_pass(PassArgs a) {
-    __shared__ TileLds<MODE, T, P, NT> L;
-    axis_tile<D, MODE, T, P, NT>(a, xcd_major((int)blockIdx.x, (int)gridDim.x), L);
+    __shared__ TileLds<double, MODE, T, P, NT> L;
+    axis_tile<double, D, MODE, T, P, NT>(a, xcd_major((int)blockIdx.x, (int)gridDim.x), L);
+}
+
+// The FP32 variant of the same pass (the reference's GpuSolver<f32>): state,
+// residual and arithmetic in float.
+template <int D, int MODE, int T, int P, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_WAVES_PER_EU))) void axis_pass_f32(
+    PassArgsT<float> a) {
+    __shared__ TileLds<float, MODE, T, P, NT> L;
+    axis_tile<float, D, MODE, T, P, NT>(a, xcd_major((int)blockIdx.x, (int)gridDim.x), L);
 }
 
 // Ensemble pass: the tiles of every instance in one persistent launch.  Work
@@ -252,7 +264,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_WAVES_PER
 // boundary conditions and device scalars (dt, CFL rate, counters).
 template <int D, int MODE, int T, int P, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_WAVES_PER_EU))) void ens_pass(EnsPassArgs e) {
-    __shared__ TileLds<MODE, T, P, NT> L;
+    __shared__ TileLds<double, MODE, T, P, NT> L;
     const XcdStride xs = xcd_stride();
     for (long long t = xs.first;; t += xs.step) {
         const long long w = xs.item(t);
@@ -266,7 +278,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_WAVES_PER
                 hi = mid;
         }
         const PassArgs& a = e.args[lo];  // read in place (global memory, cached)
-        axis_tile<D, MODE, T, P, NT>(a, (int)(w - e.prefix[lo]), L);
+        axis_tile<double, D, MODE, T, P, NT>(a, (int)(w - e.prefix[lo]), L);
         __syncthreads();
     }
 }
@@ -277,6 +289,12 @@ static hipError_t launch_one(const PassArgs& a, hipStream_t s) {
     hipLaunchKernelGGL((axis_pass<D, MODE, T, P, NT>), dim3((unsigned)nblk), dim3(NT), 0, s, a);
     return hipGetLastError();
 }
+template <int D, int MODE, int T, int P, int NT>
+static hipError_t launch_one(const PassArgsT<float>& a, hipStream_t s) {
+    const long long nblk = (long long)a.ntile * a.ngroup * a.n[a.slow_axis];
+    hipLaunchKernelGGL((axis_pass_f32<D, MODE, T, P, NT>), dim3((unsigned)nblk), dim3(NT), 0, s, a);
+    return hipGetLastError();
+}
 
 template <int D, int MODE, int T, int P, int NT>
 static hipError_t launch_ens_one(const EnsPassArgs& e, unsigned grid, hipStream_t s) {
@@ -285,8 +303,8 @@ static hipError_t launch_ens_one(const EnsPassArgs& e, unsigned grid, hipStream_
 }
 
 // Tile table: must list the shapes of kTileCfgs (em_pass.h) in order.
-template <int D, int MODE>
-static hipError_t launch_cfg(int cfg, const PassArgs& a, hipStream_t s) {
+template <int D, int MODE, class Args>
+static hipError_t launch_cfg(int cfg, const Args& a, hipStream_t s) {
     switch (cfg) {
         case 0: return launch_one<D, MODE, 32, 16, 640>(a, s);
         case 1: return launch_one<D, MODE, 64, 8, 576>(a, s);
@@ -321,6 +339,15 @@ static hipError_t launch_ens_cfg(int cfg, const EnsPassArgs& e, unsigned grid, h
 
 // This translation unit instantiates one flux mode (EM_MODE).
 hipError_t EM_LAUNCHER(int d, int cfg, const PassArgs& a, hipStream_t s) {
+    switch (d) {
+        case 0: return launch_cfg<0, EM_MODE>(cfg, a, s);
+        case 1: return launch_cfg<1, EM_MODE>(cfg, a, s);
+        case 2: return launch_cfg<2, EM_MODE>(cfg, a, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t EM_LAUNCHER32(int d, int cfg, const PassArgsT<float>& a, hipStream_t s) {
     switch (d) {
         case 0: return launch_cfg<0, EM_MODE>(cfg, a, s);
         case 1: return launch_cfg<1, EM_MODE>(cfg, a, s);

@@ -43,8 +43,9 @@ __global__ void k_set_dt(DevScalars* sc, double dt) { sc->dt = dt; }
 
 // Zero fill on the solver's stream (a non-blocking stream does not order
 // against work on the NULL stream, so no NULL-stream memset is used).
-__global__ void k_fill_zero(double* p, long long n) {
-    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (long long)gridDim.x * 256ll) p[i] = 0.0;
+template <class Real>
+__global__ void k_fill_zero(Real* p, long long n) {
+    for (long long i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (long long)gridDim.x * 256ll) p[i] = Real(0);
 }
 
 __global__ void k_set_run(DevScalars* sc, double t_end) {
@@ -112,22 +113,23 @@ __global__ void k_dt_from_rate(DevScalars* sc, double cfl, int run_to) {
 
 // CFL rate max over interior cells: dt_cfl (solver.rs:478-523).  Wave
 // reduction by shuffles, one atomicMax per workgroup on the IEEE bits of a
-// non-negative double.
-__global__ __launch_bounds__(256) void k_rate(const double* __restrict__ U, long long ncell, Eos e, double dx0,
-                                              double dx1, double dx2, int a0, int a1, int a2, DevScalars* sc) {
+// non-negative double (a float rate is widened exactly).
+template <class Real>
+__global__ __launch_bounds__(256) void k_rate(const Real* __restrict__ U, long long ncell, EosT<Real> e, Real dx0,
+                                              Real dx1, Real dx2, int a0, int a1, int a2, DevScalars* sc) {
     __shared__ double red[4];
     double r = 0.0;
     for (long long c = blockIdx.x * 256ll + threadIdx.x; c < ncell; c += (long long)gridDim.x * 256ll) {
-        double u[5], w[5];
+        Real u[5], w[5];
 #pragma unroll
         for (int k = 0; k < 5; ++k) u[k] = U[k * ncell + c];
         to_prim(e, u, w);
-        const double cs = sound_speed(e, w[0], w[4]);
-        double ac = -0.0;
+        const Real cs = sound_speed(e, w[0], w[4]);
+        Real ac = Real(-0.0);
         if (a0) ac += (fabs(w[1]) + cs) / dx0;
         if (a1) ac += (fabs(w[2]) + cs) / dx1;
         if (a2) ac += (fabs(w[3]) + cs) / dx2;
-        r = fmax(r, ac);
+        r = fmax(r, (double)ac);
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) r = fmax(r, __shfl_xor(r, off));
@@ -141,8 +143,9 @@ __global__ __launch_bounds__(256) void k_rate(const double* __restrict__ U, long
 }
 
 // Interior SoA -> ghost-padded SoA (padded extents m[3], ghost offsets g[3]).
-__global__ void k_scatter(const double* __restrict__ U, long long ncell, int n0, int n1, int n2,
-                          double* __restrict__ Pd, long long len, int m1, int m2, int g0, int g1, int g2) {
+template <class Real>
+__global__ void k_scatter(const Real* __restrict__ U, long long ncell, int n0, int n1, int n2,
+                          Real* __restrict__ Pd, long long len, int m1, int m2, int g0, int g1, int g2) {
     const long long c = blockIdx.x * 256ll + threadIdx.x;
     if (c >= ncell) return;
     const int k = (int)(c % n2);
@@ -157,9 +160,9 @@ __global__ void k_scatter(const double* __restrict__ U, long long ncell, int n0,
 // pencil over the whole padded transverse plane, layers gi = 0..ng-1 low and
 // high (solver.rs:278-348).  Sweeps run x, y, z in turn so edges and corners
 // come out as in the reference.
-template <int D>
-__global__ void k_ghost_sweep(double* __restrict__ Pd, long long len, int m0, int m1, int m2, int n, int ng, Eos e,
-                              int k0, int k1, const double* __restrict__ st) {
+template <int D, class Real>
+__global__ void k_ghost_sweep(Real* __restrict__ Pd, long long len, int m0, int m1, int m2, int n, int ng,
+                              EosT<Real> e, int k0, int k1, const Real* __restrict__ st) {
     const int mm[3] = {m0, m1, m2};
     constexpr int E0 = D == 0 ? 1 : 0;
     constexpr int E1 = D == 2 ? 1 : 2;
@@ -169,15 +172,15 @@ __global__ void k_ghost_sweep(double* __restrict__ Pd, long long len, int m0, in
     const long long sd[3] = {(long long)m1 * m2, (long long)m2, 1ll};
     const long long base = (long long)a * sd[E0] + (long long)b * sd[E1];
     const int kind[2] = {k0, k1};
-    double state[2][5];
+    Real state[2][5];
 #pragma unroll
     for (int q = 0; q < 10; ++q) state[q / 5][q % 5] = st[q];
-    auto ld = [&](int x, double* o) {
+    auto ld = [&](int x, Real* o) {
 #pragma unroll
         for (int v = 0; v < 5; ++v) o[v] = Pd[v * len + base + (long long)(ng + x) * sd[D]];
     };
     for (int gi = 0; gi < ng; ++gi) {
-        double u[5];
+        Real u[5];
         bc_ghost<D>(e, n, gi - ng, kind, state, ld, u);
         const long long lo = base + (long long)gi * sd[D];
 #pragma unroll
@@ -253,25 +256,29 @@ struct Grid {
         stride[2] = 1;
     }
 
-    PassArgs pass_args(int d, DevScalars* sc) const {
-        PassArgs a{};
+    PassArgs pass_args(int d, DevScalars* sc) const { return pass_args_t<double>(d, sc); }
+    // The pass arguments in real type Real (mesh spacing, EOS, boundary
+    // states and tolerance rounded to Real, as the reference's R::from_f64).
+    template <class Real>
+    PassArgsT<Real> pass_args_t(int d, DevScalars* sc) const {
+        PassArgsT<Real> a{};
         a.sc = sc;
         for (int e = 0; e < 3; ++e) {
             a.n[e] = (int)n[e];
             a.stride[e] = stride[e];
-            a.dx_all[e] = dx[e];
+            a.dx_all[e] = Real(dx[e]);
             a.active[e] = act[e];
         }
         a.ncell = ncell;
-        a.eos = eos;
+        a.eos = eos_as<Real>(eos);
         for (int sd = 0; sd < 2; ++sd) {
             a.bc_kind[sd] = bc_kind[d][sd];
-            for (int q = 0; q < 5; ++q) a.bc_state[sd][q] = bc_state[d][sd][q];
+            for (int q = 0; q < 5; ++q) a.bc_state[sd][q] = Real(bc_state[d][sd][q]);
         }
-        a.inv_dx = 1.0 / dx[d];
+        a.inv_dx = Real(1.0 / dx[d]);
         a.dx_d = dx[d];
         a.ndim_pos = positivity ? (double)nactive : 0.0;
-        a.tol = tol;
+        a.tol = Real(tol);
         // transverse axes: the pencils of a workgroup are neighbours along the
         // innermost active transverse axis
         int t[2], nt = 0;
@@ -299,14 +306,21 @@ struct Grid {
     }
 };
 
-struct Solver : Grid {
+// Device state, stepping, CFL, graphs and autotuner of one uniform grid in
+// real type Real: Real = double is the FP64 solver (the adjoint builds on
+// it, below); Real = float is the FP32 variant (the reference's
+// GpuSolver<f32>), whose host interface still takes and returns doubles.
+template <class Real>
+struct SolverCore : Grid {
     int precise = 1;  // reference arithmetic by default
     hipStream_t stream = nullptr;
     hipStream_t rstream = nullptr;  // the backward sweep's segment recompute (default priority)
-    double* U[3] = {nullptr, nullptr, nullptr};  // U^n, stage 1, stage 2
-    double* R = nullptr;
-    double* pad = nullptr;
-    double* bcdev = nullptr;  // [3][2][5] boundary states for the ghost sweep
+    Real* U[3] = {nullptr, nullptr, nullptr};  // U^n, stage 1, stage 2
+    Real* R = nullptr;
+    Real* pad = nullptr;
+    Real* bcdev = nullptr;  // [3][2][5] boundary states for the ghost sweep
+    double* dt_hist = nullptr;  // device dt history of a CFL-driven forward run
+    int64_t dt_hist_cap = 0;
     DevScalars* sc = nullptr;
     // per run mode: 0 run_steps, 1 run_to, 2 forward run recording dt
     hipGraphExec_t graph[3] = {nullptr, nullptr, nullptr};
@@ -314,15 +328,17 @@ struct Solver : Grid {
     int graph_cfg[3][3];
     bool rate_valid = false;
 
-    Solver(const int64_t nn[3], const double lo_[3], const double hi[3], int ng_, const double e[4],
-           const int32_t bk[6], const double bs[30], int mode_, int pos, double cfl_, double tol_, double t0)
+    SolverCore(const int64_t nn[3], const double lo_[3], const double hi[3], int ng_, const double e[4],
+               const int32_t bk[6], const double bs[30], int mode_, int pos, double cfl_, double tol_, double t0)
         : Grid(nn, lo_, hi, ng_, e, bk, bs, mode_, pos, cfl_, tol_) {
         HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-        for (int b = 0; b < 3; ++b) HIPCHK(hipMalloc(&U[b], sizeof(double) * 5 * ncell));
-        HIPCHK(hipMalloc(&R, sizeof(double) * 5 * ncell));
+        for (int b = 0; b < 3; ++b) HIPCHK(hipMalloc(&U[b], sizeof(Real) * 5 * ncell));
+        HIPCHK(hipMalloc(&R, sizeof(Real) * 5 * ncell));
         HIPCHK(hipMalloc(&sc, sizeof(DevScalars)));
-        HIPCHK(hipMalloc(&bcdev, sizeof(double) * 30));
-        HIPCHK(hipMemcpyAsync(bcdev, &bc_state[0][0][0], sizeof(double) * 30, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMalloc(&bcdev, sizeof(Real) * 30));
+        std::vector<Real> bsr(30);
+        for (int q = 0; q < 30; ++q) bsr[q] = Real((&bc_state[0][0][0])[q]);
+        HIPCHK(hipMemcpyAsync(bcdev, bsr.data(), sizeof(Real) * 30, hipMemcpyHostToDevice, stream));
         std::vector<DevScalars> h(1);
         std::memset(h.data(), 0, sizeof(DevScalars));
         h[0].t = t0;
@@ -334,8 +350,8 @@ struct Solver : Grid {
         for (int d = 0; d < 3; ++d) cfg[d] = 6;
     }
 
-    ~Solver() {
-        free_adjoint();
+    ~SolverCore() {
+        if (dt_hist) (void)hipFree(dt_hist);
         for (auto& gx : graph)
             if (gx) (void)hipGraphExecDestroy(gx);
         for (auto* p : U)
@@ -349,16 +365,16 @@ struct Solver : Grid {
     }
 
     void upload(const double* u) {
-        std::vector<double> h(5 * ncell);
+        std::vector<Real> h(5 * ncell);
         const int m1 = m(1), m2 = m(2);
         for (long long i = 0; i < n[0]; ++i)
             for (long long j = 0; j < n[1]; ++j)
                 for (long long k = 0; k < n[2]; ++k) {
                     const long long q = ((i + g(0)) * m1 + (j + g(1))) * m2 + (k + g(2));
                     const long long c = (i * n[1] + j) * n[2] + k;
-                    for (int v = 0; v < 5; ++v) h[v * ncell + c] = u[q * 5 + v];
+                    for (int v = 0; v < 5; ++v) h[v * ncell + c] = Real(u[q * 5 + v]);
                 }
-        HIPCHK(hipMemcpyAsync(U[0], h.data(), sizeof(double) * 5 * ncell, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(U[0], h.data(), sizeof(Real) * 5 * ncell, hipMemcpyHostToDevice, stream));
         HIPCHK(hipStreamSynchronize(stream));
         rate_valid = false;
     }
@@ -368,14 +384,14 @@ struct Solver : Grid {
     void fill_pad() {
         const long long L = len();
         if (!pad) {
-            HIPCHK(hipMalloc(&pad, sizeof(double) * 5 * L));
+            HIPCHK(hipMalloc(&pad, sizeof(Real) * 5 * L));
             const long long nz = 5 * L;
-            hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)std::min<long long>((nz + 255) / 256, 4096)), dim3(256), 0,
+            hipLaunchKernelGGL(k_fill_zero<Real>, dim3((unsigned)std::min<long long>((nz + 255) / 256, 4096)), dim3(256), 0,
                                stream, pad, nz);
             HIPCHK(hipGetLastError());
         }
         const unsigned nb = (unsigned)((ncell + 255) / 256);
-        hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, stream, U[0], ncell, (int)n[0], (int)n[1], (int)n[2],
+        hipLaunchKernelGGL(k_scatter<Real>, dim3(nb), dim3(256), 0, stream, U[0], ncell, (int)n[0], (int)n[1], (int)n[2],
                            pad, L, m(1), m(2), g(0), g(1), g(2));
         HIPCHK(hipGetLastError());
         for (int d = 0; d < 3; ++d) {
@@ -383,16 +399,17 @@ struct Solver : Grid {
             const int e0 = d == 0 ? 1 : 0, e1 = d == 2 ? 1 : 2;
             const long long np = (long long)m(e0) * m(e1);
             const unsigned gb = (unsigned)((np + 127) / 128);
-            const double* st = bcdev + d * 10;
+            const Real* st = bcdev + d * 10;
+            const EosT<Real> er = eos_as<Real>(eos);
             if (d == 0)
-                hipLaunchKernelGGL(k_ghost_sweep<0>, dim3(gb), dim3(128), 0, stream, pad, L, m(0), m(1), m(2),
-                                   (int)n[0], ng, eos, bc_kind[0][0], bc_kind[0][1], st);
+                hipLaunchKernelGGL((k_ghost_sweep<0, Real>), dim3(gb), dim3(128), 0, stream, pad, L, m(0), m(1), m(2),
+                                   (int)n[0], ng, er, bc_kind[0][0], bc_kind[0][1], st);
             else if (d == 1)
-                hipLaunchKernelGGL(k_ghost_sweep<1>, dim3(gb), dim3(128), 0, stream, pad, L, m(0), m(1), m(2),
-                                   (int)n[1], ng, eos, bc_kind[1][0], bc_kind[1][1], st);
+                hipLaunchKernelGGL((k_ghost_sweep<1, Real>), dim3(gb), dim3(128), 0, stream, pad, L, m(0), m(1), m(2),
+                                   (int)n[1], ng, er, bc_kind[1][0], bc_kind[1][1], st);
             else
-                hipLaunchKernelGGL(k_ghost_sweep<2>, dim3(gb), dim3(128), 0, stream, pad, L, m(0), m(1), m(2),
-                                   (int)n[2], ng, eos, bc_kind[2][0], bc_kind[2][1], st);
+                hipLaunchKernelGGL((k_ghost_sweep<2, Real>), dim3(gb), dim3(128), 0, stream, pad, L, m(0), m(1), m(2),
+                                   (int)n[2], ng, er, bc_kind[2][0], bc_kind[2][1], st);
             HIPCHK(hipGetLastError());
         }
     }
@@ -400,24 +417,24 @@ struct Solver : Grid {
     void download(double* u) {
         fill_pad();
         const long long L = len();
-        std::vector<double> h(5 * L);
-        HIPCHK(hipMemcpyAsync(h.data(), pad, sizeof(double) * 5 * L, hipMemcpyDeviceToHost, stream));
+        std::vector<Real> h(5 * L);
+        HIPCHK(hipMemcpyAsync(h.data(), pad, sizeof(Real) * 5 * L, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
         for (long long q = 0; q < L; ++q)
-            for (int v = 0; v < 5; ++v) u[q * 5 + v] = h[v * L + q];
+            for (int v = 0; v < 5; ++v) u[q * 5 + v] = (double)h[v * L + q];
     }
 
-    PassArgs pass_args(int d) const { return Grid::pass_args(d, sc); }
+    PassArgsT<Real> pass_args(int d) const { return Grid::pass_args_t<Real>(d, sc); }
 
-    hipError_t launch(int d, const PassArgs& a, hipStream_t st = nullptr) {
+    hipError_t launch(int d, const PassArgsT<Real>& a, hipStream_t st = nullptr) {
         if (!st) st = stream;
         return precise ? precise::launch_pass(d, mode, cfg[d], a, st) : fast::launch_pass(d, mode, cfg[d], a, st);
     }
 
     // One stage: residual of Uin over the active axes in order, fused with
     // the SSP-RK3 combination on the last axis (rk_stage 0: residual only).
-    void stage(const double* Uin, double* Uout, int rk_stage, bool want_rate, int stage_dt_host, double stage_dt,
-               hipEvent_t* ev, const double* base = nullptr, hipStream_t st = nullptr) {
+    void stage(const Real* Uin, Real* Uout, int rk_stage, bool want_rate, int stage_dt_host, double stage_dt,
+               hipEvent_t* ev, const Real* base = nullptr, hipStream_t st = nullptr) {
         if (!st) st = stream;
         int last = -1;
         for (int d = 0; d < 3; ++d)
@@ -425,7 +442,7 @@ struct Solver : Grid {
         bool first = true;
         for (int d = 0; d < 3; ++d) {
             if (!active(d)) continue;
-            PassArgs a = pass_args(d);
+            PassArgsT<Real> a = pass_args(d);
             a.U = Uin;
             a.R = R;
             a.U0 = base ? base : U[0];
@@ -466,8 +483,8 @@ struct Solver : Grid {
     void compute_rate() {
         hipLaunchKernelGGL(k_zero_rate, dim3(1), dim3(kSlots), 0, stream, sc);
         long long nb = std::min<long long>((ncell + 255) / 256, 2048);
-        hipLaunchKernelGGL(k_rate, dim3((unsigned)nb), dim3(256), 0, stream, U[0], ncell, eos, dx[0], dx[1], dx[2],
-                           act[0], act[1], act[2], sc);
+        hipLaunchKernelGGL(k_rate<Real>, dim3((unsigned)nb), dim3(256), 0, stream, U[0], ncell, eos_as<Real>(eos),
+                           Real(dx[0]), Real(dx[1]), Real(dx[2]), act[0], act[1], act[2], sc);
         HIPCHK(hipGetLastError());
         rate_valid = true;
     }
@@ -508,8 +525,8 @@ struct Solver : Grid {
         Scalars before = read_scalars();
         reset_run(0.0);
         stage(U[0], nullptr, 0, false, 1, stage_dt, nullptr);
-        std::vector<double> h(5 * ncell);
-        HIPCHK(hipMemcpyAsync(h.data(), R, sizeof(double) * 5 * ncell, hipMemcpyDeviceToHost, stream));
+        std::vector<Real> h(5 * ncell);
+        HIPCHK(hipMemcpyAsync(h.data(), R, sizeof(Real) * 5 * ncell, hipMemcpyDeviceToHost, stream));
         Scalars after = read_scalars();
         const long long L = len();
         std::fill(out, out + 5 * L, 0.0);
@@ -519,7 +536,7 @@ struct Solver : Grid {
                 for (long long k = 0; k < n[2]; ++k) {
                     const long long q = ((i + g(0)) * m1 + (j + g(1))) * m2 + (k + g(2));
                     const long long c = (i * n[1] + j) * n[2] + k;
-                    for (int v = 0; v < 5; ++v) out[q * 5 + v] = h[v * ncell + c];
+                    for (int v = 0; v < 5; ++v) out[q * 5 + v] = (double)h[v * ncell + c];
                 }
         if (stats) {
             stats[0] = faces_per_stage();
@@ -541,7 +558,7 @@ struct Solver : Grid {
     // written straight into the store (u1, u2; unext = U^{n+1}, or null to
     // skip the last stage), with no CFL rate and no clock update (the sweep
     // restores the scalars at its end).
-    void recompute_step(const double* un, double* u1, double* u2, double* unext, double dt, hipStream_t st) {
+    void recompute_step(const Real* un, Real* u1, Real* u2, Real* unext, double dt, hipStream_t st) {
         hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, st, sc, dt);
         HIPCHK(hipGetLastError());
         stage(un, u1, 1, false, 0, 0.0, nullptr, un, st);
@@ -648,7 +665,7 @@ struct Solver : Grid {
             for (int c = 0; c < kNumTileCfgs; ++c) {
                 if (kTileCfgs[c].T > 2 * n[d] && c != 0) continue;
                 cfg[d] = c;
-                PassArgs a = pass_args(d);
+                PassArgsT<Real> a = pass_args(d);
                 a.U = U[0];
                 a.R = R;
                 a.U0 = U[0];
@@ -678,14 +695,21 @@ struct Solver : Grid {
             for (int d = 0; d < 3; ++d) out[d] = cfg[d];
     }
 
+};
+
+// The FP64 solver: SolverCore<double> plus the discrete adjoint.
+struct Solver : SolverCore<double> {
+    Solver(const int64_t nn[3], const double lo_[3], const double hi[3], int ng_, const double e[4],
+           const int32_t bk[6], const double bs[30], int mode_, int pos, double cfl_, double tol_, double t0)
+        : SolverCore<double>(nn, lo_, hi, ng_, e, bk, bs, mode_, pos, cfl_, tol_, t0) {}
+    ~Solver() { free_adjoint(); }
+
     // ---------------------------------------------------------- adjoint
     // Checkpoints of the last forward run (every ck_every steps), its dt
     // sequence, and the device buffers of the backward sweep.
     std::vector<double*> ckpt;  // kept across forward runs; the last run uses nck_used
     int64_t nck_used = 0;
     std::vector<double> fwd_dts;
-    double* dt_hist = nullptr;  // device dt history of a CFL-driven forward run
-    int64_t dt_hist_cap = 0;
     int ck_every = 0;
     double* adjbuf = nullptr;  // A, B, Z, T: 4 x 5 ncell
     int adj_tile_cap = kAdjTileCells;  // interior cells per adjoint tile at most
@@ -1719,7 +1743,8 @@ static void rhs_noalloc(const Grid& G, int precise, double stage_dt, const doubl
 // ------------------------------------------------------------------ C ABI
 
 struct em_solver {
-    em::Solver* s;
+    em::Solver* s;                  // FP64 solver (null for an FP32 handle)
+    em::SolverCore<float>* s32;     // FP32 solver (null for an FP64 handle)
 };
 
 struct em_ensemble {
@@ -1742,10 +1767,25 @@ static int guarded(F&& f) {
 }
 
 #define NEED(h)                                                              \
-    if (!(h) || !(h)->s) {                                                   \
+    if (!(h) || (!(h)->s && !(h)->s32)) {                                    \
         em::g_last_error = "null solver handle";                             \
         return EM_ERR_ARG;                                                   \
     }
+
+// entry points of the FP64 solver only (adjoint, ensemble)
+#define NEED64(h)                                                            \
+    NEED(h)                                                                  \
+    if (!(h)->s) {                                                           \
+        em::g_last_error = "this entry point takes an FP64 solver (em_solver_create)"; \
+        return EM_ERR_ARG;                                                   \
+    }
+
+// f(core) on the handle's solver, FP64 or FP32, with the error convention
+template <class F>
+static int on_core(em_solver* h, F&& f) {
+    if (h->s32) return guarded([&] { f(*h->s32); });
+    return guarded([&] { f(static_cast<em::SolverCore<double>&>(*h->s)); });
+}
 
 extern "C" {
 
@@ -1766,21 +1806,41 @@ int em_solver_create(em_solver** out, const int64_t n[3], const double lo[3], co
             delete s;
             throw;
         }
-        *out = new em_solver{s};
+        *out = new em_solver{s, nullptr};
+    });
+}
+
+int em_solver_create_f32(em_solver** out, const int64_t n[3], const double lo[3], const double hi[3], int32_t ng,
+                         const double eos[4], const int32_t bc_kind[6], const double bc_state[30], int32_t mode,
+                         int32_t positivity, double cfl, double tol, double t0, const double* u) {
+    if (!out || !n || !lo || !hi || !eos || !bc_kind || !u) {
+        em::g_last_error = "null argument";
+        return EM_ERR_ARG;
+    }
+    return guarded([&] {
+        auto* s = new em::SolverCore<float>(n, lo, hi, ng, eos, bc_kind, bc_state, mode, positivity, cfl, tol, t0);
+        s->precise = 0;  // contracted, as the reference's kernels are compiled
+        try {
+            s->upload(u);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        *out = new em_solver{nullptr, s};
     });
 }
 
 int em_solver_destroy(em_solver* h) {
     if (!h) return EM_OK;
     delete h->s;
+    delete h->s32;
     delete h;
     return EM_OK;
 }
 
 int em_solver_set_variant(em_solver* h, int32_t precise) {
     NEED(h);
-    h->s->precise = precise ? 1 : 0;
-    return EM_OK;
+    return on_core(h, [&](auto& c) { c.precise = precise ? 1 : 0; });
 }
 
 int em_solver_set_tiles(em_solver* h, const int32_t cfg[3]) {
@@ -1790,73 +1850,74 @@ int em_solver_set_tiles(em_solver* h, const int32_t cfg[3]) {
             em::g_last_error = "tile configuration out of range";
             return EM_ERR_ARG;
         }
-    for (int d = 0; d < 3; ++d) h->s->cfg[d] = cfg[d];
-    return EM_OK;
+    return on_core(h, [&](auto& c) {
+        for (int d = 0; d < 3; ++d) c.cfg[d] = cfg[d];
+    });
 }
 
 int em_solver_autotune(em_solver* h, int32_t repeats, int32_t cfg_out[3]) {
     NEED(h);
-    return guarded([&] { h->s->autotune(repeats > 0 ? repeats : 3, cfg_out); });
+    return on_core(h, [&](auto& c) { c.autotune(repeats > 0 ? repeats : 3, cfg_out); });
 }
 
 int em_solver_upload(em_solver* h, const double* u) {
     NEED(h);
-    return guarded([&] { h->s->upload(u); });
+    return on_core(h, [&](auto& c) { c.upload(u); });
 }
 
 int em_solver_download(em_solver* h, double* u) {
     NEED(h);
-    return guarded([&] { h->s->download(u); });
+    return on_core(h, [&](auto& c) { c.download(u); });
 }
 
 int em_solver_rhs(em_solver* h, double stage_dt, double* rhs, uint64_t stats[4]) {
     NEED(h);
-    return guarded([&] { h->s->rhs(stage_dt, rhs, stats); });
+    return on_core(h, [&](auto& c) { c.rhs(stage_dt, rhs, stats); });
 }
 
 int em_solver_step(em_solver* h, double dt) {
     NEED(h);
-    return guarded([&] { h->s->step(dt); });
+    return on_core(h, [&](auto& c) { c.step(dt); });
 }
 
 int em_solver_dt_cfl(em_solver* h, double* dt) {
     NEED(h);
-    return guarded([&] { *dt = h->s->dt_cfl(); });
+    return on_core(h, [&](auto& c) { *dt = c.dt_cfl(); });
 }
 
 int em_solver_run_to(em_solver* h, double t_end, int64_t* steps) {
     NEED(h);
-    return guarded([&] {
-        int64_t k = h->s->run_to(t_end);
+    return on_core(h, [&](auto& c) {
+        int64_t k = c.run_to(t_end);
         if (steps) *steps = k;
     });
 }
 
 int em_solver_run_steps(em_solver* h, int64_t nsteps, double pass_ms[3]) {
     NEED(h);
-    return guarded([&] { h->s->run_steps(nsteps, pass_ms); });
+    return on_core(h, [&](auto& c) { c.run_steps(nsteps, pass_ms); });
 }
 
 int em_solver_counters(em_solver* h, uint64_t out[3]) {
     NEED(h);
-    return guarded([&] {
-        auto d = h->s->read_scalars();
+    return on_core(h, [&](auto& c) {
+        auto d = c.read_scalars();
         for (int q = 0; q < 3; ++q) out[q] = d.counters[q];
     });
 }
 
 int em_solver_time(em_solver* h, double* t) {
     NEED(h);
-    return guarded([&] { *t = h->s->read_scalars().t; });
+    return on_core(h, [&](auto& c) { *t = c.read_scalars().t; });
 }
 
 int em_solver_synchronize(em_solver* h) {
     NEED(h);
-    return guarded([&] { em::hip_check(hipStreamSynchronize(h->s->stream), "hipStreamSynchronize"); });
+    return on_core(h, [&](auto& c) { em::hip_check(hipStreamSynchronize(c.stream), "hipStreamSynchronize"); });
 }
 
 int em_solver_adjoint_rhs(em_solver* h, double stage_dt, const double* lam, double* out) {
-    NEED(h);
+    NEED64(h);
     if (!lam || !out) {
         em::g_last_error = "null argument";
         return EM_ERR_ARG;
@@ -1865,7 +1926,7 @@ int em_solver_adjoint_rhs(em_solver* h, double stage_dt, const double* lam, doub
 }
 
 int em_solver_set_adjoint_tile(em_solver* h, int32_t cells) {
-    NEED(h);
+    NEED64(h);
     if (cells < 10) {
         em::g_last_error = "adjoint tile size must be at least 10 cells";
         return EM_ERR_ARG;
@@ -1883,12 +1944,12 @@ int em_solver_set_adjoint_tile(em_solver* h, int32_t cells) {
 }
 
 int em_solver_forward(em_solver* h, int64_t nsteps, int32_t checkpoint_every, const double* dts, double* dts_out) {
-    NEED(h);
+    NEED64(h);
     return guarded([&] { h->s->forward(nsteps, checkpoint_every, dts, dts_out); });
 }
 
 int em_solver_backward(em_solver* h, const double* dj_final, double* dj_initial) {
-    NEED(h);
+    NEED64(h);
     if (!dj_final || !dj_initial) {
         em::g_last_error = "null argument";
         return EM_ERR_ARG;

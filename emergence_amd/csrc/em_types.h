@@ -28,25 +28,29 @@ struct alignas(256) DevScalars {
 };
 
 // One axis pass over interior-only SoA state arrays (variable plane stride
-// `ncell`, cell index (i*n1 + j)*n2 + k).
-struct PassArgs {
-    const double* U;       // stage input
-    double* R;             // residual accumulator
-    const double* U0;      // U^n (RK combination)
-    double* Uout;          // RK output (may equal U0 at stage 3)
+// `ncell`, cell index (i*n1 + j)*n2 + k), in real type Real (double, or
+// float for the FP32 variant).  The limiter's lambda is formed in double
+// from dt (the device scalars hold dt in double) and then rounded, as the
+// reference's R::from_f64(ndim * dt / dx).
+template <class Real>
+struct PassArgsT {
+    const Real* U;         // stage input
+    Real* R;               // residual accumulator
+    const Real* U0;        // U^n (RK combination)
+    Real* Uout;            // RK output (may equal U0 at stage 3)
     DevScalars* sc;
     int n[3];
     long long stride[3];
     long long ncell;
-    Eos eos;
+    EosT<Real> eos;
     int bc_kind[2];
-    double bc_state[2][5];
-    double inv_dx;         // 1 / dx_d (divergence)
+    Real bc_state[2][5];
+    Real inv_dx;           // 1 / dx_d (divergence)
     double ndim_pos;       // D if the positivity limiter is on, else 0
     double dx_d;           // dx_d (limiter lambda = ndim*dt/dx_d)
-    double dx_all[3];      // CFL rate: sum over active axes of max_speed / dx
+    Real dx_all[3];        // CFL rate: sum over active axes of max_speed / dx
     int active[3];
-    double tol;            // hybrid sensor tolerance
+    Real tol;              // hybrid sensor tolerance
     int first;             // residual input is zero (first active axis)
     int rk_stage;          // 0: store residual only; 1..3: fused SSP-RK3 stage
     int want_rate;         // accumulate the CFL rate of the RK output
@@ -56,6 +60,7 @@ struct PassArgs {
     int stage_dt_host;     // 0: lambda from sc->dt
     double stage_dt;       // used when stage_dt_host != 0
 };
+using PassArgs = PassArgsT<double>;
 
 // Ensemble pass: per-instance pass arguments and the prefix sums of the
 // instances' tile counts (ninst + 1 entries; total = prefix[ninst]).

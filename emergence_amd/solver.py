@@ -39,9 +39,17 @@ def case_init(case_id, n, lo, hi, ng=6, params=None):
 
 class HipSolver:
     def __init__(self, n, lo, hi, u, ng=6, eos=(1.4, 0.0, 0.0, 0.0), bc=None, mode="hybrid", positivity=True,
-                 cfl=0.5, tol=1e-3, t0=0.0, precise=True):
+                 cfl=0.5, tol=1e-3, t0=0.0, precise=None, real="f64"):
         """bc: [[kind_lo, kind_hi]] * 3 with kinds as names, or (name, state5)
-        tuples for Dirichlet / characteristic sides; default periodic."""
+        tuples for Dirichlet / characteristic sides; default periodic.
+        real: "f64" (the default) or "f32" (state and arithmetic in FP32 on
+        the device, em_solver_create_f32).  precise: no FMA contraction;
+        default True for f64, False for f32."""
+        if real not in ("f64", "f32"):
+            raise ValueError("real must be 'f64' or 'f32'")
+        self.real = real
+        if precise is None:
+            precise = real == "f64"
         self.lib = _lib.load()
         self.n = tuple(int(v) for v in n)
         self.ng = int(ng)
@@ -59,9 +67,10 @@ class HipSolver:
                         states[(2 * d + s) * 5 + q] = float(st[q])
         u = np.ascontiguousarray(u, dtype=np.float64).reshape(self.shape)
         h = C.c_void_p()
-        _lib.check(self.lib.em_solver_create(C.byref(h), (C.c_int64 * 3)(*self.n), (C.c_double * 3)(*lo),
-                                             (C.c_double * 3)(*hi), self.ng, (C.c_double * 4)(*eos), kinds, states,
-                                             MODE[mode], int(positivity), cfl, tol, t0, _dp(u)))
+        create = self.lib.em_solver_create if real == "f64" else self.lib.em_solver_create_f32
+        _lib.check(create(C.byref(h), (C.c_int64 * 3)(*self.n), (C.c_double * 3)(*lo), (C.c_double * 3)(*hi),
+                          self.ng, (C.c_double * 4)(*eos), kinds, states, MODE[mode], int(positivity), cfl, tol, t0,
+                          _dp(u)))
         self.h = h
         self.set_variant(precise)
 

@@ -61,6 +61,19 @@ int em_solver_create(em_solver** out, const int64_t n[3], const double lo[3], co
                      const double eos[4], const int32_t bc_kind[6], const double bc_state[30], int32_t mode,
                      int32_t positivity, double cfl, double tol, double t0, const double* u);
 
+/* GpuSolver::<f32>::from_cpu (lib.rs:26-48 Real for f32, 186-282): the same
+ * snapshot with state, residual and arithmetic in FP32 on the device (EOS,
+ * mesh spacing, boundary states and tolerance rounded to float, the
+ * limiter's lambda formed in double from dt and rounded, as the reference's
+ * R::from_f64).  The host side still passes and receives doubles: upload
+ * rounds to float, download/rhs widen.  The returned handle takes every
+ * em_solver_* call below except the adjoint ones (FP64 only, EM_ERR_ARG).
+ * Multiply-adds are contracted by default, as the reference's kernels are
+ * compiled (em_solver_set_variant(s, 1) turns contraction off). */
+int em_solver_create_f32(em_solver** out, const int64_t n[3], const double lo[3], const double hi[3], int32_t ng,
+                         const double eos[4], const int32_t bc_kind[6], const double bc_state[30], int32_t mode,
+                         int32_t positivity, double cfl, double tol, double t0, const double* u);
+
 int em_solver_destroy(em_solver* s);
 
 /* Arithmetic variant: precise != 0 evaluates every expression without FMA
