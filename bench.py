@@ -72,6 +72,9 @@ def parse():
     ap.add_argument("--no-amr", action="store_true", help="skip the block-AMR workload")
     ap.add_argument("--amr-only", action="store_true", help="only the block-AMR workload (profiling)")
     ap.add_argument("--amr-steps", type=int, default=100)
+    ap.add_argument("--no-amr-group", action="store_true", help="skip the uneven-block AMR group")
+    ap.add_argument("--amr-group-only", action="store_true", help="only the uneven-block AMR group")
+    ap.add_argument("--amr-group-steps", type=int, default=20)
     ap.add_argument("--amr-per-block-steps", type=int, default=3)
     ap.add_argument("--amr-no-graph", action="store_true",
                     help="batched AMR steps launched eagerly, not replayed as graphs (for counter collection)")
@@ -368,6 +371,63 @@ def adjoint_bench(em, args):
 AMR_BC = [["outflow", "outflow"], ["reflect", "reflect"], ["reflect", "reflect"]]
 AMR_CASE = dict(base=(8, 4, 4), b=8, levels=3, lo=(0.0, 0.0, 0.0), hi=(2.0, 1.0, 1.0),
                 params=[1.22, 0.5, 1.0, 0.5, 0.5, 0.25, 0.138], eps=1e-2, regrid_every=4, mode="hybrid", cfl=0.4)
+
+
+AMR_GROUP = [dict(b=8, base=(8, 4, 4)), dict(b=16, base=(4, 2, 2)), dict(b=32, base=(2, 1, 1))]
+
+
+def amr_group_bench(em, args):
+    """Hierarchies of uneven block size in one batched step: the 3-D
+    shock-bubble as three hierarchies with 8^3, 16^3 and 32^3 blocks (3
+    levels each), stepped as a group (one flux-pass launch per axis and
+    stage over the leaves of all three) and, for comparison, one after
+    another with their own batched, graph-replayed steps; the states must
+    agree bit for bit."""
+    c = AMR_CASE
+    kw = dict(eps=c["eps"], regrid_every=c["regrid_every"], bc=AMR_BC, mode=c["mode"], cfl=c["cfl"])
+    hiers = []
+    for m in AMR_GROUP:
+        root = em.HipAmr(m["base"], m["b"], c["levels"], c["lo"], c["hi"], case="shock_bubble", params=c["params"],
+                         **kw)
+        hiers.append((m, root.blocks() + (root.download(),)))
+        del root
+    make = [lambda m=m, h=h: em.HipAmr(m["base"], m["b"], c["levels"], c["lo"], c["hi"], blocks=h, **kw)
+            for m, h in hiers]
+    steps = args.amr_group_steps
+    solo = [f() for f in make]
+    for s in solo:
+        s.autotune(3)
+        s.run_steps(1)  # graph capture outside the timed region
+    cells = sum(s.leaf_count() * s.b ** 3 for s in solo)
+    el_solo = 0.0
+    for s in solo:
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.run_steps(steps)
+        s.synchronize()
+        el_solo += time.perf_counter() - t0
+    grp_members = [f() for f in make]
+    grp = em.HipAmrGroup(grp_members)
+    grp.run_steps(1)
+    for g in grp_members:
+        g.synchronize()
+    t0 = time.perf_counter()
+    grp.run_steps(steps)
+    for g in grp_members:
+        g.synchronize()
+    el = time.perf_counter() - t0
+    same = all(np.array_equal(a.download(), b.download()) for a, b in zip(grp_members, solo))
+    return {
+        "workload": "3-D shock-bubble as three hierarchies with 8^3, 16^3 and 32^3 blocks, 3 levels, Hybrid, CFL 0.4, "
+                    "regrid every 4; one flux-pass launch per axis and stage over the leaves of all three",
+        "members": [{"b": s.b, "leaves": s.leaf_count(), "cells": s.leaf_count() * s.b ** 3} for s in solo],
+        "value": cells * steps * 3 / el,
+        "unit": "cell-updates/s per RK stage",
+        "ms_per_step": el / steps * 1e3,
+        "steps": steps,
+        "one_after_another": {"value": cells * steps * 3 / el_solo, "ms_per_step": el_solo / steps * 1e3},
+        "identical_to_members_alone": bool(same),
+    }
 
 
 def amr_bench(em, args):
@@ -684,6 +744,9 @@ def main():
     if args.amr_only:
         print(json.dumps({"amr": amr_bench(em, args)}), flush=True)
         return
+    if args.amr_group_only:
+        print(json.dumps({"amr_group": amr_group_bench(em, args)}), flush=True)
+        return
     if args.adjoint_only:
         print(json.dumps({"adjoint": adjoint_bench(em, args)}), flush=True)
         return
@@ -757,6 +820,8 @@ def main():
         rec["f32"] = f32_bench(em, args, n, u)
     if not args.no_amr:
         rec["amr"] = amr_bench(em, args)
+    if not args.no_amr_group:
+        rec["amr_group"] = amr_group_bench(em, args)
     if not args.no_adjoint:
         rec["adjoint"] = adjoint_bench(em, args)
         rec["adjoint_forward_cost_ratio"] = rec["adjoint"]["cost_ratio"]

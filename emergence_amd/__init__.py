@@ -8,6 +8,6 @@ this package is its Python plumbing.
 """
 from ._lib import EmError, load  # noqa: F401
 from .solver import BC, MODE, HipSolver, case_init, padded_shape  # noqa: F401
-from .amr import AMR_CASE, HipAmr  # noqa: F401
+from .amr import AMR_CASE, HipAmr, HipAmrGroup  # noqa: F401
 from .ensemble import HipEnsemble, post_shock, shock_bubble_2d  # noqa: F401
 from .fiveeq import HipFiveEq  # noqa: F401

@@ -72,6 +72,9 @@ EXPORTS = (
     "em_amr_counters",
     "em_amr_time",
     "em_amr_synchronize",
+    "em_amr_group_create",
+    "em_amr_group_destroy",
+    "em_amr_group_run_steps",
     "em_fiveeq_last_error",
     "em_fiveeq_create",
     "em_fiveeq_destroy",
@@ -183,6 +186,9 @@ def load():
     lib.em_amr_counters.argtypes = [P, U64]
     lib.em_amr_time.argtypes = [P, D]
     lib.em_amr_synchronize.argtypes = [P]
+    lib.em_amr_group_create.argtypes = [C.POINTER(P), C.POINTER(P), C.c_int32]
+    lib.em_amr_group_destroy.argtypes = [P]
+    lib.em_amr_group_run_steps.argtypes = [P, C.c_int64]
     lib.em_fiveeq_last_error.argtypes = []
     lib.em_fiveeq_create.argtypes = [C.POINTER(P), I64, D, D, C.c_int32, D, C.c_int32, I32, D, C.c_int32, C.c_int32,
                                      C.c_double, C.c_double, C.c_double, D]

@@ -190,3 +190,28 @@ class HipAmr:
 
     def synchronize(self):
         _lib.check_amr(self.lib.em_amr_synchronize(self.h))
+
+
+class HipAmrGroup:
+    """Hierarchies of uneven block size stepped together (em_amr_group_*):
+    one flux-pass launch per axis and stage over the leaves of all members.
+    The group borrows its members; keep them alive while it exists."""
+
+    def __init__(self, members):
+        self.lib = _lib.load()
+        self.members = list(members)
+        hs = (C.c_void_p * len(self.members))(*[m.h for m in self.members])
+        h = C.c_void_p()
+        _lib.check_amr(self.lib.em_amr_group_create(C.byref(h), hs, len(self.members)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.em_amr_group_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def run_steps(self, nsteps):
+        _lib.check_amr(self.lib.em_amr_group_run_steps(self.h, int(nsteps)))

@@ -2254,6 +2254,173 @@ static void amr_case(const double x[3], const double* p, double w[5]) {
     }
 }
 
+// Several hierarchies stepped together, each with its own block size (8,
+// 16 or 32 cells a side), dt, tables and regrid schedule: per stage, the
+// restriction, ghost, reflux and RK launches of every member, and one
+// persistent flux-pass launch per axis over the leaves of all members
+// (amr_multi).  Members must share the flux mode, arithmetic variant and
+// active axes, and run batched.  The group borrows the members (it does not
+// own them) and runs them on its own stream; a member's results are those
+// of the member stepped alone, bit for bit.
+struct AmrGroup {
+    std::vector<Amr*> m;
+    hipStream_t stream = nullptr;
+    AmrPassArgs* d_args[3] = {nullptr, nullptr, nullptr};
+    long long* d_prefix[3] = {nullptr, nullptr, nullptr};
+    long long total[3] = {0, 0, 0};
+    int mode = 0, precise = 1;
+    // host images of the uploaded arguments (kept alive for the async copies;
+    // an unchanged step skips the upload)
+    std::vector<AmrPassArgs> h_args[3];
+    std::vector<long long> h_prefix[3];
+
+    explicit AmrGroup(const std::vector<Amr*>& members) : m(members) {
+        if (m.empty()) throw AmrError(EM_ERR_ARG, "an AMR group needs at least one member");
+        mode = m[0]->mode;
+        precise = m[0]->precise;
+        for (Amr* a : m) {
+            if (!a) throw AmrError(EM_ERR_ARG, "null AMR member");
+            if (a->mode != mode || a->precise != precise)
+                throw AmrError(EM_ERR_ARG, "AMR group members must share the flux mode and arithmetic variant");
+            for (int d = 0; d < 3; ++d)
+                if (a->act[d] != m[0]->act[d])
+                    throw AmrError(EM_ERR_ARG, "AMR group members must share their active axes");
+            if (!a->batched) throw AmrError(EM_ERR_ARG, "AMR group members must run batched");
+        }
+        hchk(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+        const int n = (int)m.size();
+        for (int d = 0; d < 3; ++d) {
+            hchk(hipMalloc(&d_args[d], sizeof(AmrPassArgs) * n), "hipMalloc");
+            hchk(hipMalloc(&d_prefix[d], sizeof(long long) * (n + 1)), "hipMalloc");
+        }
+    }
+    ~AmrGroup() {
+        for (int d = 0; d < 3; ++d) {
+            if (d_args[d]) (void)hipFree(d_args[d]);
+            if (d_prefix[d]) (void)hipFree(d_prefix[d]);
+        }
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    // the members' work goes to the group's stream while the group runs
+    struct Borrow {
+        AmrGroup& g;
+        std::vector<hipStream_t> own;
+        explicit Borrow(AmrGroup& g_) : g(g_) {
+            for (Amr* a : g.m) {
+                hchk(hipStreamSynchronize(a->stream), "hipStreamSynchronize");
+                own.push_back(a->stream);
+                a->stream = g.stream;
+            }
+        }
+        ~Borrow() {
+            (void)hipStreamSynchronize(g.stream);
+            for (size_t i = 0; i < g.m.size(); ++i) g.m[i]->stream = own[i];
+        }
+    };
+
+    // per-axis pass arguments of every member (their tables are fixed for a step)
+    void build_args() {
+        const int n = (int)m.size();
+        int last = -1;
+        for (int d = 0; d < 3; ++d)
+            if (m[0]->act[d]) last = d;
+        bool first = true;
+        for (int d = 0; d < 3; ++d) {
+            if (!m[0]->act[d]) continue;
+            std::vector<AmrPassArgs> args(n);
+            std::vector<long long> prefix(n + 1, 0);
+            for (int i = 0; i < n; ++i) {
+                Amr& a = *m[i];
+                const int P = amr_multi_pencils(a.B);
+                AmrPassArgs& pa = args[i];
+                pa.a = a.dev;
+                pa.leaves = a.d_leaves;
+                pa.leaf_level = a.d_leaf_level;
+                pa.leaf0 = 0;
+                pa.nleaves = (int)a.leaves.size();
+                pa.ngroup = (a.bcells / a.bd[d] + P - 1) / P;
+                pa.first = first ? 1 : 0;
+                pa.dfast = d == last ? 1 : 0;
+                pa.ndim_pos = a.positivity ? (double)a.ndim : 0.0;
+                pa.tol = a.tol;
+                pa.stage_dt_host = 0;
+                pa.stage_dt = 0.0;
+                pa.gsrc = a.d_gsrc;
+                prefix[i + 1] = prefix[i] + (long long)pa.nleaves * pa.ngroup;
+            }
+            total[d] = prefix[n];
+            first = false;
+            const bool same = h_args[d].size() == args.size() &&
+                              std::memcmp(h_args[d].data(), args.data(), sizeof(AmrPassArgs) * n) == 0 &&
+                              h_prefix[d] == prefix;
+            if (same) continue;
+            // the previous step's kernels may still read the old arguments
+            hchk(hipStreamSynchronize(stream), "hipStreamSynchronize");
+            h_args[d] = std::move(args);
+            h_prefix[d] = std::move(prefix);
+            hchk(hipMemcpyAsync(d_args[d], h_args[d].data(), sizeof(AmrPassArgs) * n, hipMemcpyHostToDevice, stream),
+                 "hipMemcpyAsync");
+            hchk(hipMemcpyAsync(d_prefix[d], h_prefix[d].data(), sizeof(long long) * (n + 1), hipMemcpyHostToDevice,
+                                stream),
+                 "hipMemcpyAsync");
+        }
+    }
+
+    void passes() {
+        for (int d = 0; d < 3; ++d) {
+            if (!m[0]->act[d] || total[d] == 0) continue;
+            const AmrMultiArgs ma{d_args[d], d_prefix[d], (int)m.size(), total[d]};
+            const unsigned grid = (unsigned)std::min<long long>(total[d], 2048);
+            hchk(precise ? em::precise::launch_amr_multi(d, mode, ma, grid, stream)
+                         : em::fast::launch_amr_multi(d, mode, ma, grid, stream),
+                 "amr_multi");
+        }
+    }
+
+    // nsteps CFL steps of every member (dt of each member from its own
+    // state, on the device), with each member's regrids when due.
+    void run_steps(int64_t nsteps) {
+        Borrow b(*this);
+        for (Amr* a : m) hipLaunchKernelGGL(k_amr_set_run, dim3(1), dim3(1), 0, stream, a->sc);
+        for (int64_t q = 0; q < nsteps; ++q) {
+            for (Amr* a : m) a->tables();
+            build_args();
+            for (Amr* a : m) {
+                a->rate_kernels();
+                hipLaunchKernelGGL(k_amr_dt_from_rate, dim3(1), dim3(1), 0, stream, a->sc, a->cfl);
+                a->rk(0);
+            }
+            for (int s = 1; s <= 3; ++s) {
+                for (Amr* a : m) {
+                    a->restrict_all();
+                    a->fill_ghosts(false);
+                }
+                passes();
+                for (Amr* a : m) {
+                    a->refluxes();
+                    a->rk(s);
+                }
+            }
+            for (Amr* a : m) {
+                hipLaunchKernelGGL(k_amr_advance, dim3(1), dim3(1), 0, stream, a->sc);
+                a->steps += 1;
+            }
+            hchk(hipGetLastError(), "AMR group step");
+            for (Amr* a : m)
+                if (a->steps >= a->next_regrid) {
+                    a->t = a->read_t();
+                    a->check_err();
+                    a->regrid_if_due();
+                }
+        }
+        for (Amr* a : m) {
+            a->t = a->read_t();
+            a->check_err();
+        }
+    }
+};
+
 thread_local std::string g_last_error_amr;
 
 }  // namespace em
@@ -2262,6 +2429,10 @@ thread_local std::string g_last_error_amr;
 
 struct em_amr {
     em::Amr* a;
+};
+
+struct em_amr_group {
+    em::AmrGroup* g;
 };
 
 template <class F>
@@ -2481,6 +2652,37 @@ int em_amr_time(em_amr* h, double* t) {
 int em_amr_synchronize(em_amr* h) {
     ANEED(h);
     return aguard([&] { em::hchk(hipStreamSynchronize(h->a->stream), "hipStreamSynchronize"); });
+}
+
+int em_amr_group_create(em_amr_group** out, em_amr* const* members, int32_t n) {
+    if (!out || !members || n < 1) {
+        em::g_last_error_amr = "null argument or empty group";
+        return EM_ERR_ARG;
+    }
+    std::vector<em::Amr*> m;
+    for (int i = 0; i < n; ++i) {
+        if (!members[i] || !members[i]->a) {
+            em::g_last_error_amr = "null AMR handle";
+            return EM_ERR_ARG;
+        }
+        m.push_back(members[i]->a);
+    }
+    return aguard([&] { *out = new em_amr_group{new em::AmrGroup(m)}; });
+}
+
+int em_amr_group_destroy(em_amr_group* g) {
+    if (!g) return EM_OK;
+    delete g->g;
+    delete g;
+    return EM_OK;
+}
+
+int em_amr_group_run_steps(em_amr_group* g, int64_t nsteps) {
+    if (!g || !g->g) {
+        em::g_last_error_amr = "null AMR group handle";
+        return EM_ERR_ARG;
+    }
+    return aguard([&] { g->g->run_steps(nsteps); });
 }
 
 }  // extern "C"

@@ -81,6 +81,21 @@ struct AmrPassArgs {
     const int* gsrc;
 };
 
+// Several hierarchies (of any block sizes 8, 16, 32) in one batched pass:
+// per-hierarchy pass arguments (leaf0 = 0, nleaves = all its leaves, ngroup
+// for the group kernel's pencils per work item amr_multi_pencils(b)) and the
+// prefix sums of their work items (n + 1 entries, total = prefix[n]).
+struct AmrMultiArgs {
+    const AmrPassArgs* args;
+    const long long* prefix;
+    int n;
+    long long total;
+};
+constexpr int kAmrMultiThreads = 256;
+// pencils per work item of the group kernel: (b + 5) * P midpoints fill
+// about the 256 threads (208, 252, 222)
+inline int amr_multi_pencils(int b) { return b == 8 ? 16 : b == 16 ? 12 : 6; }
+
 // Shapes of the batched pass with a compiled kernel (launch_amr_b): P pencils
 // per work item, NT threads; shape 0 is the default, em_amr_autotune times
 // the others.

@@ -31,6 +31,17 @@ constexpr int kNumTileCfgs = 10;
             default: return hipErrorInvalidValue;                                          \
         }                                                                                  \
     }                                                                                      \
+    hipError_t launch_amr_multi_hybrid(int d, const AmrMultiArgs& m, unsigned grid, hipStream_t s);   \
+    hipError_t launch_amr_multi_weno(int d, const AmrMultiArgs& m, unsigned grid, hipStream_t s);     \
+    hipError_t launch_amr_multi_central(int d, const AmrMultiArgs& m, unsigned grid, hipStream_t s);  \
+    inline hipError_t launch_amr_multi(int d, int mode, const AmrMultiArgs& m, unsigned grid, hipStream_t s) { \
+        switch (mode) {                                                                    \
+            case MODE_HYBRID: return launch_amr_multi_hybrid(d, m, grid, s);               \
+            case MODE_WENO: return launch_amr_multi_weno(d, m, grid, s);                   \
+            case MODE_CENTRAL: return launch_amr_multi_central(d, m, grid, s);             \
+            default: return hipErrorInvalidValue;                                          \
+        }                                                                                  \
+    }                                                                                      \
     hipError_t launch_ens_hybrid(int d, int cfg, const EnsPassArgs& e, unsigned grid, hipStream_t s); \
     hipError_t launch_ens_weno(int d, int cfg, const EnsPassArgs& e, unsigned grid, hipStream_t s);   \
     hipError_t launch_ens_central(int d, int cfg, const EnsPassArgs& e, unsigned grid, hipStream_t s); \

@@ -38,16 +38,19 @@
 #define EM_LAUNCHER launch_pass_hybrid
 #define EM_LAUNCHER32 launch_pass32_hybrid
 #define EM_AMR_LAUNCHER launch_amr_hybrid
+#define EM_AMR_MULTI_LAUNCHER launch_amr_multi_hybrid
 #define EM_ENS_LAUNCHER launch_ens_hybrid
 #elif EM_MODE == 1
 #define EM_LAUNCHER launch_pass_weno
 #define EM_LAUNCHER32 launch_pass32_weno
 #define EM_AMR_LAUNCHER launch_amr_weno
+#define EM_AMR_MULTI_LAUNCHER launch_amr_multi_weno
 #define EM_ENS_LAUNCHER launch_ens_weno
 #else
 #define EM_LAUNCHER launch_pass_central
 #define EM_LAUNCHER32 launch_pass32_central
 #define EM_AMR_LAUNCHER launch_amr_central
+#define EM_AMR_MULTI_LAUNCHER launch_amr_multi_central
 #define EM_ENS_LAUNCHER launch_ens_central
 #endif
 #ifndef EM_WAVES_PER_EU
@@ -361,6 +364,15 @@ hipError_t EM_ENS_LAUNCHER(int d, int cfg, const EnsPassArgs& e, unsigned grid, 
         case 0: return launch_ens_cfg<0, EM_MODE>(cfg, e, grid, s);
         case 1: return launch_ens_cfg<1, EM_MODE>(cfg, e, grid, s);
         case 2: return launch_ens_cfg<2, EM_MODE>(cfg, e, grid, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t EM_AMR_MULTI_LAUNCHER(int d, const AmrMultiArgs& m, unsigned grid, hipStream_t s) {
+    switch (d) {
+        case 0: return launch_amr_multi_d<0, EM_MODE>(m, grid, s);
+        case 1: return launch_amr_multi_d<1, EM_MODE>(m, grid, s);
+        case 2: return launch_amr_multi_d<2, EM_MODE>(m, grid, s);
         default: return hipErrorInvalidValue;
     }
 }

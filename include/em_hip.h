@@ -376,6 +376,22 @@ int em_amr_time(em_amr* h, double* t);
 
 int em_amr_synchronize(em_amr* h);
 
+/* Hierarchies of uneven block size in one batched step (no counterpart in
+ * the reference, whose hierarchy has one block size, amr.rs:49-63): a group
+ * of AMR handles, each with its own block size (8, 16 or 32 cells a side),
+ * dt, tables and regrid schedule, stepped together.  Per RK stage the flux
+ * pass is one persistent launch per axis over the leaves of every member
+ * (all block sizes); restriction, ghosts, refluxing and the RK update are
+ * the members' own launches.  Members must share the flux mode, arithmetic
+ * variant and active axes and run batched; the group borrows them (destroy
+ * the group first) and a member's results are those of em_amr_run_steps on
+ * it alone, bit for bit.  Error messages come from em_amr_last_error(). */
+typedef struct em_amr_group em_amr_group;
+int em_amr_group_create(em_amr_group** out, em_amr* const* members, int32_t n);
+int em_amr_group_destroy(em_amr_group* g);
+/* nsteps CFL steps of every member (each member's dt from its own state). */
+int em_amr_group_run_steps(em_amr_group* g, int64_t nsteps);
+
 /* ------------------------------------------------------------------------
  * Two-phase Kapila five-equation system on a uniform grid: drop-in for
  *   GpuSolver<f64> built with GpuSystem::FiveEq { eos, kapila }
