@@ -81,6 +81,7 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload")
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint workload (profiling)")
     ap.add_argument("--adj-n", type=int, default=512)
+    ap.add_argument("--adj-tile", type=int, default=0, help="interior cells per adjoint tile at most (0: library default)")
     ap.add_argument("--adj-steps", type=int, default=400)
     ap.add_argument("--adj-checkpoint", type=int, default=20)
     ap.add_argument("--adj-fd-cells", type=int, default=64, help="central-difference checks (seeded cells)")
@@ -277,6 +278,8 @@ def adjoint_bench(em, args):
     bc = [["outflow", "outflow"], ["outflow", "outflow"], ["periodic", "periodic"]]
     mk = lambda u: em.HipSolver((n, n, 1), (0, 0, 0), (1, 1, 1), u, bc=bc, mode="weno", cfl=0.5)
     g = mk(u0)
+    if args.adj_tile:
+        g.set_adjoint_tile(args.adj_tile)
     # warm-up: one full forward/backward pair of the same length, so that the
     # timed pair allocates nothing (checkpoints, dt history, segment stores,
     # adjoint work arrays) and replays already-instantiated graphs
