@@ -306,6 +306,69 @@ __global__ __launch_bounds__(kGhostCopyThreads) void k_amr_ghost_copy(AmrDev a, 
     }
 }
 
+// Coarse-fine ghost slabs (task kind 2) with the prediction's gathered
+// coarse patch (capA cells and their validity) and its intermediate passes
+// (capB cells) in LDS, the last pass storing straight into the slab: the
+// arithmetic of k_amr_ghost without its global scratch.  Launched when the
+// largest slab's buffers fit in LDS.
+__global__ __launch_bounds__(kGhostThreads) void k_amr_ghost_lds(AmrDev a, AmrWeights W,
+                                                      const GhostTask* __restrict__ tasks, int n, int capA,
+                                                      int capB) {
+    extern __shared__ double lds[];
+    double* A = lds;
+    double* ok = A + 5 * capA;
+    double* Bf = ok + capA;
+    for (int ti = blockIdx.x; ti < n; ti += gridDim.x) {
+        const GhostTask tk = tasks[ti];
+        const int d = tk.axis, side = tk.side;
+        const int E0 = d == 0 ? 1 : 0, E1 = d == 2 ? 1 : 2;
+        const int B = a.bd[d];
+        const int ng = a.ng;
+        const int s = 2 * d + side;
+        int fmin[3], fmax[3];
+        for (int e = 0; e < 3; ++e) {
+            if (!a.act[e]) {
+                fmin[e] = fmax[e] = 0;
+            } else if (e == d) {
+                fmin[e] = side == 0 ? tk.idx[e] * B - ng : tk.idx[e] * B + B;
+                fmax[e] = fmin[e] + ng - 1;
+            } else {
+                fmin[e] = tk.idx[e] * a.bd[e];
+                fmax[e] = fmin[e] + a.bd[e] - 1;
+            }
+        }
+        predict_box(a, W, tk.level, fmin, fmax, A, ok, capA, Bf, capB, [&](int, const int c[3], const double* v) {
+            const int t = c[E0] * a.bd[E1] + c[E1];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) a.G[amr_g(a, q, tk.slot, s, c[d], t)] = v[q];
+        });
+    }
+}
+
+// Restriction of the ancestors of one level cut into sub-boxes
+// (restrict_subbox), one (ancestor, sub-box) per workgroup at a time, the
+// passes after the first in LDS (cap1 + cap2 cells and their validity).
+// Same entries and write masks as k_amr_restrict; no blocks are excluded.
+constexpr int kRestrictSubThreads = 256;
+__global__ __launch_bounds__(kRestrictSubThreads) void k_amr_restrict_sub(AmrDev a, AmrWeights W,
+                                                               const int* __restrict__ ids,
+                                                               const int* __restrict__ ent, int n, int level, int w,
+                                                               RestrictSub rs) {
+    extern __shared__ double lds[];
+    double* T1 = lds;
+    double* ok1 = T1 + 5 * rs.cap1;
+    double* T2 = ok1 + rs.cap1;
+    double* ok2 = T2 + 5 * rs.cap2;
+    __shared__ int bids[kBoxBlocks];
+    __shared__ int pmap[3 * kPatchLen];
+    const long long total = (long long)n * rs.nsubt;
+    for (long long wi = blockIdx.x; wi < total; wi += gridDim.x) {
+        const int i = (int)(wi / rs.nsubt), sb = (int)(wi - (long long)i * rs.nsubt);
+        const int bi[3] = {ent[4 * i], ent[4 * i + 1], ent[4 * i + 2]};
+        restrict_subbox(a, W, level, ids[i], bi, ent[4 * i + 3], w, rs, sb, T1, ok1, T2, ok2, bids, pmap);
+    }
+}
+
 // Restriction of a list of ancestors of one level.  Entry i holds the block
 // coordinates and a write mask: bit (o0+1)*9 + (o1+1)*3 + (o2+1) selects the
 // edge box towards neighbour offset o (cells within w of that side), bit 27
@@ -593,6 +656,11 @@ struct Amr {
     double* small_scr = nullptr;
     long long big_cap = 0, small_cap = 0;
     int big_wg = 64, small_wg = 512;
+    // LDS launches of the batched stages (0 bytes: the global-scratch kernel)
+    int ghost_capA = 0, ghost_capB = 0;
+    size_t ghost_lds = 0;
+    RestrictSub rsub{};
+    size_t rsub_lds = 0;
     // tables
     std::vector<int> leaves, leaf_level;
     std::vector<std::vector<int>> edge, edge_ent;
@@ -716,6 +784,7 @@ struct Amr {
         big_wg = (int)std::max<long long>(16, std::min<long long>(1024, (2ll << 30) / (96ll * big_cap)));
         HCHK(hipMalloc(&big_scr, sizeof(double) * 12 * big_cap * big_wg));
         HCHK(hipMalloc(&small_scr, sizeof(double) * 12 * small_cap * small_wg));
+        lds_plans();
         dev.map = d_map;
         dev.sc = sc;
         for (int d = 0; d < 3; ++d) {
@@ -1594,6 +1663,99 @@ struct Amr {
         apply_edits();
     }
 
+    // Buffers of the LDS kernels, in cells.  k_amr_ghost_lds: the coarse
+    // patch of a slab and the outputs of the passes before the last, over
+    // every slab axis, side and parity of the slab's first cell (as
+    // predict_box sizes them).  k_amr_restrict_sub: sub-boxes shaped as
+    // below.  Either runs when its buffers fit in 76 KiB (two workgroups per
+    // CU) and, for the restriction, no axis has a characteristic boundary
+    // (its patches take restrict_block's gathered path).
+    void lds_plans() {
+        constexpr size_t kLdsCap = 76 * 1024;
+        const int dlast = act[2] ? 2 : act[1] ? 1 : 0;
+        long long ca = 0, cb = 0;
+        for (int ds = 0; ds < 3; ++ds) {
+            if (!act[ds]) continue;
+            for (int side = 0; side < 2; ++side)
+                for (int k = 0; k < 4; ++k) {
+                    long long fmin[3], fmax[3], sz[3];
+                    for (int e = 0; e < 3; ++e) {
+                        if (!act[e]) {
+                            fmin[e] = fmax[e] = 0;
+                        } else if (e == ds) {
+                            fmin[e] = side == 0 ? (long long)k * bd[e] - ng : (long long)k * bd[e] + bd[e];
+                            fmax[e] = fmin[e] + ng - 1;
+                        } else {
+                            fmin[e] = (long long)k * bd[e];
+                            fmax[e] = fmin[e] + bd[e] - 1;
+                        }
+                        sz[e] = act[e] ? (hdiv(fmax[e], 2) + 3) - (hdiv(fmin[e], 2) - 3) + 1 : 1;
+                    }
+                    ca = std::max(ca, sz[0] * sz[1] * sz[2]);
+                    int pass = 0;
+                    for (int d = 0; d < 3; ++d) {
+                        if (!act[d]) continue;
+                        sz[d] = fmax[d] - fmin[d] + 1;
+                        if (d == dlast) break;
+                        long long& c = (pass & 1) ? ca : cb;  // Bf, then A, then Bf
+                        c = std::max(c, sz[0] * sz[1] * sz[2]);
+                        ++pass;
+                    }
+                }
+        }
+        const size_t gb = sizeof(double) * (6 * ca + 5 * cb);
+        if (gb <= kLdsCap) {
+            ghost_capA = (int)ca;
+            ghost_capB = (int)cb;
+            ghost_lds = gb;
+            HCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_amr_ghost_lds),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)gb));
+        }
+        bool charac = false;
+        int d0 = -1;
+        for (int d = 0; d < 3; ++d)
+            if (act[d]) {
+                charac = charac || bck[d][0] == BC_CHARACTERISTIC || bck[d][1] == BC_CHARACTERISTIC;
+                if (d0 < 0) d0 = d;
+            }
+        RestrictSub r{};
+        auto F = [&](int e) { return act[e] ? 2 * r.S[e] + 8 : 1; };
+        auto rbytes = [&] {
+            r.cap1 = r.S[d0];
+            for (int e = 0; e < 3; ++e)
+                if (e != d0) r.cap1 *= F(e);
+            r.cap2 = ndim == 3 ? r.S[0] * r.S[1] * F(2) : 0;
+            return sizeof(double) * 6 * ((size_t)r.cap1 + r.cap2);
+        };
+        for (int e = 0; e < 3; ++e) r.S[e] = act[e] ? bd[e] : 1;
+        if (d0 >= 0) {
+            // cuts across the first pass's axis repeat no work: one coarse
+            // layer per box in 3-D, at most 16 in 2-D; then halve the widest
+            // other axis while the buffers exceed the cap (each halving
+            // recomputes part of the first pass's patch positions)
+            r.S[d0] = ndim == 3 ? 1 : std::min(bd[d0], 16);
+            while (rbytes() > kLdsCap) {
+                int e = -1;
+                for (int k = 0; k < 3; ++k)
+                    if (act[k] && k != d0 && r.S[k] > 1 && (e < 0 || r.S[k] >= r.S[e])) e = k;
+                if (e < 0 || r.S[e] <= 4) break;
+                r.S[e] /= 2;
+            }
+        }
+        r.nsubt = 1;
+        for (int e = 0; e < 3; ++e) {
+            r.nsub[e] = bd[e] / r.S[e];
+            r.nsubt *= r.nsub[e];
+        }
+        const size_t rb = rbytes();
+        if (!charac && d0 >= 0 && B >= 4 && 2 * B + 8 <= kPatchLen && rb <= kLdsCap) {
+            rsub = r;
+            rsub_lds = rb;
+            HCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_amr_restrict_sub),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)rb));
+        }
+    }
+
     // width of restrict_all's edge boxes (amr.rs:782)
     int box_w() const { return std::min(ng + 3, std::max(bd[0], std::max(bd[1], bd[2]))); }
 
@@ -1648,7 +1810,11 @@ struct Amr {
         for (int l = levels - 2; l >= 0; --l) {
             const int n = (int)edge[l].size();
             if (!n) continue;
-            if (batched) {
+            if (batched && rsub_lds) {
+                const unsigned g = (unsigned)std::min<long long>((long long)n * rsub.nsubt, 16384);
+                hipLaunchKernelGGL(k_amr_restrict_sub, dim3(g), dim3(kRestrictSubThreads), rsub_lds, stream, dev, W,
+                                   d_edge_ids[l], d_edge_idx[l], n, l, box_w(), rsub);
+            } else if (batched) {
                 const unsigned g = (unsigned)std::min(n, big_wg);
                 hipLaunchKernelGGL(k_amr_restrict, dim3(g), dim3(kPatchThreads), 0, stream, dev, W, d_edge_ids[l], d_edge_idx[l],
                                    n, l, box_w(), big_scr, big_cap, nullptr, nullptr);
@@ -1677,7 +1843,11 @@ struct Amr {
                                    per);
                 HCHK(hipGetLastError());
             }
-            if (np) {
+            if (np && ghost_lds) {
+                hipLaunchKernelGGL(k_amr_ghost_lds, dim3((unsigned)std::min(np, 16384)), dim3(kGhostThreads), ghost_lds,
+                                   stream, dev, W, d_ghost_pred, np, ghost_capA, ghost_capB);
+                HCHK(hipGetLastError());
+            } else if (np) {
                 const unsigned g = (unsigned)std::min(np, small_wg);
                 hipLaunchKernelGGL(k_amr_ghost, dim3(g), dim3(kGhostThreads), 0, stream, dev, W, d_ghost_pred, np, small_scr,
                                    small_cap);
@@ -1864,6 +2034,7 @@ struct Amr {
         HCHK(hipStreamSynchronize(stream));
         if (h[0].err & 6)
             throw AmrError(EM_ERR_STATE, "ghost cells could not be filled (hierarchy not 2:1 balanced?)");
+        if (h[0].err & 8) throw AmrError(EM_ERR_STATE, "restriction patch outside the sub-box kernel's range");
         if (h[0].err & 1) throw AmrError(EM_ERR_COLLAPSE, "time step collapsed");
     }
 

@@ -61,7 +61,7 @@ constexpr int kBoxBlocks = 512;
 struct BoxBlocks {
     int b0[3], nbk[3];
 };
-__device__ bool box_blocks(const AmrDev& a, int l, const int g0[3], const int sz[3], int* ids, BoxBlocks& bb) {
+__device__ __forceinline__ bool box_blocks(const AmrDev& a, int l, const int g0[3], const int sz[3], int* ids, BoxBlocks& bb) {
     int total = 1;
     for (int d = 0; d < 3; ++d) {
         if (a.act[d]) {
@@ -183,15 +183,17 @@ __device__ void box_bc(const AmrDev& a, int l, const int g0[3], const int sz[3],
 }
 
 // predict_box of amr.rs:548-586 by one workgroup: the level-(lf-1) patch is
-// gathered into scratch (gather_box: blocks of that level, periodic wrap,
+// gathered into A / ok (gather_box: blocks of that level, periodic wrap,
 // bc_box for cells outside the domain), then one 6-point pass per active
-// axis maps coarse to fine indices.  The result (box fmin..fmax of level lf)
-// is left in scratch; returns the buffer holding it.
-__device__ double* predict_region(const AmrDev& a, const AmrWeights& W, int lf, const int fmin[3], const int fmax[3],
-                                  double* scr, long long cap) {
-    double* A = scr;
-    double* Bf = scr + 5 * cap;
-    double* ok = scr + 10 * cap;
+// axis maps coarse to fine indices, writing Bf, then A, then Bf.  The last
+// pass hands every cell of the result (box fmin..fmax of level lf) to
+// fin(i, c, v) instead: its row-major index and coordinates in the box and
+// its five values.  A and ok hold capA cells, Bf capB; the buffers may be
+// global scratch or LDS.  Ends with a barrier.
+template <class Fin>
+__device__ __forceinline__ void predict_box(const AmrDev& a, const AmrWeights& W, int lf, const int fmin[3],
+                                            const int fmax[3], double* A, double* ok, long long capA, double* Bf,
+                                            long long capB, Fin&& fin) {
     __shared__ int any_out;
     const int lc = lf - 1;
     int cmin[3], sz[3];
@@ -217,14 +219,14 @@ __device__ double* predict_region(const AmrDev& a, const AmrWeights& W, int lf, 
         if (id == -2) {
             ok[i] = 0.0;
             any_out = 1;
-            for (int q = 0; q < 5; ++q) A[q * cap + i] = 0.0;
+            for (int q = 0; q < 5; ++q) A[q * capA + i] = 0.0;
             continue;
         }
         if (id < 0) {
             atomicOr(&a.sc->err, 2);  // the hierarchy is 2:1 balanced (see below)
-            for (int q = 0; q < 5; ++q) A[q * cap + i] = 0.0;
+            for (int q = 0; q < 5; ++q) A[q * capA + i] = 0.0;
         } else {
-            for (int q = 0; q < 5; ++q) A[q * cap + i] = a.U[amr_u(a, q, id, cell)];
+            for (int q = 0; q < 5; ++q) A[q * capA + i] = a.U[amr_u(a, q, id, cell)];
         }
         ok[i] = 1.0;
     }
@@ -248,7 +250,7 @@ __device__ double* predict_region(const AmrDev& a, const AmrWeights& W, int lf, 
         if (out) {
             ok[i] = 0.0;
             any_out = 1;
-            for (int q = 0; q < 5; ++q) A[q * cap + i] = 0.0;
+            for (int q = 0; q < 5; ++q) A[q * capA + i] = 0.0;
             continue;
         }
         int cell;
@@ -257,21 +259,23 @@ __device__ double* predict_region(const AmrDev& a, const AmrWeights& W, int lf, 
             // the hierarchy is 2:1 balanced, so the level below a leaf's
             // neighbourhood exists; report otherwise
             atomicOr(&a.sc->err, 2);
-            for (int q = 0; q < 5; ++q) A[q * cap + i] = 0.0;
+            for (int q = 0; q < 5; ++q) A[q * capA + i] = 0.0;
         } else {
-            for (int q = 0; q < 5; ++q) A[q * cap + i] = a.U[amr_u(a, q, id, cell)];
+            for (int q = 0; q < 5; ++q) A[q * capA + i] = a.U[amr_u(a, q, id, cell)];
         }
         ok[i] = 1.0;
     }
     __syncthreads();
     if (any_out) {
-        box_bc(a, lc, cmin, sz, A, ok, cap);
+        box_bc(a, lc, cmin, sz, A, ok, capA);
         for (int i = threadIdx.x; i < tot; i += blockDim.x)
             if (ok[i] == 0.0) atomicOr(&a.sc->err, 4);
         __syncthreads();
     }
     double* in = A;
     double* out = Bf;
+    long long icap = capA, ocap = capB;
+    const int dlast = a.act[2] ? 2 : a.act[1] ? 1 : 0;
     for (int d = 0; d < 3; ++d) {
         if (!a.act[d]) continue;
         int nsz[3] = {sz[0], sz[1], sz[2]};
@@ -290,18 +294,38 @@ __device__ double* predict_region(const AmrDev& a, const AmrWeights& W, int lf, 
                 const int src = (s3[0] * sz[1] + s3[1]) * sz[2] + s3[2];
                 const double wgt = W.wp[par][kk];
 #pragma unroll
-                for (int q = 0; q < 5; ++q) acc[q] += wgt * in[q * cap + src];
+                for (int q = 0; q < 5; ++q) acc[q] += wgt * in[q * icap + src];
             }
+            if (d == dlast) {
+                fin(i, c, acc);
+            } else {
 #pragma unroll
-            for (int q = 0; q < 5; ++q) out[q * cap + i] = acc[q];
+                for (int q = 0; q < 5; ++q) out[q * ocap + i] = acc[q];
+            }
         }
         __syncthreads();
         double* tmp = in;
         in = out;
         out = tmp;
+        const long long tc = icap;
+        icap = ocap;
+        ocap = tc;
         for (int e = 0; e < 3; ++e) sz[e] = nsz[e];
     }
-    return in;
+}
+
+// predict_box with the buffers in global scratch (12 arrays of cap doubles);
+// the result is left in scratch, in the buffer returned.
+__device__ double* predict_region(const AmrDev& a, const AmrWeights& W, int lf, const int fmin[3], const int fmax[3],
+                                  double* scr, long long cap) {
+    const int nact = a.act[0] + a.act[1] + a.act[2];
+    double* res = (nact & 1) ? scr + 5 * cap : scr;  // the last pass's output buffer
+    predict_box(a, W, lf, fmin, fmax, scr, scr + 10 * cap, cap, scr + 5 * cap, cap,
+                [&](int i, const int*, const double* v) {
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) res[q * cap + i] = v[q];
+                });
+    return res;
 }
 
 // A box of level-l cells g0 .. g0+sz-1 read in place: at box cell c, the
@@ -332,7 +356,7 @@ struct FinePatch {
 
 // map[d][p] for every box position p along every axis; every thread of the
 // workgroup calls it, a barrier must follow before the map is read
-__device__ void patch_map(const AmrDev& a, const FinePatch& P, int* map) {
+__device__ __forceinline__ void patch_map(const AmrDev& a, const FinePatch& P, int* map) {
     for (int i = threadIdx.x; i < 3 * kPatchLen; i += blockDim.x) {
         const int d = i / kPatchLen, p = i - d * kPatchLen;
         const int g = P.f0[d] + p;
@@ -450,14 +474,19 @@ __device__ __forceinline__ void patch_load(const AmrDev& a, int r, long long bas
 // output cell c of the (patch with b cells along D) box takes the first of
 // the shifted 6-point stencils whose cells are all valid, else the average
 // of its two children, else zero and invalid.
+// The output box (extents nsz) starts at off: coarse cell off[D] along D,
+// patch position off[e] along the other axes.
 template <int D>
 __device__ __forceinline__ void restrict_first_pass(const AmrDev& a, const AmrWeights& W, const FinePatch& P,
-                                                    const int nsz[3], double* out, double* ook, long long cap) {
+                                                    const int nsz[3], const int off[3], double* out, double* ook,
+                                                    long long cap) {
     const int shifts[5] = {0, -1, 1, -2, 2};
     const int ntot = nsz[0] * nsz[1] * nsz[2];
     for (int i = threadIdx.x; i < ntot; i += blockDim.x) {
         int c[3];
         unflat(i, nsz, c);
+#pragma unroll
+        for (int e = 0; e < 3; ++e) c[e] += off[e];
         int s3[3] = {c[0], c[1], c[2]};
         const int base = 2 * c[D] + 2;
         unsigned okw = 0;  // bit j: patch cell base - 2 + j is valid
@@ -512,6 +541,71 @@ __device__ __forceinline__ void restrict_first_pass(const AmrDev& a, const AmrWe
     }
 }
 
+// One output cell c of a restriction pass along d over a box held in a
+// buffer (values in[q * cap + p], validity iok[p], extents sz, coarse cell
+// c[d] over box positions 2 c[d] .. 2 c[d] + 9): the first shifted 6-point
+// stencil whose cells are all valid, else the average of the two children,
+// else zero; returns the validity (1 or 0).
+__device__ __forceinline__ double restrict_cell(const AmrWeights& W, int d, const int c[3], const int sz[3],
+                                                const double* in, const double* iok, long long cap, double acc[5]) {
+    const int shifts[5] = {0, -1, 1, -2, 2};
+    int s3[3] = {c[0], c[1], c[2]};
+    auto at_src = [&](int pos) {
+        s3[d] = pos;
+        return (s3[0] * sz[1] + s3[1]) * sz[2] + s3[2];
+    };
+    const int base = 2 * c[d] + 2;
+    bool done = false;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) acc[q] = 0.0;
+    for (int m = 0; m < 5 && !done; ++m) {
+        const int st = base + shifts[m];
+        if (st < 0 || st + 5 >= sz[d]) continue;
+        bool all = true;
+        for (int kk = 0; kk < 6; ++kk) all = all && iok[at_src(st + kk)] != 0.0;
+        if (all) {
+            for (int kk = 0; kk < 6; ++kk) {
+                const int sidx = at_src(st + kk);
+#pragma unroll
+                for (int q = 0; q < 5; ++q) acc[q] += W.wts[m][kk] * in[q * cap + sidx];
+            }
+            done = true;
+        }
+    }
+    double okv = done ? 1.0 : 0.0;
+    if (!done) {
+        const int pa = at_src(base + 2), pb = at_src(base + 3);
+        if (iok[pa] != 0.0 && iok[pb] != 0.0) {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) acc[q] = 0.5 * (in[q * cap + pa] + in[q * cap + pb]);
+            okv = 1.0;
+        }
+    }
+    return okv;
+}
+
+// Whether the write mask of k_amr_restrict (bit 27: the whole block; bit
+// (o0+1)*9 + (o1+1)*3 + (o2+1): the edge box of width w towards offset o)
+// selects any cell of the block box lo .. hi (inclusive; lo == hi: one cell).
+__device__ __forceinline__ bool restrict_selects(const AmrDev& a, int mask, int w, const int lo[3], const int hi[3]) {
+    if ((mask >> 27) & 1) return true;
+    // per axis, offset -1 holds cells c < w, 0 all, 1 cells c >= b - w
+    for (int o0 = -1; o0 <= 1; ++o0)
+        for (int o1 = -1; o1 <= 1; ++o1)
+            for (int o2 = -1; o2 <= 1; ++o2) {
+                if (!((mask >> ((o0 + 1) * 9 + (o1 + 1) * 3 + (o2 + 1))) & 1)) continue;
+                const int o[3] = {o0, o1, o2};
+                bool meets = true;
+                for (int d = 0; d < 3; ++d) {
+                    const int blo = o[d] == 1 ? (a.bd[d] >= w ? a.bd[d] - w : 0) : 0;
+                    const int bhi = o[d] == -1 ? (w < a.bd[d] ? w : a.bd[d]) : a.bd[d];
+                    meets = meets && hi[d] >= blo && lo[d] < bhi;
+                }
+                if (meets) return true;
+            }
+    return false;
+}
+
 // restrict_box of amr.rs:622-730 for a whole ancestor block (level l) by
 // one workgroup: over the (2b+8)^D fine patch with its validity mask (level
 // l+1 blocks, periodic wrap, bc_box), one pass per axis with the centred
@@ -563,7 +657,6 @@ __device__ __forceinline__ void restrict_block(const AmrDev& a, const AmrWeights
         __syncthreads();
     }
     general = general || !P.cached;  // the in-place reads use the block-id cache
-    const int shifts[5] = {0, -1, 1, -2, 2};
     double* in = A;
     double* out = Bf;
     double* iok = okA;
@@ -601,12 +694,13 @@ __device__ __forceinline__ void restrict_block(const AmrDev& a, const AmrWeights
     } else if (d0 < 3) {
         int nsz[3] = {sz[0], sz[1], sz[2]};
         nsz[d0] = a.bd[d0];
+        const int zero3[3] = {0, 0, 0};
         if (d0 == 0)
-            restrict_first_pass<0>(a, W, P, nsz, Bf, okB, cap);
+            restrict_first_pass<0>(a, W, P, nsz, zero3, Bf, okB, cap);
         else if (d0 == 1)
-            restrict_first_pass<1>(a, W, P, nsz, Bf, okB, cap);
+            restrict_first_pass<1>(a, W, P, nsz, zero3, Bf, okB, cap);
         else
-            restrict_first_pass<2>(a, W, P, nsz, Bf, okB, cap);
+            restrict_first_pass<2>(a, W, P, nsz, zero3, Bf, okB, cap);
         __syncthreads();
         in = Bf;
         out = A;
@@ -623,37 +717,8 @@ __device__ __forceinline__ void restrict_block(const AmrDev& a, const AmrWeights
         for (int i = threadIdx.x; i < ntot; i += blockDim.x) {
             int c[3];
             unflat(i, nsz, c);
-            int s3[3] = {c[0], c[1], c[2]};
-            auto at_src = [&](int pos) {
-                s3[d] = pos;
-                return (s3[0] * sz[1] + s3[1]) * sz[2] + s3[2];
-            };
-            const int base = 2 * c[d] + 2;
-            bool done = false;
-            double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-            for (int m = 0; m < 5 && !done; ++m) {
-                const int st = base + shifts[m];
-                if (st < 0 || st + 5 >= sz[d]) continue;
-                bool all = true;
-                for (int kk = 0; kk < 6; ++kk) all = all && iok[at_src(st + kk)] != 0.0;
-                if (all) {
-                    for (int kk = 0; kk < 6; ++kk) {
-                        const int sidx = at_src(st + kk);
-#pragma unroll
-                        for (int q = 0; q < 5; ++q) acc[q] += W.wts[m][kk] * in[q * cap + sidx];
-                    }
-                    done = true;
-                }
-            }
-            double okv = done ? 1.0 : 0.0;
-            if (!done) {
-                const int pa = at_src(base + 2), pb = at_src(base + 3);
-                if (iok[pa] != 0.0 && iok[pb] != 0.0) {
-#pragma unroll
-                    for (int q = 0; q < 5; ++q) acc[q] = 0.5 * (in[q * cap + pa] + in[q * cap + pb]);
-                    okv = 1.0;
-                }
-            }
+            double acc[5];
+            const double okv = restrict_cell(W, d, c, sz, in, iok, cap, acc);
 #pragma unroll
             for (int q = 0; q < 5; ++q) out[q * cap + i] = acc[q];
             ook[i] = okv;
@@ -668,31 +733,123 @@ __device__ __forceinline__ void restrict_block(const AmrDev& a, const AmrWeights
         for (int e = 0; e < 3; ++e) sz[e] = nsz[e];
     }
     for (int i = threadIdx.x; i < a.bcells; i += blockDim.x) {
-        bool wr = (mask >> 27) & 1;
-        if (!wr) {
-            int c[3];
-            unflat(i, a.bd, c);
-            // offsets whose edge box holds the cell: per axis -1 (c < w),
-            // 0 (always), 1 (c >= b - w)
-            for (int o0 = -1; o0 <= 1 && !wr; ++o0)
-                for (int o1 = -1; o1 <= 1 && !wr; ++o1)
-                    for (int o2 = -1; o2 <= 1 && !wr; ++o2) {
-                        if (!((mask >> ((o0 + 1) * 9 + (o1 + 1) * 3 + (o2 + 1))) & 1)) continue;
-                        const int o[3] = {o0, o1, o2};
-                        bool in_box = true;
-                        for (int d = 0; d < 3; ++d) {
-                            const int lo = o[d] == 1 ? (a.bd[d] >= w ? a.bd[d] - w : 0) : 0;
-                            const int hi = o[d] == -1 ? (w < a.bd[d] ? w : a.bd[d]) : a.bd[d];
-                            in_box = in_box && c[d] >= lo && c[d] < hi;
-                        }
-                        wr = in_box;
-                    }
-        }
-        if (wr)
+        int c[3];
+        unflat(i, a.bd, c);
+        if (restrict_selects(a, mask, w, c, c))
 #pragma unroll
             for (int q = 0; q < 5; ++q) a.U[amr_u(a, q, slot, i)] = in[q * cap + i];
     }
     __syncthreads();
+}
+
+// Restriction of one sub-box of an ancestor block (k_amr_restrict_sub).
+// The coarse block is cut into boxes of S[e] cells per axis (nsub[e] of
+// them); the stencils of box cells cs .. cs + S - 1 reach patch positions
+// 2 cs .. 2 cs + 2 S + 7 along every active axis (F = 2 S + 8 of them), so
+// in 3-D the first pass (along x, read in place from the fine blocks)
+// produces an S0 x F1 x F2 box, the second an S0 x S1 x F2 box, both held
+// in LDS, and the last pass stores into the block.  The fine patch, its validity and every stencil are those of
+// restrict_block, in whole-block patch coordinates: each cell gets the same
+// value, whichever box computes it.
+struct RestrictSub {
+    int S[3], nsub[3], nsubt;
+    int cap1, cap2;  // cells of the first and second pass outputs
+};
+
+__device__ __forceinline__ void restrict_subbox(const AmrDev& a, const AmrWeights& W, int l, int slot,
+                                                const int idx[3], int mask, int w, const RestrictSub& rs, int sb,
+                                                double* T1, double* ok1, double* T2, double* ok2, int* bids,
+                                                int* pmap) {
+    int cs[3], ce[3];
+    unflat(sb, rs.nsub, cs);
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        cs[e] *= rs.S[e];
+        ce[e] = cs[e] + rs.S[e] - 1;
+    }
+    if (!restrict_selects(a, mask, w, cs, ce)) return;  // the same for every thread
+    FinePatch P;
+    P.l = l + 1;
+    int d0 = 3;
+    for (int d = 0; d < 3; ++d) {
+        P.f0[d] = a.act[d] ? 2 * idx[d] * a.bd[d] - 4 : 0;
+        P.sz[d] = a.act[d] ? 2 * a.bd[d] + 8 : 1;
+        P.n3[d] = amr_n_level(a, P.l, d);
+        if (a.act[d] && d < d0) d0 = d;
+    }
+    P.map = pmap;
+    P.xs = nullptr;
+    P.nx = 0;
+    patch_map(a, P, pmap);
+    P.cached = box_blocks(a, P.l, P.f0, P.sz, bids, P.bb);  // synchronises
+    P.bids = bids;
+    if (!P.cached || d0 == 3) {  // excluded by the launch (blocks of 4 cells or more)
+        if (threadIdx.x == 0) atomicOr(&a.sc->err, 8);
+        __syncthreads();
+        return;
+    }
+    int sz[3], off[3];
+    for (int e = 0; e < 3; ++e) {
+        sz[e] = !a.act[e] ? 1 : e == d0 ? rs.S[e] : 2 * rs.S[e] + 8;
+        off[e] = !a.act[e] ? 0 : e == d0 ? cs[e] : 2 * cs[e];
+    }
+    if (d0 == 0)
+        restrict_first_pass<0>(a, W, P, sz, off, T1, ok1, rs.cap1);
+    else if (d0 == 1)
+        restrict_first_pass<1>(a, W, P, sz, off, T1, ok1, rs.cap1);
+    else
+        restrict_first_pass<2>(a, W, P, sz, off, T1, ok1, rs.cap1);
+    __syncthreads();
+    const int dlast = a.act[2] ? 2 : a.act[1] ? 1 : 0;
+    auto store = [&](int i, const int c[3], const double* v) {
+        int g[3];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) g[e] = cs[e] + c[e];
+        if (restrict_selects(a, mask, w, g, g)) {
+            const int cell = (g[0] * a.bd[1] + g[1]) * a.bd[2] + g[2];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) a.U[amr_u(a, q, slot, cell)] = v[q];
+        }
+    };
+    if (dlast == d0) {  // one active axis: the first pass was the last
+        for (int i = threadIdx.x; i < sz[0] * sz[1] * sz[2]; i += blockDim.x) {
+            int c[3];
+            unflat(i, sz, c);
+            double v[5];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) v[q] = T1[q * rs.cap1 + i];
+            store(i, c, v);
+        }
+        __syncthreads();
+        return;
+    }
+    const double* in = T1;
+    const double* iok = ok1;
+    long long icap = rs.cap1;
+    for (int d = d0 + 1; d < 3; ++d) {
+        if (!a.act[d]) continue;
+        int nsz[3] = {sz[0], sz[1], sz[2]};
+        nsz[d] = rs.S[d];
+        const int ntot = nsz[0] * nsz[1] * nsz[2];
+        for (int i = threadIdx.x; i < ntot; i += blockDim.x) {
+            int c[3];
+            unflat(i, nsz, c);
+            double acc[5];
+            const double okv = restrict_cell(W, d, c, sz, in, iok, icap, acc);
+            if (d == dlast) {
+                store(i, c, acc);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 5; ++q) T2[q * rs.cap2 + i] = acc[q];
+                ok2[i] = okv;
+            }
+        }
+        __syncthreads();
+        in = T2;
+        iok = ok2;
+        icap = rs.cap2;
+        for (int e = 0; e < 3; ++e) sz[e] = nsz[e];
+    }
 }
 
 }  // namespace em
