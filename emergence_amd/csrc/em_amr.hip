@@ -318,7 +318,10 @@ __global__ __launch_bounds__(kGhostThreads) void k_amr_ghost_lds(AmrDev a, AmrWe
     double* A = lds;
     double* ok = A + 5 * capA;
     double* Bf = ok + capA;
-    for (int ti = blockIdx.x; ti < n; ti += gridDim.x) {
+    const XcdStride xs = xcd_stride();  // neighbouring slabs share one L2
+    for (long long t = xs.first;; t += xs.step) {
+        const long long ti = xs.item(t);
+        if (ti >= n) break;
         const GhostTask tk = tasks[ti];
         const int d = tk.axis, side = tk.side;
         const int E0 = d == 0 ? 1 : 0, E1 = d == 2 ? 1 : 2;
@@ -350,7 +353,7 @@ __global__ __launch_bounds__(kGhostThreads) void k_amr_ghost_lds(AmrDev a, AmrWe
 // passes after the first in LDS (cap1 + cap2 cells and their validity).
 // Same entries and write masks as k_amr_restrict; no blocks are excluded.
 constexpr int kRestrictSubThreads = 256;
-__global__ __launch_bounds__(kRestrictSubThreads) void k_amr_restrict_sub(AmrDev a, AmrWeights W,
+__global__ __launch_bounds__(kRestrictSubThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_amr_restrict_sub(AmrDev a, AmrWeights W,
                                                                const int* __restrict__ ids,
                                                                const int* __restrict__ ent, int n, int level, int w,
                                                                RestrictSub rs) {
@@ -362,7 +365,12 @@ __global__ __launch_bounds__(kRestrictSubThreads) void k_amr_restrict_sub(AmrDev
     __shared__ int bids[kBoxBlocks];
     __shared__ int pmap[3 * kPatchLen];
     const long long total = (long long)n * rs.nsubt;
-    for (long long wi = blockIdx.x; wi < total; wi += gridDim.x) {
+    // the sub-boxes of an ancestor read overlapping fine data: keep them on
+    // one XCD (its L2), in chunks of consecutive items
+    const XcdStride xs = xcd_stride();
+    for (long long t = xs.first;; t += xs.step) {
+        const long long wi = xs.item(t);
+        if (wi >= total) break;
         const int i = (int)(wi / rs.nsubt), sb = (int)(wi - (long long)i * rs.nsubt);
         const int bi[3] = {ent[4 * i], ent[4 * i + 1], ent[4 * i + 2]};
         restrict_subbox(a, W, level, ids[i], bi, ent[4 * i + 3], w, rs, sb, T1, ok1, T2, ok2, bids, pmap);
@@ -1811,7 +1819,7 @@ struct Amr {
             const int n = (int)edge[l].size();
             if (!n) continue;
             if (batched && rsub_lds) {
-                const unsigned g = (unsigned)std::min<long long>((long long)n * rsub.nsubt, 16384);
+                const unsigned g = (unsigned)std::min<long long>(((long long)n * rsub.nsubt + 7) / 8 * 8, 16384);
                 hipLaunchKernelGGL(k_amr_restrict_sub, dim3(g), dim3(kRestrictSubThreads), rsub_lds, stream, dev, W,
                                    d_edge_ids[l], d_edge_idx[l], n, l, box_w(), rsub);
             } else if (batched) {
@@ -1844,7 +1852,7 @@ struct Amr {
                 HCHK(hipGetLastError());
             }
             if (np && ghost_lds) {
-                hipLaunchKernelGGL(k_amr_ghost_lds, dim3((unsigned)std::min(np, 16384)), dim3(kGhostThreads), ghost_lds,
+                hipLaunchKernelGGL(k_amr_ghost_lds, dim3((unsigned)std::min((np + 7) / 8 * 8, 16384)), dim3(kGhostThreads), ghost_lds,
                                    stream, dev, W, d_ghost_pred, np, ghost_capA, ghost_capB);
                 HCHK(hipGetLastError());
             } else if (np) {

@@ -276,7 +276,8 @@ __device__ __forceinline__ void predict_box(const AmrDev& a, const AmrWeights& W
     double* out = Bf;
     long long icap = capA, ocap = capB;
     const int dlast = a.act[2] ? 2 : a.act[1] ? 1 : 0;
-    for (int d = 0; d < 3; ++d) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {  // unrolled: arrays indexed by d stay in registers
         if (!a.act[d]) continue;
         int nsz[3] = {sz[0], sz[1], sz[2]};
         nsz[d] = fmax[d] - fmin[d] + 1;
@@ -359,11 +360,16 @@ struct FinePatch {
 __device__ __forceinline__ void patch_map(const AmrDev& a, const FinePatch& P, int* map) {
     for (int i = threadIdx.x; i < 3 * kPatchLen; i += blockDim.x) {
         const int d = i / kPatchLen, p = i - d * kPatchLen;
-        const int g = P.f0[d] + p;
+        // per-axis fields by selection, not by a runtime index (which would
+        // place the arrays in scratch)
+        auto pick = [d](const int* v) { return d == 0 ? v[0] : d == 1 ? v[1] : v[2]; };
+        const int n3 = pick(P.n3), f0 = pick(P.f0), psz = pick(P.sz), act = pick(a.act);
+        const int g = f0 + p;
         int e = p;
-        if (a.act[d] && p < P.sz[d] && (g < 0 || g >= P.n3[d])) {
-            const int side = g >= P.n3[d] ? 1 : 0;
-            const int kind = a.bc_kind[d][side];
+        if (act && p < psz && (g < 0 || g >= n3)) {
+            const int side = g >= n3 ? 1 : 0;
+            const int k2[3] = {a.bc_kind[0][side], a.bc_kind[1][side], a.bc_kind[2][side]};
+            const int kind = pick(k2);
             // box_bc's source position: the boundary cell (outflow) or the
             // mirror cell (reflect), which lies inside the domain because a
             // patch spans -4 .. n3 + 3 and n3 >= 16
@@ -372,17 +378,17 @@ __device__ __forceinline__ void patch_map(const AmrDev& a, const FinePatch& P, i
             if (kind == BC_DIRICHLET) {
                 e = kMapDirichlet | side;
             } else if (kind == BC_OUTFLOW) {
-                src = side == 0 ? 0 : P.n3[d] - 1;
+                src = side == 0 ? 0 : n3 - 1;
                 moved = true;
             } else if (kind == BC_REFLECT) {
-                src = side == 0 ? -1 - g : 2 * P.n3[d] - 1 - g;
+                src = side == 0 ? -1 - g : 2 * n3 - 1 - g;
                 moved = true;
             } else if (kind != BC_PERIODIC) {
                 e = kMapInvalid;  // characteristic boxes take the general path
             }
             if (moved) {
-                const int ps = src - P.f0[d];
-                e = ps >= 0 && ps < P.sz[d] ? ps | (kind == BC_REFLECT ? kMapFlip : 0) : kMapInvalid;
+                const int ps = src - f0;
+                e = ps >= 0 && ps < psz ? ps | (kind == BC_REFLECT ? kMapFlip : 0) : kMapInvalid;
             }
         }
         map[i] = e;
@@ -402,57 +408,6 @@ __device__ __forceinline__ int patch_block(const AmrDev& a, const FinePatch& P, 
     return id;
 }
 
-// source cell s of box cell c; returns -1 (s is read), -2 (invalid) or the
-// Dirichlet boundary 2 d + side; neg collects the negated axes
-__device__ __forceinline__ int patch_source(const FinePatch& P, const int c[3], int s[3], unsigned& neg) {
-    int r = -1;
-    neg = 0;
-#pragma unroll
-    for (int d = 2; d >= 0; --d) {
-        const int e = P.map[d * kPatchLen + c[d]];
-        s[d] = e & 0xfff;
-        if (r == -1) {
-            if (e & kMapDirichlet)
-                r = 2 * d + (e & 1);
-            else if (e & kMapInvalid)
-                r = -2;
-            else if (e & kMapFlip)
-                neg |= 1u << d;
-        }
-    }
-    return r;
-}
-
-// validity of box cell c (needs P.cached)
-__device__ __forceinline__ bool patch_ok(const AmrDev& a, const FinePatch& P, const int c[3]) {
-    int s[3];
-    unsigned neg;
-    const int r = patch_source(P, c, s, neg);
-    if (r != -1) return r >= 0;
-    int cell;
-    const int g[3] = {P.f0[0] + s[0], P.f0[1] + s[1], P.f0[2] + s[2]};
-    return P.bids[box_block_of(a, P.bb, g, &cell)] >= 0;
-}
-
-// box cell c as a load (needs P.cached): returns -1 (block cell, U index base + q * vs),
-// -2 (invalid) or the Dirichlet boundary 2 d + side; neg as patch_source
-__device__ __forceinline__ int patch_resolve(const AmrDev& a, const FinePatch& P, const int c[3], long long& base,
-                                             unsigned& neg) {
-    int s[3];
-    int r = patch_source(P, c, s, neg);
-    base = 0;
-    if (r == -1) {
-        int cell;
-        const int g[3] = {P.f0[0] + s[0], P.f0[1] + s[1], P.f0[2] + s[2]};
-        const int id = P.bids[box_block_of(a, P.bb, g, &cell)];
-        if (id >= 0)
-            base = (long long)id * a.bcells + cell;
-        else
-            r = -2;
-    }
-    return r;
-}
-
 // value of a resolved box cell (zero where invalid)
 __device__ __forceinline__ void patch_load(const AmrDev& a, int r, long long base, unsigned neg, double v[5]) {
     if (r == -1) {
@@ -470,6 +425,70 @@ __device__ __forceinline__ void patch_load(const AmrDev& a, int r, long long bas
         if ((neg >> d) & 1) v[1 + d] *= -1.0;
 }
 
+// Box cells along axis D at fixed other coordinates c, resolved for a load
+// (needs P.cached).  A box cell's source walks the axes from the last to
+// the first through the per-axis maps: the first axis whose entry is a
+// Dirichlet side or invalid decides the cell (a Dirichlet state, or zero and
+// invalid); reflecting entries met before it negate that axis's momentum;
+// with no deciding axis the cell is read from its block at the mapped
+// position (invalid where that block is absent).  The other axes' part of
+// the walk (those above D come before D, those below after it) and of the
+// block and cell index is computed once per row.
+template <int D>
+struct PatchRow {
+    int r_hi = -1, r_lo = -1;  // status set by the axes above / below D
+    unsigned neg_hi = 0, neg_lo = 0;
+    int bi = 0, cell = 0;      // block-index and cell-index terms of the other axes
+    __device__ __forceinline__ PatchRow(const AmrDev& a, const FinePatch& P, const int c[3]) {
+#pragma unroll
+        for (int d = 2; d >= 0; --d) {
+            if (d == D) continue;
+            const int e = P.map[d * kPatchLen + c[d]];
+            int& r = d > D ? r_hi : r_lo;
+            unsigned& ng = d > D ? neg_hi : neg_lo;
+            if (r == -1) {
+                if (e & kMapDirichlet)
+                    r = 2 * d + (e & 1);
+                else if (e & kMapInvalid)
+                    r = -2;
+                else if (e & kMapFlip)
+                    ng |= 1u << d;
+            }
+            const int g = P.f0[d] + (e & 0xfff);
+            bi += term_b(a, P, d, g);
+            cell += term_c(a, d, g);
+        }
+    }
+    __device__ __forceinline__ static int term_b(const AmrDev& a, const FinePatch& P, int d, int g) {
+        const int k = (a.act[d] ? g >> a.lb[d] : 0) - P.bb.b0[d];
+        return d == 0 ? k * P.bb.nbk[1] * P.bb.nbk[2] : d == 1 ? k * P.bb.nbk[2] : k;
+    }
+    __device__ __forceinline__ static int term_c(const AmrDev& a, int d, int g) {
+        const int k = g & (a.bd[d] - 1);
+        return d == 0 ? k * a.bd[1] * a.bd[2] : d == 1 ? k * a.bd[2] : k;
+    }
+    // box cell at position p along D: -1 (block cell, U index base + q * vs),
+    // -2 (invalid) or the Dirichlet boundary 2 d + side; neg collects the negated axes
+    __device__ __forceinline__ int resolve(const AmrDev& a, const FinePatch& P, int p, long long& base,
+                                           unsigned& neg) const {
+        int r = r_hi;
+        neg = neg_hi;
+        base = 0;
+        if (r != -1) return r;
+        const int e = P.map[D * kPatchLen + p];
+        if (e & kMapDirichlet) return 2 * D + (e & 1);
+        if (e & kMapInvalid) return -2;
+        if (e & kMapFlip) neg |= 1u << D;
+        neg |= neg_lo;
+        if (r_lo != -1) return r_lo;
+        const int g = P.f0[D] + (e & 0xfff);
+        const int id = P.bids[bi + term_b(a, P, D, g)];
+        if (id < 0) return -2;
+        base = (long long)id * a.bcells + cell + term_c(a, D, g);
+        return -1;
+    }
+};
+
 // First restriction pass, along axis D, reading the fine patch in place:
 // output cell c of the (patch with b cells along D) box takes the first of
 // the shifted 6-point stencils whose cells are all valid, else the average
@@ -480,37 +499,36 @@ template <int D>
 __device__ __forceinline__ void restrict_first_pass(const AmrDev& a, const AmrWeights& W, const FinePatch& P,
                                                     const int nsz[3], const int off[3], double* out, double* ook,
                                                     long long cap) {
-    const int shifts[5] = {0, -1, 1, -2, 2};
     const int ntot = nsz[0] * nsz[1] * nsz[2];
     for (int i = threadIdx.x; i < ntot; i += blockDim.x) {
         int c[3];
         unflat(i, nsz, c);
 #pragma unroll
         for (int e = 0; e < 3; ++e) c[e] += off[e];
-        int s3[3] = {c[0], c[1], c[2]};
+        const PatchRow<D> row(a, P, c);
         const int base = 2 * c[D] + 2;
         unsigned okw = 0;  // bit j: patch cell base - 2 + j is valid
-#pragma unroll 1
+#pragma unroll
         for (int j = 0; j < 10; ++j) {
-            s3[D] = base - 2 + j;
-            if (s3[D] >= 0 && s3[D] < P.sz[D] && patch_ok(a, P, s3)) okw |= 1u << j;
+            const int p = base - 2 + j;
+            long long bj;
+            unsigned nj;
+            if (p >= 0 && p < P.sz[D] && row.resolve(a, P, p, bj, nj) != -2) okw |= 1u << j;
         }
         bool done = false;
         double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll 1
         for (int m = 0; m < 5 && !done; ++m) {
-            const int st = base + shifts[m];
+            const int sh = (m & 1) ? -((m + 1) >> 1) : (m >> 1);  // shifts 0, -1, 1, -2, 2
+            const int st = base + sh;
             if (st < 0 || st + 5 >= P.sz[D]) continue;
-            if (((okw >> (st - base + 2)) & 0x3fu) != 0x3fu) continue;
+            if (((okw >> (sh + 2)) & 0x3fu) != 0x3fu) continue;
             // resolve the six cells, then issue their loads together
             long long bs[6];
             int rr[6];
             unsigned ng[6];
 #pragma unroll
-            for (int kk = 0; kk < 6; ++kk) {
-                s3[D] = st + kk;
-                rr[kk] = patch_resolve(a, P, s3, bs[kk], ng[kk]);
-            }
+            for (int kk = 0; kk < 6; ++kk) rr[kk] = row.resolve(a, P, st + kk, bs[kk], ng[kk]);
             double v[6][5];
 #pragma unroll
             for (int kk = 0; kk < 6; ++kk) patch_load(a, rr[kk], bs[kk], ng[kk], v[kk]);
@@ -525,10 +543,8 @@ __device__ __forceinline__ void restrict_first_pass(const AmrDev& a, const AmrWe
             double va[5], vb[5];
             long long ba, bb;
             unsigned na, nb;
-            s3[D] = base + 2;
-            const int ra = patch_resolve(a, P, s3, ba, na);
-            s3[D] = base + 3;
-            const int rb = patch_resolve(a, P, s3, bb, nb);
+            const int ra = row.resolve(a, P, base + 2, ba, na);
+            const int rb = row.resolve(a, P, base + 3, bb, nb);
             patch_load(a, ra, ba, na, va);
             patch_load(a, rb, bb, nb, vb);
 #pragma unroll
@@ -709,8 +725,9 @@ __device__ __forceinline__ void restrict_block(const AmrDev& a, const AmrWeights
         for (int e = 0; e < 3; ++e) sz[e] = nsz[e];
         dstart = d0 + 1;
     }
-    for (int d = dstart; d < 3; ++d) {
-        if (!a.act[d]) continue;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {  // unrolled: arrays indexed by d stay in registers
+        if (d < dstart || !a.act[d]) continue;
         int nsz[3] = {sz[0], sz[1], sz[2]};
         nsz[d] = a.bd[d];
         const int ntot = nsz[0] * nsz[1] * nsz[2];
@@ -826,8 +843,9 @@ __device__ __forceinline__ void restrict_subbox(const AmrDev& a, const AmrWeight
     const double* in = T1;
     const double* iok = ok1;
     long long icap = rs.cap1;
-    for (int d = d0 + 1; d < 3; ++d) {
-        if (!a.act[d]) continue;
+#pragma unroll
+    for (int d = 1; d < 3; ++d) {  // unrolled: arrays indexed by d stay in registers
+        if (d <= d0 || !a.act[d]) continue;
         int nsz[3] = {sz[0], sz[1], sz[2]};
         nsz[d] = rs.S[d];
         const int ntot = nsz[0] * nsz[1] * nsz[2];
