@@ -208,27 +208,49 @@ __device__ __forceinline__ void predict_box(const AmrDev& a, const AmrWeights& W
     BoxBlocks bb;
     const bool cached = box_blocks(a, lc, cmin, sz, bids, bb);  // synchronises
     const int tot = sz[0] * sz[1] * sz[2];
-    for (int i = threadIdx.x; i < tot && cached; i += blockDim.x) {
-        int c[3];
-        unflat(i, sz, c);
-        int g[3] = {cmin[0] + c[0], cmin[1] + c[1], cmin[2] + c[2]};
-        for (int d = 0; d < 3; ++d)
-            if (!a.act[d]) g[d] = 0;
-        int cell;
-        const int id = bids[box_block_of(a, bb, g, &cell)];
-        if (id == -2) {
-            ok[i] = 0.0;
-            any_out = 1;
-            for (int q = 0; q < 5; ++q) A[q * capA + i] = 0.0;
-            continue;
+    // four cells per thread and round, their loads issued together (a cell
+    // with no block reads U[0] and keeps zero)
+    constexpr int kBatch = 4;
+    for (int i0 = threadIdx.x; i0 < tot && cached; i0 += kBatch * blockDim.x) {
+        long long src[kBatch];
+        int id[kBatch];
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) {
+            const int i = i0 + k * blockDim.x;
+            id[k] = -3;  // beyond the box
+            src[k] = 0;
+            if (i >= tot) continue;
+            int c[3];
+            unflat(i, sz, c);
+            int g[3] = {cmin[0] + c[0], cmin[1] + c[1], cmin[2] + c[2]};
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                if (!a.act[d]) g[d] = 0;
+            int cell;
+            id[k] = bids[box_block_of(a, bb, g, &cell)];
+            if (id[k] >= 0) src[k] = (long long)id[k] * a.bcells + cell;
         }
-        if (id < 0) {
-            atomicOr(&a.sc->err, 2);  // the hierarchy is 2:1 balanced (see below)
-            for (int q = 0; q < 5; ++q) A[q * capA + i] = 0.0;
-        } else {
-            for (int q = 0; q < 5; ++q) A[q * capA + i] = a.U[amr_u(a, q, id, cell)];
+        double v[kBatch][5];
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k)
+#pragma unroll
+            for (int q = 0; q < 5; ++q) {
+                const double x = a.U[q * a.vs + src[k]];
+                v[k][q] = id[k] >= 0 ? x : 0.0;
+            }
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) {
+            const int i = i0 + k * blockDim.x;
+            if (id[k] == -3) continue;
+            if (id[k] == -2) {
+                any_out = 1;
+            } else if (id[k] < 0) {
+                atomicOr(&a.sc->err, 2);  // the hierarchy is 2:1 balanced (see below)
+            }
+#pragma unroll
+            for (int q = 0; q < 5; ++q) A[q * capA + i] = v[k][q];
+            ok[i] = id[k] == -2 ? 0.0 : 1.0;
         }
-        ok[i] = 1.0;
     }
     for (int i = threadIdx.x; i < tot && !cached; i += blockDim.x) {
         int c[3];
