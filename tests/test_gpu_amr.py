@@ -229,6 +229,35 @@ def test_16cubed_blocks_3d(em):
     assert np.array_equal(per.download(), o.get_u())
 
 
+def test_32cubed_blocks_3d(em):
+    # 3-D shock-bubble on 32^3 blocks over two levels: the restriction's
+    # sub-boxes (1 x 16 x 8 coarse cells, edge boxes of width 9 selecting
+    # part of them) and the global-scratch ghost predictions of this block
+    # size, bit-identical to the oracle through steps
+    print("amr 3-D shock-bubble weno, 32^3 blocks", flush=True)
+    bc = [["outflow", "outflow"], ["reflect", "reflect"], ["reflect", "reflect"]]
+    prm = [2.0, 0.15, 0.3, 0.25, 0.25, 0.12, 0.1]  # shock and bubble in the first level-0 block
+    o = OracleAmr("shock_bubble", prm, (2, 1, 1), 32, 2, (0, 0, 0), (1, 0.5, 0.5), eps=0.5, bc=bc, mode="weno",
+                  regrid_every=2)
+    lv, _, leaf = o.blocks()
+    assert len(set(lv[leaf])) == 2
+    ro, so = o.compute_rhs(1e-3)
+    for shape in range(2):
+        g = _from_oracle(em, o, bc=bc, mode="weno", eps=0.5, regrid_every=2)
+        g.set_shapes([shape] * 3)
+        rg, sg = g.rhs(1e-3)
+        assert np.array_equal(rg, ro), shape
+        assert sg == so, shape
+        assert np.array_equal(g.download(), o.get_u()), shape
+    for _ in range(2):
+        dt = o.dt_cfl()
+        assert g.dt_cfl() == dt
+        o.step(dt)
+        g.step(dt)
+    _same_tree(g, o)
+    assert np.array_equal(g.download(), o.get_u())
+
+
 def test_details_and_regrid(em):
     print("amr details + regrid 2-D riemann", flush=True)
     o = _riemann("weno")
