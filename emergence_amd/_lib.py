@@ -77,6 +77,7 @@ EXPORTS = (
     "em_amr_group_run_steps",
     "em_fiveeq_last_error",
     "em_fiveeq_create",
+    "em_fiveeq_create_f32",
     "em_fiveeq_destroy",
     "em_fiveeq_upload",
     "em_fiveeq_download",
@@ -191,6 +192,8 @@ def load():
     lib.em_amr_group_run_steps.argtypes = [P, C.c_int64]
     lib.em_fiveeq_last_error.argtypes = []
     lib.em_fiveeq_create.argtypes = [C.POINTER(P), I64, D, D, C.c_int32, D, C.c_int32, I32, D, C.c_int32, C.c_int32,
+                                     C.c_double, C.c_double, C.c_double, D]
+    lib.em_fiveeq_create_f32.argtypes = [C.POINTER(P), I64, D, D, C.c_int32, D, C.c_int32, I32, D, C.c_int32, C.c_int32,
                                      C.c_double, C.c_double, C.c_double, D]
     lib.em_fiveeq_destroy.argtypes = [P]
     lib.em_fiveeq_upload.argtypes = [P, D]

@@ -54,33 +54,40 @@ struct alignas(256) FeScalars {
     unsigned long long counters[3][kSlots][16];
 };
 
-struct FePassArgs {
-    const double* U;  // stage input (7 planes of ncell)
-    double* R;        // residual accumulator
-    const double* U0;
-    double* Uout;
+// Pass arguments in the real type Real of the state (EOS parameters,
+// boundary states, 1/dx and the sensor tolerance rounded to Real, as the
+// reference's R::from_f64); the limiter's lambda is formed in double from
+// ndim_pos, the device dt and dx_d, then rounded.
+template <class Real>
+struct FePassArgsT {
+    const Real* U;  // stage input (7 planes of ncell)
+    Real* R;        // residual accumulator
+    const Real* U0;
+    Real* Uout;
     FeScalars* sc;
     int n[3];
     long long stride[3];
     long long ncell;
-    Sys sys;
+    SysT<Real> sys;
     int bc_kind[2];
-    double bc_state[2][7];
-    double inv_dx, ndim_pos, dx_d, tol;
+    Real bc_state[2][7];
+    Real inv_dx, tol;
+    double ndim_pos, dx_d;
     int first, rk_stage;
     int fast_axis, slow_axis, dfast, ntile, ngroup;
     int stage_dt_host;
     double stage_dt;
 };
+using FePassArgs = FePassArgsT<double>;
 
-template <int MODE, int T, int P>
+template <class Real, int MODE, int T, int P>
 struct FeLds {
     static constexpr bool HYB = MODE == MODE_HYBRID;
-    double sw[7][T + 10][P];
-    double sre[T + 10][P];  // rho e(w) of each tile cell (central pair fluxes, limiter)
-    double sm[8][T + 5][P];  // midpoints: flux + contact speed; then faces: flux + face velocity
-    double ss[3][HYB ? T + 10 : 1][P];
-    double sth[HYB ? T + 1 : 1][P];
+    Real sw[7][T + 10][P];
+    Real sre[T + 10][P];  // rho e(w) of each tile cell (central pair fluxes, limiter)
+    Real sm[8][T + 5][P];  // midpoints: flux + contact speed; then faces: flux + face velocity
+    Real ss[3][HYB ? T + 10 : 1][P];
+    Real sth[HYB ? T + 1 : 1][P];
     unsigned int scnt[3];
 };
 
@@ -88,15 +95,15 @@ struct FeLds {
 #define EM_FE_WAVES_PER_EU 2
 #endif
 
-template <int D, int MODE, int T, int P, int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_FE_WAVES_PER_EU))) void fe_pass(FePassArgs a) {
+template <class Real, int D, int MODE, int T, int P, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_FE_WAVES_PER_EU))) void fe_pass(FePassArgsT<Real> a) {
     constexpr int C = T + 10;
     constexpr int M = T + 5;
     constexpr int NF = T + 1;
     constexpr bool HYB = MODE == MODE_HYBRID;
     constexpr bool CEN = MODE == MODE_CENTRAL;
     constexpr int un = IU + D;
-    __shared__ FeLds<MODE, T, P> L;
+    __shared__ FeLds<Real, MODE, T, P> L;
     auto& sw = L.sw;
     auto& sre = L.sre;
     auto& sm = L.sm;
@@ -119,7 +126,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_FE_WAVES_
     const long long sF = a.stride[a.fast_axis];
     const long long base = (long long)sI * a.stride[a.slow_axis] + (long long)p0 * sF;
     const long long nc = a.ncell;
-    const Sys& sys = a.sys;
+    const SysT<Real>& sys = a.sys;
     if (tid < 3) L.scnt[tid] = 0;
 
     // ---- 1. load + primitives (+ sensor ln rho, ln(p + p_inf,mix), alpha)
@@ -135,13 +142,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_FE_WAVES_
         if (p >= Pv || c >= Tv + 10) continue;
         const int g = c0 - 5 + c;
         const long long pen = base + (long long)p * sF;
-        double u[7], w[7];
+        Real u[7], w[7];
         if (g >= 0 && g < nD) {
             const long long idx = pen + (long long)g * sD;
 #pragma unroll
             for (int k = 0; k < 7; ++k) u[k] = a.U[k * nc + idx];
         } else {
-            auto ld = [&](int x, double* o) {
+            auto ld = [&](int x, Real* o) {
 #pragma unroll
                 for (int k = 0; k < 7; ++k) o[k] = a.U[k * nc + pen + (long long)x * sD];
             };
@@ -153,18 +160,19 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_FE_WAVES_
         sre[c][p] = rho_e(sys, w);
         if constexpr (HYB) {
             // FiveEq::sensor, multiphase.rs:385-390
-            ss[0][c][p] = log(fmax(mix_rho(w), 1e-300));
-            ss[1][c][p] = log(fmax(w[IP] + p_inf_mix(sys, w), 1e-300));
+            ss[0][c][p] = log(fmax(mix_rho(w), Real(1e-300)));
+            ss[1][c][p] = log(fmax(w[IP] + p_inf_mix(sys, w), Real(1e-300)));
             ss[2][c][p] = w[IA];
         }
     }
     __syncthreads();
 
-    double lam = 0.0;
+    double lam_d = 0.0;
     if (a.ndim_pos > 0.0) {
         const double dt = a.stage_dt_host ? a.stage_dt : a.sc->dt;
-        lam = a.ndim_pos * dt / a.dx_d;  // solver.rs:393
+        lam_d = a.ndim_pos * dt / a.dx_d;  // solver.rs:393
     }
+    const Real lam = Real(lam_d);
     unsigned n_shock = 0, n_fb = 0, n_lim = 0;
 
     // ---- 2. hybrid blend weights (scheme.rs:171-183)
@@ -172,18 +180,18 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_FE_WAVES_
         for (int it = tid; it < NF * P; it += NT) {
             const int p = it % P, f = it / P;
             if (p >= Pv || f > Tv) continue;
-            double r = 0.0;
+            Real r = Real(0.0);
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                double q[6];
+                Real q[6];
 #pragma unroll
                 for (int j = 0; j < 6; ++j) q[j] = ss[k][f + 2 + j][p];
                 r = fmax(fmax(r, d4f(q[0], q[1], q[2], q[3], q[4])), d4f(q[1], q[2], q[3], q[4], q[5]));
             }
-            double x = (r / a.tol - 0.5) / 0.5;
-            if (x < 0.0) x = 0.0;
-            if (x > 1.0) x = 1.0;
-            sth[f][p] = isfinite(r) ? x * x * (3.0 - 2.0 * x) : 1.0;
+            Real x = (r / a.tol - Real(0.5)) / Real(0.5);
+            if (x < Real(0.0)) x = Real(0.0);
+            if (x > Real(1.0)) x = Real(1.0);
+            sth[f][p] = isfinite(r) ? x * x * (Real(3.0) - Real(2.0) * x) : Real(1.0);
         }
         __syncthreads();
     }
@@ -196,10 +204,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_FE_WAVES_
             if constexpr (HYB) {
                 bool need = false;
                 const int flo = max(0, m - 4), fhi = min(Tv, m);
-                for (int f = flo; f <= fhi; ++f) need = need || sth[f][p] > 0.0;
+                for (int f = flo; f <= fhi; ++f) need = need || sth[f][p] > Real(0.0);
                 if (!need) continue;
             }
-            double f[7], us;
+            Real f[7], us;
             const bool fb = midpoint_flux<D>(sys, [&](int j, int k) { return sw[k][m + j][p]; }, f, us);
 #pragma unroll
             for (int k = 0; k < 7; ++k) sm[k][m][p] = f[k];
@@ -215,36 +223,36 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_FE_WAVES_
 
     // ---- 4. faces (face f between tile cells f+4 and f+5)
     constexpr int FITER = (NF * P + NT - 1) / NT;
-    double FR[FITER][8];
+    Real FR[FITER][8];
 #pragma unroll
     for (int fi = 0; fi < FITER; ++fi) {
         const int it = tid + fi * NT;
         const int p = it % P, f = it / P;
         if (it >= NF * P || p >= Pv || f > Tv) continue;
         const unsigned own_face = (f < Tv || c0 + Tv == nD) ? 1u : 0u;
-        double theta;
+        Real theta;
         if constexpr (MODE == MODE_WENO)
-            theta = 1.0;
+            theta = Real(1.0);
         else if constexpr (CEN)
-            theta = 0.0;
+            theta = Real(0.0);
         else
             theta = sth[f][p];
-        double fo[7], cen[7];
-        double uo = 0.0, uc = 0.0;
+        Real fo[7], cen[7];
+        Real uo = Real(0.0), uc = Real(0.0);
 #pragma unroll
         for (int k = 0; k < 7; ++k) {
-            fo[k] = 0.0;
-            cen[k] = 0.0;
+            fo[k] = Real(0.0);
+            cen[k] = Real(0.0);
         }
-        if (theta < 1.0) {
+        if (theta < Real(1.0)) {
             // KEEP-PE 6th-order central flux and face velocity, scheme.rs:188-214
-            const double a6[3] = {0.75, -0.15, 1.0 / 60.0};
+            const Real a6[3] = {Real(0.75), -Real(0.15), Real(1.0) / Real(60.0)};
 #pragma unroll
             for (int li = 0; li < 3; ++li) {
                 const int lag = li + 1;
 #pragma unroll
                 for (int mm = 0; mm < lag; ++mm) {
-                    double wa[7], wb[7], ft[7];
+                    Real wa[7], wb[7], ft[7];
 #pragma unroll
                     for (int k = 0; k < 7; ++k) {
                         wa[k] = sw[k][f + 4 - mm][p];
@@ -252,24 +260,24 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_FE_WAVES_
                     }
                     split_flux_re<D>(wa, wb, sre[f + 4 - mm][p], sre[f + 4 - mm + lag][p], ft);
 #pragma unroll
-                    for (int k = 0; k < 7; ++k) cen[k] += 2.0 * a6[li] * ft[k];
+                    for (int k = 0; k < 7; ++k) cen[k] += Real(2.0) * a6[li] * ft[k];
                     uc += a6[li] * (wa[un] + wb[un]);
                 }
             }
         }
-        if (theta == 0.0) {
+        if (theta == Real(0.0)) {
 #pragma unroll
             for (int k = 0; k < 7; ++k) fo[k] = cen[k];
             uo = uc;
         } else {
             n_shock += own_face;
             // F = sum_s S6[s] (F_{i+s} + F_{i-s}), order s=0, i+1, i-1, i+2, i-2
-            const double s0 = 75.0 / 64.0 - 25.0 / 384.0 + 3.0 / 640.0;
-            const double s1 = -25.0 / 384.0 + 3.0 / 640.0;
-            const double s2 = 3.0 / 640.0;
+            const Real s0 = Real(75.0) / Real(64.0) - Real(25.0) / Real(384.0) + Real(3.0) / Real(640.0);
+            const Real s1 = -Real(25.0) / Real(384.0) + Real(3.0) / Real(640.0);
+            const Real s2 = Real(3.0) / Real(640.0);
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
-                double v = 0.0;
+                Real v = Real(0.0);
                 v += s0 * sm[k][f + 2][p];
                 v += s1 * sm[k][f + 3][p];
                 v += s1 * sm[k][f + 1][p];
@@ -280,15 +288,15 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_FE_WAVES_
                 else
                     uo = v;
             }
-            if (theta < 1.0) {
+            if (theta < Real(1.0)) {
 #pragma unroll
-                for (int k = 0; k < 7; ++k) fo[k] = theta * fo[k] + (1.0 - theta) * cen[k];
-                uo = theta * uo + (1.0 - theta) * uc;
+                for (int k = 0; k < 7; ++k) fo[k] = theta * fo[k] + (Real(1.0) - theta) * cen[k];
+                uo = theta * uo + (Real(1.0) - theta) * uc;
             }
         }
-        if (lam > 0.0) {
+        if (lam > Real(0.0)) {
             // positivity limiter, scheme.rs:401-450
-            double wl[7], wr[7], ul[7], ur[7], fl[7], fr[7];
+            Real wl[7], wr[7], ul[7], ur[7], fl[7], fr[7];
 #pragma unroll
             for (int k = 0; k < 7; ++k) {
                 wl[k] = sw[k][f + 4][p];
@@ -298,9 +306,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_FE_WAVES_
             to_cons_re(wr, sre[f + 5][p], ur);
             phys_flux<D>(wl, ul, fl);
             phys_flux<D>(wr, ur, fr);
-            const double tl = 2.0 * lam;
-            auto ok = [&](const double* fx) {
-                double qa[7], qb[7], pa[7], pb[7];
+            const Real tl = Real(2.0) * lam;
+            auto ok = [&](const Real* fx) {
+                Real qa[7], qb[7], pa[7], pb[7];
 #pragma unroll
                 for (int k = 0; k < 7; ++k) {
                     qa[k] = ul[k] - tl * (fx[k] - fl[k]);
@@ -314,29 +322,29 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_FE_WAVES_
 #pragma unroll
             for (int k = 0; k < 7; ++k) fin = fin && isfinite(fo[k]);
             if (!(fin && ok(fo))) {
-                const double al = fmax(fabs(wl[un]) + sound_speed(sys, wl), fabs(wr[un]) + sound_speed(sys, wr));
-                double flf[7], fh[7];
+                const Real al = fmax(fabs(wl[un]) + sound_speed(sys, wl), fabs(wr[un]) + sound_speed(sys, wr));
+                Real flf[7], fh[7];
 #pragma unroll
                 for (int k = 0; k < 7; ++k) {
-                    flf[k] = 0.5 * (fl[k] + fr[k]) - 0.5 * al * (ur[k] - ul[k]);
+                    flf[k] = Real(0.5) * (fl[k] + fr[k]) - Real(0.5) * al * (ur[k] - ul[k]);
                     fh[k] = fo[k];
                 }
                 n_lim += own_face;
-                double lo = 0.0, hi = 1.0;
+                Real lo = Real(0.0), hi = Real(1.0);
                 for (int itb = 0; itb < 30; ++itb) {
-                    const double mid = 0.5 * (lo + hi);
-                    double bl[7];
+                    const Real mid = Real(0.5) * (lo + hi);
+                    Real bl[7];
 #pragma unroll
-                    for (int k = 0; k < 7; ++k) bl[k] = isfinite(fh[k]) ? mid * fh[k] + (1.0 - mid) * flf[k] : flf[k];
+                    for (int k = 0; k < 7; ++k) bl[k] = isfinite(fh[k]) ? mid * fh[k] + (Real(1.0) - mid) * flf[k] : flf[k];
                     if (ok(bl))
                         lo = mid;
                     else
                         hi = mid;
                 }
 #pragma unroll
-                for (int k = 0; k < 7; ++k) fo[k] = isfinite(fh[k]) ? lo * fh[k] + (1.0 - lo) * flf[k] : flf[k];
-                const double um = 0.5 * (wl[un] + wr[un]);
-                uo = isfinite(uo) ? lo * uo + (1.0 - lo) * um : um;
+                for (int k = 0; k < 7; ++k) fo[k] = isfinite(fh[k]) ? lo * fh[k] + (Real(1.0) - lo) * flf[k] : flf[k];
+                const Real um = Real(0.5) * (wl[un] + wr[un]);
+                uo = isfinite(uo) ? lo * uo + (Real(1.0) - lo) * um : um;
             }
         }
 #pragma unroll
@@ -355,7 +363,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_FE_WAVES_
     __syncthreads();
 
     // ---- 5. cells: divergence, non-conservative term, fused RK stage
-    const double dt = a.sc->dt;
+    const Real dt = Real(a.sc->dt);
     for (int it = tid; it < T * P; it += NT) {
         int c, p;
         if (a.dfast) {
@@ -367,10 +375,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_FE_WAVES_
         }
         if (p >= Pv || c >= Tv) continue;
         const long long idx = base + (long long)p * sF + (long long)(c0 + c) * sD;
-        double r[7], w[7];
+        Real r[7], w[7];
 #pragma unroll
         for (int k = 0; k < 7; ++k) {
-            const double prev = a.first ? 0.0 : a.R[k * nc + idx];
+            const Real prev = a.first ? Real(0.0) : a.R[k * nc + idx];
             r[k] = prev - (sm[k][c + 1][p] - sm[k][c][p]) * a.inv_dx;
             w[k] = sw[k][c + 5][p];
         }
@@ -382,17 +390,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_FE_WAVES_
         } else {
 #pragma unroll
             for (int k = 0; k < 7; ++k) {
-                const double u0 = a.U0[k * nc + idx];
-                double o;
+                const Real u0 = a.U0[k * nc + idx];
+                Real o;
                 if (a.rk_stage == 1) {
                     o = u0 + dt * r[k];
                 } else {
-                    const double us = a.U[k * nc + idx];
+                    const Real us = a.U[k * nc + idx];
                     // solver.rs:113-116 and 716-726: dest = U^n, old = the stage
                     if (a.rk_stage == 2)
-                        o = 0.75 * u0 + 0.25 * (us + dt * r[k]);
+                        o = Real(0.75) * u0 + Real(0.25) * (us + dt * r[k]);
                     else
-                        o = u0 / 3.0 + 2.0 / 3.0 * (us + dt * r[k]);
+                        o = u0 / Real(3.0) + Real(2.0) / Real(3.0) * (us + dt * r[k]);
                 }
                 a.Uout[k * nc + idx] = o;
             }
@@ -420,9 +428,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_FE_WAVES_
 
 // Conserved neighbour of interior cell x (pencil coordinate) along d through
 // the boundary condition when x is a boundary cell (solver.rs:592-640).
-template <int D, class LD>
-__device__ __forceinline__ void neighbor_conserved(const Sys& s, int n, int x, bool high, const int kind[2],
-                                                   const double (*state)[7], LD ld, double* out) {
+template <int D, class Real, class LD>
+__device__ __forceinline__ void neighbor_conserved(const SysT<Real>& s, int n, int x, bool high, const int kind[2],
+                                                   const Real (*state)[7], LD ld, Real* out) {
     const bool boundary = high ? x == n - 1 : x == 0;
     if (!boundary) {
         ld(high ? x + 1 : x - 1, out);
@@ -438,70 +446,72 @@ __device__ __forceinline__ void neighbor_conserved(const Sys& s, int n, int x, b
             return;
         case 2:
             ld(x, out);
-            out[IU + D] *= -1.0;
+            out[IU + D] *= -Real(1.0);
             return;
         default: {
-            double ub[7], ub1[7];
+            Real ub[7], ub1[7];
             ld(x, ub);
             ld(high ? x - 1 : x + 1, ub1);
-            characteristic_ghost<D>(s, ub, ub1, state[side], side, 1.0, out);
+            characteristic_ghost<D>(s, ub, ub1, state[side], side, Real(1.0), out);
             return;
         }
     }
 }
 
-struct FeRateArgs {
-    const double* U;
+template <class Real>
+struct FeRateArgsT {
+    const Real* U;
     FeScalars* sc;
     long long ncell;
     int n[3];
     long long stride[3];
-    double dx[3];
+    Real dx[3];
     int act[3];
-    Sys sys;
+    SysT<Real> sys;
     int bc_kind[3][2];
-    double bc_state[3][2][7];
+    Real bc_state[3][2][7];
 };
 
-template <int D>
-__device__ __forceinline__ double divu_axis(const FeRateArgs& a, long long cell, int x) {
-    auto ld = [&](int y, double* o) {
+template <int D, class Real>
+__device__ __forceinline__ Real divu_axis(const FeRateArgsT<Real>& a, long long cell, int x) {
+    auto ld = [&](int y, Real* o) {
         const long long q = cell + (long long)(y - x) * a.stride[D];
 #pragma unroll
         for (int k = 0; k < 7; ++k) o[k] = a.U[k * a.ncell + q];
     };
-    double lo[7], hi[7];
+    Real lo[7], hi[7];
     neighbor_conserved<D>(a.sys, a.n[D], x, false, a.bc_kind[D], a.bc_state[D], ld, lo);
     neighbor_conserved<D>(a.sys, a.n[D], x, true, a.bc_kind[D], a.bc_state[D], ld, hi);
-    const double va = lo[IU + D] / (lo[0] + lo[1]);  // velocity_conserved, multiphase.rs:158-161
-    const double vb = hi[IU + D] / (hi[0] + hi[1]);
-    return fabs(vb - va) / (2.0 * a.dx[D]);
+    const Real va = lo[IU + D] / (lo[0] + lo[1]);  // velocity_conserved, multiphase.rs:158-161
+    const Real vb = hi[IU + D] / (hi[0] + hi[1]);
+    return fabs(vb - va) / (Real(2.0) * a.dx[D]);
 }
 
 // CFL rate and dilatation-rate limit of every interior cell (solver.rs:478-
 // 523): rate = max sum_d (|u_d| + c)/dx_d, source = max dilatation_rate(w,
 // sum_d |v_{+d} - v_{-d}| / (2 dx_d)); one atomicMax per workgroup per slot.
-__global__ __launch_bounds__(256) void k_fe_rate(FeRateArgs a) {
-    double rate = 0.0, source = 0.0;
+template <class Real>
+__global__ __launch_bounds__(256) void k_fe_rate(FeRateArgsT<Real> a) {
+    double rate = 0.0, source = 0.0;  // maxima of Real values, exact in double
     for (long long cell = blockIdx.x * 256ll + threadIdx.x; cell < a.ncell; cell += (long long)gridDim.x * 256ll) {
-        double u[7], w[7];
+        Real u[7], w[7];
 #pragma unroll
         for (int k = 0; k < 7; ++k) u[k] = a.U[k * a.ncell + cell];
         to_prim(a.sys, u, w);
-        const double c = sound_speed(a.sys, w);
-        double ac = -0.0;
+        const Real c = sound_speed(a.sys, w);
+        Real ac = -Real(0.0);
 #pragma unroll
         for (int d = 0; d < 3; ++d)
             if (a.act[d]) ac += (fabs(w[IU + d]) + c) / a.dx[d];
-        rate = fmax(rate, ac);
+        rate = fmax(rate, (double)ac);
         const int x0 = (int)(cell / a.stride[0]);
         const int x1 = (int)((cell / a.stride[1]) % a.n[1]);
         const int x2 = (int)(cell % a.n[2]);
-        double divu = 0.0;
+        Real divu = Real(0.0);
         if (a.act[0]) divu += divu_axis<0>(a, cell, x0);
         if (a.act[1]) divu += divu_axis<1>(a, cell, x1);
         if (a.act[2]) divu += divu_axis<2>(a, cell, x2);
-        source = fmax(source, dilatation_rate(a.sys, w, divu));
+        source = fmax(source, (double)dilatation_rate(a.sys, w, divu));
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -544,14 +554,14 @@ struct FeTile {
 constexpr FeTile kFeTiles[] = {{27, 8, 256}, {59, 4, 256}, {59, 8, 512}, {27, 16, 512}, {123, 4, 512}};
 constexpr int kNumFeTiles = 5;
 
-template <int D, int MODE, int T, int P, int NT>
-static hipError_t launch_fe(const FePassArgs& a, hipStream_t st) {
+template <int D, int MODE, int T, int P, int NT, class Real>
+static hipError_t launch_fe(const FePassArgsT<Real>& a, hipStream_t st) {
     const long long nblk = (long long)a.ntile * a.ngroup * a.n[a.slow_axis];
-    hipLaunchKernelGGL((fe_pass<D, MODE, T, P, NT>), dim3((unsigned)nblk), dim3(NT), 0, st, a);
+    hipLaunchKernelGGL((fe_pass<Real, D, MODE, T, P, NT>), dim3((unsigned)nblk), dim3(NT), 0, st, a);
     return hipGetLastError();
 }
-template <int D, int MODE>
-static hipError_t launch_fe_cfg(int cfg, const FePassArgs& a, hipStream_t st) {
+template <int D, int MODE, class Real>
+static hipError_t launch_fe_cfg(int cfg, const FePassArgsT<Real>& a, hipStream_t st) {
     switch (cfg) {
         case 0: return launch_fe<D, MODE, 27, 8, 256>(a, st);
         case 1: return launch_fe<D, MODE, 59, 4, 256>(a, st);
@@ -561,8 +571,8 @@ static hipError_t launch_fe_cfg(int cfg, const FePassArgs& a, hipStream_t st) {
         default: return hipErrorInvalidValue;
     }
 }
-template <int D>
-static hipError_t launch_fe_mode(int mode, int cfg, const FePassArgs& a, hipStream_t st) {
+template <int D, class Real>
+static hipError_t launch_fe_mode(int mode, int cfg, const FePassArgsT<Real>& a, hipStream_t st) {
     switch (mode) {
         case MODE_HYBRID: return launch_fe_cfg<D, MODE_HYBRID>(cfg, a, st);
         case MODE_WENO: return launch_fe_cfg<D, MODE_WENO>(cfg, a, st);
@@ -570,7 +580,8 @@ static hipError_t launch_fe_mode(int mode, int cfg, const FePassArgs& a, hipStre
         default: return hipErrorInvalidValue;
     }
 }
-static hipError_t launch_fe_pass(int d, int mode, int cfg, const FePassArgs& a, hipStream_t st) {
+template <class Real>
+static hipError_t launch_fe_pass(int d, int mode, int cfg, const FePassArgsT<Real>& a, hipStream_t st) {
     switch (d) {
         case 0: return launch_fe_mode<0>(mode, cfg, a, st);
         case 1: return launch_fe_mode<1>(mode, cfg, a, st);
@@ -579,8 +590,21 @@ static hipError_t launch_fe_pass(int d, int mode, int cfg, const FePassArgs& a, 
     }
 }
 
-// Host object: mesh, boundaries, options, device state
-struct FeSolver {
+// The system in real type Real (the reference's R::from_f64 of each phase)
+template <class Real>
+static SysT<Real> sys_as(const Sys& s) {
+    SysT<Real> r;
+    for (int k = 0; k < 2; ++k)
+        r.e[k] = PhaseT<Real>{Real(s.e[k].gamma), Real(s.e[k].p_inf), Real(s.e[k].b), Real(s.e[k].q)};
+    r.kapila = s.kapila;
+    return r;
+}
+
+// Host object: mesh, boundaries, options, device state in real type Real
+// (double: the reference's GpuSolver<f64>; float: GpuSolver<f32>).  The
+// host side (mesh, options, time, dt) stays double.
+template <class Real>
+struct FeCore {
     int64_t n[3];
     double lo[3], dx[3];
     int ng;
@@ -594,8 +618,8 @@ struct FeSolver {
     long long stride[3];
     int nactive = 0, act[3];
     hipStream_t stream = nullptr;
-    double* U[3] = {nullptr, nullptr, nullptr};
-    double* R = nullptr;
+    Real* U[3] = {nullptr, nullptr, nullptr};
+    Real* R = nullptr;
     FeScalars* sc = nullptr;
     uint64_t counters_host[3] = {0, 0, 0};
 
@@ -603,7 +627,7 @@ struct FeSolver {
     int m(int d) const { return active(d) ? (int)(n[d] + 2 * ng) : 1; }
     int g(int d) const { return active(d) ? ng : 0; }
 
-    FeSolver(const int64_t nn[3], const double lo_[3], const double hi[3], int ng_, const double ph[8], int kapila,
+    FeCore(const int64_t nn[3], const double lo_[3], const double hi[3], int ng_, const double ph[8], int kapila,
              const int32_t bk[6], const double bs[42], int mode_, int pos, double cfl_, double tol_, double t0) {
         for (int d = 0; d < 3; ++d) {
             if (nn[d] < 1 || nn[d] > (1 << 30)) throw Error(EM_ERR_ARG, "mesh extents must be in [1, 2^30]");
@@ -638,8 +662,8 @@ struct FeSolver {
         stride[2] = 1;
         for (int d = 0; d < 3; ++d) cfg[d] = stride[d] == 1 ? 1 : 0;
         FECHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-        for (auto& b : U) FECHK(hipMalloc(&b, sizeof(double) * 7 * ncell));
-        FECHK(hipMalloc(&R, sizeof(double) * 7 * ncell));
+        for (auto& b : U) FECHK(hipMalloc(&b, sizeof(Real) * 7 * ncell));
+        FECHK(hipMalloc(&R, sizeof(Real) * 7 * ncell));
         FECHK(hipMalloc(&sc, sizeof(FeScalars)));
         std::vector<FeScalars> h(1);
         std::memset(h.data(), 0, sizeof(FeScalars));
@@ -647,7 +671,7 @@ struct FeSolver {
         FECHK(hipMemcpyAsync(sc, h.data(), sizeof(FeScalars), hipMemcpyHostToDevice, stream));
         FECHK(hipStreamSynchronize(stream));
     }
-    ~FeSolver() {
+    ~FeCore() {
         for (auto* b : U)
             if (b) (void)hipFree(b);
         if (R) (void)hipFree(R);
@@ -656,22 +680,22 @@ struct FeSolver {
     }
 
     void upload(const double* u) {
-        std::vector<double> h(7 * ncell);
+        std::vector<Real> h(7 * ncell);
         const int m1 = m(1), m2 = m(2);
         for (long long i = 0; i < n[0]; ++i)
             for (long long j = 0; j < n[1]; ++j)
                 for (long long k = 0; k < n[2]; ++k) {
                     const long long q = ((i + g(0)) * m1 + (j + g(1))) * m2 + (k + g(2));
                     const long long c = (i * n[1] + j) * n[2] + k;
-                    for (int v = 0; v < 7; ++v) h[v * ncell + c] = u[q * 7 + v];
+                    for (int v = 0; v < 7; ++v) h[v * ncell + c] = Real(u[q * 7 + v]);
                 }
-        FECHK(hipMemcpyAsync(U[0], h.data(), sizeof(double) * 7 * ncell, hipMemcpyHostToDevice, stream));
+        FECHK(hipMemcpyAsync(U[0], h.data(), sizeof(Real) * 7 * ncell, hipMemcpyHostToDevice, stream));
         FECHK(hipStreamSynchronize(stream));
     }
     // ghost-padded AoS with the interior filled and zero ghosts
-    void to_host(const double* dev, double* u) {
-        std::vector<double> h(7 * ncell);
-        FECHK(hipMemcpyAsync(h.data(), dev, sizeof(double) * 7 * ncell, hipMemcpyDeviceToHost, stream));
+    void to_host(const Real* dev, double* u) {
+        std::vector<Real> h(7 * ncell);
+        FECHK(hipMemcpyAsync(h.data(), dev, sizeof(Real) * 7 * ncell, hipMemcpyDeviceToHost, stream));
         FECHK(hipStreamSynchronize(stream));
         const long long L = (long long)m(0) * m(1) * m(2);
         std::fill(u, u + 7 * L, 0.0);
@@ -681,27 +705,27 @@ struct FeSolver {
                 for (long long k = 0; k < n[2]; ++k) {
                     const long long q = ((i + g(0)) * m1 + (j + g(1))) * m2 + (k + g(2));
                     const long long c = (i * n[1] + j) * n[2] + k;
-                    for (int v = 0; v < 7; ++v) u[q * 7 + v] = h[v * ncell + c];
+                    for (int v = 0; v < 7; ++v) u[q * 7 + v] = (double)h[v * ncell + c];
                 }
     }
 
-    FePassArgs pass_args(int d) const {
-        FePassArgs a{};
+    FePassArgsT<Real> pass_args(int d) const {
+        FePassArgsT<Real> a{};
         a.sc = sc;
         for (int e = 0; e < 3; ++e) {
             a.n[e] = (int)n[e];
             a.stride[e] = stride[e];
         }
         a.ncell = ncell;
-        a.sys = sys;
+        a.sys = sys_as<Real>(sys);
         for (int sd = 0; sd < 2; ++sd) {
             a.bc_kind[sd] = bc_kind[d][sd];
-            for (int q = 0; q < 7; ++q) a.bc_state[sd][q] = bc_state[d][sd][q];
+            for (int q = 0; q < 7; ++q) a.bc_state[sd][q] = Real(bc_state[d][sd][q]);
         }
-        a.inv_dx = 1.0 / dx[d];
+        a.inv_dx = Real(1.0 / dx[d]);
         a.dx_d = dx[d];
         a.ndim_pos = positivity ? (double)nactive : 0.0;
-        a.tol = tol;
+        a.tol = Real(tol);
         int tt[2], nt = 0;
         for (int e = 0; e < 3; ++e)
             if (e != d) tt[nt++] = e;
@@ -722,14 +746,14 @@ struct FeSolver {
     }
 
     // residual of Uin over the active axes in order; fused RK stage on the last
-    void stage(const double* Uin, double* Uout, int rk_stage, int stage_dt_host, double stage_dt) {
+    void stage(const Real* Uin, Real* Uout, int rk_stage, int stage_dt_host, double stage_dt) {
         int last = -1;
         for (int d = 0; d < 3; ++d)
             if (active(d)) last = d;
         bool first = true;
         for (int d = 0; d < 3; ++d) {
             if (!active(d)) continue;
-            FePassArgs a = pass_args(d);
+            FePassArgsT<Real> a = pass_args(d);
             a.U = Uin;
             a.R = R;
             a.U0 = U[0];
@@ -786,23 +810,23 @@ struct FeSolver {
     double dt_cfl() {
         hipLaunchKernelGGL(k_fe_zero_rate, dim3(1), dim3(kSlots), 0, stream, sc);
         FECHK(hipGetLastError());
-        FeRateArgs a{};
+        FeRateArgsT<Real> a{};
         a.U = U[0];
         a.sc = sc;
         a.ncell = ncell;
-        a.sys = sys;
+        a.sys = sys_as<Real>(sys);
         for (int d = 0; d < 3; ++d) {
             a.n[d] = (int)n[d];
             a.stride[d] = stride[d];
-            a.dx[d] = dx[d];
+            a.dx[d] = Real(dx[d]);
             a.act[d] = act[d];
             for (int sd = 0; sd < 2; ++sd) {
                 a.bc_kind[d][sd] = bc_kind[d][sd];
-                for (int q = 0; q < 7; ++q) a.bc_state[d][sd][q] = bc_state[d][sd][q];
+                for (int q = 0; q < 7; ++q) a.bc_state[d][sd][q] = Real(bc_state[d][sd][q]);
             }
         }
         const long long nb = std::min<long long>((ncell + 255) / 256, 2048);
-        hipLaunchKernelGGL(k_fe_rate, dim3((unsigned)nb), dim3(256), 0, stream, a);
+        hipLaunchKernelGGL(k_fe_rate<Real>, dim3((unsigned)nb), dim3(256), 0, stream, a);
         FECHK(hipGetLastError());
         std::vector<FeScalars> h(1);
         FECHK(hipMemcpyAsync(h.data(), sc, sizeof(FeScalars), hipMemcpyDeviceToHost, stream));
@@ -836,7 +860,7 @@ struct FeSolver {
             for (int c = 0; c < kNumFeTiles; ++c) {
                 if (kFeTiles[c].T > 2 * n[d] && c != 0) continue;
                 cfg[d] = c;
-                FePassArgs a = pass_args(d);
+                FePassArgsT<Real> a = pass_args(d);
                 a.U = U[0];
                 a.R = R;
                 a.U0 = U[0];
@@ -881,13 +905,16 @@ struct FeSolver {
     }
 };
 
+using FeSolver = FeCore<double>;
+
 }  // namespace fe
 }  // namespace em
 
 // ------------------------------------------------------------------ C ABI
 
 struct em_fiveeq {
-    em::fe::FeSolver* s;
+    em::fe::FeCore<double>* s;   // FP64 solver (null for an FP32 handle)
+    em::fe::FeCore<float>* s32;  // FP32 solver (null for an FP64 handle)
 };
 
 template <class F>
@@ -905,11 +932,46 @@ static int fe_guarded(F&& f) {
     }
 }
 
-#define FENEED(h)                                \
-    if (!(h) || !(h)->s) {                       \
-        em::fe::g_err = "null FiveEq handle";    \
-        return EM_ERR_ARG;                       \
+#define FENEED(h)                                     \
+    if (!(h) || (!(h)->s && !(h)->s32)) {             \
+        em::fe::g_err = "null FiveEq handle";         \
+        return EM_ERR_ARG;                            \
     }
+
+// f(core) on the handle's solver, whichever real type it holds
+template <class F>
+static int fe_on(em_fiveeq* h, F&& f) {
+    return fe_guarded([&] {
+        if (h->s)
+            f(*h->s);
+        else
+            f(*h->s32);
+    });
+}
+
+template <class Real>
+static int fe_make(em_fiveeq** out, const int64_t n[3], const double lo[3], const double hi[3], int32_t ng,
+                   const double phases[8], int32_t kapila, const int32_t bc_kind[6], const double bc_state[42],
+                   int32_t mode, int32_t positivity, double cfl, double tol, double t0, const double* u) {
+    if (!out || !n || !lo || !hi || !phases || !bc_kind || !u) {
+        em::fe::g_err = "null argument";
+        return EM_ERR_ARG;
+    }
+    return fe_guarded([&] {
+        auto* s = new em::fe::FeCore<Real>(n, lo, hi, ng, phases, kapila, bc_kind, bc_state, mode, positivity, cfl,
+                                           tol, t0);
+        try {
+            s->upload(u);
+        } catch (...) {
+            delete s;
+            throw;
+        }
+        if constexpr (sizeof(Real) == 8)
+            *out = new em_fiveeq{s, nullptr};
+        else
+            *out = new em_fiveeq{nullptr, s};
+    });
+}
 
 extern "C" {
 
@@ -918,59 +980,52 @@ const char* em_fiveeq_last_error(void) { return em::fe::g_err.c_str(); }
 int em_fiveeq_create(em_fiveeq** out, const int64_t n[3], const double lo[3], const double hi[3], int32_t ng,
                      const double phases[8], int32_t kapila, const int32_t bc_kind[6], const double bc_state[42],
                      int32_t mode, int32_t positivity, double cfl, double tol, double t0, const double* u) {
-    if (!out || !n || !lo || !hi || !phases || !bc_kind || !u) {
-        em::fe::g_err = "null argument";
-        return EM_ERR_ARG;
-    }
-    return fe_guarded([&] {
-        auto* s = new em::fe::FeSolver(n, lo, hi, ng, phases, kapila, bc_kind, bc_state, mode, positivity, cfl,
-                                       tol, t0);
-        try {
-            s->upload(u);
-        } catch (...) {
-            delete s;
-            throw;
-        }
-        *out = new em_fiveeq{s};
-    });
+    return fe_make<double>(out, n, lo, hi, ng, phases, kapila, bc_kind, bc_state, mode, positivity, cfl, tol, t0, u);
+}
+
+int em_fiveeq_create_f32(em_fiveeq** out, const int64_t n[3], const double lo[3], const double hi[3], int32_t ng,
+                         const double phases[8], int32_t kapila, const int32_t bc_kind[6], const double bc_state[42],
+                         int32_t mode, int32_t positivity, double cfl, double tol, double t0, const double* u) {
+    return fe_make<float>(out, n, lo, hi, ng, phases, kapila, bc_kind, bc_state, mode, positivity, cfl, tol, t0, u);
 }
 
 int em_fiveeq_destroy(em_fiveeq* h) {
     if (!h) return EM_OK;
     delete h->s;
+    delete h->s32;
     delete h;
     return EM_OK;
 }
 
 int em_fiveeq_upload(em_fiveeq* h, const double* u) {
     FENEED(h);
-    return fe_guarded([&] { h->s->upload(u); });
+    return fe_on(h, [&](auto& c) { c.upload(u); });
 }
 
 int em_fiveeq_download(em_fiveeq* h, double* u) {
     FENEED(h);
-    return fe_guarded([&] { h->s->to_host(h->s->U[0], u); });
+    return fe_on(h, [&](auto& c) { c.to_host(c.U[0], u); });
 }
 
 int em_fiveeq_rhs(em_fiveeq* h, double stage_dt, double* rhs, uint64_t stats[4]) {
     FENEED(h);
-    return fe_guarded([&] { h->s->rhs(stage_dt, rhs, stats); });
+    return fe_on(h, [&](auto& c) { c.rhs(stage_dt, rhs, stats); });
 }
 
 int em_fiveeq_step(em_fiveeq* h, double dt) {
     FENEED(h);
-    return fe_guarded([&] { h->s->step(dt); });
+    return fe_on(h, [&](auto& c) { c.step(dt); });
 }
 
 int em_fiveeq_dt_cfl(em_fiveeq* h, double* dt) {
     FENEED(h);
-    return fe_guarded([&] { *dt = h->s->dt_cfl(); });
+    return fe_on(h, [&](auto& c) { *dt = c.dt_cfl(); });
 }
 
 int em_fiveeq_run_to(em_fiveeq* h, double t_end, int64_t* steps) {
     FENEED(h);
-    return fe_guarded([&] {
-        const int64_t k = h->s->run_to(t_end);
+    return fe_on(h, [&](auto& c) {
+        const int64_t k = c.run_to(t_end);
         if (steps) *steps = k;
     });
 }
@@ -982,9 +1037,9 @@ int em_fiveeq_set_tiles(em_fiveeq* h, const int32_t cfg[3]) {
             em::fe::g_err = "tile configuration out of range";
             return EM_ERR_ARG;
         }
-    for (int d = 0; d < 3; ++d) h->s->cfg[d] = cfg[d];
-    em::fe::g_err.clear();
-    return EM_OK;
+    return fe_on(h, [&](auto& c) {
+        for (int d = 0; d < 3; ++d) c.cfg[d] = cfg[d];
+    });
 }
 
 int em_fiveeq_autotune(em_fiveeq* h, int32_t repeats, int32_t cfg_out[3]) {
@@ -993,23 +1048,22 @@ int em_fiveeq_autotune(em_fiveeq* h, int32_t repeats, int32_t cfg_out[3]) {
         em::fe::g_err = "repeats must be >= 1";
         return EM_ERR_ARG;
     }
-    return fe_guarded([&] { h->s->autotune(repeats, cfg_out); });
+    return fe_on(h, [&](auto& c) { c.autotune(repeats, cfg_out); });
 }
 
 int em_fiveeq_counters(em_fiveeq* h, uint64_t out[3]) {
     FENEED(h);
-    return fe_guarded([&] { h->s->read_counters(out); });
+    return fe_on(h, [&](auto& c) { c.read_counters(out); });
 }
 
 int em_fiveeq_time(em_fiveeq* h, double* t) {
     FENEED(h);
-    *t = h->s->t;
-    return EM_OK;
+    return fe_on(h, [&](auto& c) { *t = c.t; });
 }
 
 int em_fiveeq_synchronize(em_fiveeq* h) {
     FENEED(h);
-    return fe_guarded([&] { FECHK(hipStreamSynchronize(h->s->stream)); });
+    return fe_on(h, [&](auto& c) { FECHK(hipStreamSynchronize(c.stream)); });
 }
 
 }  // extern "C"

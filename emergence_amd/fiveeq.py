@@ -33,9 +33,13 @@ def padded_shape(n, ng):
 
 class HipFiveEq:
     def __init__(self, n, lo, hi, u, ng=6, phases=WATER_AIR, kapila=True, bc=None, mode="hybrid",
-                 positivity=True, cfl=0.5, tol=1e-3, t0=0.0):
+                 positivity=True, cfl=0.5, tol=1e-3, t0=0.0, real="f64"):
         """bc: [[kind_lo, kind_hi]] * 3 with kinds as names, or (name, state7)
-        tuples for Dirichlet / characteristic sides; default periodic."""
+        tuples for Dirichlet / characteristic sides; default periodic.
+        real: "f64" (GpuSolver<f64>) or "f32" (GpuSolver<f32>: state and
+        arithmetic in FP32 on the device; arrays here stay float64)."""
+        if real not in ("f64", "f32"):
+            raise ValueError("real must be 'f64' or 'f32'")
         self.lib = _lib.load()
         self.n = tuple(int(v) for v in n)
         self.ng = int(ng)
@@ -54,9 +58,10 @@ class HipFiveEq:
         ph = (C.c_double * 8)(*[float(v) for p in phases for v in p])
         u = np.ascontiguousarray(u, dtype=np.float64).reshape(self.shape)
         h = C.c_void_p()
-        _lib.check_fiveeq(self.lib.em_fiveeq_create(C.byref(h), (C.c_int64 * 3)(*self.n), (C.c_double * 3)(*lo),
-                                                    (C.c_double * 3)(*hi), self.ng, ph, int(bool(kapila)), kinds,
-                                                    states, MODE[mode], int(positivity), cfl, tol, t0, _dp(u)))
+        create = self.lib.em_fiveeq_create_f32 if real == "f32" else self.lib.em_fiveeq_create
+        _lib.check_fiveeq(create(C.byref(h), (C.c_int64 * 3)(*self.n), (C.c_double * 3)(*lo), (C.c_double * 3)(*hi),
+                                 self.ng, ph, int(bool(kapila)), kinds, states, MODE[mode], int(positivity), cfl, tol,
+                                 t0, _dp(u)))
         self.h = h
 
     def close(self):

@@ -415,6 +415,17 @@ const char* em_fiveeq_last_error(void);
 int em_fiveeq_create(em_fiveeq** out, const int64_t n[3], const double lo[3], const double hi[3], int32_t ng,
                      const double phases[8], int32_t kapila, const int32_t bc_kind[6], const double bc_state[42],
                      int32_t mode, int32_t positivity, double cfl, double tol, double t0, const double* u);
+/* GpuSolver::<f32>::from_cpu with GpuSystem::FiveEq (lib.rs:26-48 Real for
+ * f32; the reference compiles its FiveEq kernels for float, kernels.cu:1157):
+ * state, residual and arithmetic in FP32 on the device, phase parameters,
+ * boundary states, 1/dx and the tolerance rounded to float, the limiter's
+ * lambda formed in double from dt and rounded, the float WENO-Z epsilon
+ * 1e-30 (kernels.cu:263).  Host arrays stay double (upload rounds, download
+ * and rhs widen); the handle takes every em_fiveeq_* call.  Built without
+ * multiply-add contraction, like the FP64 kernels of this system. */
+int em_fiveeq_create_f32(em_fiveeq** out, const int64_t n[3], const double lo[3], const double hi[3], int32_t ng,
+                         const double phases[8], int32_t kapila, const int32_t bc_kind[6], const double bc_state[42],
+                         int32_t mode, int32_t positivity, double cfl, double tol, double t0, const double* u);
 int em_fiveeq_destroy(em_fiveeq* h);
 /* upload / download the conserved state (download leaves the ghosts zero) */
 int em_fiveeq_upload(em_fiveeq* h, const double* u);
