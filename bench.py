@@ -67,7 +67,7 @@ def parse():
     ap.add_argument("--tiles", type=str, default=None,
                     help="fixed Weno tile shapes per axis, e.g. 6,6,5 (indices into TILE_SHAPES; no autotuning)")
     ap.add_argument("--hybrid-tiles", type=str, default=None, help="fixed Hybrid tile shapes per axis")
-    ap.add_argument("--no-f32", action="store_true", help="skip the FP32 256^3 line")
+    ap.add_argument("--no-f32", action="store_true", help="skip the FP32 lines (256^3 Euler, FiveEq probe)")
     ap.add_argument("--f32-tiles", type=str, default=None, help="fixed FP32 tile shapes per axis")
     ap.add_argument("--no-amr", action="store_true", help="skip the block-AMR workload")
     ap.add_argument("--amr-only", action="store_true", help="only the block-AMR workload (profiling)")
@@ -621,7 +621,10 @@ def fiveeq_bench(em, args):
     """The reference's `interface` probe at n^3 (examples/cpu_validate.rs:100-
     146): a water drop (stiffened gas) in air moving at 0.1 through p = 1e5,
     Kapila five-equation model, Hybrid mode, 3-D periodic; fixed dt = half the
-    CFL dt of the initial state, steps timed after a warm-up step."""
+    CFL dt of the initial state, steps timed after a warm-up step.  Then the
+    same steps on the FP32 variant (the reference's GpuSolver<f32> with
+    GpuSystem::FiveEq) at the same dt, with the relative L1 difference of its
+    mixture density from the FP64 run's."""
     n = args.fe_n
     x = (np.arange(n) + 0.5) / n
     X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
@@ -633,18 +636,23 @@ def fiveeq_bench(em, args):
     e = 1e5 * (a / (g1 - 1.0) + (1.0 - a) / (g2 - 1.0)) + g1 * pi1 * a / (g1 - 1.0) + 0.5 * rho * vx * vx
     u = np.zeros((n + 12, n + 12, n + 12, 7))
     u[6:-6, 6:-6, 6:-6] = np.stack([a1, a2, rho * vx, 0 * a, 0 * a, e, a], axis=-1)
-    g = em.HipFiveEq((n, n, n), (0, 0, 0), (1, 1, 1), u, mode="hybrid")
-    tiles = pick_tiles(g, args, args.fe_tiles)
-    dt = 0.5 * g.dt_cfl()
-    g.step(dt)
-    g.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.fe_steps):
+    del X, Y, Z, r, a, a1, a2, rho, e
+
+    def run(real, dt=None):
+        g = em.HipFiveEq((n, n, n), (0, 0, 0), (1, 1, 1), u, mode="hybrid", real=real)
+        tiles = pick_tiles(g, args, args.fe_tiles)
+        dt = 0.5 * g.dt_cfl() if dt is None else dt
         g.step(dt)
-    g.synchronize()
-    el = time.perf_counter() - t0
+        g.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.fe_steps):
+            g.step(dt)
+        g.synchronize()
+        return g, time.perf_counter() - t0, tiles, dt
+
+    g, el, tiles, dt = run("f64")
     shock = g.counters()[0] / (3 * (args.fe_steps + 1) * 3 * (n + 1) * n * n)
-    return {
+    rec = {
         "workload": f"3-D periodic {n}^3 water drop in air, Kapila five-equation model (7 variables), Hybrid, "
                     "the reference's interface probe at this size",
         "value": n ** 3 * args.fe_steps * 3 / el,
@@ -655,6 +663,20 @@ def fiveeq_bench(em, args):
         "dt": dt,
         "shock_path_face_fraction": shock,
     }
+    if not args.no_f32:
+        rho64 = g.interior(g.download())[..., :2].sum(axis=-1)
+        g.close()
+        g32, el32, tiles32, _ = run("f32", dt)
+        rho32 = g32.interior(g32.download())[..., :2].sum(axis=-1)
+        g32.close()
+        rec["f32"] = {
+            "value": n ** 3 * args.fe_steps * 3 / el32,
+            "unit": "cell-updates/s per RK stage",
+            "ms_per_step": el32 / args.fe_steps * 1e3,
+            "tiles": tiles32,
+            "density_rel_l1_vs_f64": float(np.abs(rho32 - rho64).sum() / np.abs(rho64).sum()),
+        }
+    return rec
 
 
 def f32_bench(em, args, n, u):
