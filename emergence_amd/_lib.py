@@ -50,6 +50,8 @@ EXPORTS = (
     "em_amr_last_error",
     "em_amr_create",
     "em_amr_create_case",
+    "em_amr_create_f32",
+    "em_amr_create_case_f32",
     "em_amr_destroy",
     "em_amr_set_variant",
     "em_amr_set_batched",
@@ -165,6 +167,8 @@ def load():
     lib.em_amr_last_error.argtypes = []
     lib.em_amr_create.argtypes = amr_cfg + [C.c_int64, I32, I64, U8, D]
     lib.em_amr_create_case.argtypes = amr_cfg + [C.c_int32, D]
+    lib.em_amr_create_f32.argtypes = amr_cfg + [C.c_int64, I32, I64, U8, D]
+    lib.em_amr_create_case_f32.argtypes = amr_cfg + [C.c_int32, D]
     lib.em_amr_destroy.argtypes = [P]
     lib.em_amr_set_variant.argtypes = [P, C.c_int32]
     lib.em_amr_set_batched.argtypes = [P, C.c_int32]

@@ -42,10 +42,15 @@ def _bc_arrays(bc):
 class HipAmr:
     def __init__(self, base, b, levels, lo, hi, case=None, params=None, blocks=None, eps=1e-3, regrid_every=4,
                  ghosts=5, bc=None, mode="hybrid", cfl=0.4, positivity=True, tol=1e-3, eos=(1.4, 0.0, 0.0, 0.0),
-                 precise=True, batched=True):
+                 precise=True, batched=True, real="f64"):
         """Either `case` (name of a built-in initial condition, with params) or
-        `blocks` = (level[n], idx[n, 3], leaf[n], u[n, b0, b1, b2, 5])."""
+        `blocks` = (level[n], idx[n, 3], leaf[n], u[n, b0, b1, b2, 5]).
+        real: "f64" (GpuAmr<f64>) or "f32" (GpuAmr<f32>: pool and arithmetic
+        in FP32 on the device; arrays here stay float64)."""
+        if real not in ("f64", "f32"):
+            raise ValueError("real must be 'f64' or 'f32'")
         self.lib = _lib.load()
+        self.real = real
         self.base = tuple(int(v) for v in base)
         self.b = int(b)
         self.levels = int(levels)
@@ -60,7 +65,8 @@ class HipAmr:
             prm = np.zeros(15)
             if params is not None:
                 prm[:len(params)] = params
-            _lib.check_amr(self.lib.em_amr_create_case(*cfg, AMR_CASE[case], _dp(prm)))
+            create = self.lib.em_amr_create_case_f32 if real == "f32" else self.lib.em_amr_create_case
+            _lib.check_amr(create(*cfg, AMR_CASE[case], _dp(prm)))
         else:
             level, idx, leaf, u = blocks
             level = np.ascontiguousarray(level, dtype=np.int32)
@@ -70,9 +76,10 @@ class HipAmr:
             n = len(level)
             if u.size != n * int(np.prod(self.bd)) * 5:
                 raise ValueError("block data has the wrong size")
-            _lib.check_amr(self.lib.em_amr_create(*cfg, n, level.ctypes.data_as(C.POINTER(C.c_int32)),
-                                                  idx.ctypes.data_as(C.POINTER(C.c_int64)),
-                                                  leaf.ctypes.data_as(C.POINTER(C.c_uint8)), _dp(u)))
+            create = self.lib.em_amr_create_f32 if real == "f32" else self.lib.em_amr_create
+            _lib.check_amr(create(*cfg, n, level.ctypes.data_as(C.POINTER(C.c_int32)),
+                                  idx.ctypes.data_as(C.POINTER(C.c_int64)), leaf.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                  _dp(u)))
         self.h = h
         self.set_variant(precise)
         self.set_batched(batched)

@@ -82,14 +82,16 @@ __global__ void k_amr_map_set(int* map, const long long* __restrict__ off, const
 
 // pool slots src[i] -> tmp block i, and tmp block i -> slot dst[i] (a
 // permutation of blocks applied through a staging buffer)
-__global__ void k_amr_blocks_out(AmrDev a, const int* __restrict__ src, int n, double* __restrict__ tmp) {
+template <class Real>
+__global__ void k_amr_blocks_out(AmrDevT<Real> a, const int* __restrict__ src, int n, Real* __restrict__ tmp) {
     const long long tot = (long long)n * a.bcells;
     for (long long i = blockIdx.x * 256ll + threadIdx.x; i < tot; i += (long long)gridDim.x * 256ll) {
         const long long b = i / a.bcells, c = i % a.bcells;
         for (int q = 0; q < 5; ++q) tmp[(b * 5 + q) * a.bcells + c] = a.U[amr_u(a, q, src[b], c)];
     }
 }
-__global__ void k_amr_blocks_in(AmrDev a, const int* __restrict__ dst, int n, const double* __restrict__ tmp) {
+template <class Real>
+__global__ void k_amr_blocks_in(AmrDevT<Real> a, const int* __restrict__ dst, int n, const Real* __restrict__ tmp) {
     const long long tot = (long long)n * a.bcells;
     for (long long i = blockIdx.x * 256ll + threadIdx.x; i < tot; i += (long long)gridDim.x * 256ll) {
         const long long b = i / a.bcells, c = i % a.bcells;
@@ -98,28 +100,32 @@ __global__ void k_amr_blocks_in(AmrDev a, const int* __restrict__ dst, int n, co
 }
 
 // staged host data (per block AoS: bcells x 5) -> pool slots
-__global__ void k_amr_scatter(AmrDev a, const double* __restrict__ stage, const int* __restrict__ slots, int n) {
+template <class Real>
+__global__ void k_amr_scatter(AmrDevT<Real> a, const double* __restrict__ stage, const int* __restrict__ slots, int n) {
     const long long tot = (long long)n * a.bcells;
     for (long long i = blockIdx.x * 256ll + threadIdx.x; i < tot; i += (long long)gridDim.x * 256ll) {
         const long long b = i / a.bcells, c = i % a.bcells;
-        for (int q = 0; q < 5; ++q) a.U[amr_u(a, q, slots[b], c)] = stage[i * 5 + q];
+        for (int q = 0; q < 5; ++q) a.U[amr_u(a, q, slots[b], c)] = Real(stage[i * 5 + q]);
     }
 }
 // pool slots -> staging (per block AoS)
-__global__ void k_amr_gather(AmrDev a, double* __restrict__ stage, const double* __restrict__ src, long long n) {
+template <class Real>
+__global__ void k_amr_gather(AmrDevT<Real> a, double* __restrict__ stage, const Real* __restrict__ src, long long n) {
     const long long tot = n * a.bcells;
     for (long long i = blockIdx.x * 256ll + threadIdx.x; i < tot; i += (long long)gridDim.x * 256ll) {
         const long long b = i / a.bcells, c = i % a.bcells;
-        for (int q = 0; q < 5; ++q) stage[i * 5 + q] = src[q * a.vs + b * a.bcells + c];
+        for (int q = 0; q < 5; ++q) stage[i * 5 + q] = (double)src[q * a.vs + b * a.bcells + c];
     }
 }
 // block data move (coarsening's swap-remove)
-__global__ void k_amr_move(AmrDev a, int from, int to) {
+template <class Real>
+__global__ void k_amr_move(AmrDevT<Real> a, int from, int to) {
     for (long long c = blockIdx.x * 256ll + threadIdx.x; c < a.bcells; c += (long long)gridDim.x * 256ll)
         for (int q = 0; q < 5; ++q) a.U[amr_u(a, q, to, c)] = a.U[amr_u(a, q, from, c)];
 }
 
-__global__ void k_amr_copy_u0(AmrDev a, const int* __restrict__ leaves, int n) {
+template <class Real>
+__global__ void k_amr_copy_u0(AmrDevT<Real> a, const int* __restrict__ leaves, int n) {
     const long long tot = (long long)n * a.bcells;
     for (long long i = blockIdx.x * 256ll + threadIdx.x; i < tot; i += (long long)gridDim.x * 256ll) {
         const long long s = leaves[i / a.bcells], c = i % a.bcells;
@@ -128,21 +134,22 @@ __global__ void k_amr_copy_u0(AmrDev a, const int* __restrict__ leaves, int n) {
 }
 
 // SSP-RK3 update of the leaves (amr.rs:1476-1498)
-__global__ void k_amr_rk(AmrDev a, const int* __restrict__ leaves, int n, int stage) {
-    const double dt = a.sc->dt;
+template <class Real>
+__global__ void k_amr_rk(AmrDevT<Real> a, const int* __restrict__ leaves, int n, int stage) {
+    const Real dt = Real(a.sc->dt);
     const long long tot = (long long)n * a.bcells;
     for (long long i = blockIdx.x * 256ll + threadIdx.x; i < tot; i += (long long)gridDim.x * 256ll) {
         const long long s = leaves[i / a.bcells], c = i % a.bcells;
         for (int q = 0; q < 5; ++q) {
             const long long ix = amr_u(a, q, s, c);
-            const double u = a.U[ix], r = a.R[ix];
-            double o;
+            const Real u = a.U[ix], r = a.R[ix];
+            Real o;
             if (stage == 1)
                 o = u + dt * r;
             else if (stage == 2)
-                o = 0.75 * a.U0[ix] + 0.25 * (u + dt * r);
+                o = Real(0.75) * a.U0[ix] + Real(0.25) * (u + dt * r);
             else
-                o = a.U0[ix] / 3.0 + 2.0 / 3.0 * (u + dt * r);
+                o = a.U0[ix] / Real(3.0) + Real(2.0) / Real(3.0) * (u + dt * r);
             a.U[ix] = o;
         }
     }
@@ -150,22 +157,23 @@ __global__ void k_amr_rk(AmrDev a, const int* __restrict__ leaves, int n, int st
 
 // CFL rate over leaf cells: sum over active axes of max_speed / dx_level
 // (amr.rs:1409-1430)
-__global__ __launch_bounds__(256) void k_amr_rate(AmrDev a, const int* __restrict__ leaves,
+template <class Real>
+__global__ __launch_bounds__(256) void k_amr_rate(AmrDevT<Real> a, const int* __restrict__ leaves,
                                                   const int* __restrict__ lvl, int n) {
     __shared__ double red[4];
-    double r = 0.0;
+    double r = 0.0;  // a max of Real values, exact in double
     const long long tot = (long long)n * a.bcells;
     for (long long i = blockIdx.x * 256ll + threadIdx.x; i < tot; i += (long long)gridDim.x * 256ll) {
         const long long li = i / a.bcells, c = i % a.bcells;
         const long long s = leaves[li];
-        double u[5], w[5];
+        Real u[5], w[5];
         for (int q = 0; q < 5; ++q) u[q] = a.U[amr_u(a, q, s, c)];
         to_prim(a.eos, u, w);
-        const double cs = sound_speed(a.eos, w[0], w[4]);
-        double ac = -0.0;
+        const Real cs = sound_speed(a.eos, w[0], w[4]);
+        Real ac = -Real(0.0);
         for (int d = 0; d < 3; ++d)
             if (a.act[d]) ac += (fabs(w[1 + d]) + cs) / a.dx[lvl[li]][d];
-        r = fmax(r, ac);
+        r = fmax(r, (double)ac);
     }
     for (int off = 32; off > 0; off >>= 1) r = fmax(r, __shfl_xor(r, off));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = r;
@@ -188,9 +196,10 @@ constexpr int kGhostThreads = 256;  // the slab prediction needs more registers
 // copy, physical boundary (bc_box on a face slab), or 6th-order prediction
 // (fill_scratch, amr.rs:1113-1221).
 
-__global__ __launch_bounds__(kGhostThreads) void k_amr_ghost(AmrDev a, AmrWeights W, const GhostTask* __restrict__ tasks, int n,
-                                                  double* scratch, long long cap) {
-    double* scr = scratch + (long long)blockIdx.x * 12 * cap;
+template <class Real>
+__global__ __launch_bounds__(kGhostThreads) void k_amr_ghost(AmrDevT<Real> a, AmrWeightsT<Real> W, const GhostTask* __restrict__ tasks, int n,
+                                                  Real* scratch, long long cap) {
+    Real* scr = scratch + (long long)blockIdx.x * 12 * cap;
     for (int ti = blockIdx.x; ti < n; ti += gridDim.x) {
         const GhostTask tk = tasks[ti];
         const int d = tk.axis, side = tk.side;
@@ -206,13 +215,13 @@ __global__ __launch_bounds__(kGhostThreads) void k_amr_ghost(AmrDev a, AmrWeight
                 cc[E0] = t / a.bd[E1];
                 cc[E1] = t % a.bd[E1];
                 const int x = side == 0 ? layer - ng : B + layer;
-                double u[5];
+                Real u[5];
                 if (tk.kind == 0) {
                     cc[d] = side == 0 ? B + x : x - B;
                     const long long cell = ((long long)cc[0] * a.bd[1] + cc[1]) * a.bd[2] + cc[2];
                     for (int q = 0; q < 5; ++q) u[q] = a.U[amr_u(a, q, tk.src, cell)];
                 } else {
-                    auto ld = [&](int xx, double* o) {
+                    auto ld = [&](int xx, Real* o) {
                         int c3[3] = {cc[0], cc[1], cc[2]};
                         c3[d] = xx;
                         const long long cell = ((long long)c3[0] * a.bd[1] + c3[1]) * a.bd[2] + c3[2];
@@ -241,7 +250,7 @@ __global__ __launch_bounds__(kGhostThreads) void k_amr_ghost(AmrDev a, AmrWeight
                     fmax[e] = fmin[e] + a.bd[e] - 1;
                 }
             }
-            const double* res = predict_region(a, W, tk.level, fmin, fmax, scr, cap);
+            const Real* res = predict_region(a, W, tk.level, fmin, fmax, scr, cap);
             // result box: extents ng along d, bd along the others (row-major)
             int rsz[3];
             for (int e = 0; e < 3; ++e) rsz[e] = (int)(fmax[e] - fmin[e] + 1);
@@ -265,7 +274,8 @@ __global__ __launch_bounds__(kGhostThreads) void k_amr_ghost(AmrDev a, AmrWeight
 // (kind 2) keep k_amr_ghost and its register budget.  Tasks write disjoint
 // slabs and read only block data, so the split changes no value.
 constexpr int kGhostCopyThreads = 256;
-__global__ __launch_bounds__(kGhostCopyThreads) void k_amr_ghost_copy(AmrDev a, const GhostTask* __restrict__ tasks,
+template <class Real>
+__global__ __launch_bounds__(kGhostCopyThreads) void k_amr_ghost_copy(AmrDevT<Real> a, const GhostTask* __restrict__ tasks,
                                                                        long long ntot, int per) {
     for (long long gi = blockIdx.x * (long long)kGhostCopyThreads + threadIdx.x; gi < ntot;
          gi += (long long)gridDim.x * kGhostCopyThreads) {
@@ -282,13 +292,13 @@ __global__ __launch_bounds__(kGhostCopyThreads) void k_amr_ghost_copy(AmrDev a, 
         cc[E0] = t / a.bd[E1];
         cc[E1] = t % a.bd[E1];
         const int x = side == 0 ? layer - ng : B + layer;
-        double u[5];
+        Real u[5];
         if (tk.kind == 0) {
             cc[d] = side == 0 ? B + x : x - B;
             const long long cell = ((long long)cc[0] * a.bd[1] + cc[1]) * a.bd[2] + cc[2];
             for (int q = 0; q < 5; ++q) u[q] = a.U[amr_u(a, q, tk.src, cell)];
         } else {
-            auto ld = [&](int xx, double* o) {
+            auto ld = [&](int xx, Real* o) {
                 int c3[3] = {cc[0], cc[1], cc[2]};
                 c3[d] = xx;
                 const long long cell = ((long long)c3[0] * a.bd[1] + c3[1]) * a.bd[2] + c3[2];
@@ -311,13 +321,15 @@ __global__ __launch_bounds__(kGhostCopyThreads) void k_amr_ghost_copy(AmrDev a, 
 // (capB cells) in LDS, the last pass storing straight into the slab: the
 // arithmetic of k_amr_ghost without its global scratch.  Launched when the
 // largest slab's buffers fit in LDS.
-__global__ __launch_bounds__(kGhostThreads) void k_amr_ghost_lds(AmrDev a, AmrWeights W,
+template <class Real>
+__global__ __launch_bounds__(kGhostThreads) void k_amr_ghost_lds(AmrDevT<Real> a, AmrWeightsT<Real> W,
                                                       const GhostTask* __restrict__ tasks, int n, int capA,
                                                       int capB) {
-    extern __shared__ double lds[];
-    double* A = lds;
-    double* ok = A + 5 * capA;
-    double* Bf = ok + capA;
+    extern __shared__ __align__(16) unsigned char lds_bytes[];  // one name for every Real
+    Real* lds = reinterpret_cast<Real*>(lds_bytes);
+    Real* A = lds;
+    Real* ok = A + 5 * capA;
+    Real* Bf = ok + capA;
     const XcdStride xs = xcd_stride();  // neighbouring slabs share one L2
     for (long long t = xs.first;; t += xs.step) {
         const long long ti = xs.item(t);
@@ -340,7 +352,7 @@ __global__ __launch_bounds__(kGhostThreads) void k_amr_ghost_lds(AmrDev a, AmrWe
                 fmax[e] = fmin[e] + a.bd[e] - 1;
             }
         }
-        predict_box(a, W, tk.level, fmin, fmax, A, ok, capA, Bf, capB, [&](int, const int c[3], const double* v) {
+        predict_box(a, W, tk.level, fmin, fmax, A, ok, capA, Bf, capB, [&](int, const int c[3], const Real* v) {
             const int t = c[E0] * a.bd[E1] + c[E1];
 #pragma unroll
             for (int q = 0; q < 5; ++q) a.G[amr_g(a, q, tk.slot, s, c[d], t)] = v[q];
@@ -353,15 +365,17 @@ __global__ __launch_bounds__(kGhostThreads) void k_amr_ghost_lds(AmrDev a, AmrWe
 // passes after the first in LDS (cap1 + cap2 cells and their validity).
 // Same entries and write masks as k_amr_restrict; no blocks are excluded.
 constexpr int kRestrictSubThreads = 256;
-__global__ __launch_bounds__(kRestrictSubThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_amr_restrict_sub(AmrDev a, AmrWeights W,
+template <class Real>
+__global__ __launch_bounds__(kRestrictSubThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_amr_restrict_sub(AmrDevT<Real> a, AmrWeightsT<Real> W,
                                                                const int* __restrict__ ids,
                                                                const int* __restrict__ ent, int n, int level, int w,
                                                                RestrictSub rs) {
-    extern __shared__ double lds[];
-    double* T1 = lds;
-    double* ok1 = T1 + 5 * rs.cap1;
-    double* T2 = ok1 + rs.cap1;
-    double* ok2 = T2 + 5 * rs.cap2;
+    extern __shared__ __align__(16) unsigned char lds_bytes[];  // one name for every Real
+    Real* lds = reinterpret_cast<Real*>(lds_bytes);
+    Real* T1 = lds;
+    Real* ok1 = T1 + 5 * rs.cap1;
+    Real* T2 = ok1 + rs.cap1;
+    Real* ok2 = T2 + 5 * rs.cap2;
     __shared__ int bids[kBoxBlocks];
     __shared__ int pmap[3 * kPatchLen];
     const long long total = (long long)n * rs.nsubt;
@@ -384,11 +398,12 @@ __global__ __launch_bounds__(kRestrictSubThreads) __attribute__((amdgpu_waves_pe
 // depend on the box they are computed in, so the whole block is computed and
 // only the selected cells are stored.  xoff/xs (optional, coarsening
 // replay): ancestor i treats the blocks xs[xoff[i] .. xoff[i+1]) as absent.
-__global__ __launch_bounds__(kPatchThreads) void k_amr_restrict(AmrDev a, AmrWeights W, const int* __restrict__ ids,
+template <class Real>
+__global__ __launch_bounds__(kPatchThreads) void k_amr_restrict(AmrDevT<Real> a, AmrWeightsT<Real> W, const int* __restrict__ ids,
                                                      const int* __restrict__ ent, int n, int level, int w,
-                                                     double* scratch, long long cap, const int* __restrict__ xoff,
+                                                     Real* scratch, long long cap, const int* __restrict__ xoff,
                                                      const int* __restrict__ xs) {
-    double* scr = scratch + (long long)blockIdx.x * 12 * cap;
+    Real* scr = scratch + (long long)blockIdx.x * 12 * cap;
     for (int i = blockIdx.x; i < n; i += gridDim.x) {
         const int bi[3] = {ent[4 * i], ent[4 * i + 1], ent[4 * i + 2]};
         const int nx = xoff ? xoff[i + 1] - xoff[i] : 0;
@@ -398,10 +413,11 @@ __global__ __launch_bounds__(kPatchThreads) void k_amr_restrict(AmrDev a, AmrWei
 
 // Prediction of whole refined children from their parent level
 // (refine_block, amr.rs:989-996).
-__global__ __launch_bounds__(kPatchThreads) void k_amr_predict_blocks(AmrDev a, AmrWeights W, const int* __restrict__ slots,
+template <class Real>
+__global__ __launch_bounds__(kPatchThreads) void k_amr_predict_blocks(AmrDevT<Real> a, AmrWeightsT<Real> W, const int* __restrict__ slots,
                                                            const int* __restrict__ idx, int n, int level,
-                                                           double* scratch, long long cap) {
-    double* scr = scratch + (long long)blockIdx.x * 12 * cap;
+                                                           Real* scratch, long long cap) {
+    Real* scr = scratch + (long long)blockIdx.x * 12 * cap;
     for (int i = blockIdx.x; i < n; i += gridDim.x) {
         int fmin[3], fmax[3];
         for (int e = 0; e < 3; ++e) {
@@ -409,7 +425,7 @@ __global__ __launch_bounds__(kPatchThreads) void k_amr_predict_blocks(AmrDev a, 
             fmax[e] = fmin[e] + a.bd[e] - 1;
             if (!a.act[e]) fmin[e] = fmax[e] = 0;
         }
-        const double* res = predict_region(a, W, level, fmin, fmax, scr, cap);
+        const Real* res = predict_region(a, W, level, fmin, fmax, scr, cap);
         for (int c = threadIdx.x; c < a.bcells; c += blockDim.x)
             for (int q = 0; q < 5; ++q) a.U[amr_u(a, q, slots[i], c)] = res[q * cap + c];
         __syncthreads();
@@ -419,17 +435,18 @@ __global__ __launch_bounds__(kPatchThreads) void k_amr_predict_blocks(AmrDev a, 
 // Refluxing of one (axis, side): coarse rhs += +-(mean fine flux - coarse
 // flux) / dx (amr.rs:1327-1406); one launch per (axis, side) in the
 // reference's order, so no cell is corrected twice in one launch.
-__global__ void k_amr_reflux(AmrDev a, const RefluxTask* __restrict__ t, int n, int d, int side) {
+template <class Real>
+__global__ void k_amr_reflux(AmrDevT<Real> a, const RefluxTask* __restrict__ t, int n, int d, int side) {
     for (int i = blockIdx.x * 128 + threadIdx.x; i < n; i += gridDim.x * 128) {
         const RefluxTask k = t[i];
-        const double dx = a.dx[k.level][d];
-        const double cnt = (double)k.nfine;
+        const Real dx = a.dx[k.level][d];
+        const Real cnt = Real(k.nfine);
         for (int q = 0; q < 5; ++q) {
-            double sum = 0.0;
+            Real sum = Real(0.0);
             for (int f = 0; f < k.nfine; ++f) sum += a.F[amr_f(a, q, k.fslot[f], 2 * d + (1 - side), k.fface[f])];
-            const double fc = a.F[amr_f(a, q, k.slot, 2 * d + side, k.face)];
-            const double df = sum / cnt - fc;
-            const double dr = side == 1 ? -df / dx : df / dx;
+            const Real fc = a.F[amr_f(a, q, k.slot, 2 * d + side, k.face)];
+            const Real df = sum / cnt - fc;
+            const Real dr = side == 1 ? -df / dx : df / dx;
             const long long ix = amr_u(a, q, k.slot, k.cell);
             a.R[ix] += dr;
         }
@@ -440,19 +457,20 @@ __global__ void k_amr_reflux(AmrDev a, const RefluxTask* __restrict__ t, int n, 
 // all (axis, side) lists grouped by the coarse cell they correct, each group
 // in the (axis, side) order of k_amr_reflux's launches, so a cell's residual
 // takes its corrections in the same order and every cell is one thread's.
-__global__ void k_amr_reflux_cells(AmrDev a, const RefluxTask* __restrict__ t, const int* __restrict__ grp, int ng) {
+template <class Real>
+__global__ void k_amr_reflux_cells(AmrDevT<Real> a, const RefluxTask* __restrict__ t, const int* __restrict__ grp, int ng) {
     for (int gi = blockIdx.x * 128 + threadIdx.x; gi < ng; gi += gridDim.x * 128) {
         for (int i = grp[gi]; i < grp[gi + 1]; ++i) {
             const RefluxTask k = t[i];
             const int d = k.axis, side = k.side;
-            const double dx = a.dx[k.level][d];
-            const double cnt = (double)k.nfine;
+            const Real dx = a.dx[k.level][d];
+            const Real cnt = Real(k.nfine);
             for (int q = 0; q < 5; ++q) {
-                double sum = 0.0;
+                Real sum = Real(0.0);
                 for (int f = 0; f < k.nfine; ++f) sum += a.F[amr_f(a, q, k.fslot[f], 2 * d + (1 - side), k.fface[f])];
-                const double fc = a.F[amr_f(a, q, k.slot, 2 * d + side, k.face)];
-                const double df = sum / cnt - fc;
-                const double dr = side == 1 ? -df / dx : df / dx;
+                const Real fc = a.F[amr_f(a, q, k.slot, 2 * d + side, k.face)];
+                const Real df = sum / cnt - fc;
+                const Real dr = side == 1 ? -df / dx : df / dx;
                 const long long ix = amr_u(a, q, k.slot, k.cell);
                 a.R[ix] += dr;
             }
@@ -463,17 +481,18 @@ __global__ void k_amr_reflux_cells(AmrDev a, const RefluxTask* __restrict__ t, c
 // Largest relative wavelet detail of each leaf (block_detail, amr.rs:823-867):
 // |s - sum_m wr[m] s(+-1, +-3, +-5)| of the sensor values ln rho,
 // ln(p + p_inf) along every active axis; infinite if not finite.
-__global__ __launch_bounds__(256) void k_amr_details(AmrDev a, AmrWeights W, const int* __restrict__ leaves, int n,
+template <class Real>
+__global__ __launch_bounds__(256) void k_amr_details(AmrDevT<Real> a, AmrWeightsT<Real> W, const int* __restrict__ leaves, int n,
                                                     double* __restrict__ det) {
-    __shared__ double red[4];
+    __shared__ Real red[4];
     for (int li = blockIdx.x; li < n; li += gridDim.x) {
         const long long slot = leaves[li];
-        double mx = 0.0;
+        Real mx = Real(0.0);
         for (int c = threadIdx.x; c < a.bcells; c += blockDim.x) {
             int ci[3];
             unflat(c, a.bd, ci);
-            auto sens = [&](int d, int x, double s[2]) {
-                double u[5], w[5];
+            auto sens = [&](int d, int x, Real s[2]) {
+                Real u[5], w[5];
                 if (x >= 0 && x < a.bd[d]) {
                     int c3[3] = {ci[0], ci[1], ci[2]};
                     c3[d] = x;
@@ -486,22 +505,22 @@ __global__ __launch_bounds__(256) void k_amr_details(AmrDev a, AmrWeights W, con
                     for (int q = 0; q < 5; ++q) u[q] = a.G[amr_g(a, q, slot, 2 * d + (x < 0 ? 0 : 1), layer, t)];
                 }
                 to_prim(a.eos, u, w);
-                s[0] = log(fmax(w[0], 1e-300));
-                s[1] = log(fmax(w[4] + a.eos.p_inf, 1e-300));
+                s[0] = log(fmax(w[0], Real(1e-300)));
+                s[1] = log(fmax(w[4] + a.eos.p_inf, Real(1e-300)));
             };
             for (int d = 0; d < 3; ++d) {
                 if (!a.act[d]) continue;
-                double s0[2];
+                Real s0[2];
                 sens(d, ci[d], s0);
-                double pred[2] = {0.0, 0.0};
+                Real pred[2] = {Real(0.0), Real(0.0)};
                 const int offs[6] = {-5, -3, -1, 1, 3, 5};
                 for (int m = 0; m < 6; ++m) {
-                    double sv[2];
+                    Real sv[2];
                     sens(d, ci[d] + offs[m], sv);
                     for (int q = 0; q < 2; ++q) pred[q] += W.wr[m] * sv[q];
                 }
                 for (int q = 0; q < 2; ++q) {
-                    const double dd = fabs(s0[q] - pred[q]);
+                    const Real dd = fabs(s0[q] - pred[q]);
                     mx = isfinite(dd) ? fmax(mx, dd) : INFINITY;
                 }
             }
@@ -509,7 +528,7 @@ __global__ __launch_bounds__(256) void k_amr_details(AmrDev a, AmrWeights W, con
         for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
         if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
         __syncthreads();
-        if (threadIdx.x == 0) det[li] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        if (threadIdx.x == 0) det[li] = (double)fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
         __syncthreads();
     }
 }
@@ -520,21 +539,22 @@ __global__ __launch_bounds__(256) void k_amr_details(AmrDev a, AmrWeights W, con
 // once per stencil that reads it; the values and the order of the sums are
 // those of k_amr_details.
 constexpr int kDetCells = 512, kDetFace = 64;
-__global__ __launch_bounds__(256) void k_amr_details_lds(AmrDev a, AmrWeights W, const int* __restrict__ leaves, int n,
+template <class Real>
+__global__ __launch_bounds__(256) void k_amr_details_lds(AmrDevT<Real> a, AmrWeightsT<Real> W, const int* __restrict__ leaves, int n,
                                                         double* __restrict__ det) {
-    __shared__ double red[4];
-    __shared__ double si[2][kDetCells];
-    __shared__ double sg[2][6][5][kDetFace];
-    auto sensor = [&](const double u[5], double s[2]) {
-        double w[5];
+    __shared__ Real red[4];
+    __shared__ Real si[2][kDetCells];
+    __shared__ Real sg[2][6][5][kDetFace];
+    auto sensor = [&](const Real u[5], Real s[2]) {
+        Real w[5];
         to_prim(a.eos, u, w);
-        s[0] = log(fmax(w[0], 1e-300));
-        s[1] = log(fmax(w[4] + a.eos.p_inf, 1e-300));
+        s[0] = log(fmax(w[0], Real(1e-300)));
+        s[1] = log(fmax(w[4] + a.eos.p_inf, Real(1e-300)));
     };
     for (int li = blockIdx.x; li < n; li += gridDim.x) {
         const long long slot = leaves[li];
         for (int c = threadIdx.x; c < a.bcells; c += blockDim.x) {
-            double u[5], sv[2];
+            Real u[5], sv[2];
             for (int q = 0; q < 5; ++q) u[q] = a.U[amr_u(a, q, slot, c)];
             sensor(u, sv);
             si[0][c] = sv[0];
@@ -546,18 +566,18 @@ __global__ __launch_bounds__(256) void k_amr_details_lds(AmrDev a, AmrWeights W,
             const int d = f >> 1, side = f & 1;
             if (!a.act[d] || t >= a.bcells / a.bd[d]) continue;
             const int layer = side == 0 ? a.ng - 1 - k : k;
-            double u[5], sv[2];
+            Real u[5], sv[2];
             for (int q = 0; q < 5; ++q) u[q] = a.G[amr_g(a, q, slot, f, layer, t)];
             sensor(u, sv);
             sg[0][f][k][t] = sv[0];
             sg[1][f][k][t] = sv[1];
         }
         __syncthreads();
-        double mx = 0.0;
+        Real mx = Real(0.0);
         for (int c = threadIdx.x; c < a.bcells; c += blockDim.x) {
             int ci[3];
             unflat(c, a.bd, ci);
-            auto sens = [&](int d, int x, double s[2]) {
+            auto sens = [&](int d, int x, Real s[2]) {
                 if (x >= 0 && x < a.bd[d]) {
                     int c3[3] = {ci[0], ci[1], ci[2]};
                     c3[d] = x;
@@ -575,17 +595,17 @@ __global__ __launch_bounds__(256) void k_amr_details_lds(AmrDev a, AmrWeights W,
             };
             for (int d = 0; d < 3; ++d) {
                 if (!a.act[d]) continue;
-                double s0[2];
+                Real s0[2];
                 sens(d, ci[d], s0);
-                double pred[2] = {0.0, 0.0};
+                Real pred[2] = {Real(0.0), Real(0.0)};
                 const int offs[6] = {-5, -3, -1, 1, 3, 5};
                 for (int m = 0; m < 6; ++m) {
-                    double sv[2];
+                    Real sv[2];
                     sens(d, ci[d] + offs[m], sv);
                     for (int q = 0; q < 2; ++q) pred[q] += W.wr[m] * sv[q];
                 }
                 for (int q = 0; q < 2; ++q) {
-                    const double dd = fabs(s0[q] - pred[q]);
+                    const Real dd = fabs(s0[q] - pred[q]);
                     mx = isfinite(dd) ? fmax(mx, dd) : INFINITY;
                 }
             }
@@ -593,7 +613,7 @@ __global__ __launch_bounds__(256) void k_amr_details_lds(AmrDev a, AmrWeights W,
         for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
         if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
         __syncthreads();
-        if (threadIdx.x == 0) det[li] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        if (threadIdx.x == 0) det[li] = (double)fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
         __syncthreads();
     }
 }
@@ -630,7 +650,12 @@ struct HBlock {
 
 using InitFn = void (*)(const double x[3], const double* prm, double w[5]);
 
-struct Amr {
+// The host hierarchy and the device pool of one block-AMR solver; Real is
+// the type of the pool data and its arithmetic (double; float for the FP32
+// variant, the reference's GpuAmr<f32>).  Times, dt, mesh spacing,
+// wavelet details and the tree logic stay double on the host.
+template <class Real>
+struct AmrT {
     // configuration
     int B, levels, ng, regrid_every;
     double eps;
@@ -646,7 +671,7 @@ struct Amr {
     bool act[3];
     int bd[3];
     int ndim, bcells, tmax;
-    AmrWeights W;
+    AmrWeightsT<Real> W;
     // hierarchy (ids = pool slots)
     std::vector<HBlock> blocks;
     std::vector<std::vector<int>> map;  // per level, dense
@@ -655,13 +680,13 @@ struct Amr {
     int64_t steps = 0, regrid_interval, next_regrid;
     // device
     hipStream_t stream = nullptr;
-    AmrDev dev{};
+    AmrDevT<Real> dev{};
     long long cap = 0;
     DevScalars* sc = nullptr;
     int* d_map = nullptr;
     long long map_total = 0;
-    double* big_scr = nullptr;
-    double* small_scr = nullptr;
+    Real* big_scr = nullptr;
+    Real* small_scr = nullptr;
     long long big_cap = 0, small_cap = 0;
     int big_wg = 64, small_wg = 512;
     // LDS launches of the batched stages (0 bytes: the global-scratch kernel)
@@ -705,7 +730,7 @@ struct Amr {
         return v;
     }
 
-    Amr(const int64_t base_[3], int B_, int levels_, int ng_, double eps_, int regrid_every_, const double lo_[3],
+    AmrT(const int64_t base_[3], int B_, int levels_, int ng_, double eps_, int regrid_every_, const double lo_[3],
         const double hi_[3], const double e[4], const int32_t bk[6], const double bs[30], int mode_, int pos_,
         double cfl_, double tol_) {
         B = B_;
@@ -741,12 +766,21 @@ struct Amr {
         tol = tol_;
         regrid_interval = regrid_every;
         next_regrid = regrid_every;
-        lagrange6_h(-3.0, -0.25, W.wp[0]);
-        lagrange6_h(-2.0, 0.25, W.wp[1]);
+        // the operator weights in double, then rounded to Real (the
+        // reference's R::from_f64 of its f64 weight tables)
+        AmrWeightsT<double> wd;
+        lagrange6_h(-3.0, -0.25, wd.wp[0]);
+        lagrange6_h(-2.0, 0.25, wd.wp[1]);
         const double wr[6] = {3.0 / 256.0, -25.0 / 256.0, 150.0 / 256.0, 150.0 / 256.0, -25.0 / 256.0, 3.0 / 256.0};
-        for (int k = 0; k < 6; ++k) W.wr[k] = W.wts[0][k] = wr[k];
+        for (int k = 0; k < 6; ++k) wd.wr[k] = wd.wts[0][k] = wr[k];
         const int shifts[5] = {0, -1, 1, -2, 2};
-        for (int m = 1; m < 5; ++m) lagrange6_h(-2.5 + (double)shifts[m], 0.0, W.wts[m]);
+        for (int m = 1; m < 5; ++m) lagrange6_h(-2.5 + (double)shifts[m], 0.0, wd.wts[m]);
+        for (int k = 0; k < 6; ++k) {
+            W.wp[0][k] = Real(wd.wp[0][k]);
+            W.wp[1][k] = Real(wd.wp[1][k]);
+            W.wr[k] = Real(wd.wr[k]);
+            for (int m = 0; m < 5; ++m) W.wts[m][k] = Real(wd.wts[m][k]);
+        }
         // dense index
         map.resize(levels);
         map_off.resize(levels);
@@ -790,8 +824,8 @@ struct Amr {
         // workgroups of the restriction / block-prediction kernels: up to
         // 1024, with their scratch held under 2 GiB
         big_wg = (int)std::max<long long>(16, std::min<long long>(1024, (2ll << 30) / (96ll * big_cap)));
-        HCHK(hipMalloc(&big_scr, sizeof(double) * 12 * big_cap * big_wg));
-        HCHK(hipMalloc(&small_scr, sizeof(double) * 12 * small_cap * small_wg));
+        HCHK(hipMalloc(&big_scr, sizeof(Real) * 12 * big_cap * big_wg));
+        HCHK(hipMalloc(&small_scr, sizeof(Real) * 12 * small_cap * small_wg));
         lds_plans();
         dev.map = d_map;
         dev.sc = sc;
@@ -802,7 +836,7 @@ struct Amr {
             dev.nb0[d] = (int)nbl(0, d);
             for (int s = 0; s < 2; ++s) {
                 dev.bc_kind[d][s] = bck[d][s];
-                for (int q = 0; q < 5; ++q) dev.bc_state[d][s][q] = bcs[d][s][q];
+                for (int q = 0; q < 5; ++q) dev.bc_state[d][s][q] = Real(bcs[d][s][q]);
             }
         }
         dev.ndim = ndim;
@@ -812,13 +846,13 @@ struct Amr {
         dev.levels = levels;
         for (int l = 0; l < levels; ++l) {
             dev.map_off[l] = map_off[l];
-            for (int d = 0; d < 3; ++d) dev.dx[l][d] = dxl(l, d);
+            for (int d = 0; d < 3; ++d) dev.dx[l][d] = Real(dxl(l, d));
         }
-        dev.eos = eos;
+        dev.eos = eos_as<Real>(eos);
         ensure_capacity(64);
     }
 
-    ~Amr() {
+    ~AmrT() {
         auto f = [](void* p) {
             if (p) (void)hipFree(p);
         };
@@ -857,16 +891,16 @@ struct Amr {
     void ensure_capacity(long long n) {
         if (n <= cap) return;
         long long nc = std::max<long long>(n, cap * 3 / 2 + 16);
-        double *U, *U0, *R, *G, *F;
+        Real *U, *U0, *R, *G, *F;
         const long long vs = nc * bcells, gs = nc * 6 * ng * (long long)tmax, fs = nc * 6 * (long long)tmax;
-        HCHK(hipMalloc(&U, sizeof(double) * 5 * vs));
-        HCHK(hipMalloc(&U0, sizeof(double) * 5 * vs));
-        HCHK(hipMalloc(&R, sizeof(double) * 5 * vs));
-        HCHK(hipMalloc(&G, sizeof(double) * 5 * gs));
-        HCHK(hipMalloc(&F, sizeof(double) * 5 * fs));
+        HCHK(hipMalloc(&U, sizeof(Real) * 5 * vs));
+        HCHK(hipMalloc(&U0, sizeof(Real) * 5 * vs));
+        HCHK(hipMalloc(&R, sizeof(Real) * 5 * vs));
+        HCHK(hipMalloc(&G, sizeof(Real) * 5 * gs));
+        HCHK(hipMalloc(&F, sizeof(Real) * 5 * fs));
         if (cap > 0) {
             for (int q = 0; q < 5; ++q)
-                HCHK(hipMemcpyAsync(U + q * vs, dev.U + q * dev.vs, sizeof(double) * dev.vs, hipMemcpyDeviceToDevice,
+                HCHK(hipMemcpyAsync(U + q * vs, dev.U + q * dev.vs, sizeof(Real) * dev.vs, hipMemcpyDeviceToDevice,
                                     stream));
             HCHK(hipStreamSynchronize(stream));
             (void)hipFree(dev.U);
@@ -1095,7 +1129,7 @@ struct Amr {
         HCHK(hipMemcpyAsync(stage, data, sizeof(double) * 5 * bcells * n, hipMemcpyHostToDevice, stream));
         HCHK(hipMemcpyAsync(ds, slots.data(), sizeof(int) * n, hipMemcpyHostToDevice, stream));
         const unsigned g = (unsigned)std::min<long long>((n * bcells + 255) / 256, 4096);
-        hipLaunchKernelGGL(k_amr_scatter, dim3(g), dim3(256), 0, stream, dev, stage, ds, (int)n);
+        hipLaunchKernelGGL(k_amr_scatter<Real>, dim3(g), dim3(256), 0, stream, dev, stage, ds, (int)n);
         HCHK(hipGetLastError());
         HCHK(hipStreamSynchronize(stream));
         (void)hipFree(stage);
@@ -1191,7 +1225,7 @@ struct Amr {
             }
             HCHK(hipMemcpyAsync(d_tmp + off, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, stream));
             const unsigned g = (unsigned)std::min(n, big_wg);
-            hipLaunchKernelGGL(k_amr_predict_blocks, dim3(g), dim3(kPatchThreads), 0, stream, dev, W, d_tmp + off,
+            hipLaunchKernelGGL(k_amr_predict_blocks<Real>, dim3(g), dim3(kPatchThreads), 0, stream, dev, W, d_tmp + off,
                                d_tmp + off + n, n, pending[i][0], big_scr, big_cap);
             HCHK(hipGetLastError());
             pending_host.push_back(std::move(h));
@@ -1293,7 +1327,7 @@ struct Amr {
     // Then the removals are applied to the index, the swap-remove moves at
     // once as one block permutation, and the index entries of moved blocks
     // rewritten.
-    double* d_co_tmp = nullptr;
+    Real* d_co_tmp = nullptr;
     long long d_co_tmp_n = 0;
     int* d_co_int = nullptr;
 
@@ -1366,7 +1400,7 @@ struct Amr {
         const long long need = (long long)src.size() * 5 * bcells;
         if (need > d_co_tmp_n) {
             if (d_co_tmp) HCHK(hipFree(d_co_tmp));
-            HCHK(hipMalloc(&d_co_tmp, sizeof(double) * (size_t)need));
+            HCHK(hipMalloc(&d_co_tmp, sizeof(Real) * (size_t)need));
             d_co_tmp_n = need;
         }
         auto edits = [&](int lo, int hi) {
@@ -1377,7 +1411,7 @@ struct Amr {
         };
         for (const Launch& L : rl) {
             const unsigned g = (unsigned)std::min(L.n, big_wg);
-            hipLaunchKernelGGL(k_amr_restrict, dim3(g), dim3(kPatchThreads), 0, stream, dev, W, d_co_int + L.ids,
+            hipLaunchKernelGGL(k_amr_restrict<Real>, dim3(g), dim3(kPatchThreads), 0, stream, dev, W, d_co_int + L.ids,
                                d_co_int + L.ent, L.n, L.level, box_w(), big_scr, big_cap, d_co_int + L.xoff,
                                d_co_int + L.xs);
             HCHK(hipGetLastError());
@@ -1386,9 +1420,9 @@ struct Amr {
         if (!src.empty()) {
             const int np = (int)src.size();
             const unsigned g = (unsigned)std::min<long long>(((long long)np * bcells + 255) / 256, 4096);
-            hipLaunchKernelGGL(k_amr_blocks_out, dim3(g), dim3(256), 0, stream, dev, d_co_int + psrc, np, d_co_tmp);
+            hipLaunchKernelGGL(k_amr_blocks_out<Real>, dim3(g), dim3(256), 0, stream, dev, d_co_int + psrc, np, d_co_tmp);
             HCHK(hipGetLastError());
-            hipLaunchKernelGGL(k_amr_blocks_in, dim3(g), dim3(256), 0, stream, dev, d_co_int + pdst, np, d_co_tmp);
+            hipLaunchKernelGGL(k_amr_blocks_in<Real>, dim3(g), dim3(256), 0, stream, dev, d_co_int + pdst, np, d_co_tmp);
             HCHK(hipGetLastError());
         }
         edits(nrem, nall);
@@ -1711,12 +1745,12 @@ struct Amr {
                     }
                 }
         }
-        const size_t gb = sizeof(double) * (6 * ca + 5 * cb);
+        const size_t gb = sizeof(Real) * (6 * ca + 5 * cb);
         if (gb <= kLdsCap) {
             ghost_capA = (int)ca;
             ghost_capB = (int)cb;
             ghost_lds = gb;
-            HCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_amr_ghost_lds),
+            HCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_amr_ghost_lds<Real>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)gb));
         }
         bool charac = false;
@@ -1733,7 +1767,7 @@ struct Amr {
             for (int e = 0; e < 3; ++e)
                 if (e != d0) r.cap1 *= F(e);
             r.cap2 = ndim == 3 ? r.S[0] * r.S[1] * F(2) : 0;
-            return sizeof(double) * 6 * ((size_t)r.cap1 + r.cap2);
+            return sizeof(Real) * 6 * ((size_t)r.cap1 + r.cap2);
         };
         for (int e = 0; e < 3; ++e) r.S[e] = act[e] ? bd[e] : 1;
         if (d0 >= 0) {
@@ -1759,7 +1793,7 @@ struct Amr {
         if (!charac && d0 >= 0 && B >= 4 && 2 * B + 8 <= kPatchLen && rb <= kLdsCap) {
             rsub = r;
             rsub_lds = rb;
-            HCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_amr_restrict_sub),
+            HCHK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_amr_restrict_sub<Real>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)rb));
         }
     }
@@ -1820,15 +1854,15 @@ struct Amr {
             if (!n) continue;
             if (batched && rsub_lds) {
                 const unsigned g = (unsigned)std::min<long long>(((long long)n * rsub.nsubt + 7) / 8 * 8, 16384);
-                hipLaunchKernelGGL(k_amr_restrict_sub, dim3(g), dim3(kRestrictSubThreads), rsub_lds, stream, dev, W,
+                hipLaunchKernelGGL(k_amr_restrict_sub<Real>, dim3(g), dim3(kRestrictSubThreads), rsub_lds, stream, dev, W,
                                    d_edge_ids[l], d_edge_idx[l], n, l, box_w(), rsub);
             } else if (batched) {
                 const unsigned g = (unsigned)std::min(n, big_wg);
-                hipLaunchKernelGGL(k_amr_restrict, dim3(g), dim3(kPatchThreads), 0, stream, dev, W, d_edge_ids[l], d_edge_idx[l],
+                hipLaunchKernelGGL(k_amr_restrict<Real>, dim3(g), dim3(kPatchThreads), 0, stream, dev, W, d_edge_ids[l], d_edge_idx[l],
                                    n, l, box_w(), big_scr, big_cap, nullptr, nullptr);
             } else {
                 for (int i = 0; i < n; ++i)
-                    hipLaunchKernelGGL(k_amr_restrict, dim3(1), dim3(kPatchThreads), 0, stream, dev, W, d_edge_ids[l] + i,
+                    hipLaunchKernelGGL(k_amr_restrict<Real>, dim3(1), dim3(kPatchThreads), 0, stream, dev, W, d_edge_ids[l] + i,
                                        d_edge_idx[l] + 4 * i, 1, l, box_w(), big_scr, big_cap, nullptr, nullptr);
             }
             HCHK(hipGetLastError());
@@ -1847,17 +1881,17 @@ struct Amr {
                 const int per = ng * (bcells / B);
                 const long long ntot = (long long)nc * per;
                 const unsigned g = (unsigned)std::min<long long>((ntot + kGhostCopyThreads - 1) / kGhostCopyThreads, 65536);
-                hipLaunchKernelGGL(k_amr_ghost_copy, dim3(g), dim3(kGhostCopyThreads), 0, stream, dev, d_ghost_copy, ntot,
+                hipLaunchKernelGGL(k_amr_ghost_copy<Real>, dim3(g), dim3(kGhostCopyThreads), 0, stream, dev, d_ghost_copy, ntot,
                                    per);
                 HCHK(hipGetLastError());
             }
             if (np && ghost_lds) {
-                hipLaunchKernelGGL(k_amr_ghost_lds, dim3((unsigned)std::min((np + 7) / 8 * 8, 16384)), dim3(kGhostThreads), ghost_lds,
+                hipLaunchKernelGGL(k_amr_ghost_lds<Real>, dim3((unsigned)std::min((np + 7) / 8 * 8, 16384)), dim3(kGhostThreads), ghost_lds,
                                    stream, dev, W, d_ghost_pred, np, ghost_capA, ghost_capB);
                 HCHK(hipGetLastError());
             } else if (np) {
                 const unsigned g = (unsigned)std::min(np, small_wg);
-                hipLaunchKernelGGL(k_amr_ghost, dim3(g), dim3(kGhostThreads), 0, stream, dev, W, d_ghost_pred, np, small_scr,
+                hipLaunchKernelGGL(k_amr_ghost<Real>, dim3(g), dim3(kGhostThreads), 0, stream, dev, W, d_ghost_pred, np, small_scr,
                                    small_cap);
                 HCHK(hipGetLastError());
             }
@@ -1865,7 +1899,7 @@ struct Amr {
             // one launch per leaf (its 2*D slabs)
             const int per = 2 * ndim;
             for (int i = 0; i < n; i += per) {
-                hipLaunchKernelGGL(k_amr_ghost, dim3((unsigned)per), dim3(kGhostThreads), 0, stream, dev, W, d_ghost + i, per,
+                hipLaunchKernelGGL(k_amr_ghost<Real>, dim3((unsigned)per), dim3(kGhostThreads), 0, stream, dev, W, d_ghost + i, per,
                                    small_scr, small_cap);
                 HCHK(hipGetLastError());
             }
@@ -1874,7 +1908,7 @@ struct Amr {
 
     int shape[3] = {0, 0, 0};  // pass shape per axis (amr_shape)
 
-    hipError_t launch_pass(int d, const AmrPassArgs& pa, unsigned grid) {
+    hipError_t launch_pass(int d, const AmrPassArgsT<Real>& pa, unsigned grid) {
         return precise ? em::precise::launch_amr_pass(d, mode, B, shape[d], pa, grid, stream)
                        : em::fast::launch_amr_pass(d, mode, B, shape[d], pa, grid, stream);
     }
@@ -1888,7 +1922,7 @@ struct Amr {
             if (!act[d]) continue;
             if (only_axis >= 0 && d != only_axis) continue;
             const int P = amr_shape(B, shape[d]).P;
-            AmrPassArgs pa{};
+            AmrPassArgsT<Real> pa{};
             pa.a = dev;
             pa.leaves = d_leaves;
             pa.leaf_level = d_leaf_level;
@@ -1896,7 +1930,7 @@ struct Amr {
             pa.first = first ? 1 : 0;
             pa.dfast = d == last ? 1 : 0;
             pa.ndim_pos = positivity ? (double)ndim : 0.0;
-            pa.tol = tol;
+            pa.tol = Real(tol);
             pa.stage_dt_host = stage_dt_host;
             pa.stage_dt = stage_dt;
             pa.gsrc = batched ? d_gsrc : nullptr;
@@ -1922,7 +1956,7 @@ struct Amr {
         if (batched) {
             const int ng = (int)reflux_grp.size() - 1;
             if (ng > 0) {
-                hipLaunchKernelGGL(k_amr_reflux_cells, dim3((unsigned)std::min((ng + 127) / 128, 4096)), dim3(128), 0,
+                hipLaunchKernelGGL(k_amr_reflux_cells<Real>, dim3((unsigned)std::min((ng + 127) / 128, 4096)), dim3(128), 0,
                                    stream, dev, d_reflux_cells, d_reflux_grp, ng);
                 HCHK(hipGetLastError());
             }
@@ -1933,7 +1967,7 @@ struct Amr {
             for (int s = 0; s < 2; ++s) {
                 const int n = (int)reflux[d][s].size();
                 if (!n) continue;
-                hipLaunchKernelGGL(k_amr_reflux, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, stream, dev,
+                hipLaunchKernelGGL(k_amr_reflux<Real>, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, stream, dev,
                                    d_reflux[d][s], n, d, s);
                 HCHK(hipGetLastError());
             }
@@ -1954,16 +1988,16 @@ struct Amr {
         if (batched) {
             const unsigned g = (unsigned)std::min<long long>(((long long)n * bcells + 255) / 256, 8192);
             if (stage == 0)
-                hipLaunchKernelGGL(k_amr_copy_u0, dim3(g), dim3(256), 0, stream, dev, d_leaves, n);
+                hipLaunchKernelGGL(k_amr_copy_u0<Real>, dim3(g), dim3(256), 0, stream, dev, d_leaves, n);
             else
-                hipLaunchKernelGGL(k_amr_rk, dim3(g), dim3(256), 0, stream, dev, d_leaves, n, stage);
+                hipLaunchKernelGGL(k_amr_rk<Real>, dim3(g), dim3(256), 0, stream, dev, d_leaves, n, stage);
         } else {
             const unsigned g = (unsigned)std::max(1, (bcells + 255) / 256);
             for (int li = 0; li < n; ++li) {
                 if (stage == 0)
-                    hipLaunchKernelGGL(k_amr_copy_u0, dim3(g), dim3(256), 0, stream, dev, d_leaves + li, 1);
+                    hipLaunchKernelGGL(k_amr_copy_u0<Real>, dim3(g), dim3(256), 0, stream, dev, d_leaves + li, 1);
                 else
-                    hipLaunchKernelGGL(k_amr_rk, dim3(g), dim3(256), 0, stream, dev, d_leaves + li, 1, stage);
+                    hipLaunchKernelGGL(k_amr_rk<Real>, dim3(g), dim3(256), 0, stream, dev, d_leaves + li, 1, stage);
             }
         }
         HCHK(hipGetLastError());
@@ -2186,7 +2220,7 @@ struct Amr {
         hipLaunchKernelGGL(k_amr_zero_rate, dim3(1), dim3(kSlots), 0, stream, sc);
         const int n = (int)leaves.size();
         const unsigned g = (unsigned)std::min<long long>(((long long)n * bcells + 255) / 256, 2048);
-        hipLaunchKernelGGL(k_amr_rate, dim3(g), dim3(256), 0, stream, dev, d_leaves, d_leaf_level, n);
+        hipLaunchKernelGGL(k_amr_rate<Real>, dim3(g), dim3(256), 0, stream, dev, d_leaves, d_leaf_level, n);
         HCHK(hipGetLastError());
     }
 
@@ -2213,10 +2247,10 @@ struct Amr {
         std::vector<double> det(blocks.size(), 0.0);
         if (n) {
             if (bcells <= kDetCells && tmax <= kDetFace)
-                hipLaunchKernelGGL(k_amr_details_lds, dim3((unsigned)std::min(n, 2048)), dim3(256), 0, stream, dev, W,
+                hipLaunchKernelGGL(k_amr_details_lds<Real>, dim3((unsigned)std::min(n, 2048)), dim3(256), 0, stream, dev, W,
                                    d_leaves, n, d_det);
             else
-                hipLaunchKernelGGL(k_amr_details, dim3((unsigned)std::min(n, 2048)), dim3(256), 0, stream, dev, W, d_leaves,
+                hipLaunchKernelGGL(k_amr_details<Real>, dim3((unsigned)std::min(n, 2048)), dim3(256), 0, stream, dev, W, d_leaves,
                                    n, d_det);
             HCHK(hipGetLastError());
             std::vector<double> h(n);
@@ -2319,12 +2353,12 @@ struct Amr {
         if (u) gather(dev.U, u, n);
     }
 
-    void gather(const double* src, double* out, long long n) {
+    void gather(const Real* src, double* out, long long n) {
         if (n == 0) return;
         double* stage;
         HCHK(hipMalloc(&stage, sizeof(double) * 5 * bcells * n));
         const unsigned g = (unsigned)std::min<long long>((n * bcells + 255) / 256, 4096);
-        hipLaunchKernelGGL(k_amr_gather, dim3(g), dim3(256), 0, stream, dev, stage, src, n);
+        hipLaunchKernelGGL(k_amr_gather<Real>, dim3(g), dim3(256), 0, stream, dev, stage, src, n);
         HCHK(hipGetLastError());
         HCHK(hipMemcpyAsync(out, stage, sizeof(double) * 5 * bcells * n, hipMemcpyDeviceToHost, stream));
         HCHK(hipStreamSynchronize(stream));
@@ -2373,6 +2407,7 @@ struct Amr {
         }
     }
 };
+using Amr = AmrT<double>;
 
 // ------------------------------------------------------------------ cases
 // Primitive initial conditions of the AMR workloads (ideal gas, gamma 1.4):
@@ -2607,7 +2642,8 @@ thread_local std::string g_last_error_amr;
 // ------------------------------------------------------------------ C ABI
 
 struct em_amr {
-    em::Amr* a;
+    em::AmrT<double>* a;   // FP64 hierarchy (null for an FP32 handle)
+    em::AmrT<float>* a32;  // FP32 hierarchy (null for an FP64 handle)
 };
 
 struct em_amr_group {
@@ -2630,53 +2666,68 @@ static int aguard(F&& f) {
 }
 
 #define ANEED(h)                                           \
-    if (!(h) || !(h)->a) {                                 \
+    if (!(h) || (!(h)->a && !(h)->a32)) {                  \
         em::g_last_error_amr = "null AMR handle";          \
         return EM_ERR_ARG;                                 \
     }
 
-extern "C" {
+// f(hierarchy) on the handle's hierarchy, whichever real type it holds
+template <class F>
+static void aeach(em_amr* h, F&& f) {
+    if (h->a)
+        f(*h->a);
+    else
+        f(*h->a32);
+}
+template <class F>
+static int aon(em_amr* h, F&& f) {
+    return aguard([&] { aeach(h, f); });
+}
 
-const char* em_amr_last_error(void) { return em::g_last_error_amr.c_str(); }
-
+template <class Real, class Fill>
 static int amr_make(em_amr** out, const int64_t base[3], int32_t b, int32_t levels, int32_t ng, double eps,
                     int32_t regrid_every, const double lo[3], const double hi[3], const double eos[4],
                     const int32_t bc_kind[6], const double bc_state[30], int32_t mode, int32_t positivity, double cfl,
-                    double tol, const std::function<void(em::Amr*)>& fill) {
+                    double tol, Fill&& fill) {
     if (!out || !base || !lo || !hi || !eos || !bc_kind) {
         em::g_last_error_amr = "null argument";
         return EM_ERR_ARG;
     }
     return aguard([&] {
-        auto* a = new em::Amr(base, b, levels, ng, eps, regrid_every, lo, hi, eos, bc_kind, bc_state, mode, positivity,
-                              cfl, tol);
+        auto* a = new em::AmrT<Real>(base, b, levels, ng, eps, regrid_every, lo, hi, eos, bc_kind, bc_state, mode,
+                                     positivity, cfl, tol);
         try {
-            fill(a);
+            fill(*a);
         } catch (...) {
             delete a;
             throw;
         }
-        *out = new em_amr{a};
+        if constexpr (sizeof(Real) == 8)
+            *out = new em_amr{a, nullptr};
+        else
+            *out = new em_amr{nullptr, a};
     });
 }
 
-int em_amr_create(em_amr** out, const int64_t base[3], int32_t b, int32_t levels, int32_t ng, double eps,
-                  int32_t regrid_every, const double lo[3], const double hi[3], const double eos[4],
-                  const int32_t bc_kind[6], const double bc_state[30], int32_t mode, int32_t positivity, double cfl,
-                  double tol, int64_t nblocks, const int32_t* level, const int64_t* idx, const uint8_t* leaf,
-                  const double* u) {
+template <class Real>
+static int amr_create_t(em_amr** out, const int64_t base[3], int32_t b, int32_t levels, int32_t ng, double eps,
+                        int32_t regrid_every, const double lo[3], const double hi[3], const double eos[4],
+                        const int32_t bc_kind[6], const double bc_state[30], int32_t mode, int32_t positivity,
+                        double cfl, double tol, int64_t nblocks, const int32_t* level, const int64_t* idx,
+                        const uint8_t* leaf, const double* u) {
     if (nblocks < 1 || !level || !idx || !leaf || !u) {
         em::g_last_error_amr = "block arrays missing";
         return EM_ERR_ARG;
     }
-    return amr_make(out, base, b, levels, ng, eps, regrid_every, lo, hi, eos, bc_kind, bc_state, mode, positivity, cfl,
-                    tol, [&](em::Amr* a) { a->build_from_blocks(nblocks, level, idx, leaf, u); });
+    return amr_make<Real>(out, base, b, levels, ng, eps, regrid_every, lo, hi, eos, bc_kind, bc_state, mode,
+                          positivity, cfl, tol, [&](auto& a) { a.build_from_blocks(nblocks, level, idx, leaf, u); });
 }
 
-int em_amr_create_case(em_amr** out, const int64_t base[3], int32_t b, int32_t levels, int32_t ng, double eps,
-                       int32_t regrid_every, const double lo[3], const double hi[3], const double eos[4],
-                       const int32_t bc_kind[6], const double bc_state[30], int32_t mode, int32_t positivity,
-                       double cfl, double tol, int32_t case_id, const double* params) {
+template <class Real>
+static int amr_create_case_t(em_amr** out, const int64_t base[3], int32_t b, int32_t levels, int32_t ng, double eps,
+                             int32_t regrid_every, const double lo[3], const double hi[3], const double eos[4],
+                             const int32_t bc_kind[6], const double bc_state[30], int32_t mode, int32_t positivity,
+                             double cfl, double tol, int32_t case_id, const double* params) {
     if (case_id < 0 || case_id > 4) {
         em::g_last_error_amr = "unknown AMR case";
         return EM_ERR_ARG;
@@ -2685,28 +2736,71 @@ int em_amr_create_case(em_amr** out, const int64_t base[3], int32_t b, int32_t l
     if (params)
         for (int i = 0; i < 15; ++i) prm[i] = params[i];
     prm[15] = (double)case_id;
-    return amr_make(out, base, b, levels, ng, eps, regrid_every, lo, hi, eos, bc_kind, bc_state, mode, positivity, cfl,
-                    tol, [&](em::Amr* a) { a->build_from_case(em::amr_case, prm); });
+    return amr_make<Real>(out, base, b, levels, ng, eps, regrid_every, lo, hi, eos, bc_kind, bc_state, mode,
+                          positivity, cfl, tol, [&](auto& a) { a.build_from_case(em::amr_case, prm); });
+}
+
+extern "C" {
+
+const char* em_amr_last_error(void) { return em::g_last_error_amr.c_str(); }
+
+int em_amr_create(em_amr** out, const int64_t base[3], int32_t b, int32_t levels, int32_t ng, double eps,
+                  int32_t regrid_every, const double lo[3], const double hi[3], const double eos[4],
+                  const int32_t bc_kind[6], const double bc_state[30], int32_t mode, int32_t positivity, double cfl,
+                  double tol, int64_t nblocks, const int32_t* level, const int64_t* idx, const uint8_t* leaf,
+                  const double* u) {
+    return amr_create_t<double>(out, base, b, levels, ng, eps, regrid_every, lo, hi, eos, bc_kind, bc_state, mode,
+                                positivity, cfl, tol, nblocks, level, idx, leaf, u);
+}
+
+int em_amr_create_f32(em_amr** out, const int64_t base[3], int32_t b, int32_t levels, int32_t ng, double eps,
+                      int32_t regrid_every, const double lo[3], const double hi[3], const double eos[4],
+                      const int32_t bc_kind[6], const double bc_state[30], int32_t mode, int32_t positivity,
+                      double cfl, double tol, int64_t nblocks, const int32_t* level, const int64_t* idx,
+                      const uint8_t* leaf, const double* u) {
+    return amr_create_t<float>(out, base, b, levels, ng, eps, regrid_every, lo, hi, eos, bc_kind, bc_state, mode,
+                               positivity, cfl, tol, nblocks, level, idx, leaf, u);
+}
+
+int em_amr_create_case(em_amr** out, const int64_t base[3], int32_t b, int32_t levels, int32_t ng, double eps,
+                       int32_t regrid_every, const double lo[3], const double hi[3], const double eos[4],
+                       const int32_t bc_kind[6], const double bc_state[30], int32_t mode, int32_t positivity,
+                       double cfl, double tol, int32_t case_id, const double* params) {
+    return amr_create_case_t<double>(out, base, b, levels, ng, eps, regrid_every, lo, hi, eos, bc_kind, bc_state,
+                                     mode, positivity, cfl, tol, case_id, params);
+}
+
+int em_amr_create_case_f32(em_amr** out, const int64_t base[3], int32_t b, int32_t levels, int32_t ng, double eps,
+                           int32_t regrid_every, const double lo[3], const double hi[3], const double eos[4],
+                           const int32_t bc_kind[6], const double bc_state[30], int32_t mode, int32_t positivity,
+                           double cfl, double tol, int32_t case_id, const double* params) {
+    return amr_create_case_t<float>(out, base, b, levels, ng, eps, regrid_every, lo, hi, eos, bc_kind, bc_state,
+                                    mode, positivity, cfl, tol, case_id, params);
 }
 
 int em_amr_destroy(em_amr* h) {
     if (!h) return EM_OK;
     delete h->a;
+    delete h->a32;
     delete h;
     return EM_OK;
 }
 
 int em_amr_set_variant(em_amr* h, int32_t precise) {
     ANEED(h);
-    h->a->precise = precise ? 1 : 0;
-    ++h->a->version;
+    aeach(h, [&](auto& a) {
+        a.precise = precise ? 1 : 0;
+        ++a.version;
+    });
     return EM_OK;
 }
 
 int em_amr_set_batched(em_amr* h, int32_t batched) {
     ANEED(h);
-    h->a->batched = batched != 0;
-    ++h->a->version;
+    aeach(h, [&](auto& a) {
+        a.batched = batched != 0;
+        ++a.version;
+    });
     return EM_OK;
 }
 
@@ -2716,121 +2810,128 @@ int em_amr_set_shapes(em_amr* h, const int32_t shape[3]) {
         em::g_last_error_amr = "null argument";
         return EM_ERR_ARG;
     }
+    const int bsz = h->a ? h->a->B : h->a32->B;
     for (int d = 0; d < 3; ++d)
-        if (shape[d] < 0 || shape[d] >= em::amr_num_shapes(h->a->B)) {
+        if (shape[d] < 0 || shape[d] >= em::amr_num_shapes(bsz)) {
             em::g_last_error_amr = "no such pass shape";
             return EM_ERR_ARG;
         }
-    for (int d = 0; d < 3; ++d) h->a->shape[d] = shape[d];
-    ++h->a->version;
+    aeach(h, [&](auto& a) {
+        for (int d = 0; d < 3; ++d) a.shape[d] = shape[d];
+        ++a.version;
+    });
     return EM_OK;
 }
 
 int em_amr_autotune(em_amr* h, int32_t repeats, int32_t shape_out[3]) {
     ANEED(h);
-    return aguard([&] { h->a->autotune(repeats > 0 ? repeats : 3, shape_out); });
+    return aon(h, [&](auto& a) { a.autotune(repeats > 0 ? repeats : 3, shape_out); });
 }
 
 int em_amr_set_graph(em_amr* h, int32_t on) {
     ANEED(h);
-    h->a->use_graph = on != 0;
+    aeach(h, [&](auto& a) { a.use_graph = on != 0; });
     return EM_OK;
 }
 
 int em_amr_set_profile(em_amr* h, int32_t on) {
     ANEED(h);
-    h->a->profile = on != 0;
+    aeach(h, [&](auto& a) { a.profile = on != 0; });
     return EM_OK;
 }
 
 int em_amr_profile(em_amr* h, double out[5]) {
     ANEED(h);
-    for (int i = 0; i < 5; ++i) out[i] = h->a->prof_ms[i];
+    aeach(h, [&](auto& a) {
+        for (int i = 0; i < 5; ++i) out[i] = a.prof_ms[i];
+    });
     return EM_OK;
 }
 
 int em_amr_set_next_regrid(em_amr* h, int64_t step) {
     ANEED(h);
-    h->a->next_regrid = step;
+    aeach(h, [&](auto& a) { a.next_regrid = step; });
     return EM_OK;
 }
 
 int em_amr_nblocks(em_amr* h, int64_t* n, int64_t* leaves) {
     ANEED(h);
-    if (n) *n = (int64_t)h->a->blocks.size();
-    if (leaves) {
-        int64_t k = 0;
-        for (auto& b : h->a->blocks) k += b.leaf ? 1 : 0;
-        *leaves = k;
-    }
+    aeach(h, [&](auto& a) {
+        if (n) *n = (int64_t)a.blocks.size();
+        if (leaves) {
+            int64_t k = 0;
+            for (auto& b : a.blocks) k += b.leaf ? 1 : 0;
+            *leaves = k;
+        }
+    });
     return EM_OK;
 }
 
 int em_amr_download(em_amr* h, int32_t* level, int64_t* idx, uint8_t* leaf, double* u) {
     ANEED(h);
-    return aguard([&] { h->a->download(level, idx, leaf, u); });
+    return aon(h, [&](auto& a) { a.download(level, idx, leaf, u); });
 }
 
 int em_amr_upload(em_amr* h, const double* u) {
     ANEED(h);
-    return aguard([&] { h->a->upload(u); });
+    return aon(h, [&](auto& a) { a.upload(u); });
 }
 
 int em_amr_rhs(em_amr* h, double stage_dt, double* rhs, uint64_t stats[4]) {
     ANEED(h);
-    return aguard([&] { h->a->rhs(stage_dt, rhs, stats); });
+    return aon(h, [&](auto& a) { a.rhs(stage_dt, rhs, stats); });
 }
 
 int em_amr_step(em_amr* h, double dt) {
     ANEED(h);
-    return aguard([&] { h->a->step(dt); });
+    return aon(h, [&](auto& a) { a.step(dt); });
 }
 
 int em_amr_run_steps(em_amr* h, int64_t nsteps) {
     ANEED(h);
-    return aguard([&] { h->a->run_steps(nsteps); });
+    return aon(h, [&](auto& a) { a.run_steps(nsteps); });
 }
 
 int em_amr_run_to(em_amr* h, double t_end, int64_t* steps) {
     ANEED(h);
-    return aguard([&] {
-        const int64_t n = h->a->run_to(t_end);
+    return aon(h, [&](auto& a) {
+        const int64_t n = a.run_to(t_end);
         if (steps) *steps = n;
     });
 }
 
 int em_amr_dt_cfl(em_amr* h, double* dt) {
     ANEED(h);
-    return aguard([&] { *dt = h->a->dt_cfl(); });
+    return aon(h, [&](auto& a) { *dt = a.dt_cfl(); });
 }
 
 int em_amr_details(em_amr* h, double* det) {
     ANEED(h);
-    return aguard([&] {
-        std::vector<double> d = h->a->details();
+    return aon(h, [&](auto& a) {
+        std::vector<double> d = a.details();
         std::copy(d.begin(), d.end(), det);
     });
 }
 
 int em_amr_regrid(em_amr* h) {
     ANEED(h);
-    return aguard([&] { h->a->regrid(); });
+    return aon(h, [&](auto& a) { a.regrid(); });
 }
 
 int em_amr_counters(em_amr* h, uint64_t out[3]) {
     ANEED(h);
-    return aguard([&] { h->a->counters(out); });
+    return aon(h, [&](auto& a) { a.counters(out); });
 }
 
 int em_amr_time(em_amr* h, double* t) {
     ANEED(h);
-    *t = h->a->t;
+    aeach(h, [&](auto& a) { *t = a.t; });
     return EM_OK;
 }
 
 int em_amr_synchronize(em_amr* h) {
     ANEED(h);
-    return aguard([&] { em::hchk(hipStreamSynchronize(h->a->stream), "hipStreamSynchronize"); });
+    return aon(h, [&](auto& a) { em::hchk(hipStreamSynchronize(a.stream), "hipStreamSynchronize"); });
 }
 
 int em_amr_group_create(em_amr_group** out, em_amr* const* members, int32_t n) {
@@ -2840,8 +2941,12 @@ int em_amr_group_create(em_amr_group** out, em_amr* const* members, int32_t n) {
     }
     std::vector<em::Amr*> m;
     for (int i = 0; i < n; ++i) {
-        if (!members[i] || !members[i]->a) {
+        if (!members[i] || (!members[i]->a && !members[i]->a32)) {
             em::g_last_error_amr = "null AMR handle";
+            return EM_ERR_ARG;
+        }
+        if (!members[i]->a) {
+            em::g_last_error_amr = "AMR group members must be FP64 hierarchies";
             return EM_ERR_ARG;
         }
         m.push_back(members[i]->a);

@@ -12,10 +12,11 @@ namespace em {
 
 // Weights of the separable 6th-order operators (amr.rs:41-47, 305-308,
 // 675-676): prediction by fine-cell parity, centred and shifted restriction.
-struct AmrWeights {
-    double wp[2][6];
-    double wr[6];
-    double wts[5][6];  // restriction stencils for shifts 0, -1, 1, -2, 2
+template <class Real>
+struct AmrWeightsT {
+    Real wp[2][6];
+    Real wr[6];
+    Real wts[5][6];  // restriction stencils for shifts 0, -1, 1, -2, 2
 };
 
 // Index arithmetic is 32-bit throughout: global cell indices of one level
@@ -25,13 +26,15 @@ __device__ __forceinline__ int rem_e(int a, int b) {
     return r < 0 ? r + (b < 0 ? -b : b) : r;
 }
 
-__device__ __forceinline__ int amr_n_level(const AmrDev& a, int l, int d) {
+template <class Real>
+__device__ __forceinline__ int amr_n_level(const AmrDevT<Real>& a, int l, int d) {
     return a.act[d] ? (a.nb0[d] * a.bd[d]) << l : 1;
 }
 
 // Block id holding global cell g (inside the domain) at level l, -1 if
 // absent; *cell receives the cell index inside the block.
-__device__ __forceinline__ int amr_find(const AmrDev& a, int l, const int g[3], int* cell) {
+template <class Real>
+__device__ __forceinline__ int amr_find(const AmrDevT<Real>& a, int l, const int g[3], int* cell) {
     int s = 0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -61,7 +64,8 @@ constexpr int kBoxBlocks = 512;
 struct BoxBlocks {
     int b0[3], nbk[3];
 };
-__device__ __forceinline__ bool box_blocks(const AmrDev& a, int l, const int g0[3], const int sz[3], int* ids, BoxBlocks& bb) {
+template <class Real>
+__device__ __forceinline__ bool box_blocks(const AmrDevT<Real>& a, int l, const int g0[3], const int sz[3], int* ids, BoxBlocks& bb) {
     int total = 1;
     for (int d = 0; d < 3; ++d) {
         if (a.act[d]) {
@@ -98,7 +102,8 @@ __device__ __forceinline__ bool box_blocks(const AmrDev& a, int l, const int g0[
     return true;
 }
 // entry of ids[] for the (unwrapped) cell g, and its cell index in the block
-__device__ __forceinline__ int box_block_of(const AmrDev& a, const BoxBlocks& bb, const int g[3], int* cell) {
+template <class Real>
+__device__ __forceinline__ int box_block_of(const AmrDevT<Real>& a, const BoxBlocks& bb, const int g[3], int* cell) {
     int bi = 0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) bi = bi * bb.nbk[d] + ((a.act[d] ? g[d] >> a.lb[d] : 0) - bb.b0[d]);
@@ -109,7 +114,8 @@ __device__ __forceinline__ int box_block_of(const AmrDev& a, const BoxBlocks& bb
 // bc_box of amr.rs:450-504 on a box held in scratch (values val[q*cap+i],
 // validity ok[i]): cells outside a non-periodic boundary are filled from
 // cells of the same box, one axis sweep at a time; called by one workgroup.
-__device__ void box_bc(const AmrDev& a, int l, const int g0[3], const int sz[3], double* val, double* ok,
+template <class Real>
+__device__ void box_bc(const AmrDevT<Real>& a, int l, const int g0[3], const int sz[3], Real* val, Real* ok,
                        long long cap) {
     const int n3[3] = {amr_n_level(a, l, 0), amr_n_level(a, l, 1), amr_n_level(a, l, 2)};
     const int tot = sz[0] * sz[1] * sz[2];
@@ -133,11 +139,11 @@ __device__ void box_bc(const AmrDev& a, int l, const int g0[3], const int sz[3],
                 int s3[3] = {c[0], c[1], c[2]};
                 s3[d] = cc;
                 const int p = (s3[0] * sz[1] + s3[1]) * sz[2] + s3[2];
-                return ok[p] != 0.0 ? p : -1;
+                return ok[p] != Real(0.0) ? p : -1;
             };
             const int gi = g[d] < 0 ? 0 : (g[d] > n3[d] - 1 ? n3[d] - 1 : g[d]);
             const int kind = a.bc_kind[d][side];
-            double v[5];
+            Real v[5];
             bool have = false;
             if (kind == BC_DIRICHLET) {
                 for (int q = 0; q < 5; ++q) v[q] = a.bc_state[d][side][q];
@@ -152,18 +158,18 @@ __device__ void box_bc(const AmrDev& a, int l, const int g0[3], const int sz[3],
                 const int p = local(side == 0 ? -1 - g[d] : 2 * n3[d] - 1 - g[d]);
                 if (p >= 0) {
                     for (int q = 0; q < 5; ++q) v[q] = val[q * cap + p];
-                    v[1 + d] *= -1.0;
+                    v[1 + d] *= -Real(1.0);
                     have = true;
                 }
             } else if (kind == BC_CHARACTERISTIC) {
                 const int pb = local(gi), p1 = local(gi + (side == 1 ? -1 : 1));
                 if (pb >= 0 && p1 >= 0) {
-                    double ub[5], ub1[5];
+                    Real ub[5], ub1[5];
                     for (int q = 0; q < 5; ++q) {
                         ub[q] = val[q * cap + pb];
                         ub1[q] = val[q * cap + p1];
                     }
-                    const double m = (double)(g[d] > gi ? g[d] - gi : gi - g[d]);
+                    const Real m = Real(g[d] > gi ? g[d] - gi : gi - g[d]);
                     if (d == 0)
                         characteristic_ghost<0>(a.eos, ub, ub1, a.bc_state[d][side], side, m, v);
                     else if (d == 1)
@@ -175,7 +181,7 @@ __device__ void box_bc(const AmrDev& a, int l, const int g0[3], const int sz[3],
             }
             if (have) {
                 for (int q = 0; q < 5; ++q) val[q * cap + i] = v[q];
-                ok[i] = 1.0;
+                ok[i] = Real(1.0);
             }
         }
         __syncthreads();
@@ -190,9 +196,9 @@ __device__ void box_bc(const AmrDev& a, int l, const int g0[3], const int sz[3],
 // fin(i, c, v) instead: its row-major index and coordinates in the box and
 // its five values.  A and ok hold capA cells, Bf capB; the buffers may be
 // global scratch or LDS.  Ends with a barrier.
-template <class Fin>
-__device__ __forceinline__ void predict_box(const AmrDev& a, const AmrWeights& W, int lf, const int fmin[3],
-                                            const int fmax[3], double* A, double* ok, long long capA, double* Bf,
+template <class Real, class Fin>
+__device__ __forceinline__ void predict_box(const AmrDevT<Real>& a, const AmrWeightsT<Real>& W, int lf, const int fmin[3],
+                                            const int fmax[3], Real* A, Real* ok, long long capA, Real* Bf,
                                             long long capB, Fin&& fin) {
     __shared__ int any_out;
     const int lc = lf - 1;
@@ -230,13 +236,13 @@ __device__ __forceinline__ void predict_box(const AmrDev& a, const AmrWeights& W
             id[k] = bids[box_block_of(a, bb, g, &cell)];
             if (id[k] >= 0) src[k] = (long long)id[k] * a.bcells + cell;
         }
-        double v[kBatch][5];
+        Real v[kBatch][5];
 #pragma unroll
         for (int k = 0; k < kBatch; ++k)
 #pragma unroll
             for (int q = 0; q < 5; ++q) {
-                const double x = a.U[q * a.vs + src[k]];
-                v[k][q] = id[k] >= 0 ? x : 0.0;
+                const Real x = a.U[q * a.vs + src[k]];
+                v[k][q] = id[k] >= 0 ? x : Real(0.0);
             }
 #pragma unroll
         for (int k = 0; k < kBatch; ++k) {
@@ -249,7 +255,7 @@ __device__ __forceinline__ void predict_box(const AmrDev& a, const AmrWeights& W
             }
 #pragma unroll
             for (int q = 0; q < 5; ++q) A[q * capA + i] = v[k][q];
-            ok[i] = id[k] == -2 ? 0.0 : 1.0;
+            ok[i] = id[k] == -2 ? Real(0.0) : Real(1.0);
         }
     }
     for (int i = threadIdx.x; i < tot && !cached; i += blockDim.x) {
@@ -270,9 +276,9 @@ __device__ __forceinline__ void predict_box(const AmrDev& a, const AmrWeights& W
             }
         }
         if (out) {
-            ok[i] = 0.0;
+            ok[i] = Real(0.0);
             any_out = 1;
-            for (int q = 0; q < 5; ++q) A[q * capA + i] = 0.0;
+            for (int q = 0; q < 5; ++q) A[q * capA + i] = Real(0.0);
             continue;
         }
         int cell;
@@ -281,21 +287,21 @@ __device__ __forceinline__ void predict_box(const AmrDev& a, const AmrWeights& W
             // the hierarchy is 2:1 balanced, so the level below a leaf's
             // neighbourhood exists; report otherwise
             atomicOr(&a.sc->err, 2);
-            for (int q = 0; q < 5; ++q) A[q * capA + i] = 0.0;
+            for (int q = 0; q < 5; ++q) A[q * capA + i] = Real(0.0);
         } else {
             for (int q = 0; q < 5; ++q) A[q * capA + i] = a.U[amr_u(a, q, id, cell)];
         }
-        ok[i] = 1.0;
+        ok[i] = Real(1.0);
     }
     __syncthreads();
     if (any_out) {
         box_bc(a, lc, cmin, sz, A, ok, capA);
         for (int i = threadIdx.x; i < tot; i += blockDim.x)
-            if (ok[i] == 0.0) atomicOr(&a.sc->err, 4);
+            if (ok[i] == Real(0.0)) atomicOr(&a.sc->err, 4);
         __syncthreads();
     }
-    double* in = A;
-    double* out = Bf;
+    Real* in = A;
+    Real* out = Bf;
     long long icap = capA, ocap = capB;
     const int dlast = a.act[2] ? 2 : a.act[1] ? 1 : 0;
 #pragma unroll
@@ -310,12 +316,12 @@ __device__ __forceinline__ void predict_box(const AmrDev& a, const AmrWeights& W
             const int f = fmin[d] + c[d];
             const int par = f & 1;
             const int base = (f >> 1) + (par == 1 ? -2 : -3) - cmin[d];
-            double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            Real acc[5] = {Real(0.0), Real(0.0), Real(0.0), Real(0.0), Real(0.0)};
             int s3[3] = {c[0], c[1], c[2]};
             for (int kk = 0; kk < 6; ++kk) {
                 s3[d] = base + kk;
                 const int src = (s3[0] * sz[1] + s3[1]) * sz[2] + s3[2];
-                const double wgt = W.wp[par][kk];
+                const Real wgt = W.wp[par][kk];
 #pragma unroll
                 for (int q = 0; q < 5; ++q) acc[q] += wgt * in[q * icap + src];
             }
@@ -327,7 +333,7 @@ __device__ __forceinline__ void predict_box(const AmrDev& a, const AmrWeights& W
             }
         }
         __syncthreads();
-        double* tmp = in;
+        Real* tmp = in;
         in = out;
         out = tmp;
         const long long tc = icap;
@@ -339,12 +345,13 @@ __device__ __forceinline__ void predict_box(const AmrDev& a, const AmrWeights& W
 
 // predict_box with the buffers in global scratch (12 arrays of cap doubles);
 // the result is left in scratch, in the buffer returned.
-__device__ double* predict_region(const AmrDev& a, const AmrWeights& W, int lf, const int fmin[3], const int fmax[3],
-                                  double* scr, long long cap) {
+template <class Real>
+__device__ Real* predict_region(const AmrDevT<Real>& a, const AmrWeightsT<Real>& W, int lf, const int fmin[3], const int fmax[3],
+                                  Real* scr, long long cap) {
     const int nact = a.act[0] + a.act[1] + a.act[2];
-    double* res = (nact & 1) ? scr + 5 * cap : scr;  // the last pass's output buffer
+    Real* res = (nact & 1) ? scr + 5 * cap : scr;  // the last pass's output buffer
     predict_box(a, W, lf, fmin, fmax, scr, scr + 10 * cap, cap, scr + 5 * cap, cap,
-                [&](int i, const int*, const double* v) {
+                [&](int i, const int*, const Real* v) {
 #pragma unroll
                     for (int q = 0; q < 5; ++q) res[q * cap + i] = v[q];
                 });
@@ -379,7 +386,8 @@ struct FinePatch {
 
 // map[d][p] for every box position p along every axis; every thread of the
 // workgroup calls it, a barrier must follow before the map is read
-__device__ __forceinline__ void patch_map(const AmrDev& a, const FinePatch& P, int* map) {
+template <class Real>
+__device__ __forceinline__ void patch_map(const AmrDevT<Real>& a, const FinePatch& P, int* map) {
     for (int i = threadIdx.x; i < 3 * kPatchLen; i += blockDim.x) {
         const int d = i / kPatchLen, p = i - d * kPatchLen;
         // per-axis fields by selection, not by a runtime index (which would
@@ -418,7 +426,8 @@ __device__ __forceinline__ void patch_map(const AmrDev& a, const FinePatch& P, i
 }
 
 // block id (-1: absent) and cell index of a box cell with no outside axis
-__device__ __forceinline__ int patch_block(const AmrDev& a, const FinePatch& P, const int c[3], int* cell) {
+template <class Real>
+__device__ __forceinline__ int patch_block(const AmrDevT<Real>& a, const FinePatch& P, const int c[3], int* cell) {
     int g[3] = {P.f0[0] + c[0], P.f0[1] + c[1], P.f0[2] + c[2]};
     if (P.cached) return P.bids[box_block_of(a, P.bb, g, cell)];
 #pragma unroll
@@ -431,7 +440,8 @@ __device__ __forceinline__ int patch_block(const AmrDev& a, const FinePatch& P, 
 }
 
 // value of a resolved box cell (zero where invalid)
-__device__ __forceinline__ void patch_load(const AmrDev& a, int r, long long base, unsigned neg, double v[5]) {
+template <class Real>
+__device__ __forceinline__ void patch_load(const AmrDevT<Real>& a, int r, long long base, unsigned neg, Real v[5]) {
     if (r == -1) {
 #pragma unroll
         for (int q = 0; q < 5; ++q) v[q] = a.U[q * a.vs + base];
@@ -440,11 +450,11 @@ __device__ __forceinline__ void patch_load(const AmrDev& a, int r, long long bas
         for (int q = 0; q < 5; ++q) v[q] = a.bc_state[r >> 1][r & 1][q];
     } else {
 #pragma unroll
-        for (int q = 0; q < 5; ++q) v[q] = 0.0;
+        for (int q = 0; q < 5; ++q) v[q] = Real(0.0);
     }
 #pragma unroll
     for (int d = 0; d < 3; ++d)
-        if ((neg >> d) & 1) v[1 + d] *= -1.0;
+        if ((neg >> d) & 1) v[1 + d] *= -Real(1.0);
 }
 
 // Box cells along axis D at fixed other coordinates c, resolved for a load
@@ -456,12 +466,12 @@ __device__ __forceinline__ void patch_load(const AmrDev& a, int r, long long bas
 // position (invalid where that block is absent).  The other axes' part of
 // the walk (those above D come before D, those below after it) and of the
 // block and cell index is computed once per row.
-template <int D>
+template <int D, class Real>
 struct PatchRow {
     int r_hi = -1, r_lo = -1;  // status set by the axes above / below D
     unsigned neg_hi = 0, neg_lo = 0;
     int bi = 0, cell = 0;      // block-index and cell-index terms of the other axes
-    __device__ __forceinline__ PatchRow(const AmrDev& a, const FinePatch& P, const int c[3]) {
+    __device__ __forceinline__ PatchRow(const AmrDevT<Real>& a, const FinePatch& P, const int c[3]) {
 #pragma unroll
         for (int d = 2; d >= 0; --d) {
             if (d == D) continue;
@@ -481,17 +491,17 @@ struct PatchRow {
             cell += term_c(a, d, g);
         }
     }
-    __device__ __forceinline__ static int term_b(const AmrDev& a, const FinePatch& P, int d, int g) {
+    __device__ __forceinline__ static int term_b(const AmrDevT<Real>& a, const FinePatch& P, int d, int g) {
         const int k = (a.act[d] ? g >> a.lb[d] : 0) - P.bb.b0[d];
         return d == 0 ? k * P.bb.nbk[1] * P.bb.nbk[2] : d == 1 ? k * P.bb.nbk[2] : k;
     }
-    __device__ __forceinline__ static int term_c(const AmrDev& a, int d, int g) {
+    __device__ __forceinline__ static int term_c(const AmrDevT<Real>& a, int d, int g) {
         const int k = g & (a.bd[d] - 1);
         return d == 0 ? k * a.bd[1] * a.bd[2] : d == 1 ? k * a.bd[2] : k;
     }
     // box cell at position p along D: -1 (block cell, U index base + q * vs),
     // -2 (invalid) or the Dirichlet boundary 2 d + side; neg collects the negated axes
-    __device__ __forceinline__ int resolve(const AmrDev& a, const FinePatch& P, int p, long long& base,
+    __device__ __forceinline__ int resolve(const AmrDevT<Real>& a, const FinePatch& P, int p, long long& base,
                                            unsigned& neg) const {
         int r = r_hi;
         neg = neg_hi;
@@ -517,9 +527,9 @@ struct PatchRow {
 // of its two children, else zero and invalid.
 // The output box (extents nsz) starts at off: coarse cell off[D] along D,
 // patch position off[e] along the other axes.
-template <int D>
-__device__ __forceinline__ void restrict_first_pass(const AmrDev& a, const AmrWeights& W, const FinePatch& P,
-                                                    const int nsz[3], const int off[3], double* out, double* ook,
+template <int D, class Real>
+__device__ __forceinline__ void restrict_first_pass(const AmrDevT<Real>& a, const AmrWeightsT<Real>& W, const FinePatch& P,
+                                                    const int nsz[3], const int off[3], Real* out, Real* ook,
                                                     long long cap) {
     const int ntot = nsz[0] * nsz[1] * nsz[2];
     for (int i = threadIdx.x; i < ntot; i += blockDim.x) {
@@ -527,7 +537,7 @@ __device__ __forceinline__ void restrict_first_pass(const AmrDev& a, const AmrWe
         unflat(i, nsz, c);
 #pragma unroll
         for (int e = 0; e < 3; ++e) c[e] += off[e];
-        const PatchRow<D> row(a, P, c);
+        const PatchRow<D, Real> row(a, P, c);
         const int base = 2 * c[D] + 2;
         unsigned okw = 0;  // bit j: patch cell base - 2 + j is valid
 #pragma unroll
@@ -538,7 +548,7 @@ __device__ __forceinline__ void restrict_first_pass(const AmrDev& a, const AmrWe
             if (p >= 0 && p < P.sz[D] && row.resolve(a, P, p, bj, nj) != -2) okw |= 1u << j;
         }
         bool done = false;
-        double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        Real acc[5] = {Real(0.0), Real(0.0), Real(0.0), Real(0.0), Real(0.0)};
 #pragma unroll 1
         for (int m = 0; m < 5 && !done; ++m) {
             const int sh = (m & 1) ? -((m + 1) >> 1) : (m >> 1);  // shifts 0, -1, 1, -2, 2
@@ -551,7 +561,7 @@ __device__ __forceinline__ void restrict_first_pass(const AmrDev& a, const AmrWe
             unsigned ng[6];
 #pragma unroll
             for (int kk = 0; kk < 6; ++kk) rr[kk] = row.resolve(a, P, st + kk, bs[kk], ng[kk]);
-            double v[6][5];
+            Real v[6][5];
 #pragma unroll
             for (int kk = 0; kk < 6; ++kk) patch_load(a, rr[kk], bs[kk], ng[kk], v[kk]);
 #pragma unroll
@@ -560,9 +570,9 @@ __device__ __forceinline__ void restrict_first_pass(const AmrDev& a, const AmrWe
                 for (int q = 0; q < 5; ++q) acc[q] += W.wts[m][kk] * v[kk][q];
             done = true;
         }
-        double okv = done ? 1.0 : 0.0;
+        Real okv = done ? Real(1.0) : Real(0.0);
         if (!done && ((okw >> 4) & 3u) == 3u) {
-            double va[5], vb[5];
+            Real va[5], vb[5];
             long long ba, bb;
             unsigned na, nb;
             const int ra = row.resolve(a, P, base + 2, ba, na);
@@ -570,8 +580,8 @@ __device__ __forceinline__ void restrict_first_pass(const AmrDev& a, const AmrWe
             patch_load(a, ra, ba, na, va);
             patch_load(a, rb, bb, nb, vb);
 #pragma unroll
-            for (int q = 0; q < 5; ++q) acc[q] = 0.5 * (va[q] + vb[q]);
-            okv = 1.0;
+            for (int q = 0; q < 5; ++q) acc[q] = Real(0.5) * (va[q] + vb[q]);
+            okv = Real(1.0);
         }
 #pragma unroll
         for (int q = 0; q < 5; ++q) out[q * cap + i] = acc[q];
@@ -584,8 +594,9 @@ __device__ __forceinline__ void restrict_first_pass(const AmrDev& a, const AmrWe
 // c[d] over box positions 2 c[d] .. 2 c[d] + 9): the first shifted 6-point
 // stencil whose cells are all valid, else the average of the two children,
 // else zero; returns the validity (1 or 0).
-__device__ __forceinline__ double restrict_cell(const AmrWeights& W, int d, const int c[3], const int sz[3],
-                                                const double* in, const double* iok, long long cap, double acc[5]) {
+template <class Real>
+__device__ __forceinline__ Real restrict_cell(const AmrWeightsT<Real>& W, int d, const int c[3], const int sz[3],
+                                                const Real* in, const Real* iok, long long cap, Real acc[5]) {
     const int shifts[5] = {0, -1, 1, -2, 2};
     int s3[3] = {c[0], c[1], c[2]};
     auto at_src = [&](int pos) {
@@ -595,12 +606,12 @@ __device__ __forceinline__ double restrict_cell(const AmrWeights& W, int d, cons
     const int base = 2 * c[d] + 2;
     bool done = false;
 #pragma unroll
-    for (int q = 0; q < 5; ++q) acc[q] = 0.0;
+    for (int q = 0; q < 5; ++q) acc[q] = Real(0.0);
     for (int m = 0; m < 5 && !done; ++m) {
         const int st = base + shifts[m];
         if (st < 0 || st + 5 >= sz[d]) continue;
         bool all = true;
-        for (int kk = 0; kk < 6; ++kk) all = all && iok[at_src(st + kk)] != 0.0;
+        for (int kk = 0; kk < 6; ++kk) all = all && iok[at_src(st + kk)] != Real(0.0);
         if (all) {
             for (int kk = 0; kk < 6; ++kk) {
                 const int sidx = at_src(st + kk);
@@ -610,13 +621,13 @@ __device__ __forceinline__ double restrict_cell(const AmrWeights& W, int d, cons
             done = true;
         }
     }
-    double okv = done ? 1.0 : 0.0;
+    Real okv = done ? Real(1.0) : Real(0.0);
     if (!done) {
         const int pa = at_src(base + 2), pb = at_src(base + 3);
-        if (iok[pa] != 0.0 && iok[pb] != 0.0) {
+        if (iok[pa] != Real(0.0) && iok[pb] != Real(0.0)) {
 #pragma unroll
-            for (int q = 0; q < 5; ++q) acc[q] = 0.5 * (in[q * cap + pa] + in[q * cap + pb]);
-            okv = 1.0;
+            for (int q = 0; q < 5; ++q) acc[q] = Real(0.5) * (in[q * cap + pa] + in[q * cap + pb]);
+            okv = Real(1.0);
         }
     }
     return okv;
@@ -625,7 +636,8 @@ __device__ __forceinline__ double restrict_cell(const AmrWeights& W, int d, cons
 // Whether the write mask of k_amr_restrict (bit 27: the whole block; bit
 // (o0+1)*9 + (o1+1)*3 + (o2+1): the edge box of width w towards offset o)
 // selects any cell of the block box lo .. hi (inclusive; lo == hi: one cell).
-__device__ __forceinline__ bool restrict_selects(const AmrDev& a, int mask, int w, const int lo[3], const int hi[3]) {
+template <class Real>
+__device__ __forceinline__ bool restrict_selects(const AmrDevT<Real>& a, int mask, int w, const int lo[3], const int hi[3]) {
     if ((mask >> 27) & 1) return true;
     // per axis, offset -1 holds cells c < w, 0 all, 1 cells c >= b - w
     for (int o0 = -1; o0 <= 1; ++o0)
@@ -656,13 +668,14 @@ __device__ __forceinline__ bool restrict_selects(const AmrDev& a, int mask, int 
 // stencils stay inside its own box's fine patch, so its value is the same
 // whichever box the reference restricts it in.  The nx blocks xs are treated
 // as absent from the index (a coarsening that removed them came first).
-__device__ __forceinline__ void restrict_block(const AmrDev& a, const AmrWeights& W, int l, int slot,
-                                               const int idx[3], int mask, int w, double* scr, long long cap,
+template <class Real>
+__device__ __forceinline__ void restrict_block(const AmrDevT<Real>& a, const AmrWeightsT<Real>& W, int l, int slot,
+                                               const int idx[3], int mask, int w, Real* scr, long long cap,
                                                const int* xs, int nx) {
-    double* A = scr;
-    double* Bf = scr + 5 * cap;
-    double* okA = scr + 10 * cap;
-    double* okB = scr + 11 * cap;
+    Real* A = scr;
+    Real* Bf = scr + 5 * cap;
+    Real* okA = scr + 10 * cap;
+    Real* okB = scr + 11 * cap;
     int sz[3];
     FinePatch P;
     P.l = l + 1;
@@ -695,17 +708,17 @@ __device__ __forceinline__ void restrict_block(const AmrDev& a, const AmrWeights
         __syncthreads();
     }
     general = general || !P.cached;  // the in-place reads use the block-id cache
-    double* in = A;
-    double* out = Bf;
-    double* iok = okA;
-    double* ook = okB;
+    Real* in = A;
+    Real* out = Bf;
+    Real* iok = okA;
+    Real* ook = okB;
     int dstart = d0;
     if (general) {
         const int tot = sz[0] * sz[1] * sz[2];
         for (int i = threadIdx.x; i < tot; i += blockDim.x) {
             int c[3];
             unflat(i, sz, c);
-            double v[5];
+            Real v[5];
             bool ok = false;
             bool inside = true;
             for (int d = 0; d < 3; ++d) {
@@ -718,14 +731,14 @@ __device__ __forceinline__ void restrict_block(const AmrDev& a, const AmrWeights
                 const int id = patch_block(a, P, c, &cell);
                 ok = id >= 0;
 #pragma unroll
-                for (int q = 0; q < 5; ++q) v[q] = ok ? a.U[amr_u(a, q, id, cell)] : 0.0;
+                for (int q = 0; q < 5; ++q) v[q] = ok ? a.U[amr_u(a, q, id, cell)] : Real(0.0);
             } else {
 #pragma unroll
-                for (int q = 0; q < 5; ++q) v[q] = 0.0;
+                for (int q = 0; q < 5; ++q) v[q] = Real(0.0);
             }
 #pragma unroll
             for (int q = 0; q < 5; ++q) A[q * cap + i] = v[q];
-            okA[i] = ok ? 1.0 : 0.0;
+            okA[i] = ok ? Real(1.0) : Real(0.0);
         }
         __syncthreads();
         box_bc(a, P.l, P.f0, sz, A, okA, cap);
@@ -756,14 +769,14 @@ __device__ __forceinline__ void restrict_block(const AmrDev& a, const AmrWeights
         for (int i = threadIdx.x; i < ntot; i += blockDim.x) {
             int c[3];
             unflat(i, nsz, c);
-            double acc[5];
-            const double okv = restrict_cell(W, d, c, sz, in, iok, cap, acc);
+            Real acc[5];
+            const Real okv = restrict_cell(W, d, c, sz, in, iok, cap, acc);
 #pragma unroll
             for (int q = 0; q < 5; ++q) out[q * cap + i] = acc[q];
             ook[i] = okv;
         }
         __syncthreads();
-        double* t = in;
+        Real* t = in;
         in = out;
         out = t;
         t = iok;
@@ -795,9 +808,10 @@ struct RestrictSub {
     int cap1, cap2;  // cells of the first and second pass outputs
 };
 
-__device__ __forceinline__ void restrict_subbox(const AmrDev& a, const AmrWeights& W, int l, int slot,
+template <class Real>
+__device__ __forceinline__ void restrict_subbox(const AmrDevT<Real>& a, const AmrWeightsT<Real>& W, int l, int slot,
                                                 const int idx[3], int mask, int w, const RestrictSub& rs, int sb,
-                                                double* T1, double* ok1, double* T2, double* ok2, int* bids,
+                                                Real* T1, Real* ok1, Real* T2, Real* ok2, int* bids,
                                                 int* pmap) {
     int cs[3], ce[3];
     unflat(sb, rs.nsub, cs);
@@ -840,7 +854,7 @@ __device__ __forceinline__ void restrict_subbox(const AmrDev& a, const AmrWeight
         restrict_first_pass<2>(a, W, P, sz, off, T1, ok1, rs.cap1);
     __syncthreads();
     const int dlast = a.act[2] ? 2 : a.act[1] ? 1 : 0;
-    auto store = [&](int i, const int c[3], const double* v) {
+    auto store = [&](int i, const int c[3], const Real* v) {
         int g[3];
 #pragma unroll
         for (int e = 0; e < 3; ++e) g[e] = cs[e] + c[e];
@@ -854,7 +868,7 @@ __device__ __forceinline__ void restrict_subbox(const AmrDev& a, const AmrWeight
         for (int i = threadIdx.x; i < sz[0] * sz[1] * sz[2]; i += blockDim.x) {
             int c[3];
             unflat(i, sz, c);
-            double v[5];
+            Real v[5];
 #pragma unroll
             for (int q = 0; q < 5; ++q) v[q] = T1[q * rs.cap1 + i];
             store(i, c, v);
@@ -862,8 +876,8 @@ __device__ __forceinline__ void restrict_subbox(const AmrDev& a, const AmrWeight
         __syncthreads();
         return;
     }
-    const double* in = T1;
-    const double* iok = ok1;
+    const Real* in = T1;
+    const Real* iok = ok1;
     long long icap = rs.cap1;
 #pragma unroll
     for (int d = 1; d < 3; ++d) {  // unrolled: arrays indexed by d stay in registers
@@ -874,8 +888,8 @@ __device__ __forceinline__ void restrict_subbox(const AmrDev& a, const AmrWeight
         for (int i = threadIdx.x; i < ntot; i += blockDim.x) {
             int c[3];
             unflat(i, nsz, c);
-            double acc[5];
-            const double okv = restrict_cell(W, d, c, sz, in, iok, icap, acc);
+            Real acc[5];
+            const Real okv = restrict_cell(W, d, c, sz, in, iok, icap, acc);
             if (d == dlast) {
                 store(i, c, acc);
             } else {

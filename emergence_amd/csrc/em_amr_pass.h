@@ -19,23 +19,23 @@ namespace em {
 namespace EM_VARIANT {
 
 // LDS of one work item (whole pencils of a B-cell block, P of them)
-template <int MODE, int B, int P>
+template <class Real, int MODE, int B, int P>
 struct AmrLds {
     static constexpr bool HYB = MODE == MODE_HYBRID;
-    double sw[5][B + 10][P];
-    double sre[B + 10][P];  // rho_e of each cell
-    double sm[5][B + 5][P];
-    double ss[2][HYB ? B + 10 : 1][P];
-    double sth[HYB ? B + 1 : 1][P];
+    Real sw[5][B + 10][P];
+    Real sre[B + 10][P];  // rho_e of each cell
+    Real sm[5][B + 5][P];
+    Real ss[2][HYB ? B + 10 : 1][P];
+    Real sth[HYB ? B + 1 : 1][P];
 };
 
 // Work item w of one batched AMR pass: the pencil group w % ngroup of leaf
 // leaf0 + w / ngroup (every thread of the workgroup calls it; it returns
 // after a barrier).  Counters of shock-path faces, fallbacks and limited
 // faces are added to the calling thread's n_*.
-template <int D, int MODE, int B, int P, int NT>
-__device__ __forceinline__ void amr_item(const AmrPassArgs& pa, long long w, AmrLds<MODE, B, P>& L, unsigned& n_shock,
-                                         unsigned& n_fb, unsigned& n_lim) {
+template <int D, int MODE, int B, int P, int NT, class Real>
+__device__ __forceinline__ void amr_item(const AmrPassArgsT<Real>& pa, long long w, AmrLds<Real, MODE, B, P>& L,
+                                         unsigned& n_shock, unsigned& n_fb, unsigned& n_lim) {
     constexpr int C = B + 10;
     constexpr bool HYB = MODE == MODE_HYBRID;
     constexpr int E0 = D == 0 ? 1 : 0;
@@ -45,18 +45,18 @@ __device__ __forceinline__ void amr_item(const AmrPassArgs& pa, long long w, Amr
     auto& sm = L.sm;
     auto& ss = L.ss;
     auto& sth = L.sth;
-    const AmrDev& a = pa.a;
+    const AmrDevT<Real>& a = pa.a;
     const int tid = threadIdx.x;
     const int be1 = a.bd[E1];
     const int nt = a.bcells / B;  // pencils of a block along D
     const int ngh = a.ng;
-    const Eos e = a.eos;
+    const EosT<Real> e = a.eos;
     const double dtv = pa.stage_dt_host ? pa.stage_dt : a.sc->dt;
     const int li = pa.leaf0 + (int)(w / pa.ngroup);
     const int gp = (int)(w % pa.ngroup);
     const long long slot = pa.leaves[li];
     const int lvl = pa.leaf_level[li];
-    const double dx = a.dx[lvl][D];
+    const Real dx = a.dx[lvl][D];
     const int t0 = gp * P;
     const int Pv = min(P, nt - t0);
     // ---- load the pencils (interior and same-level ghosts from the pool, others from the slabs)
@@ -72,7 +72,7 @@ __device__ __forceinline__ void amr_item(const AmrPassArgs& pa, long long w, Amr
         if (p >= Pv) continue;
         const int t = t0 + p;
         const int x = c - 5;
-        double u[5], wv[5];
+        Real u[5], wv[5];
         if (x >= 0 && x < B) {
             int cc[3];
             cc[D] = x;
@@ -104,13 +104,14 @@ __device__ __forceinline__ void amr_item(const AmrPassArgs& pa, long long w, Amr
         for (int q = 0; q < 5; ++q) sw[q][c][p] = wv[q];
         sre[c][p] = eos_rho_e(e, wv[0], wv[4]);
         if constexpr (HYB) {
-            ss[0][c][p] = log(fmax(wv[0], 1e-300));
-            ss[1][c][p] = log(fmax(wv[4] + e.p_inf, 1e-300));
+            ss[0][c][p] = log(fmax(wv[0], Real(1e-300)));
+            ss[1][c][p] = log(fmax(wv[4] + e.p_inf, Real(1e-300)));
         }
     }
     __syncthreads();
-    const double lam = pa.ndim_pos > 0.0 ? pa.ndim_pos * dtv / dx : 0.0;  // amr.rs:1254
-    tile_faces<D, MODE, B, P, NT>(sw, sre, sm, ss, sth, B, Pv, 0, B, e, lam, pa.tol, n_shock, n_fb, n_lim);
+    // amr.rs:1254; formed in double from the device dt and rounded to Real
+    const double lam = pa.ndim_pos > 0.0 ? pa.ndim_pos * dtv / (double)dx : 0.0;
+    tile_faces<D, MODE, B, P, NT>(sw, sre, sm, ss, sth, B, Pv, 0, B, e, Real(lam), pa.tol, n_shock, n_fb, n_lim);
     // ---- residual and boundary face fluxes
     for (int it = tid; it < B * P; it += NT) {
         int c, p;
@@ -131,7 +132,7 @@ __device__ __forceinline__ void amr_item(const AmrPassArgs& pa, long long w, Amr
 #pragma unroll
         for (int q = 0; q < 5; ++q) {
             const long long ix = amr_u(a, q, slot, cell);
-            const double prev = pa.first ? 0.0 : a.R[ix];
+            const Real prev = pa.first ? Real(0.0) : a.R[ix];
             a.R[ix] = prev - (sm[q][c + 1][p] - sm[q][c][p]) / dx;
         }
     }
@@ -145,12 +146,12 @@ __device__ __forceinline__ void amr_item(const AmrPassArgs& pa, long long w, Amr
     __syncthreads();
 }
 
-template <int D, int MODE, int B, int P, int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_WAVES_PER_EU))) void amr_pass(AmrPassArgs pa) {
-    __shared__ AmrLds<MODE, B, P> L;
+template <int D, int MODE, int B, int P, int NT, class Real>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(EM_WAVES_PER_EU))) void amr_pass(AmrPassArgsT<Real> pa) {
+    __shared__ AmrLds<Real, MODE, B, P> L;
     __shared__ unsigned int scnt[3];
 
-    const AmrDev& a = pa.a;
+    const AmrDevT<Real>& a = pa.a;
     if (a.sc->active == 0) return;
     const int tid = threadIdx.x;
     if (tid < 3) scnt[tid] = 0;
@@ -221,9 +222,9 @@ template <int D, int MODE>
 __global__ __launch_bounds__(kAmrMultiThreads) __attribute__((amdgpu_waves_per_eu(3))) void amr_multi(AmrMultiArgs m) {
     constexpr int NT = kAmrMultiThreads;
     __shared__ union {
-        AmrLds<MODE, 8, 16> b8;
-        AmrLds<MODE, 16, 12> b16;
-        AmrLds<MODE, 32, 6> b32;
+        AmrLds<double, MODE, 8, 16> b8;
+        AmrLds<double, MODE, 16, 12> b16;
+        AmrLds<double, MODE, 32, 6> b32;
     } L;
     __shared__ unsigned int scnt[3];
     unsigned int n_shock = 0, n_fb = 0, n_lim = 0;
@@ -261,9 +262,10 @@ static hipError_t launch_amr_multi_d(const AmrMultiArgs& m, unsigned grid, hipSt
 
 // Block sizes with a compiled pass: B = 8, 16, 32, in the shapes of
 // amr_shape (P pencils, NT threads).
-template <int D, int MODE>
-static hipError_t launch_amr_b(int b, int shape, const AmrPassArgs& pa, unsigned grid, hipStream_t s) {
-#define EM_AMR_CASE(B_, P_, NT_) hipLaunchKernelGGL((amr_pass<D, MODE, B_, P_, NT_>), dim3(grid), dim3(NT_), 0, s, pa)
+template <int D, int MODE, class Real>
+static hipError_t launch_amr_b(int b, int shape, const AmrPassArgsT<Real>& pa, unsigned grid, hipStream_t s) {
+#define EM_AMR_CASE(B_, P_, NT_) \
+    hipLaunchKernelGGL((amr_pass<D, MODE, B_, P_, NT_, Real>), dim3(grid), dim3(NT_), 0, s, pa)
     switch (b * 16 + shape) {
         case 8 * 16 + 0: EM_AMR_CASE(8, 8, 128); break;
         case 8 * 16 + 1: EM_AMR_CASE(8, 16, 256); break;

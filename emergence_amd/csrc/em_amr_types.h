@@ -20,12 +20,16 @@ namespace em {
 
 constexpr int kMaxLevels = 8;
 
-struct AmrDev {
-    double* U;
-    double* U0;
-    double* R;
-    double* G;
-    double* F;
+// Real: the type of the pool data and of the arithmetic on it (double, or
+// float for the FP32 variant); the device scalars (dt, time, CFL rate) stay
+// double.
+template <class Real>
+struct AmrDevT {
+    Real* U;
+    Real* U0;
+    Real* R;
+    Real* G;
+    Real* F;
     long long cap;        // pool capacity (blocks)
     long long vs;         // variable stride of U/U0/R = cap * bcells
     long long gs;         // variable stride of G = cap * 6 * ng * tmax
@@ -41,12 +45,13 @@ struct AmrDev {
     int nb0[3];           // level-0 blocks per axis (1 on inactive axes)
     long long map_off[kMaxLevels];  // dense per-level block index offsets
     const int* map;       // block id at (level, block coords) or -1
-    double dx[kMaxLevels][3];
-    Eos eos;
+    Real dx[kMaxLevels][3];
+    EosT<Real> eos;
     int bc_kind[3][2];
-    double bc_state[3][2][5];
+    Real bc_state[3][2][5];
     DevScalars* sc;
 };
+using AmrDev = AmrDevT<double>;
 
 // One ghost slab of one leaf: kind 0 same-level copy from block `src`,
 // 1 physical boundary condition, 2 6th-order prediction from level-1.
@@ -63,8 +68,9 @@ struct RefluxTask {
     int axis, side;
 };
 
-struct AmrPassArgs {
-    AmrDev a;
+template <class Real>
+struct AmrPassArgsT {
+    AmrDevT<Real> a;
     const int* leaves;      // leaf slots (block ids)
     const int* leaf_level;  // level of each leaf
     int leaf0, nleaves;     // leaves [leaf0, leaf0 + nleaves) of the list
@@ -72,7 +78,7 @@ struct AmrPassArgs {
     int first;              // residual input is zero (first active axis)
     int dfast;              // the pass axis is the contiguous one in a block
     double ndim_pos;        // active dimensions if the limiter is on, else 0
-    double tol;
+    Real tol;
     int stage_dt_host;
     double stage_dt;
     // same-level source block of each (block, face) ghost slab, indexed
@@ -80,6 +86,7 @@ struct AmrPassArgs {
     // every ghost is read from the slabs
     const int* gsrc;
 };
+using AmrPassArgs = AmrPassArgsT<double>;
 
 // Several hierarchies (of any block sizes 8, 16, 32) in one batched pass:
 // per-hierarchy pass arguments (leaf0 = 0, nleaves = all its leaves, ngroup
@@ -110,13 +117,16 @@ inline AmrShape amr_shape(int b, int shape) {
     return b == 8 ? s8[shape] : b == 16 ? s16[shape] : s32[shape];
 }
 
-__host__ __device__ inline long long amr_u(const AmrDev& a, int q, long long slot, long long cell) {
+template <class Real>
+__host__ __device__ inline long long amr_u(const AmrDevT<Real>& a, int q, long long slot, long long cell) {
     return q * a.vs + slot * a.bcells + cell;
 }
-__host__ __device__ inline long long amr_g(const AmrDev& a, int q, long long slot, int s, int layer, int t) {
+template <class Real>
+__host__ __device__ inline long long amr_g(const AmrDevT<Real>& a, int q, long long slot, int s, int layer, int t) {
     return q * a.gs + ((slot * 6 + s) * a.ng + layer) * (long long)a.tmax + t;
 }
-__host__ __device__ inline long long amr_f(const AmrDev& a, int q, long long slot, int s, int t) {
+template <class Real>
+__host__ __device__ inline long long amr_f(const AmrDevT<Real>& a, int q, long long slot, int s, int t) {
     return q * a.fs + (slot * 6 + s) * (long long)a.tmax + t;
 }
 

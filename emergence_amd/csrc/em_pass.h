@@ -31,6 +31,21 @@ constexpr int kNumTileCfgs = 10;
             default: return hipErrorInvalidValue;                                          \
         }                                                                                  \
     }                                                                                      \
+    hipError_t launch_amr32_hybrid(int d, int b, int shape, const AmrPassArgsT<float>& pa, unsigned grid,  \
+                                   hipStream_t s);                                                          \
+    hipError_t launch_amr32_weno(int d, int b, int shape, const AmrPassArgsT<float>& pa, unsigned grid,    \
+                                 hipStream_t s);                                                            \
+    hipError_t launch_amr32_central(int d, int b, int shape, const AmrPassArgsT<float>& pa, unsigned grid, \
+                                    hipStream_t s);                                                         \
+    inline hipError_t launch_amr_pass(int d, int mode, int b, int shape, const AmrPassArgsT<float>& pa,    \
+                                      unsigned grid, hipStream_t s) {                                      \
+        switch (mode) {                                                                    \
+            case MODE_HYBRID: return launch_amr32_hybrid(d, b, shape, pa, grid, s);        \
+            case MODE_WENO: return launch_amr32_weno(d, b, shape, pa, grid, s);            \
+            case MODE_CENTRAL: return launch_amr32_central(d, b, shape, pa, grid, s);      \
+            default: return hipErrorInvalidValue;                                          \
+        }                                                                                  \
+    }                                                                                      \
     hipError_t launch_amr_multi_hybrid(int d, const AmrMultiArgs& m, unsigned grid, hipStream_t s);   \
     hipError_t launch_amr_multi_weno(int d, const AmrMultiArgs& m, unsigned grid, hipStream_t s);     \
     hipError_t launch_amr_multi_central(int d, const AmrMultiArgs& m, unsigned grid, hipStream_t s);  \

@@ -38,18 +38,21 @@
 #define EM_LAUNCHER launch_pass_hybrid
 #define EM_LAUNCHER32 launch_pass32_hybrid
 #define EM_AMR_LAUNCHER launch_amr_hybrid
+#define EM_AMR_LAUNCHER32 launch_amr32_hybrid
 #define EM_AMR_MULTI_LAUNCHER launch_amr_multi_hybrid
 #define EM_ENS_LAUNCHER launch_ens_hybrid
 #elif EM_MODE == 1
 #define EM_LAUNCHER launch_pass_weno
 #define EM_LAUNCHER32 launch_pass32_weno
 #define EM_AMR_LAUNCHER launch_amr_weno
+#define EM_AMR_LAUNCHER32 launch_amr32_weno
 #define EM_AMR_MULTI_LAUNCHER launch_amr_multi_weno
 #define EM_ENS_LAUNCHER launch_ens_weno
 #else
 #define EM_LAUNCHER launch_pass_central
 #define EM_LAUNCHER32 launch_pass32_central
 #define EM_AMR_LAUNCHER launch_amr_central
+#define EM_AMR_LAUNCHER32 launch_amr32_central
 #define EM_AMR_MULTI_LAUNCHER launch_amr_multi_central
 #define EM_ENS_LAUNCHER launch_ens_central
 #endif
@@ -378,6 +381,15 @@ hipError_t EM_AMR_MULTI_LAUNCHER(int d, const AmrMultiArgs& m, unsigned grid, hi
 }
 
 hipError_t EM_AMR_LAUNCHER(int d, int b, int shape, const AmrPassArgs& pa, unsigned grid, hipStream_t s) {
+    switch (d) {
+        case 0: return launch_amr_b<0, EM_MODE>(b, shape, pa, grid, s);
+        case 1: return launch_amr_b<1, EM_MODE>(b, shape, pa, grid, s);
+        case 2: return launch_amr_b<2, EM_MODE>(b, shape, pa, grid, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t EM_AMR_LAUNCHER32(int d, int b, int shape, const AmrPassArgsT<float>& pa, unsigned grid, hipStream_t s) {
     switch (d) {
         case 0: return launch_amr_b<0, EM_MODE>(b, shape, pa, grid, s);
         case 1: return launch_amr_b<1, EM_MODE>(b, shape, pa, grid, s);

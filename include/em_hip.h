@@ -294,6 +294,23 @@ int em_amr_create_case(em_amr** out, const int64_t base[3], int32_t b, int32_t l
                        int32_t regrid_every, const double lo[3], const double hi[3], const double eos[4],
                        const int32_t bc_kind[6], const double bc_state[30], int32_t mode, int32_t positivity,
                        double cfl, double tol, int32_t case_id, const double* params);
+/* GpuAmr::<f32>::new (compressible-cuda/src/amr.rs:95-184 with R = f32,
+ * lib.rs:26-48): the same constructors with the pool (state, residual,
+ * ghost slabs, face fluxes) and its arithmetic in FP32 on the device; EOS,
+ * boundary states, mesh spacing and the prediction/restriction weights
+ * rounded to float, the float WENO-Z epsilon 1e-30 (kernels.cu:263).  Host
+ * arrays, times, dt and wavelet details stay double (details are computed
+ * in float and widened).  The handle takes every em_amr_* call; an AMR group
+ * takes FP64 handles only (EM_ERR_ARG). */
+int em_amr_create_f32(em_amr** out, const int64_t base[3], int32_t b, int32_t levels, int32_t ng, double eps,
+                      int32_t regrid_every, const double lo[3], const double hi[3], const double eos[4],
+                      const int32_t bc_kind[6], const double bc_state[30], int32_t mode, int32_t positivity,
+                      double cfl, double tol, int64_t nblocks, const int32_t* level, const int64_t* idx,
+                      const uint8_t* leaf, const double* u);
+int em_amr_create_case_f32(em_amr** out, const int64_t base[3], int32_t b, int32_t levels, int32_t ng, double eps,
+                           int32_t regrid_every, const double lo[3], const double hi[3], const double eos[4],
+                           const int32_t bc_kind[6], const double bc_state[30], int32_t mode, int32_t positivity,
+                           double cfl, double tol, int32_t case_id, const double* params);
 
 int em_amr_destroy(em_amr* h);
 
