@@ -67,7 +67,7 @@ def parse():
     ap.add_argument("--tiles", type=str, default=None,
                     help="fixed Weno tile shapes per axis, e.g. 6,6,5 (indices into TILE_SHAPES; no autotuning)")
     ap.add_argument("--hybrid-tiles", type=str, default=None, help="fixed Hybrid tile shapes per axis")
-    ap.add_argument("--no-f32", action="store_true", help="skip the FP32 lines (256^3 Euler, FiveEq probe)")
+    ap.add_argument("--no-f32", action="store_true", help="skip the FP32 lines (256^3 Euler, FiveEq probe, AMR)")
     ap.add_argument("--f32-tiles", type=str, default=None, help="fixed FP32 tile shapes per axis")
     ap.add_argument("--no-amr", action="store_true", help="skip the block-AMR workload")
     ap.add_argument("--amr-only", action="store_true", help="only the block-AMR workload (profiling)")
@@ -481,6 +481,30 @@ def amr_bench(em, args):
     flop = share * FLOP_PER_CELL_STAGE + (1.0 - share) * FLOP_PER_CELL_STAGE_CENTRAL
     achieved = cells * 3 * flop / (phase[2] * 1e-3)
     cpu = None if args.no_cpu_baseline else amr_cpu_baseline(nleaves * c["b"] ** 3, args.cpu_repeats)
+    f32 = None
+    if not args.no_f32:
+        # the same hierarchy and steps on the FP32 pool (the reference's
+        # GpuAmr<f32>, the precision of its own 3-D bubble runs)
+        leaves_f64 = bat.leaf_count()
+        del bat
+        g32 = em.HipAmr(c["base"], c["b"], c["levels"], c["lo"], c["hi"], blocks=hier, real="f32", **kw)
+        shapes32 = g32.autotune(3)
+        g32.synchronize()
+        l0 = g32.leaf_count()
+        t0 = time.perf_counter()
+        g32.run_steps(args.amr_steps)
+        g32.synchronize()
+        el32 = time.perf_counter() - t0
+        l1 = g32.leaf_count()
+        f32 = {
+            "value": 0.5 * (l0 + l1) * c["b"] ** 3 * args.amr_steps * 3 / el32,
+            "unit": "cell-updates/s per RK stage",
+            "ms_per_step": el32 / args.amr_steps * 1e3,
+            "pass_shapes": shapes32,
+            "leaves_after": l1,
+            "leaves_after_f64": leaves_f64,
+        }
+        del g32
     return {
         "workload": "3-D shock-bubble block AMR, 8^3 blocks, 3 levels, Hybrid, CFL 0.4, regrid every 4",
         "blocks": nblocks,
@@ -499,6 +523,7 @@ def amr_bench(em, args):
                      "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64, "flop_per_cell_stage": flop},
         "cpu_baseline": cpu,
+        "f32": f32,
     }
 
 
