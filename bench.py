@@ -295,6 +295,24 @@ def adjoint_bench(em, args):
     t0 = time.perf_counter()
     du0 = g.backward(dj)
     t_bwd = time.perf_counter() - t0
+    # the same pair with every step's u_n and stage states kept on the device
+    # (checkpoint_every = 0: 120 B per cell and step), so that the backward
+    # recomputes nothing; warm-up pair first, as above
+    g.upload(u0)
+    g.forward(args.adj_steps, 0)
+    g.backward(dj)
+    g.upload(u0)
+    g.synchronize()
+    t0 = time.perf_counter()
+    dts_k = g.forward(args.adj_steps, 0)
+    g.synchronize()
+    t_fwd_k = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    du0_k = g.backward(dj)
+    t_bwd_k = time.perf_counter() - t0
+    kept = {"forward_s": t_fwd_k, "backward_s": t_bwd_k, "cost_ratio": t_bwd_k / t_fwd_k,
+            "device_bytes": 120 * n * n * args.adj_steps,
+            "gradient_identical_to_checkpointed": bool(np.array_equal(du0_k, du0) and np.array_equal(dts_k, dts))}
     q0 = u0[6:6 + n, 6:6 + n, 0]
     vel = q0[..., 1:4] / q0[..., :1]
     dirn = np.concatenate([np.ones_like(q0[..., :1]), vel, 0.5 * np.sum(vel * vel, axis=-1, keepdims=True)], -1)
@@ -357,6 +375,7 @@ def adjoint_bench(em, args):
         "backward_s": t_bwd,
         "cost_ratio": t_bwd / t_fwd,
         "checkpoint_every": args.adj_checkpoint,
+        "kept_states": kept,
         "J": J,
         "grad_max_abs": scale,
         "fd_cells": {"cells": len(errs), "h": "1e-7 rho", "drawn_from": "|dJ/drho_0| >= 1e-2 max",

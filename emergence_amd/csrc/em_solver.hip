@@ -717,6 +717,11 @@ struct Solver : SolverCore<double> {
     double* store2 = nullptr;  // a second segment (recomputed while the first is reversed), if it fits
     long long store_steps = 0;
     double* final_state = nullptr;
+    // checkpoint_every = 0: u_n, u1, u2 of every step of the forward run,
+    // kept in device memory, so that the backward recomputes nothing
+    bool fwd_all = false;
+    double* hist = nullptr;
+    long long hist_steps = 0;
 
     void free_adjoint() {
         for (auto* p : ckpt) (void)hipFree(p);
@@ -729,8 +734,11 @@ struct Solver : SolverCore<double> {
         if (store) (void)hipFree(store);
         if (store2) (void)hipFree(store2);
         if (final_state) (void)hipFree(final_state);
-        adjbuf = store = store2 = final_state = nullptr;
+        if (hist) (void)hipFree(hist);
+        adjbuf = store = store2 = final_state = hist = nullptr;
         store_steps = 0;
+        hist_steps = 0;
+        fwd_all = false;
     }
 
     void adj_check() {
@@ -832,6 +840,19 @@ struct Solver : SolverCore<double> {
         if (!adjbuf) HIPCHK(hipMalloc(&adjbuf, sizeof(double) * 20 * ncell));
     }
 
+    // One step reversed over its kept states st (u_n, u1, u2): T = L'(u2)^T A,
+    // then L'(u1)^T B, then L'(u^n)^T B with the RK updates fused into each
+    // stage's last axis; stage 2's new B goes to A's array (free once stage 3
+    // has read it), stage 1 leaves the step's input cotangent in Zb, and A
+    // and Zb are swapped, so A holds it on return.
+    void reverse_step(const double* st, double dt, double*& A, double* B, double*& Zb, double* T) {
+        const long long nv = 5 * ncell;
+        adj_apply(st + 2 * nv, A, T, dt, 3, A, nullptr, Zb, B);
+        adj_apply(st + nv, B, T, dt, 2, nullptr, B, Zb, A);
+        adj_apply(st, A, T, dt, 1, nullptr, A, Zb, nullptr);
+        std::swap(A, Zb);
+    }
+
     // (dR/du)^T lam at the current state (one residual evaluation)
     void adjoint_rhs(double stage_dt, const double* lam, double* out) {
         adj_check();
@@ -843,33 +864,69 @@ struct Solver : SolverCore<double> {
         soa_to_host(T, out);
     }
 
-    // Forward run of nsteps SSP-RK3 steps with checkpoints every k steps;
-    // dt from dts or, if null, the CFL dt of each step, computed and recorded
-    // on the device (graph-replayed steps, no host wait per step).  Device
-    // buffers of earlier runs (checkpoints, dt history, final state) are
-    // reused, so a repeated run of the same length allocates nothing.
+    // Forward run of nsteps SSP-RK3 steps with checkpoints every k steps, or
+    // with k = 0 every step's u_n and both stage states kept (3 states per
+    // step in device memory; the backward then recomputes nothing); dt from
+    // dts or, if null, the CFL dt of each step, computed and recorded on the
+    // device (graph-replayed steps, no host wait per step).  Device buffers
+    // of earlier runs (checkpoints, dt history, kept states, final state)
+    // are reused, so a repeated run of the same length allocates nothing.
     void forward(int64_t nsteps, int k, const double* dts, double* dts_out) {
-        if (nsteps < 1 || k < 1) throw Error(EM_ERR_ARG, "nsteps and checkpoint_every must be >= 1");
+        if (nsteps < 1 || k < 0) throw Error(EM_ERR_ARG, "nsteps must be >= 1 and checkpoint_every >= 0");
         adj_check();
         if (dts)
             for (int64_t s = 0; s < nsteps; ++s)
                 if (!(std::isfinite(dts[s]) && dts[s] > 0.0)) throw Error(EM_ERR_COLLAPSE, "time step collapsed");
+        const size_t sz = sizeof(double) * 5 * ncell;
+        const bool all = k == 0;
+        nck_used = 0;
+        fwd_all = false;
+        if (all) {
+            if (hist_steps < nsteps) {
+                if (hist) HIPCHK(hipFree(hist));
+                hist = nullptr;
+                hist_steps = 0;
+                if (hipMalloc(&hist, sz * 3 * (size_t)nsteps) != hipSuccess) {
+                    (void)hipGetLastError();
+                    hist = nullptr;
+                    throw Error(EM_ERR_ARG, "keeping every stage state of " + std::to_string(nsteps) + " steps needs " +
+                                                std::to_string(sz * 3 * (size_t)nsteps) +
+                                                " bytes of device memory; use checkpoints");
+                }
+                hist_steps = nsteps;
+            }
+            k = (int)std::min<int64_t>(nsteps, INT32_MAX);
+        } else {
+            const int64_t nck = (nsteps + k - 1) / k;
+            while ((int64_t)ckpt.size() < nck) {
+                double* p;
+                HIPCHK(hipMalloc(&p, sz));
+                ckpt.push_back(p);
+            }
+        }
         ck_every = k;
         fwd_dts.assign((size_t)nsteps, 0.0);
-        const int64_t nck = (nsteps + k - 1) / k;
-        while ((int64_t)ckpt.size() < nck) {
-            double* p;
-            HIPCHK(hipMalloc(&p, sizeof(double) * 5 * ncell));
-            ckpt.push_back(p);
-        }
-        nck_used = nck;
-        if (!final_state) HIPCHK(hipMalloc(&final_state, sizeof(double) * 5 * ncell));
-        const size_t sz = sizeof(double) * 5 * ncell;
+        if (!final_state) HIPCHK(hipMalloc(&final_state, sz));
+        const long long nv = 5 * ncell;
+        // before step s: its checkpoint or kept u_n; after it, the kept stage
+        // states (U[1], U[2] hold u1, u2 at the end of a step)
+        auto keep_before = [&](int64_t s) {
+            if (all)
+                HIPCHK(hipMemcpyAsync(hist + 3 * s * nv, U[0], sz, hipMemcpyDeviceToDevice, stream));
+            else if (s % k == 0)
+                HIPCHK(hipMemcpyAsync(ckpt[s / k], U[0], sz, hipMemcpyDeviceToDevice, stream));
+        };
+        auto keep_after = [&](int64_t s) {
+            if (!all) return;
+            HIPCHK(hipMemcpyAsync(hist + (3 * s + 1) * nv, U[1], sz, hipMemcpyDeviceToDevice, stream));
+            HIPCHK(hipMemcpyAsync(hist + (3 * s + 2) * nv, U[2], sz, hipMemcpyDeviceToDevice, stream));
+        };
         if (dts) {
             for (int64_t s = 0; s < nsteps; ++s) {
-                if (s % k == 0) HIPCHK(hipMemcpyAsync(ckpt[s / k], U[0], sz, hipMemcpyDeviceToDevice, stream));
+                keep_before(s);
                 fwd_dts[s] = dts[s];
                 step_async(dts[s]);
+                keep_after(s);
             }
         } else {
             if (dt_hist_cap < nsteps) {
@@ -885,8 +942,9 @@ struct Solver : SolverCore<double> {
             compute_rate();
             hipGraphExec_t gx = get_graph(2);
             for (int64_t s = 0; s < nsteps; ++s) {
-                if (s % k == 0) HIPCHK(hipMemcpyAsync(ckpt[s / k], U[0], sz, hipMemcpyDeviceToDevice, stream));
+                keep_before(s);
                 HIPCHK(hipGraphLaunch(gx, stream));
+                keep_after(s);
             }
             HIPCHK(hipMemcpyAsync(fwd_dts.data(), dt_hist, sizeof(double) * nsteps, hipMemcpyDeviceToHost, stream));
             Scalars h = read_scalars();  // waits for the stream
@@ -895,17 +953,37 @@ struct Solver : SolverCore<double> {
         HIPCHK(hipMemcpyAsync(final_state, U[0], sz, hipMemcpyDeviceToDevice, stream));
         HIPCHK(hipStreamSynchronize(stream));
         rate_valid = false;
+        nck_used = all ? 1 : (nsteps + k - 1) / k;
+        fwd_all = all;
         if (dts_out) std::copy(fwd_dts.begin(), fwd_dts.end(), dts_out);
     }
 
     // dJ/du_0 from dJ/du_N of the last forward run: per segment, the states
     // are recomputed from its checkpoint (u_n and both RK stage states of
-    // every step kept), then the steps are reversed.
+    // every step kept), then the steps are reversed.  After a forward run
+    // that kept every state (checkpoint_every = 0) the steps are reversed
+    // over those, with nothing recomputed.
     void backward(const double* dj_final, double* dj_initial) {
         if (nck_used == 0) throw Error(EM_ERR_STATE, "backward needs a forward run with checkpoints first");
         ensure_adjbuf();
         const long long nv = 5 * ncell;
         const size_t sz = sizeof(double) * nv;
+        if (fwd_all) {
+            std::vector<DevScalars> saved = save_scalars();
+            double* A = adjbuf;
+            double* B = adjbuf + nv;
+            double* Zb = adjbuf + 2 * nv;
+            double* T = adjbuf + 3 * nv;
+            host_to_soa(dj_final, A);
+            for (int64_t s = (int64_t)fwd_dts.size() - 1; s >= 0; --s)
+                reverse_step(hist + 3 * s * nv, fwd_dts[s], A, B, Zb, T);
+            soa_to_host(A, dj_initial);
+            HIPCHK(hipMemcpyAsync(U[0], final_state, sz, hipMemcpyDeviceToDevice, stream));
+            HIPCHK(hipMemcpyAsync(sc, saved.data(), sizeof(DevScalars), hipMemcpyHostToDevice, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            rate_valid = false;
+            return;
+        }
         if (store_steps < ck_every) {
             if (store) HIPCHK(hipFree(store));
             if (store2) HIPCHK(hipFree(store2));
@@ -969,18 +1047,7 @@ struct Solver : SolverCore<double> {
             HIPCHK(hipStreamWaitEvent(stream, ev_rec[seg & 1], 0));
             const int64_t s0 = seg * ck_every, s1 = std::min<int64_t>(s0 + ck_every, nsteps);
             const double* sto = sbuf[seg & 1];
-            // per step: T = L'(u2)^T A, then L'(u1)^T B, then L'(u^n)^T B with
-            // the RK updates fused into each stage's last axis; stage 2's new
-            // B goes to A's array (free once stage 3 has read it), stage 1
-            // leaves the step's input cotangent in Zb
-            for (int64_t s = s1 - 1; s >= s0; --s) {
-                const double dt = fwd_dts[s];
-                const double* st = sto + 3 * (s - s0) * nv;
-                adj_apply(st + 2 * nv, A, T, dt, 3, A, nullptr, Zb, B);
-                adj_apply(st + nv, B, T, dt, 2, nullptr, B, Zb, A);
-                adj_apply(st, A, T, dt, 1, nullptr, A, Zb, nullptr);
-                std::swap(A, Zb);
-            }
+            for (int64_t s = s1 - 1; s >= s0; --s) reverse_step(sto + 3 * (s - s0) * nv, fwd_dts[s], A, B, Zb, T);
             HIPCHK(hipEventRecord(ev_done[seg & 1], stream));
             if (!store2 && seg > 0) {
                 HIPCHK(hipStreamWaitEvent(rstream, ev_done[seg & 1], 0));

@@ -156,8 +156,9 @@ class HipSolver:
         return out
 
     def forward(self, nsteps, checkpoint_every, dts=None):
-        """nsteps steps with checkpoints; dts frozen (or CFL if None).
-        Returns the dt sequence used."""
+        """nsteps steps with checkpoints every checkpoint_every steps (0: every
+        step's states kept on the device, nothing recomputed by backward);
+        dts frozen (or CFL if None).  Returns the dt sequence used."""
         out = np.empty(int(nsteps), dtype=np.float64)
         d = None if dts is None else _dp(np.ascontiguousarray(dts, dtype=np.float64))
         _lib.check(self.lib.em_solver_forward(self.h, int(nsteps), int(checkpoint_every), d, _dp(out)))

@@ -159,15 +159,20 @@ int em_solver_set_adjoint_tile(em_solver* s, int32_t cells);
 int em_solver_adjoint_rhs(em_solver* s, double stage_dt, const double* lam, double* out);
 
 /* Forward run of nsteps SSP-RK3 steps from the current state, keeping a
- * checkpoint every checkpoint_every steps.  dt per step from dts (frozen
+ * checkpoint every checkpoint_every steps.  checkpoint_every = 0 keeps every
+ * step's u_n and both RK stage states in device memory instead (3 states per
+ * step, 120 bytes per cell and step; EM_ERR_ARG if they do not fit), so that
+ * the backward sweep recomputes nothing.  dt per step from dts (frozen
  * sequence) or, with dts == NULL, the CFL dt of each step; dts_out (may be
  * NULL) receives the sequence used. */
 int em_solver_forward(em_solver* s, int64_t nsteps, int32_t checkpoint_every, const double* dts, double* dts_out);
 
 /* Backward sweep of the last forward run: dJ/du_0 from dJ/du_N (CPU
  * layout).  Each checkpoint segment is recomputed forward keeping u_n and
- * both RK stage states of every step, then the steps are reversed.  The
- * state and time are left as after the forward run. */
+ * both RK stage states of every step, then the steps are reversed (after a
+ * forward run with checkpoint_every = 0, over the kept states, with the same
+ * result bit for bit).  The state and time are left as after the forward
+ * run. */
 int em_solver_backward(em_solver* s, const double* dj_final, double* dj_initial);
 
 /* ------------------------------------------------------------------------

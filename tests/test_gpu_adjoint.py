@@ -202,3 +202,35 @@ def test_adjoint_results_do_not_depend_on_the_tiling(em, case):
     for cells in (37, 10, 300):
         g.set_adjoint_tile(cells)
         assert np.array_equal(g.adjoint_rhs(0.2, lam), ref), cells
+
+
+def test_backward_over_kept_states_equals_the_checkpointed_sweep(em):
+    # checkpoint_every = 0 keeps every step's u_n, u1 and u2 on the device:
+    # the backward reverses the steps over them without recomputing, and the
+    # gradient is the checkpointed sweep's bit for bit (with CFL dts recorded
+    # on the device and with a frozen sequence); the tape bounds both
+    print("adjoint backward over kept states vs checkpoints, 2-D riemann 24x20, 9 steps", flush=True)
+    from emergence_amd import EmError
+
+    n = (24, 20, 1)
+    o = OracleAdj(n, (0, 0, 0), (1, 1, 1), bc=RIEMANN_BC)
+    u0 = fill(o, lambda i, j, k: [v * (1.0 + 0.01 * np.sin(2 * i + 3 * j)) if q == 0 else v
+                                    for q, v in enumerate(riemann(i, j, 24, 0.55))])
+    g = em.HipSolver(n, (0, 0, 0), (1, 1, 1), u0, bc=RIEMANN_BC, mode="weno")
+    dts = g.forward(9, 4)
+    uN = g.download()
+    _, dj = pressure_window(o, uN, 0.3, 0.7)
+    ref = g.backward(dj)
+    g.upload(u0)
+    assert np.array_equal(g.forward(9, 0), dts)  # CFL dts recorded on the device
+    assert np.array_equal(g.download(), uN)
+    got = g.backward(dj)
+    assert np.array_equal(got, ref)
+    assert np.array_equal(g.download(), uN)
+    assert np.array_equal(g.backward(dj), got)
+    g.upload(u0)
+    g.forward(9, 0, dts=dts)  # frozen sequence, kept buffers reused
+    assert np.array_equal(g.backward(dj), ref)
+    assert _rel(o.interior(ref), o.interior(o.gradient(u0, dts, dj))) < 1e-10
+    with pytest.raises(EmError):
+        g.forward(9, -1)
