@@ -133,8 +133,9 @@ int em_solver_time(em_solver* s, double* t);
 int em_solver_synchronize(em_solver* s);
 
 /* ------------------------------------------------------------------------
- * Discrete adjoint (no counterpart in the reference, which has none; the
- * forward/backward pair the survey of the path asks for).  FluxMode::Weno
+ * Discrete adjoint (no counterpart in the reference, which has none: a
+ * checkpointed forward/backward pair for sensitivities of the flow to its
+ * initial state).  FluxMode::Weno
  * only, periodic / outflow / reflect (n >= 5) / Dirichlet boundaries, axes of
  * any length (pencils are cut into tiles of at most 128 cells with their
  * halo; an axis periodic on one side only keeps whole pencils in LDS, up to
