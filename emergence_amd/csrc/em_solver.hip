@@ -908,25 +908,31 @@ struct Solver : SolverCore<double> {
         fwd_dts.assign((size_t)nsteps, 0.0);
         if (!final_state) HIPCHK(hipMalloc(&final_state, sz));
         const long long nv = 5 * ncell;
-        // before step s: its checkpoint or kept u_n; after it, the kept stage
-        // states (U[1], U[2] hold u1, u2 at the end of a step)
-        auto keep_before = [&](int64_t s) {
-            if (all)
-                HIPCHK(hipMemcpyAsync(hist + 3 * s * nv, U[0], sz, hipMemcpyDeviceToDevice, stream));
-            else if (s % k == 0)
-                HIPCHK(hipMemcpyAsync(ckpt[s / k], U[0], sz, hipMemcpyDeviceToDevice, stream));
+        // kept states: the stages of step s write straight into slots 3s + 1
+        // and 3s + 2 and U^{s+1} into slot 3s + 3 (the last step into U[0]),
+        // launched eagerly (the buffers change every step)
+        auto kept_step = [&](int64_t s) {
+            double* h = hist + 3 * s * nv;
+            double* next = s + 1 < nsteps ? h + 3 * nv : U[0];
+            stage(h, h + nv, 1, false, 0, 0.0, nullptr, h);
+            stage(h + nv, h + 2 * nv, 2, false, 0, 0.0, nullptr, h);
+            hipLaunchKernelGGL(k_zero_rate, dim3(1), dim3(kSlots), 0, stream, sc);
+            stage(h + 2 * nv, next, 3, true, 0, 0.0, nullptr, h);
+            hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, stream, sc);
+            HIPCHK(hipGetLastError());
         };
-        auto keep_after = [&](int64_t s) {
-            if (!all) return;
-            HIPCHK(hipMemcpyAsync(hist + (3 * s + 1) * nv, U[1], sz, hipMemcpyDeviceToDevice, stream));
-            HIPCHK(hipMemcpyAsync(hist + (3 * s + 2) * nv, U[2], sz, hipMemcpyDeviceToDevice, stream));
-        };
+        if (all) HIPCHK(hipMemcpyAsync(hist, U[0], sz, hipMemcpyDeviceToDevice, stream));
         if (dts) {
             for (int64_t s = 0; s < nsteps; ++s) {
-                keep_before(s);
                 fwd_dts[s] = dts[s];
-                step_async(dts[s]);
-                keep_after(s);
+                if (all) {
+                    reset_run(0.0);
+                    hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(1), 0, stream, sc, dts[s]);
+                    kept_step(s);
+                } else {
+                    if (s % k == 0) HIPCHK(hipMemcpyAsync(ckpt[s / k], U[0], sz, hipMemcpyDeviceToDevice, stream));
+                    step_async(dts[s]);
+                }
             }
         } else {
             if (dt_hist_cap < nsteps) {
@@ -940,11 +946,19 @@ struct Solver : SolverCore<double> {
             }
             reset_run(0.0);
             compute_rate();
-            hipGraphExec_t gx = get_graph(2);
-            for (int64_t s = 0; s < nsteps; ++s) {
-                keep_before(s);
-                HIPCHK(hipGraphLaunch(gx, stream));
-                keep_after(s);
+            if (all) {
+                // device_step(2) on the kept buffers
+                for (int64_t s = 0; s < nsteps; ++s) {
+                    hipLaunchKernelGGL(k_dt_from_rate, dim3(1), dim3(1), 0, stream, sc, cfl, 0);
+                    hipLaunchKernelGGL(k_record_dt, dim3(1), dim3(1), 0, stream, sc, dt_hist);
+                    kept_step(s);
+                }
+            } else {
+                hipGraphExec_t gx = get_graph(2);
+                for (int64_t s = 0; s < nsteps; ++s) {
+                    if (s % k == 0) HIPCHK(hipMemcpyAsync(ckpt[s / k], U[0], sz, hipMemcpyDeviceToDevice, stream));
+                    HIPCHK(hipGraphLaunch(gx, stream));
+                }
             }
             HIPCHK(hipMemcpyAsync(fwd_dts.data(), dt_hist, sizeof(double) * nsteps, hipMemcpyDeviceToHost, stream));
             Scalars h = read_scalars();  // waits for the stream
