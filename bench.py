@@ -282,25 +282,32 @@ def adjoint_bench(em, args):
         g.set_adjoint_tile(args.adj_tile)
     # warm-up: one full forward/backward pair of the same length, so that the
     # timed pair allocates nothing (checkpoints, dt history, segment stores,
-    # adjoint work arrays) and replays already-instantiated graphs
+    # adjoint work arrays) and replays already-instantiated graphs.  Both
+    # timed runs take and leave their data in device memory: the backward's
+    # cotangents are device arrays (em_solver_backward_device), as the
+    # forward's state is, so no host layout conversion is timed.
     g.forward(args.adj_steps, args.adj_checkpoint)
-    g.backward(pressure_window(g.download(), n)[1])
+    J, dj = pressure_window(g.download(), n)
+    dj_d = em.DeviceArray.from_host(g.to_soa(dj))
+    out_d = em.DeviceArray(dj_d.n)
+    g.backward_device(dj_d, out_d)
     g.upload(u0)
     g.synchronize()
     t0 = time.perf_counter()
     dts = g.forward(args.adj_steps, args.adj_checkpoint)  # CFL dt computed and recorded on the device
     g.synchronize()
     t_fwd = time.perf_counter() - t0
-    J, dj = pressure_window(g.download(), n)
     t0 = time.perf_counter()
-    du0 = g.backward(dj)
+    g.backward_device(dj_d, out_d)
     t_bwd = time.perf_counter() - t0
+    du0 = g.from_soa(out_d.to_host())
+    host_api_same = bool(np.array_equal(g.backward(dj), du0))  # the host-array entry point, untimed
     # the same pair with every step's u_n and stage states kept on the device
     # (checkpoint_every = 0: 120 B per cell and step), so that the backward
     # recomputes nothing; warm-up pair first, as above
     g.upload(u0)
     g.forward(args.adj_steps, 0)
-    g.backward(dj)
+    g.backward_device(dj_d, out_d)
     g.upload(u0)
     g.synchronize()
     t0 = time.perf_counter()
@@ -308,8 +315,9 @@ def adjoint_bench(em, args):
     g.synchronize()
     t_fwd_k = time.perf_counter() - t0
     t0 = time.perf_counter()
-    du0_k = g.backward(dj)
+    g.backward_device(dj_d, out_d)
     t_bwd_k = time.perf_counter() - t0
+    du0_k = g.from_soa(out_d.to_host())
     kept = {"forward_s": t_fwd_k, "backward_s": t_bwd_k, "cost_ratio": t_bwd_k / t_fwd_k,
             "device_bytes": 120 * n * n * args.adj_steps,
             "gradient_identical_to_checkpointed": bool(np.array_equal(du0_k, du0) and np.array_equal(dts_k, dts))}
@@ -375,6 +383,8 @@ def adjoint_bench(em, args):
         "backward_s": t_bwd,
         "cost_ratio": t_bwd / t_fwd,
         "checkpoint_every": args.adj_checkpoint,
+        "backward_entry_point": "em_solver_backward_device (cotangents in device memory)",
+        "host_array_entry_point_identical": host_api_same,
         "kept_states": kept,
         "J": J,
         "grad_max_abs": scale,

@@ -6,6 +6,8 @@ no CPU fallback; a missing or unloadable library raises.
 import ctypes as C
 import os
 
+import numpy as np
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libemhip.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "em_hip.h")
@@ -33,6 +35,7 @@ EXPORTS = (
     "em_solver_set_adjoint_tile",
     "em_solver_forward",
     "em_solver_backward",
+    "em_solver_backward_device",
     "em_case_init",
     "em_rhs_noalloc",
     "em_ensemble_create",
@@ -143,6 +146,7 @@ def load():
     lib.em_solver_set_adjoint_tile.argtypes = [P, C.c_int32]
     lib.em_solver_forward.argtypes = [P, C.c_int64, C.c_int32, D, D]
     lib.em_solver_backward.argtypes = [P, D, D]
+    lib.em_solver_backward_device.argtypes = [P, P, P]
     lib.em_case_init.argtypes = [C.c_int32, I64, D, D, C.c_int32, D, D]
     # device pointers (u, rhs, stats) and the stream travel as plain addresses
     lib.em_rhs_noalloc.argtypes = [I64, D, D, D, I32, D, C.c_int32, C.c_int32, C.c_double, C.c_int32, C.c_double,
@@ -233,3 +237,52 @@ def check_amr(rc):
     if rc != 0:
         raise EmError(rc, load().em_amr_last_error().decode())
     return rc
+
+
+class DeviceArray:
+    """A float64 buffer in device memory (HIP runtime through ctypes), for
+    the entry points that take device arrays; freed with the object."""
+
+    _hip = None
+
+    @classmethod
+    def _rt(cls):
+        if cls._hip is None:
+            load()  # libemhip.so brings the HIP runtime in
+            try:
+                hip = C.CDLL("libamdhip64.so")
+            except OSError:
+                hip = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+            hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            hip.hipFree.argtypes = [C.c_void_p]
+            hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            cls._hip = hip
+        return cls._hip
+
+    def __init__(self, n):
+        self.n = int(n)
+        self.ptr = C.c_void_p()
+        rc = self._rt().hipMalloc(C.byref(self.ptr), 8 * max(self.n, 1))
+        if rc != 0:
+            raise EmError(rc, "hipMalloc failed")
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        d = cls(a.size)
+        rc = cls._rt().hipMemcpy(d.ptr, a.ctypes.data, 8 * a.size, 1)  # host to device
+        if rc != 0:
+            raise EmError(rc, "hipMemcpy failed")
+        return d
+
+    def to_host(self):
+        a = np.empty(self.n, dtype=np.float64)
+        rc = self._rt().hipMemcpy(a.ctypes.data, self.ptr, 8 * self.n, 2)  # device to host
+        if rc != 0:
+            raise EmError(rc, "hipMemcpy failed")
+        return a
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and self.ptr.value and self._hip is not None:
+            self._hip.hipFree(self.ptr)
+            self.ptr = C.c_void_p()

@@ -977,21 +977,36 @@ struct Solver : SolverCore<double> {
     // every step kept), then the steps are reversed.  After a forward run
     // that kept every state (checkpoint_every = 0) the steps are reversed
     // over those, with nothing recomputed.
-    void backward(const double* dj_final, double* dj_initial) {
+    // dev: dj_final and dj_initial are device arrays of the interior cells,
+    // variable-major (5 x ncell, the solver's own layout), else host arrays
+    // in the CPU layout.
+    void backward(const double* dj_final, double* dj_initial, bool dev = false) {
         if (nck_used == 0) throw Error(EM_ERR_STATE, "backward needs a forward run with checkpoints first");
         ensure_adjbuf();
         const long long nv = 5 * ncell;
         const size_t sz = sizeof(double) * nv;
+        auto load_in = [&](double* A) {
+            if (dev)
+                HIPCHK(hipMemcpyAsync(A, dj_final, sz, hipMemcpyDeviceToDevice, stream));
+            else
+                host_to_soa(dj_final, A);
+        };
+        auto store_out = [&](const double* A) {
+            if (dev)
+                HIPCHK(hipMemcpyAsync(dj_initial, A, sz, hipMemcpyDeviceToDevice, stream));
+            else
+                soa_to_host(A, dj_initial);
+        };
         if (fwd_all) {
             std::vector<DevScalars> saved = save_scalars();
             double* A = adjbuf;
             double* B = adjbuf + nv;
             double* Zb = adjbuf + 2 * nv;
             double* T = adjbuf + 3 * nv;
-            host_to_soa(dj_final, A);
+            load_in(A);
             for (int64_t s = (int64_t)fwd_dts.size() - 1; s >= 0; --s)
                 reverse_step(hist + 3 * s * nv, fwd_dts[s], A, B, Zb, T);
-            soa_to_host(A, dj_initial);
+            store_out(A);
             HIPCHK(hipMemcpyAsync(U[0], final_state, sz, hipMemcpyDeviceToDevice, stream));
             HIPCHK(hipMemcpyAsync(sc, saved.data(), sizeof(DevScalars), hipMemcpyHostToDevice, stream));
             HIPCHK(hipStreamSynchronize(stream));
@@ -1017,7 +1032,7 @@ struct Solver : SolverCore<double> {
         double* B = adjbuf + nv;
         double* Zb = adjbuf + 2 * nv;
         double* T = adjbuf + 3 * nv;
-        host_to_soa(dj_final, A);
+        load_in(A);
         reset_run(0.0);
         const int64_t nsteps = (int64_t)fwd_dts.size();
         // Segments are recomputed on rstream into store buffer (seg & 1) and
@@ -1068,7 +1083,7 @@ struct Solver : SolverCore<double> {
                 recompute(seg - 1);
             }
         }
-        soa_to_host(A, dj_initial);
+        store_out(A);
         HIPCHK(hipMemcpyAsync(U[0], final_state, sz, hipMemcpyDeviceToDevice, stream));
         HIPCHK(hipMemcpyAsync(sc, saved.data(), sizeof(DevScalars), hipMemcpyHostToDevice, stream));
         HIPCHK(hipStreamSynchronize(stream));
@@ -2036,6 +2051,15 @@ int em_solver_backward(em_solver* h, const double* dj_final, double* dj_initial)
         return EM_ERR_ARG;
     }
     return guarded([&] { h->s->backward(dj_final, dj_initial); });
+}
+
+int em_solver_backward_device(em_solver* h, const double* dj_final, double* dj_initial) {
+    NEED64(h);
+    if (!dj_final || !dj_initial) {
+        em::g_last_error = "null argument";
+        return EM_ERR_ARG;
+    }
+    return guarded([&] { h->s->backward(dj_final, dj_initial, true); });
 }
 
 // ---------------------------------------------------------------- ensemble

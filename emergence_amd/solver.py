@@ -171,6 +171,26 @@ class HipSolver:
         _lib.check(self.lib.em_solver_backward(self.h, _dp(dj), _dp(out)))
         return out
 
+    def backward_device(self, dj_final, dj_initial):
+        """backward on device arrays (DeviceArray) of the interior cells,
+        variable-major (to_soa / from_soa convert): nothing is converted
+        on the host."""
+        nv = 5 * int(np.prod(self.n))
+        if dj_final.n != nv or dj_initial.n != nv:
+            raise ValueError(f"device arrays must hold 5 x {nv // 5} values")
+        _lib.check(self.lib.em_solver_backward_device(self.h, dj_final.ptr, dj_initial.ptr))
+
+    def to_soa(self, a):
+        """CPU layout -> interior cells, variable-major (5 x ncell)."""
+        a = np.asarray(a, dtype=np.float64).reshape(self.shape)
+        return np.ascontiguousarray(np.moveaxis(self.interior(a), -1, 0)).ravel()
+
+    def from_soa(self, v):
+        """Inverse of to_soa (ghost cells zero)."""
+        out = np.zeros(self.shape, dtype=np.float64)
+        self.interior(out)[...] = np.moveaxis(np.asarray(v, dtype=np.float64).reshape((5,) + tuple(self.n)), 0, -1)
+        return out
+
     def interior(self, a):
         g = [self.ng if v > 1 else 0 for v in self.n]
         return a[g[0]:g[0] + self.n[0], g[1]:g[1] + self.n[1], g[2]:g[2] + self.n[2]]

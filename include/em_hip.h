@@ -175,6 +175,11 @@ int em_solver_forward(em_solver* s, int64_t nsteps, int32_t checkpoint_every, co
  * result bit for bit).  The state and time are left as after the forward
  * run. */
 int em_solver_backward(em_solver* s, const double* dj_final, double* dj_initial);
+/* em_solver_backward on device arrays (no counterpart in the reference):
+ * dj_final and dj_initial hold the interior cells variable-major, value q of
+ * cell (i, j, k) at q * ncell + (i * n1 + j) * n2 + k, so nothing is
+ * converted on the host.  Returns when dj_initial is written. */
+int em_solver_backward_device(em_solver* s, const double* dj_final, double* dj_initial);
 
 /* ------------------------------------------------------------------------
  * Ensemble: independent uniform-grid solvers (own mesh extents, domain,

@@ -232,5 +232,17 @@ def test_backward_over_kept_states_equals_the_checkpointed_sweep(em):
     g.forward(9, 0, dts=dts)  # frozen sequence, kept buffers reused
     assert np.array_equal(g.backward(dj), ref)
     assert _rel(o.interior(ref), o.interior(o.gradient(u0, dts, dj))) < 1e-10
+    # the device-array entry point: the same bits after either forward run
+    dj_d = em.DeviceArray.from_host(g.to_soa(dj))
+    out_d = em.DeviceArray(dj_d.n)
+    assert np.array_equal(o.interior(g.from_soa(dj_d.to_host())), o.interior(dj))
+    g.backward_device(dj_d, out_d)
+    assert np.array_equal(g.from_soa(out_d.to_host()), ref)
+    g.upload(u0)
+    g.forward(9, 4)
+    g.backward_device(dj_d, out_d)
+    assert np.array_equal(g.from_soa(out_d.to_host()), ref)
+    with pytest.raises(ValueError):
+        g.backward_device(dj_d, em.DeviceArray(7))
     with pytest.raises(EmError):
         g.forward(9, -1)
