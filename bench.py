@@ -903,7 +903,7 @@ def main():
         rec["amr_group"] = amr_group_bench(em, args)
     if not args.no_adjoint:
         rec["adjoint"] = adjoint_bench(em, args)
-        rec["adjoint_forward_cost_ratio"] = rec["adjoint"]["cost_ratio"]  # checkpoints every 20
+        rec["adjoint_forward_cost_ratio"] = rec["adjoint"]["cost_ratio"]  # checkpoints every --adj-checkpoint steps
         rec["adjoint_forward_cost_ratio_kept_states"] = rec["adjoint"]["kept_states"]["cost_ratio"]
     if not args.no_ensemble:
         rec["ensemble"] = ensemble_bench(em, args)
