@@ -2483,6 +2483,7 @@ struct AmrGroup {
     long long* d_prefix[3] = {nullptr, nullptr, nullptr};
     long long total[3] = {0, 0, 0};
     int mode = 0, precise = 1;
+    int ncu = 256;  // compute units of the device
     // host images of the uploaded arguments (kept alive for the async copies;
     // an unchanged step skips the upload)
     std::vector<AmrPassArgs> h_args[3];
@@ -2502,6 +2503,9 @@ struct AmrGroup {
             if (!a->batched) throw AmrError(EM_ERR_ARG, "AMR group members must run batched");
         }
         hchk(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+        int dev = 0;
+        hchk(hipGetDevice(&dev), "hipGetDevice");
+        hchk(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
         const int n = (int)m.size();
         for (int d = 0; d < 3; ++d) {
             hchk(hipMalloc(&d_args[d], sizeof(AmrPassArgs) * n), "hipMalloc");
@@ -2585,7 +2589,10 @@ struct AmrGroup {
         for (int d = 0; d < 3; ++d) {
             if (!m[0]->act[d] || total[d] == 0) continue;
             const AmrMultiArgs ma{d_args[d], d_prefix[d], (int)m.size(), total[d]};
-            const unsigned grid = (unsigned)std::min<long long>(total[d], 2048);
+            // three full rounds of resident workgroups (three per CU): a
+            // partial last round (2,048 workgroups were 2.67 rounds) left
+            // CUs idle at the end of each pass, 2% of the group's step
+            const unsigned grid = (unsigned)std::min<long long>(total[d], 9ll * ncu);
             hchk(precise ? em::precise::launch_amr_multi(d, mode, ma, grid, stream)
                          : em::fast::launch_amr_multi(d, mode, ma, grid, stream),
                  "amr_multi");
